@@ -1,0 +1,141 @@
+"""Plain-data body descriptors, the POD form of a geometry that crosses the C ABI.
+
+The reference takes any SDFLibrary geometry or callable ``x -> phi`` (``sdf_value``,
+``src/marching_squares/cache.jl:69``) and, for PLIC, ``get_sdf`` at a cell centroid
+(``src/plic/plic.jl:74``).  A closure cannot cross a C ABI, so a body here is one of a fixed set of
+descriptors (``kind`` + up to 24 parameters + an optional primitive table).  An arbitrary field is
+still supported by sampling it into ``phi`` yourself and calling ``update_cache_from_phi``.
+
+SDFLibrary is not vendored with the reference; these are the build's own signed-distance functions
+(negative inside, ``||.||`` accumulated as ``((dx^2 + dy^2) + dz^2)``):
+
+* ``SDFSphere`` / ``SDFCircle``: ``||x - c|| - r``; normal ``(x - c)/||x - c||``.
+* ``SDFTorus``: in the frame ``p = R^T (x - c)``, ``q = (sqrt(px^2 + py^2) - R, pz)``,
+  ``phi = ||q|| - r``; normal is the analytic gradient rotated back.
+* ``SDFBox``: the exact box distance in its rotated frame; normal from ``sign(p)`` (zero on a
+  medial plane, which PLIC reports as a degenerate fit).
+* ``SDFPlane``: ``n . (x - p)``.
+* ``SDFSphereUnion``: spheres folded left with the polynomial smooth minimum
+  ``h = max(k - |a-b|, 0)/k; min(a, b) - h*h*k/4``.
+* ``EmptyGeo``: the constant field ``value`` (the reference seeds every cache with ``EmptyGeo(1)``,
+  ``src/marching_squares/cache.jl:46``).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+GEO_EMPTY, GEO_SPHERE, GEO_TORUS, GEO_BOX, GEO_PLANE, GEO_SPHERE_UNION = range(6)
+
+_EYE = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+
+
+def _vec3(v):
+    v = [float(x) for x in v]
+    return v + [0.0] * (3 - len(v))
+
+
+def _rot(r):
+    if r is None:
+        return list(_EYE)
+    r = np.asarray(r, dtype=np.float64).reshape(-1)
+    if r.size == 4:  # a 2x2 rotation, embedded
+        return [r[0], r[1], 0.0, r[2], r[3], 0.0, 0.0, 0.0, 1.0]
+    return [float(x) for x in r]
+
+
+@dataclass
+class EmptyGeo:
+    value: float = 1.0
+    kind = GEO_EMPTY
+    prims = None
+
+    @property
+    def params(self):
+        return [float(self.value)]
+
+
+@dataclass
+class SDFSphere:
+    center: tuple
+    radius: float
+    kind = GEO_SPHERE
+    prims = None
+
+    @property
+    def params(self):
+        return _vec3(self.center) + [float(self.radius)]
+
+
+SDFCircle = SDFSphere  # the same descriptor in 2D (the third centre coordinate is ignored)
+
+
+@dataclass
+class SDFTorus:
+    center: tuple
+    major_radius: float
+    minor_radius: float
+    rotation: object = None  # 3x3, row-major; the torus axis is R @ e_z
+    kind = GEO_TORUS
+    prims = None
+
+    @property
+    def params(self):
+        return (_vec3(self.center) + [float(self.major_radius), float(self.minor_radius)] +
+                _rot(self.rotation))
+
+
+@dataclass
+class SDFBox:
+    center: tuple
+    half_extents: tuple
+    rotation: object = None
+    kind = GEO_BOX
+    prims = None
+
+    @property
+    def params(self):
+        return _vec3(self.center) + _vec3(self.half_extents) + _rot(self.rotation)
+
+
+@dataclass
+class SDFPlane:
+    normal: tuple
+    point: tuple
+    kind = GEO_PLANE
+    prims = None
+
+    @property
+    def params(self):
+        return _vec3(self.normal) + _vec3(self.point)
+
+
+@dataclass
+class SDFSphereUnion:
+    centers: np.ndarray
+    radii: np.ndarray
+    smoothing: float = 0.02
+    kind = GEO_SPHERE_UNION
+    _prims: np.ndarray = field(default=None, repr=False)
+
+    def __post_init__(self):
+        c = np.asarray(self.centers, dtype=np.float64).reshape(-1, 3)
+        r = np.asarray(self.radii, dtype=np.float64).reshape(-1, 1)
+        self._prims = np.ascontiguousarray(np.hstack([c, r]))
+
+    @property
+    def prims(self):
+        return self._prims
+
+    @property
+    def params(self):
+        return [float(self.smoothing)]
+
+
+def random_rotation(rng):
+    """A uniformly random rotation matrix from a unit quaternion drawn by ``rng`` (numpy)."""
+    q = rng.standard_normal(4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])

@@ -1,0 +1,993 @@
+/* CPU oracle for the cut-cell update hot path -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+ *
+ * A plain-C restatement of CutCellMethods.jl's Float64 update_cache! (nodal marching squares,
+ * nodal marching cubes, PLIC), written to be the checker the HIP kernels are compared against.
+ * Only tests, the smoke check and the benchmark's CPU-baseline leg load it.
+ *
+ * Every function cites the reference file:line it restates.  Arithmetic follows the reference
+ * expression by expression: StaticArrays reductions are left folds in written order, Julia's n-ary
+ * `+` is a left fold, and the file is compiled with -ffp-contract=off so no a*b+c is fused.
+ * Indices are 0-based throughout; the reference's 1-based tables and dispatch constants are shifted
+ * at each use and the shift is noted there.
+ *
+ * Third-party pieces the reference calls but does not contain (un-vendored):
+ *   - Lewiner tables (MarchingCubes.jl 0.1.11): lewiner_tables.h, data only.
+ *   - get_volume_fraction (CartesianMeshes 0.2.3): restated from Scardovelli & Zaleski (2000),
+ *     J. Comput. Phys. 164, 228-237.  Bits unpinned against CartesianMeshes; pinned by the
+ *     marching-cubes plane tests (two unrelated algorithms agreeing to 1e-14).
+ *   - get_node / get_elem_centroid (CartesianMeshes): x0 + i*d and x0 + (i + 0.5)*d.
+ *   - sdf_value / get_sdf (SDFLibrary 0.10.5): the build's own geometry (sphere, torus, box, plane,
+ *     polynomial smooth union); parity unpinned against SDFLibrary, which has no literal outputs in
+ *     the reference's tests.
+ */
+#include "cutcell_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define CC_LUT_QUAL static const
+#include "../cutcellmethods.jl_amd/csrc/lewiner_tables.h"
+
+#define EPS64 2.220446049250313e-16 /* eps(Float64) */
+
+enum { CELL_INSIDE = 0, CELL_OUTSIDE = 1, CELL_CUT = 2 }; /* src/CutCellMethods.jl:56-58 */
+
+/* =========================================================================================== */
+/* Geometry (the build's own; SDFLibrary is un-vendored)                                        */
+/* =========================================================================================== */
+
+static double sgn(double v) { return v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : 0.0); }
+
+/* p = R^T (x - c), R row-major 3x3 at r[0..8]. */
+static void to_local(const double *r, const double *c, const double *x, int dim, double *p)
+{
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < dim; ++a) v[a] = x[a] - c[a];
+    for (int i = 0; i < dim; ++i) {
+        double s = r[0 * 3 + i] * v[0];
+        for (int j = 1; j < dim; ++j) s = s + r[j * 3 + i] * v[j];
+        p[i] = s;
+    }
+}
+
+static void from_local(const double *r, const double *gl, int dim, double *n)
+{
+    for (int i = 0; i < dim; ++i) {
+        double s = r[i * 3 + 0] * gl[0];
+        for (int j = 1; j < dim; ++j) s = s + r[i * 3 + j] * gl[j];
+        n[i] = s;
+    }
+}
+
+static double sphere_dist(const double *c, double r, int dim, const double *x, double *nr_out,
+                          double *v)
+{
+    double s = 0.0;
+    for (int a = 0; a < dim; ++a) v[a] = x[a] - c[a];
+    s = v[0] * v[0];
+    for (int a = 1; a < dim; ++a) s = s + v[a] * v[a];
+    double nr = sqrt(s);
+    if (nr_out) *nr_out = nr;
+    return nr - r;
+}
+
+static double smin_poly(double a, double b, double k)
+{
+    /* h = max(k - |a-b|, 0)/k; smin = min(a,b) - h*h*k/4 (polynomial smooth minimum). */
+    double h = k - fabs(a - b);
+    h = (h > 0.0 ? h : 0.0) / k;
+    double m = a < b ? a : b;
+    return m - ((h * h) * k) / 4.0;
+}
+
+double orc_get_sdf(const orc_geometry *g, int dim, const double *x, double *normal)
+{
+    double v[3] = {0.0, 0.0, 0.0}, p[3] = {0.0, 0.0, 0.0}, gl[3] = {0.0, 0.0, 0.0};
+    switch (g->kind) {
+    case ORC_GEO_EMPTY:
+        for (int a = 0; a < dim; ++a) normal[a] = 0.0;
+        return g->p[0];
+    case ORC_GEO_SPHERE: {
+        double c[3] = {g->p[0], g->p[1], g->p[2]}, nr;
+        double dist = sphere_dist(c, g->p[3], dim, x, &nr, v);
+        for (int a = 0; a < dim; ++a) normal[a] = v[a] / nr;
+        return dist;
+    }
+    case ORC_GEO_TORUS: { /* 3D only */
+        const double *r = g->p + 5;
+        to_local(r, g->p, x, 3, p);
+        double rho = sqrt(p[0] * p[0] + p[1] * p[1]);
+        double qx = rho - g->p[3], qz = p[2];
+        double nq = sqrt(qx * qx + qz * qz);
+        double u = qx / nq;
+        gl[0] = u * (p[0] / rho);
+        gl[1] = u * (p[1] / rho);
+        gl[2] = qz / nq;
+        from_local(r, gl, 3, normal);
+        return nq - g->p[4];
+    }
+    case ORC_GEO_BOX: {
+        const double *r = g->p + 6;
+        double q[3] = {0.0, 0.0, 0.0}, mx[3] = {0.0, 0.0, 0.0};
+        to_local(r, g->p, x, dim, p);
+        for (int a = 0; a < dim; ++a) {
+            q[a] = fabs(p[a]) - g->p[3 + a];
+            mx[a] = q[a] > 0.0 ? q[a] : 0.0;
+        }
+        double o = mx[0] * mx[0];
+        for (int a = 1; a < dim; ++a) o = o + mx[a] * mx[a];
+        o = sqrt(o);
+        double qm = q[0];
+        int am = 0;
+        for (int a = 1; a < dim; ++a)
+            if (q[a] > qm) { qm = q[a]; am = a; }
+        double inner = qm < 0.0 ? qm : 0.0;
+        if (qm > 0.0) {
+            for (int a = 0; a < dim; ++a) gl[a] = (mx[a] / o) * sgn(p[a]);
+        } else {
+            for (int a = 0; a < dim; ++a) gl[a] = 0.0;
+            gl[am] = sgn(p[am]);
+        }
+        from_local(r, gl, dim, normal);
+        return o + inner;
+    }
+    case ORC_GEO_PLANE: {
+        double s = g->p[0] * (x[0] - g->p[3]);
+        for (int a = 1; a < dim; ++a) s = s + g->p[a] * (x[a] - g->p[3 + a]);
+        for (int a = 0; a < dim; ++a) normal[a] = g->p[a];
+        return s;
+    }
+    case ORC_GEO_SPHERE_UNION: { /* distance only; no analytic normal is offered */
+        for (int a = 0; a < dim; ++a) normal[a] = 0.0;
+        return orc_sdf_value(g, dim, x);
+    }
+    }
+    for (int a = 0; a < dim; ++a) normal[a] = 0.0;
+    return NAN;
+}
+
+double orc_sdf_value(const orc_geometry *g, int dim, const double *x)
+{
+    double v[3] = {0.0, 0.0, 0.0};
+    if (g->kind == ORC_GEO_SPHERE_UNION) {
+        const double *s = g->prims;
+        double k = g->p[0];
+        double acc = sphere_dist(s, s[3], dim, x, NULL, v);
+        for (int i = 1; i < g->n_prims; ++i) {
+            const double *si = s + 4 * i;
+            acc = smin_poly(acc, sphere_dist(si, si[3], dim, x, NULL, v), k);
+        }
+        return acc;
+    }
+    if (g->kind == ORC_GEO_SPHERE) {
+        double c[3] = {g->p[0], g->p[1], g->p[2]};
+        return sphere_dist(c, g->p[3], dim, x, NULL, v);
+    }
+    double nrm[3];
+    return orc_get_sdf(g, dim, x, nrm);
+}
+
+/* src/marching_squares/cache.jl:67-70: phi[ni] = sdf_value(geo, get_node(grid, ni)). */
+void orc_sample_nodes(const orc_geometry *g, int dim, const double *x0, const double *d,
+                      const int64_t *n, double *phi)
+{
+    int64_t m0 = n[0] + 1, m1 = n[1] + 1, m2 = dim == 3 ? n[2] + 1 : 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < m2; ++k)
+        for (int64_t j = 0; j < m1; ++j)
+            for (int64_t i = 0; i < m0; ++i) {
+                double x[3] = {x0[0] + (double)i * d[0], x0[1] + (double)j * d[1],
+                               dim == 3 ? x0[2] + (double)k * d[2] : 0.0};
+                phi[i + m0 * (j + m1 * k)] = orc_sdf_value(g, dim, x);
+            }
+}
+
+/* =========================================================================================== */
+/* Marching squares primitives: src/marching_squares/marching_squares.jl                       */
+/* =========================================================================================== */
+
+static inline int ms_next(int k) { return k == 3 ? 0 : k + 1; } /* :179 */
+static inline int ms_prev(int k) { return k == 0 ? 3 : k - 1; } /* :180 */
+
+/* MS_DIRECTION_EDGE (:82) = (4,2,1,3) 1-based -> 0-based edge per 0-based direction. */
+static const int MS_DIR_EDGE[4] = {3, 1, 0, 2};
+
+/* _open_fraction, :192-198 */
+static inline double open_fraction(double a, double b)
+{
+    int fa = a >= 0.0, fb = b >= 0.0;
+    if (fa && fb) return 1.0;
+    if (!fa && !fb) return 0.0;
+    return fa ? a / (a - b) : b / (b - a);
+}
+
+/* _open_segment, :208-217 */
+static inline void open_segment(const double *pa, const double *pb, double fa, double fb,
+                                double frac, double *sa, double *sb)
+{
+    if (fa >= 0.0 && fb >= 0.0) {
+        sa[0] = pa[0]; sa[1] = pa[1]; sb[0] = pb[0]; sb[1] = pb[1];
+    } else if (fa >= 0.0) {
+        sa[0] = pa[0]; sa[1] = pa[1];
+        sb[0] = pa[0] + frac * (pb[0] - pa[0]);
+        sb[1] = pa[1] + frac * (pb[1] - pa[1]);
+    } else {
+        sa[0] = pb[0] + frac * (pa[0] - pb[0]);
+        sa[1] = pb[1] + frac * (pa[1] - pb[1]);
+        sb[0] = pb[0]; sb[1] = pb[1];
+    }
+}
+
+/* cell_boundary_walk, :240-251 */
+static void boundary_walk(const double nodes[4][2], const double phi[4], double fracs[4],
+                          int present[4], double seg_a[4][2], double seg_b[4][2])
+{
+    for (int k = 0; k < 4; ++k) {
+        int j = ms_next(k);
+        fracs[k] = open_fraction(phi[k], phi[j]);
+        present[k] = phi[k] >= 0.0 || phi[j] >= 0.0;
+        open_segment(nodes[k], nodes[j], phi[k], phi[j], fracs[k], seg_a[k], seg_b[k]);
+    }
+}
+
+/* _next_present, :255-263 */
+static int next_present(const int present[4], int k)
+{
+    int j = ms_next(k);
+    if (present[j]) return j;
+    j = ms_next(j);
+    if (present[j]) return j;
+    j = ms_next(j);
+    if (present[j]) return j;
+    return k;
+}
+
+/* cell_gap_segments, :291-317 */
+static int gap_segments(const int present[4], double seg_a[4][2], double seg_b[4][2],
+                        double ga[2][2], double gb[2][2])
+{
+    int n = 0;
+    ga[0][0] = ga[0][1] = ga[1][0] = ga[1][1] = 0.0;
+    gb[0][0] = gb[0][1] = gb[1][0] = gb[1][1] = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        if (!present[k]) continue;
+        int j = next_present(present, k);
+        const double *a = seg_b[k], *b = seg_a[j];
+        if (a[0] != b[0] || a[1] != b[1]) {
+            n += 1;
+            if (n <= 2) {
+                ga[n - 1][0] = a[0]; ga[n - 1][1] = a[1];
+                gb[n - 1][0] = b[0]; gb[n - 1][1] = b[1];
+            }
+        }
+    }
+    return n < 2 ? n : 2;
+}
+
+/* =========================================================================================== */
+/* 2D moments: src/marching_squares/moments.jl:39-163                                           */
+/* =========================================================================================== */
+
+/* _full_face_centre (:39-42) then _face_offset (:48-51): the stored 1-component offset. */
+static double ms_full_face_offset(const double nodes[4][2], const double origin[2], int dir)
+{
+    int k = MS_DIR_EDGE[dir], k2 = ms_next(k);
+    int p = 1 - (dir >> 1); /* 3 - direction_axis(dir), 0-based */
+    double x = 0.5 * (nodes[k][p] + nodes[k2][p]);
+    return x - origin[p];
+}
+
+static void ms_cut_cell_moments(const double nodes[4][2], const double phi[4],
+                                const double cs[2], int8_t *kind, uint8_t *amb, double *vf,
+                                double centroid[2], double ff[4], double fcl[4], double ic[2])
+{
+    const double *origin = nodes[0];
+    double centre[2] = {origin[0] + 0.5 * cs[0], origin[1] + 0.5 * cs[1]};
+    double vol_cell = cs[0] * cs[1];
+    int noutside = 0;
+    for (int k = 0; k < 4; ++k) noutside += phi[k] >= 0.0 ? 1 : 0;
+
+    if (noutside == 4 || noutside == 0) { /* :84-93 */
+        int outside = noutside == 4;
+        *kind = outside ? CELL_OUTSIDE : CELL_INSIDE;
+        *amb = 0;
+        *vf = outside ? 1.0 : 0.0;
+        centroid[0] = centre[0]; centroid[1] = centre[1];
+        for (int dir = 0; dir < 4; ++dir) {
+            ff[dir] = outside ? 1.0 : 0.0;
+            fcl[dir] = ms_full_face_offset(nodes, origin, dir);
+        }
+        ic[0] = centre[0]; ic[1] = centre[1];
+        return;
+    }
+
+    double fracs[4], seg_a[4][2], seg_b[4][2];
+    int present[4];
+    boundary_walk(nodes, phi, fracs, present, seg_a, seg_b); /* :99 */
+
+    for (int dir = 0; dir < 4; ++dir) { /* :105-110 */
+        int k = MS_DIR_EDGE[dir];
+        ff[dir] = fracs[k];
+        int p = 1 - (dir >> 1);
+        if (present[k]) {
+            double x = 0.5 * (seg_a[k][p] + seg_b[k][p]);
+            fcl[dir] = x - origin[p];
+        } else {
+            fcl[dir] = ms_full_face_offset(nodes, origin, dir);
+        }
+    }
+
+    double area2 = 0.0, cm0 = 0.0, cm1 = 0.0; /* :117-127 */
+    for (int k = 0; k < 4; ++k) {
+        if (!present[k]) continue;
+        double la0 = seg_a[k][0] - origin[0], la1 = seg_a[k][1] - origin[1];
+        double lb0 = seg_b[k][0] - origin[0], lb1 = seg_b[k][1] - origin[1];
+        double cr = la0 * lb1 - la1 * lb0;
+        area2 += cr;
+        cm0 += (la0 + lb0) * cr;
+        cm1 += (la1 + lb1) * cr;
+    }
+
+    double ga[2][2], gb[2][2];
+    int nif = gap_segments(present, seg_a, seg_b, ga, gb); /* :129 */
+    double ilen = 0.0, icm0 = 0.0, icm1 = 0.0;
+    for (int s = 0; s < nif; ++s) { /* :133-150 */
+        const double *a = ga[s], *b = gb[s];
+        double la0 = a[0] - origin[0], la1 = a[1] - origin[1];
+        double lb0 = b[0] - origin[0], lb1 = b[1] - origin[1];
+        double cr = la0 * lb1 - la1 * lb0;
+        area2 += cr;
+        cm0 += (la0 + lb0) * cr;
+        cm1 += (la1 + lb1) * cr;
+        double v0 = b[0] - a[0], v1 = b[1] - a[1];
+        double len = sqrt(v0 * v0 + v1 * v1);
+        ilen += len;
+        double hl = 0.5 * len;
+        icm0 += hl * (a[0] + b[0]);
+        icm1 += hl * (a[1] + b[1]);
+    }
+
+    double area = 0.5 * area2; /* :152-163 */
+    if (area2 != 0.0) {
+        double den = 3.0 * area2;
+        centroid[0] = origin[0] + cm0 / den;
+        centroid[1] = origin[1] + cm1 / den;
+    } else {
+        centroid[0] = centre[0]; centroid[1] = centre[1];
+    }
+    if (ilen > 0.0) {
+        ic[0] = icm0 / ilen; ic[1] = icm1 / ilen;
+    } else {
+        ic[0] = centre[0]; ic[1] = centre[1];
+    }
+    *kind = CELL_CUT;
+    *amb = nif > 1;
+    *vf = area / vol_cell;
+}
+
+/* _nodal_cells_kernel! (src/marching_squares/cache.jl:75-78) over a 2D grid. */
+void orc_ms_cells(const double *phi, const double *x0, const double *d, const int64_t *n,
+                  const orc_cols *out)
+{
+    int64_t m0 = n[0] + 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n[1]; ++j)
+        for (int64_t i = 0; i < n[0]; ++i) {
+            static const int B[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}}; /* MS_NODE_BITS :68 */
+            double nodes[4][2], ph[4];
+            for (int k = 0; k < 4; ++k) {
+                int64_t ii = i + B[k][0], jj = j + B[k][1];
+                nodes[k][0] = x0[0] + (double)ii * d[0];
+                nodes[k][1] = x0[1] + (double)jj * d[1];
+                ph[k] = phi[ii + m0 * jj];
+            }
+            int64_t c = i + n[0] * j;
+            ms_cut_cell_moments(nodes, ph, d, &out->kind[c], &out->ambiguous[c],
+                                &out->volume_fraction[c], out->centroid + 2 * c,
+                                out->face_fraction + 4 * c, out->face_centroid_local + 4 * c,
+                                out->interface_centroid + 2 * c);
+        }
+}
+
+/* =========================================================================================== */
+/* Marching cubes: src/marching_cubes/marching_cubes.jl                                         */
+/* =========================================================================================== */
+
+/* MC_NODE_BITS (:106-107), 0-based corner p -> ((p^(p>>1))&1, (p>>1)&1, (p>>2)&1). */
+static const int MC_BITS[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                  {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+/* MC_EDGE_NODES (:135-137), 0-based. */
+static const int MC_EDGE[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
+                                   {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+/* MC_FACE_AXES (:146-149) and MC_FACE_NODES (:169-180), 0-based. */
+static const int MC_FACE_AX[6][2] = {{1, 2}, {1, 2}, {0, 2}, {0, 2}, {0, 1}, {0, 1}};
+static const int MC_FACE_NODE[6][4] = {{0, 3, 7, 4}, {1, 2, 6, 5}, {0, 1, 5, 4},
+                                       {3, 2, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
+/* MC_TEST_FACE_QUADS (:372-373), 0-based corners; MC_FACE_TEST_CODE (:385-388). */
+static const int MC_TEST_QUAD[6][4] = {{0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3},
+                                       {3, 7, 4, 0}, {0, 3, 2, 1}, {4, 7, 6, 5}};
+static const int MC_FACE_TEST_CODE[6] = {4, 2, 1, 3, 5, 6};
+
+/* nudge_zeros, :240-243 */
+static void nudge_zeros(const double *raw, double *phi)
+{
+    for (int k = 0; k < 8; ++k) phi[k] = fabs(raw[k]) < EPS64 ? EPS64 : raw[k];
+}
+
+/* cell_case, :343-349 (returns the 0-based mask; the reference adds 1 to index its tables). */
+static int cell_mask(const double *phi)
+{
+    int lut = 0;
+    for (int p = 0; p < 8; ++p)
+        if (phi[p] >= 0.0) lut |= 1 << p;
+    return lut;
+}
+
+/* _test_face, :401-406 */
+static int test_face(const double *cb, int face)
+{
+    const int *q = MC_TEST_QUAD[abs(face) - 1];
+    double A = cb[q[0]], B = cb[q[1]], C = cb[q[2]], D = cb[q[3]];
+    if (fabs(A * C - B * D) < EPS64) return face >= 0;
+    return (double)face * A * (A * C - B * D) >= 0.0;
+}
+
+/* _test_interior, :427-504.  `case1` is the reference's 1-based case number; `subcfg` 1-based. */
+static int test_interior(int case1, const double *c, int cfg, int subcfg, int s)
+{
+    double At = 0.0, Bt = 0.0, Ct = 0.0, Dt = 0.0, t;
+    if (case1 == 5 || case1 == 11) {
+        double a = (c[4] - c[0]) * (c[6] - c[2]) - (c[7] - c[3]) * (c[5] - c[1]);
+        double b = c[2] * (c[4] - c[0]) + c[0] * (c[6] - c[2]) - c[1] * (c[7] - c[3]) -
+                   c[3] * (c[5] - c[1]);
+        t = -b / (2.0 * a);
+        if (t < 0.0 || t > 1.0) return s > 0;
+        At = c[0] + (c[4] - c[0]) * t;
+        Bt = c[3] + (c[7] - c[3]) * t;
+        Ct = c[2] + (c[6] - c[2]) * t;
+        Dt = c[1] + (c[5] - c[1]) * t;
+    } else {
+        int edge;
+        if (case1 == 7)       edge = LW_TEST6[cfg][2];
+        else if (case1 == 8)  edge = LW_TEST7[cfg][4];
+        else if (case1 == 13) edge = LW_TEST12[cfg][3];
+        else                  edge = LW_TILING13_5_1[cfg][subcfg - 1][0]; /* 0-based already */
+        switch (edge) {
+        case 0:  t = c[0] / (c[0] - c[1]); Bt = c[3] + (c[2] - c[3]) * t; Ct = c[7] + (c[6] - c[7]) * t; Dt = c[4] + (c[5] - c[4]) * t; break;
+        case 1:  t = c[1] / (c[1] - c[2]); Bt = c[0] + (c[3] - c[0]) * t; Ct = c[4] + (c[7] - c[4]) * t; Dt = c[5] + (c[6] - c[5]) * t; break;
+        case 2:  t = c[2] / (c[2] - c[3]); Bt = c[1] + (c[0] - c[1]) * t; Ct = c[5] + (c[4] - c[5]) * t; Dt = c[6] + (c[7] - c[6]) * t; break;
+        case 3:  t = c[3] / (c[3] - c[0]); Bt = c[2] + (c[1] - c[2]) * t; Ct = c[6] + (c[5] - c[6]) * t; Dt = c[7] + (c[4] - c[7]) * t; break;
+        case 4:  t = c[4] / (c[4] - c[5]); Bt = c[7] + (c[6] - c[7]) * t; Ct = c[3] + (c[2] - c[3]) * t; Dt = c[0] + (c[1] - c[0]) * t; break;
+        case 5:  t = c[5] / (c[5] - c[6]); Bt = c[4] + (c[7] - c[4]) * t; Ct = c[0] + (c[3] - c[0]) * t; Dt = c[1] + (c[2] - c[1]) * t; break;
+        case 6:  t = c[6] / (c[6] - c[7]); Bt = c[5] + (c[4] - c[5]) * t; Ct = c[1] + (c[0] - c[1]) * t; Dt = c[2] + (c[3] - c[2]) * t; break;
+        case 7:  t = c[7] / (c[7] - c[4]); Bt = c[6] + (c[5] - c[6]) * t; Ct = c[2] + (c[1] - c[2]) * t; Dt = c[3] + (c[0] - c[3]) * t; break;
+        case 8:  t = c[0] / (c[0] - c[4]); Bt = c[3] + (c[7] - c[3]) * t; Ct = c[2] + (c[6] - c[2]) * t; Dt = c[1] + (c[5] - c[1]) * t; break;
+        case 9:  t = c[1] / (c[1] - c[5]); Bt = c[0] + (c[4] - c[0]) * t; Ct = c[3] + (c[7] - c[3]) * t; Dt = c[2] + (c[6] - c[2]) * t; break;
+        case 10: t = c[2] / (c[2] - c[6]); Bt = c[1] + (c[5] - c[1]) * t; Ct = c[0] + (c[4] - c[0]) * t; Dt = c[3] + (c[7] - c[3]) * t; break;
+        default: t = c[3] / (c[3] - c[7]); Bt = c[2] + (c[6] - c[2]) * t; Ct = c[1] + (c[5] - c[1]) * t; Dt = c[0] + (c[4] - c[0]) * t; break;
+        }
+    }
+    int test = 0;
+    if (At >= 0.0) test += 1;
+    if (Bt >= 0.0) test += 2;
+    if (Ct >= 0.0) test += 4;
+    if (Dt >= 0.0) test += 8;
+    if (test == 6 || test == 8 || test == 9 || test == 12 || (test >= 0 && test <= 4)) return s > 0;
+    if (test == 5) return (At * Ct - Bt * Dt < EPS64) ? s > 0 : s < 0;
+    if (test == 10) return (At * Ct - Bt * Dt >= EPS64) ? s > 0 : s < 0;
+    return s < 0;
+}
+
+static int tris_from(const signed char *tile, int n, int8_t *codes)
+{
+    for (int i = 0; i < 3 * n; ++i) codes[i] = tile[i];
+    return n;
+}
+
+/* cell_triangles, :538-651.  phi already nudged.  Returns n; codes are 0-based (12 = interior).
+ * A SUBCONFIG13 entry of -1 (no valid tiling) yields -1. */
+static int cell_triangles(const double *phi, int8_t *codes)
+{
+    int lut = cell_mask(phi);
+    int cs = LW_CASES[lut][0], cfg = LW_CASES[lut][1];
+    int subcfg = 1;
+    switch (cs) { /* the reference's case numbers are cs + 1 */
+    case 0: return 0;
+    case 1: return tris_from(LW_TILING1[cfg], 1, codes);
+    case 2: return tris_from(LW_TILING2[cfg], 2, codes);
+    case 3:
+        return test_face(phi, LW_TEST3[cfg]) ? tris_from(LW_TILING3_2[cfg], 4, codes)
+                                             : tris_from(LW_TILING3_1[cfg], 2, codes);
+    case 4:
+        return test_interior(5, phi, cfg, subcfg, LW_TEST4[cfg])
+                   ? tris_from(LW_TILING4_1[cfg], 2, codes)
+                   : tris_from(LW_TILING4_2[cfg], 6, codes);
+    case 5: return tris_from(LW_TILING5[cfg], 3, codes);
+    case 6:
+        if (test_face(phi, LW_TEST6[cfg][0])) return tris_from(LW_TILING6_2[cfg], 5, codes);
+        if (test_interior(7, phi, cfg, subcfg, LW_TEST6[cfg][1]))
+            return tris_from(LW_TILING6_1_1[cfg], 3, codes);
+        return tris_from(LW_TILING6_1_2[cfg], 9, codes);
+    case 7:
+        if (test_face(phi, LW_TEST7[cfg][0])) subcfg += 1;
+        if (test_face(phi, LW_TEST7[cfg][1])) subcfg += 2;
+        if (test_face(phi, LW_TEST7[cfg][2])) subcfg += 4;
+        switch (subcfg) {
+        case 1: return tris_from(LW_TILING7_1[cfg], 3, codes);
+        case 2: case 3: return tris_from(LW_TILING7_2[cfg][subcfg - 2], 5, codes);
+        case 4: return tris_from(LW_TILING7_3[cfg][0], 9, codes);
+        case 5: return tris_from(LW_TILING7_2[cfg][2], 5, codes);
+        case 6: case 7: return tris_from(LW_TILING7_3[cfg][subcfg - 5], 9, codes);
+        default:
+            return test_interior(8, phi, cfg, subcfg, LW_TEST7[cfg][3])
+                       ? tris_from(LW_TILING7_4_2[cfg], 9, codes)
+                       : tris_from(LW_TILING7_4_1[cfg], 5, codes);
+        }
+    case 8: return tris_from(LW_TILING8[cfg], 2, codes);
+    case 9: return tris_from(LW_TILING9[cfg], 4, codes);
+    case 10:
+        if (test_face(phi, LW_TEST10[cfg][0]))
+            return test_face(phi, LW_TEST10[cfg][1]) ? tris_from(LW_TILING10_1_1_[cfg], 4, codes)
+                                                      : tris_from(LW_TILING10_2[cfg], 8, codes);
+        if (test_face(phi, LW_TEST10[cfg][1])) return tris_from(LW_TILING10_2_[cfg], 8, codes);
+        if (test_interior(11, phi, cfg, subcfg, LW_TEST10[cfg][2]))
+            return tris_from(LW_TILING10_1_1[cfg], 4, codes);
+        return tris_from(LW_TILING10_1_2[cfg], 8, codes);
+    case 11: return tris_from(LW_TILING11[cfg], 4, codes);
+    case 12:
+        if (test_face(phi, LW_TEST12[cfg][0]))
+            return test_face(phi, LW_TEST12[cfg][1]) ? tris_from(LW_TILING12_1_1_[cfg], 4, codes)
+                                                      : tris_from(LW_TILING12_2[cfg], 8, codes);
+        if (test_face(phi, LW_TEST12[cfg][1])) return tris_from(LW_TILING12_2_[cfg], 8, codes);
+        if (test_interior(13, phi, cfg, subcfg, LW_TEST12[cfg][2]))
+            return tris_from(LW_TILING12_1_1[cfg], 4, codes);
+        return tris_from(LW_TILING12_1_2[cfg], 8, codes);
+    case 13: {
+        for (int f = 0; f < 6; ++f)
+            if (test_face(phi, LW_TEST13[cfg][f])) subcfg += 1 << f;
+        int sc = LW_SUBCONFIG13[subcfg - 1];
+        if (sc < 0) return -1;
+        if (sc == 0) return tris_from(LW_TILING13_1[cfg], 4, codes);
+        if (sc <= 6) return tris_from(LW_TILING13_2[cfg][sc - 1], 6, codes);
+        if (sc <= 18) return tris_from(LW_TILING13_3[cfg][sc - 7], 10, codes);
+        if (sc <= 22) return tris_from(LW_TILING13_4[cfg][sc - 19], 12, codes);
+        if (sc <= 26) {
+            int sub = sc - 22;
+            /* the reference reads test13[cfg][6] (1-based) = [5] 0-based, :631-637 */
+            return test_interior(14, phi, cfg, sub, LW_TEST13[cfg][5])
+                       ? tris_from(LW_TILING13_5_1[cfg][sub - 1], 6, codes)
+                       : tris_from(LW_TILING13_5_2[cfg][sub - 1], 10, codes);
+        }
+        if (sc <= 38) return tris_from(LW_TILING13_3_[cfg][sc - 27], 10, codes);
+        if (sc <= 44) return tris_from(LW_TILING13_2_[cfg][sc - 39], 6, codes);
+        return tris_from(LW_TILING13_1_[cfg], 4, codes);
+    }
+    default: return tris_from(LW_TILING14[cfg], 4, codes);
+    }
+}
+
+/* _edge_crossing (:284-287) and edge_crossings (:306-330), on cell-local nodes. */
+static void edge_crossings(const double ln[8][3], const double *phi, double pts[13][3])
+{
+    double acc[3] = {0.0, 0.0, 0.0};
+    int n = 0;
+    for (int e = 0; e < 12; ++e) {
+        int a = MC_EDGE[e][0], b = MC_EDGE[e][1];
+        int crossed = (phi[a] >= 0.0) != (phi[b] >= 0.0);
+        if (crossed) {
+            double s = open_fraction(phi[a], phi[b]);
+            for (int x = 0; x < 3; ++x)
+                pts[e][x] = phi[a] >= 0.0 ? ln[a][x] + s * (ln[b][x] - ln[a][x])
+                                          : ln[b][x] + s * (ln[a][x] - ln[b][x]);
+            for (int x = 0; x < 3; ++x) acc[x] += pts[e][x];
+            n += 1;
+        } else {
+            pts[e][0] = pts[e][1] = pts[e][2] = 0.0;
+        }
+    }
+    for (int x = 0; x < 3; ++x) pts[12][x] = n > 0 ? acc[x] / (double)n : 0.0;
+}
+
+/* full_face_area, src/cut_cell.jl:246-253 (0-based direction). */
+static double full_face_area3(const double *cs, int dir)
+{
+    int ax = dir >> 1;
+    double p = 1.0;
+    for (int a = 0; a < 3; ++a)
+        if (a != ax) p *= cs[a];
+    return p;
+}
+
+/* _chord_target, src/marching_cubes/moments.jl:91-98 (returns -1 for "none"). */
+static int chord_target(const double fphi[4], int k, int forward)
+{
+    int j = k;
+    for (int it = 0; it < 4; ++it) {
+        j = forward ? ms_next(j) : ms_prev(j);
+        if (fphi[j] < 0.0 && fphi[ms_next(j)] >= 0.0) return j;
+    }
+    return -1;
+}
+
+/* _face_polygon, moments.jl:111-139 */
+static void face_polygon(const double fn[4][2], const double fphi[4], int connect,
+                         double *area2_out, double cmom[2])
+{
+    double fracs[4], seg_a[4][2], seg_b[4][2];
+    int present[4];
+    boundary_walk(fn, fphi, fracs, present, seg_a, seg_b);
+    double area2 = 0.0, c0 = 0.0, c1 = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        if (!present[k]) continue;
+        const double *la = seg_a[k], *lb = seg_b[k];
+        double cr = la[0] * lb[1] - la[1] * lb[0];
+        area2 += cr;
+        c0 += (la[0] + lb[0]) * cr;
+        c1 += (la[1] + lb[1]) * cr;
+    }
+    for (int k = 0; k < 4; ++k) {
+        if (!(fphi[k] >= 0.0 && fphi[ms_next(k)] < 0.0)) continue;
+        int j = chord_target(fphi, k, connect);
+        if (j < 0) continue;
+        const double *la = seg_b[k], *lb = seg_a[j];
+        double cr = la[0] * lb[1] - la[1] * lb[0];
+        area2 += cr;
+        c0 += (la[0] + lb[0]) * cr;
+        c1 += (la[1] + lb[1]) * cr;
+    }
+    *area2_out = area2;
+    cmom[0] = c0;
+    cmom[1] = c1;
+}
+
+/* _cut_cell_face, moments.jl:150-196 */
+static int cut_cell_face(int dir, const double ln[8][3], const double *phi, const double *cs,
+                         double *frac, double c2[2])
+{
+    int p = MC_FACE_AX[dir][0], q = MC_FACE_AX[dir][1];
+    double fn[4][2], fphi[4];
+    for (int k = 0; k < 4; ++k) {
+        const double *nd = ln[MC_FACE_NODE[dir][k]];
+        fn[k][0] = nd[p];
+        fn[k][1] = nd[q];
+        fphi[k] = phi[MC_FACE_NODE[dir][k]];
+    }
+    int nout = 0;
+    for (int k = 0; k < 4; ++k) nout += fphi[k] >= 0.0 ? 1 : 0;
+    int amb = nout == 2 && ((fphi[0] >= 0.0) == (fphi[2] >= 0.0));
+    double avg0 = 0.25 * (((fn[0][0] + fn[1][0]) + fn[2][0]) + fn[3][0]);
+    double avg1 = 0.25 * (((fn[0][1] + fn[1][1]) + fn[2][1]) + fn[3][1]);
+    if (nout == 4 || nout == 0) {
+        *frac = nout == 4 ? 1.0 : 0.0;
+        c2[0] = avg0;
+        c2[1] = avg1;
+    } else {
+        double area2, cm[2];
+        face_polygon(fn, fphi, test_face(phi, MC_FACE_TEST_CODE[dir]), &area2, cm);
+        *frac = 0.5 * area2 / full_face_area3(cs, dir);
+        if (area2 != 0.0) {
+            double den = 3.0 * area2;
+            c2[0] = cm[0] / den;
+            c2[1] = cm[1] / den;
+        } else {
+            c2[0] = avg0;
+            c2[1] = avg1;
+        }
+    }
+    return amb;
+}
+
+/* cut_cell_moments, moments.jl:275-346 (with cut_cell_faces, :224-257, inlined). */
+void orc_mc_cell(const double *nodes24, const double *phi_raw8, const double *cs,
+                 int8_t *kind_out, uint8_t *amb_out, double *vf, double *centroid, double *ff,
+                 double *fcl, double *ic)
+{
+    const double(*nodes)[3] = (const double(*)[3])nodes24;
+    double phi[8];
+    nudge_zeros(phi_raw8, phi);
+    const double *origin = nodes[0];
+    double centre[3], ln[8][3];
+    for (int a = 0; a < 3; ++a) centre[a] = origin[a] + 0.5 * cs[a];
+    double vol_cell = (cs[0] * cs[1]) * cs[2];
+    for (int k = 0; k < 8; ++k)
+        for (int a = 0; a < 3; ++a) ln[k][a] = nodes[k][a] - origin[a];
+
+    int8_t codes[36];
+    int n_tri = cell_triangles(phi, codes);
+
+    int noutside = 0;
+    for (int k = 0; k < 8; ++k) noutside += phi[k] >= 0.0 ? 1 : 0;
+    int kind = noutside == 8 ? CELL_OUTSIDE : (noutside == 0 ? CELL_INSIDE : CELL_CUT);
+    int amb_face = 0;
+    for (int dir = 0; dir < 6; ++dir) amb_face |= cut_cell_face(dir, ln, phi, cs, &ff[dir], &fcl[2 * dir]);
+
+    if (kind != CELL_CUT) {
+        *kind_out = (int8_t)kind;
+        *amb_out = 0;
+        *vf = kind == CELL_OUTSIDE ? 1.0 : 0.0;
+        for (int a = 0; a < 3; ++a) centroid[a] = ic[a] = centre[a];
+        return;
+    }
+
+    double pts[13][3];
+    edge_crossings(ln, phi, pts);
+
+    double vol3 = 0.0, cm2[3] = {0.0, 0.0, 0.0};
+    for (int dir = 0; dir < 6; ++dir) { /* :306-313 */
+        int ax = dir >> 1;
+        double s = (dir & 1) ? 1.0 : -1.0;
+        double x_ax = ln[MC_FACE_NODE[dir][0]][ax];
+        double a_open = ff[dir] * full_face_area3(cs, dir);
+        vol3 += s * x_ax * a_open;
+        for (int a = 0; a < 3; ++a) cm2[a] += a == ax ? s * x_ax * x_ax * a_open : 0.0;
+    }
+    double iarea = 0.0, icm[3] = {0.0, 0.0, 0.0};
+    if (n_tri < 0) n_tri = 0; /* impossible 13.x configuration: no facets (never met in tests) */
+    for (int t = 0; t < n_tri; ++t) { /* :317-332 */
+        const double *a = pts[codes[3 * t]], *b = pts[codes[3 * t + 1]], *d = pts[codes[3 * t + 2]];
+        double u[3], v[3], w[3], xc[3];
+        for (int x = 0; x < 3; ++x) { u[x] = b[x] - a[x]; v[x] = d[x] - a[x]; }
+        w[0] = 0.5 * (u[1] * v[2] - u[2] * v[1]);
+        w[1] = 0.5 * (u[2] * v[0] - u[0] * v[2]);
+        w[2] = 0.5 * (u[0] * v[1] - u[1] * v[0]);
+        for (int x = 0; x < 3; ++x) xc[x] = (a[x] + b[x] + d[x]) / 3.0;
+        vol3 -= w[0] * xc[0] + w[1] * xc[1] + w[2] * xc[2];
+        for (int x = 0; x < 3; ++x) {
+            double m2 = (a[x] * a[x] + b[x] * b[x] + d[x] * d[x] + a[x] * b[x] + b[x] * d[x] +
+                         d[x] * a[x]) / 6.0;
+            cm2[x] -= w[x] * m2;
+        }
+        double len = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        iarea += len;
+        for (int x = 0; x < 3; ++x) icm[x] += len * xc[x];
+    }
+    double volume = vol3 / 3.0;
+    for (int a = 0; a < 3; ++a) {
+        centroid[a] = volume > 0.0 ? origin[a] + cm2[a] / (2.0 * volume) : centre[a];
+        ic[a] = iarea > 0.0 ? origin[a] + icm[a] / iarea : centre[a];
+    }
+    int cs0 = LW_CASES[cell_mask(phi)][0]; /* reference 1-based {5,7,8,11,13,14} */
+    int amb = amb_face || cs0 == 4 || cs0 == 6 || cs0 == 7 || cs0 == 10 || cs0 == 12 || cs0 == 13;
+    *kind_out = CELL_CUT;
+    *amb_out = (uint8_t)amb;
+    *vf = volume / vol_cell;
+}
+
+int orc_mc_cell_case(const double *phi_raw8)
+{
+    double phi[8];
+    nudge_zeros(phi_raw8, phi);
+    return LW_CASES[cell_mask(phi)][0];
+}
+
+int orc_mc_cell_triangles(const double *phi_raw8, int8_t *codes36)
+{
+    double phi[8];
+    nudge_zeros(phi_raw8, phi);
+    return cell_triangles(phi, codes36);
+}
+
+void orc_mc_edge_crossings(const double *local_nodes24, const double *phi_raw8, double *pts39)
+{
+    double phi[8];
+    nudge_zeros(phi_raw8, phi);
+    edge_crossings((const double(*)[3])local_nodes24, phi, (double(*)[3])pts39);
+}
+
+/* _nodal_cells_kernel! (src/marching_squares/cache.jl:75-78) over cells k in [k0, k1). */
+void orc_mc_cells(const double *phi, const double *x0, const double *d, const int64_t *n,
+                  int64_t k0, int64_t k1, const orc_cols *out)
+{
+    int64_t m0 = n[0] + 1, m1 = n[1] + 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = k0; k < k1; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t i = 0; i < n[0]; ++i) {
+                double nodes[8][3], ph[8];
+                for (int c = 0; c < 8; ++c) {
+                    int64_t ii = i + MC_BITS[c][0], jj = j + MC_BITS[c][1], kk = k + MC_BITS[c][2];
+                    nodes[c][0] = x0[0] + (double)ii * d[0];
+                    nodes[c][1] = x0[1] + (double)jj * d[1];
+                    nodes[c][2] = x0[2] + (double)kk * d[2];
+                    ph[c] = phi[ii + m0 * (jj + m1 * kk)];
+                }
+                int64_t cidx = i + n[0] * (j + n[1] * (k - k0));
+                orc_mc_cell(&nodes[0][0], ph, d, &out->kind[cidx], &out->ambiguous[cidx],
+                            &out->volume_fraction[cidx], out->centroid + 3 * cidx,
+                            out->face_fraction + 6 * cidx, out->face_centroid_local + 12 * cidx,
+                            out->interface_centroid + 3 * cidx);
+            }
+}
+
+/* =========================================================================================== */
+/* PLIC: src/plic/plic.jl, src/plic/apertures.jl, src/plic/cache.jl                             */
+/* =========================================================================================== */
+
+/* CartesianMeshes.get_volume_fraction(n, a): the measure of {xi in [0,1]^D : n.xi <= a}.
+ * Scardovelli & Zaleski (2000): reflect negative components, normalise by sum|n|, use the
+ * alpha <= 1/2 half by symmetry, then the piecewise closed form over sorted m1 <= m2 (<= m3). */
+static double vf2(double n0, double n1, double a)
+{
+    if (n0 < 0.0) a -= n0;
+    if (n1 < 0.0) a -= n1;
+    double m0 = fabs(n0), m1 = fabs(n1);
+    double s = m0 + m1;
+    if (!(s > 0.0)) return a >= 0.0 ? 1.0 : 0.0;
+    m0 /= s; m1 /= s; a /= s;
+    if (a <= 0.0) return 0.0;
+    if (a >= 1.0) return 1.0;
+    double lo = m0 < m1 ? m0 : m1, hi = m0 < m1 ? m1 : m0;
+    int flip = a > 0.5;
+    double al = flip ? 1.0 - a : a, v;
+    if (al < lo) v = (al * al) / (2.0 * lo * hi);
+    else v = (al - 0.5 * lo) / hi;
+    return flip ? 1.0 - v : v;
+}
+
+static double vf3(const double *n, double a)
+{
+    double m[3];
+    for (int i = 0; i < 3; ++i) {
+        if (n[i] < 0.0) a -= n[i];
+        m[i] = fabs(n[i]);
+    }
+    double s = (m[0] + m[1]) + m[2];
+    if (!(s > 0.0)) return a >= 0.0 ? 1.0 : 0.0;
+    for (int i = 0; i < 3; ++i) m[i] /= s;
+    a /= s;
+    if (a <= 0.0) return 0.0;
+    if (a >= 1.0) return 1.0;
+    /* sort ascending */
+    double t;
+    if (m[0] > m[1]) { t = m[0]; m[0] = m[1]; m[1] = t; }
+    if (m[1] > m[2]) { t = m[1]; m[1] = m[2]; m[2] = t; }
+    if (m[0] > m[1]) { t = m[0]; m[0] = m[1]; m[1] = t; }
+    int flip = a > 0.5;
+    double al = flip ? 1.0 - a : a, v;
+    double m1 = m[0], m2 = m[1], m3 = m[2], m12 = m1 + m2;
+    /* alpha^3 - sum_i (alpha - m_i)^3_+ over 6 m1 m2 m3, with the m1 factor cancelled by hand:
+     * alpha^3 - (alpha - m1)^3 = m1 * P1, P1 = 3 alpha (alpha - m1) + m1^2.  Between m2 and m12
+     * every remaining (alpha - m_i) is below m1 (alpha <= 1/2 < m12 whenever alpha >= m3), so the
+     * subtraction below loses no digits even for a nearly axis-aligned plane. */
+    if (al < m1) {
+        v = (al * al * al) / (6.0 * m1 * m2 * m3);
+    } else {
+        double P1 = 3.0 * al * (al - m1) + m1 * m1;
+        if (al < m2) {
+            v = P1 / (6.0 * m2 * m3);
+        } else if (al < m12) {
+            double u2 = al - m2;
+            double c = u2 * u2 * (u2 / m1);
+            if (al > m3) {
+                double u3 = al - m3;
+                c = c + u3 * u3 * (u3 / m1);
+            }
+            v = (P1 - c) / (6.0 * m2 * m3);
+        } else {
+            v = (2.0 * al - m12) / (2.0 * m3);
+        }
+    }
+    return flip ? 1.0 - v : v;
+}
+
+double orc_volume_fraction(int dim, const double *normal, double a)
+{
+    return dim == 2 ? vf2(normal[0], normal[1], a) : vf3(normal, a);
+}
+
+/* cell_plane (src/plic/plic.jl:72-91) + _plic_fit_kernel! (src/plic/cache.jl:96-113). */
+void orc_plic_fit(const orc_geometry *g, int dim, const double *x0, const double *d,
+                  const int64_t *n, double *vf, int8_t *kind, double *normals,
+                  double *intercepts, uint8_t *is_valid)
+{
+    int64_t n2 = dim == 3 ? n[2] : 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n2; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t i = 0; i < n[0]; ++i) {
+                int64_t c = i + n[0] * (j + n[1] * k);
+                double x[3] = {x0[0] + ((double)i + 0.5) * d[0], x0[1] + ((double)j + 0.5) * d[1],
+                               dim == 3 ? x0[2] + ((double)k + 0.5) * d[2] : 0.0};
+                double nrm[3] = {0.0, 0.0, 0.0};
+                double dist = orc_get_sdf(g, dim, x, nrm);
+                double sum = nrm[0] + nrm[1];
+                if (dim == 3) sum = sum + nrm[2];
+                double a = 0.5 * sum - dist / d[0];
+                int zero = nrm[0] == 0.0 && nrm[1] == 0.0 && (dim == 2 || nrm[2] == 0.0);
+                double fraction;
+                int valid;
+                if (isnan(a) || zero) {
+                    for (int q = 0; q < dim; ++q) nrm[q] = 0.0;
+                    a = 0.0;
+                    fraction = dist <= 0.0 ? 1.0 : 0.0;
+                    valid = 0;
+                } else {
+                    fraction = orc_volume_fraction(dim, nrm, a);
+                    valid = 1;
+                }
+                double frac = 1.0 - fraction;
+                for (int q = 0; q < dim; ++q) normals[dim * c + q] = nrm[q];
+                intercepts[c] = a;
+                is_valid[c] = (uint8_t)valid;
+                vf[c] = frac;
+                kind[c] = frac == 0.0 ? CELL_INSIDE : (frac == 1.0 ? CELL_OUTSIDE : CELL_CUT);
+            }
+}
+
+/* _cell_wet and _face_area (src/plic/apertures.jl:73-125).  k is the 0-based direction. */
+static double face_area(int dim, const double *nv, double a, double frac, int valid,
+                        const double *cs, int k)
+{
+    double s = (k & 1) ? 1.0 : 0.0; /* isodd(k) 1-based <=> even 0-based -> 0 */
+    if (dim == 2) {
+        int c = k < 2 ? 0 : 1, p = 1 - c;
+        double len = cs[p];
+        if (!valid) return frac < 0.5 ? 0.0 : len;
+        double base = nv[c] * s - a;
+        return open_fraction(base, base + nv[p]) * len;
+    }
+    int c, p, q;
+    if (k < 2) { c = 0; p = 1; q = 2; }
+    else if (k < 4) { c = 1; p = 0; q = 2; }
+    else { c = 2; p = 0; q = 1; }
+    double U = cs[p], V = cs[q];
+    if (!valid) return frac < 0.5 ? 0.0 : U * V;
+    double base = nv[c] * s - a;
+    if (nv[p] == 0.0 && nv[q] == 0.0) return base >= 0.0 ? U * V : 0.0;
+    double solid = vf2(nv[p], nv[q], -base);
+    return U * V * (1.0 - solid);
+}
+
+/* _plic_face_fraction_kernel! and _resolved_slot (src/plic/cache.jl:118-162). */
+void orc_plic_faces(int dim, const double *d, const int64_t *n, const double *normals,
+                    const double *intercepts, const uint8_t *is_valid, const double *vf,
+                    double *face_fraction)
+{
+    int64_t n2 = dim == 3 ? n[2] : 1;
+    int64_t st[3] = {1, n[0], n[0] * n[1]};
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n2; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t i = 0; i < n[0]; ++i) {
+                int64_t c = i + n[0] * (j + n[1] * k);
+                int64_t ci[3] = {i, j, k};
+                for (int slot = 0; slot < 2 * dim; ++slot) {
+                    int ax = slot >> 1;
+                    double full;
+                    if (dim == 2) full = d[1 - ax];
+                    else full = full_face_area3(d, slot);
+                    double own = face_area(dim, normals + dim * c, intercepts[c], vf[c],
+                                           is_valid[c], d, slot);
+                    double r;
+                    if ((slot & 1) == 0) { /* low face: the neighbour below is the negative side */
+                        int has = ci[ax] >= 1;
+                        int64_t nb = has ? c - st[ax] : c;
+                        double below = face_area(dim, normals + dim * nb, intercepts[nb], vf[nb],
+                                                 is_valid[nb], d, slot + 1);
+                        double an = below / full, ap = own / full;
+                        r = has ? (an + ap) / 2.0 : ap;
+                    } else {
+                        int has = ci[ax] <= n[ax] - 2;
+                        int64_t nb = has ? c + st[ax] : c;
+                        double above = face_area(dim, normals + dim * nb, intercepts[nb], vf[nb],
+                                                 is_valid[nb], d, slot - 1);
+                        double an = own / full, ap = above / full;
+                        r = has ? (an + ap) / 2.0 : an;
+                    }
+                    face_fraction[2 * dim * c + slot] = r;
+                }
+            }
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,234 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+
+The oracle (``cutcell_oracle.c``) is a plain-C restatement of CutCellMethods.jl's Float64
+``update_cache!`` for MarchingSquaresCutCell, MarchingCubesCutCell and PLICCutCell.  Only the tests,
+``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg import this module, and only as the
+checker.  It never stands in for the HIP path.
+
+Arrays follow the reference's storage: every per-cell field is its own column, column-major over the
+grid (i fastest), vector fields interleaved per cell (``src/cut_cell.jl:112-120``,
+``src/cache.jl:101-104``).  Columns come back flat over the cell linear index with the per-cell
+components as trailing axes.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+
+FIELDS = ("kind", "ambiguous", "volume_fraction", "centroid", "face_fraction",
+          "face_centroid_local", "interface_centroid")
+
+GEO_EMPTY, GEO_SPHERE, GEO_TORUS, GEO_BOX, GEO_PLANE, GEO_SPHERE_UNION = range(6)
+
+
+class OrcCols(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_void_p), ("ambiguous", ctypes.c_void_p),
+                ("volume_fraction", ctypes.c_void_p), ("centroid", ctypes.c_void_p),
+                ("face_fraction", ctypes.c_void_p), ("face_centroid_local", ctypes.c_void_p),
+                ("interface_centroid", ctypes.c_void_p)]
+
+
+class OrcGeometry(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("n_prims", ctypes.c_int32),
+                ("p", ctypes.c_double * 24), ("prims", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def build():
+    """Compile liborc.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p
+        L.orc_sample_nodes.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
+        L.orc_ms_cells.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_mc_cells.argtypes = [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int64, vp]
+        L.orc_mc_cell_case.argtypes = [vp]
+        L.orc_mc_cell_case.restype = ctypes.c_int
+        L.orc_mc_cell_triangles.argtypes = [vp, vp]
+        L.orc_mc_cell_triangles.restype = ctypes.c_int
+        L.orc_mc_edge_crossings.argtypes = [vp, vp, vp]
+        L.orc_mc_cell.argtypes = [vp] * 10
+        L.orc_volume_fraction.argtypes = [ctypes.c_int, vp, ctypes.c_double]
+        L.orc_volume_fraction.restype = ctypes.c_double
+        L.orc_plic_fit.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_plic_faces.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_sdf_value.argtypes = [vp, ctypes.c_int, vp]
+        L.orc_sdf_value.restype = ctypes.c_double
+        L.orc_get_sdf.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.orc_get_sdf.restype = ctypes.c_double
+        L.orc_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(x, n=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    if n is not None and a.size < n:
+        a = np.concatenate([a, np.zeros(n - a.size)])
+    return a
+
+
+def _i64(x, n=3):
+    a = list(int(v) for v in x) + [1] * (n - len(x))
+    return np.asarray(a, dtype=np.int64)
+
+
+def make_geometry(geo):
+    """An OrcGeometry from any object exposing ``kind``, ``params`` (<= 24 floats) and ``prims``
+    (None or an (m, 4) array of sphere centres and radii).  The prims array is kept alive on the
+    returned struct."""
+    g = OrcGeometry()
+    g.kind = int(geo.kind)
+    params = list(float(v) for v in geo.params)
+    for i, v in enumerate(params[:24]):
+        g.p[i] = v
+    prims = getattr(geo, "prims", None)
+    if prims is not None:
+        arr = np.ascontiguousarray(np.asarray(prims, dtype=np.float64).reshape(-1, 4))
+        g.n_prims = arr.shape[0]
+        g.prims = arr.ctypes.data
+        g._keep = arr
+    else:
+        g.n_prims = 0
+        g.prims = None
+    return g
+
+
+def alloc_cols(ncell, dim, nan=False):
+    fill = np.nan if nan else 0.0
+    return {
+        "kind": np.zeros(ncell, np.int8),
+        "ambiguous": np.zeros(ncell, np.uint8),
+        "volume_fraction": np.full(ncell, fill),
+        "centroid": np.full((ncell, dim), fill),
+        "face_fraction": np.full((ncell, 2 * dim), fill),
+        "face_centroid_local": np.full((ncell, 2 * dim, dim - 1), fill),
+        "interface_centroid": np.full((ncell, dim), fill),
+    }
+
+
+def _cols_struct(cols):
+    s = OrcCols()
+    for f in FIELDS:
+        setattr(s, f, cols[f].ctypes.data)
+    return s
+
+
+def sample_nodes(geo, dim, x0, d, n):
+    """``phi`` at every node (src/marching_squares/cache.jl:67-70), flat, column-major."""
+    nn = [int(v) + 1 for v in n[:dim]]
+    phi = np.empty(int(np.prod(nn)), np.float64)
+    g = make_geometry(geo)
+    lib().orc_sample_nodes(ctypes.byref(g), dim, _p(_f64(x0, 3)), _p(_f64(d, 3)),
+                           _p(_i64(n)), _p(phi))
+    return phi
+
+
+def sdf_value(geo, dim, x):
+    g = make_geometry(geo)
+    return lib().orc_sdf_value(ctypes.byref(g), dim, _p(_f64(x, 3)))
+
+
+def get_sdf(geo, dim, x):
+    g = make_geometry(geo)
+    nrm = np.zeros(3)
+    dist = lib().orc_get_sdf(ctypes.byref(g), dim, _p(_f64(x, 3)), _p(nrm))
+    return dist, nrm[:dim]
+
+
+def ms_cells(phi, x0, d, n):
+    """Every cell of a 2D grid (src/marching_squares/cache.jl:75-78)."""
+    ncell = int(n[0]) * int(n[1])
+    cols = alloc_cols(ncell, 2)
+    s = _cols_struct(cols)
+    phi = _f64(phi)
+    lib().orc_ms_cells(_p(phi), _p(_f64(x0, 3)), _p(_f64(d, 3)), _p(_i64(n)), ctypes.byref(s))
+    return cols
+
+
+def mc_cells(phi, x0, d, n, k0=0, k1=None):
+    """Cells with k in [k0, k1) of a 3D grid (src/marching_squares/cache.jl:75-78)."""
+    if k1 is None:
+        k1 = int(n[2])
+    ncell = int(n[0]) * int(n[1]) * (k1 - k0)
+    cols = alloc_cols(ncell, 3)
+    s = _cols_struct(cols)
+    phi = _f64(phi)
+    lib().orc_mc_cells(_p(phi), _p(_f64(x0, 3)), _p(_f64(d, 3)), _p(_i64(n)), k0, k1,
+                       ctypes.byref(s))
+    return cols
+
+
+def mc_cell(nodes, phi_raw, cellsize):
+    """One cell's CutCellData (src/marching_cubes/moments.jl:275-346) as a dict."""
+    nodes = _f64(nodes).reshape(8, 3)
+    phi_raw = _f64(phi_raw)
+    kind = np.zeros(1, np.int8)
+    amb = np.zeros(1, np.uint8)
+    vf = np.zeros(1)
+    cen, ff, fcl, ic = np.zeros(3), np.zeros(6), np.zeros(12), np.zeros(3)
+    lib().orc_mc_cell(_p(nodes), _p(phi_raw), _p(_f64(cellsize)), _p(kind), _p(amb), _p(vf),
+                      _p(cen), _p(ff), _p(fcl), _p(ic))
+    return {"kind": int(kind[0]), "ambiguous": bool(amb[0]), "volume_fraction": float(vf[0]),
+            "centroid": cen, "face_fraction": ff, "face_centroid_local": fcl.reshape(6, 2),
+            "interface_centroid": ic}
+
+
+def mc_cell_case(phi_raw):
+    return lib().orc_mc_cell_case(_p(_f64(phi_raw)))
+
+
+def mc_cell_triangles(phi_raw):
+    codes = np.zeros(36, np.int8)
+    n = lib().orc_mc_cell_triangles(_p(_f64(phi_raw)), _p(codes))
+    return n, codes[:3 * max(n, 0)].reshape(-1, 3)
+
+
+def mc_edge_crossings(local_nodes, phi_raw):
+    pts = np.zeros(39)
+    lib().orc_mc_edge_crossings(_p(_f64(local_nodes)), _p(_f64(phi_raw)), _p(pts))
+    return pts.reshape(13, 3)
+
+
+def volume_fraction(normal, a):
+    nv = _f64(normal, 3)
+    return lib().orc_volume_fraction(len(normal), _p(nv), float(a))
+
+
+def plic(geo, dim, x0, d, n):
+    """A PLIC cache update (src/plic/cache.jl:77-92): returns (cols, normals, intercepts,
+    is_valid); the centroid columns are NaN and ``ambiguous`` is false as allocate_cache seeds
+    them (src/plic/cache.jl:60-75)."""
+    ncell = int(np.prod([int(v) for v in n[:dim]]))
+    cols = alloc_cols(ncell, dim, nan=True)
+    normals = np.zeros((ncell, dim))
+    intercepts = np.zeros(ncell)
+    valid = np.zeros(ncell, np.uint8)
+    g = make_geometry(geo)
+    x0a, da, na = _f64(x0, 3), _f64(d, 3), _i64(n)
+    lib().orc_plic_fit(ctypes.byref(g), dim, _p(x0a), _p(da), _p(na), _p(cols["volume_fraction"]),
+                       _p(cols["kind"]), _p(normals), _p(intercepts), _p(valid))
+    lib().orc_plic_faces(dim, _p(da), _p(na), _p(normals), _p(intercepts), _p(valid),
+                         _p(cols["volume_fraction"]), _p(cols["face_fraction"]))
+    return cols, normals, intercepts, valid
+
+
+def num_threads():
+    return lib().orc_num_threads()
