@@ -1,0 +1,546 @@
+// The C ABI of libcutcell_hip: cache lifetime, the update entry points, column access and the
+// marching-squares contour.  Mirrors the reference's cache API (src/cache.jl:15-121,
+// src/marching_squares/cache.jl, src/marching_cubes/cache.jl, src/plic/cache.jl) with status codes
+// in place of exceptions.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cutcell_hip.h"
+#include "cc_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPC(expr)                                                                            \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(CC_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));           \
+    } while (0)
+
+template <typename T> __global__ void k_fill(T *p, size_t n, T v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+constexpr int MAX_TIMED = 8;
+
+}  // namespace
+
+struct cc_cache {
+    int method = 0, dtype = 0, dim = 0, device = 0;
+    int64_t n[3] = {1, 1, 1};
+    int64_t ncell = 0, nnode = 0;
+    void *col[CC_NUM_FIELDS] = {};
+    int64_t colbytes[CC_NUM_FIELDS] = {};
+    uint32_t *cutlist = nullptr;
+    uint32_t *counter = nullptr;
+    double *prims_dev = nullptr;
+    int64_t prims_cap = 0;
+    std::vector<double> prims_host;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    hipEvent_t ev[MAX_TIMED + 1] = {};
+    const char *ev_name[MAX_TIMED] = {};
+    int nev = 0;
+    bool ev_ready = false;
+};
+
+namespace {
+
+size_t elem_size(int dtype) { return dtype == CC_FLOAT32 ? 4 : 8; }
+
+int check_grid(const cc_cache *c, const cc_grid *g) {
+    if (!g) return fail(CC_ERR_ARGUMENT, "grid is NULL");
+    if (g->dim != c->dim)
+        return fail(CC_ERR_DIMENSION_MISMATCH, "the cache is %dD but the grid is %dD", c->dim, g->dim);
+    for (int a = 0; a < c->dim; ++a)
+        if (g->n[a] != c->n[a])
+            return fail(CC_ERR_DIMENSION_MISMATCH,
+                        "the cache holds (%lld, %lld, %lld) cells, but `grid` has (%lld, %lld, %lld); a "
+                        "cache is allocated for one grid's `n`, and only its origin may move between "
+                        "updates",
+                        (long long)c->n[0], (long long)c->n[1], (long long)(c->dim == 3 ? c->n[2] : 1),
+                        (long long)g->n[0], (long long)g->n[1],
+                        (long long)(g->dim == 3 ? g->n[2] : 1));
+    for (int a = 0; a < c->dim; ++a)
+        if (!(g->d[a] > 0.0) || !std::isfinite(g->d[a]) || !std::isfinite(g->x0[a]))
+            return fail(CC_ERR_ARGUMENT, "grid.d must be positive and finite, x0 finite");
+    if (c->method == CC_PLIC) {
+        for (int a = 1; a < c->dim; ++a)
+            if (g->d[a] != g->d[0])
+                return fail(CC_ERR_ARGUMENT,
+                            "PLICCutCell needs an isotropic grid (equal cell size on every axis), got "
+                            "grid.d = (%g, %g%s)",
+                            g->d[0], g->d[1], c->dim == 3 ? ", ..." : "");
+    }
+    return CC_OK;
+}
+
+template <typename T> cc::GridT<T> grid_t(const cc_grid *g, int dim) {
+    cc::GridT<T> r;
+    for (int a = 0; a < 3; ++a) {
+        r.x0[a] = a < dim ? T(g->x0[a]) : T(0);
+        r.d[a] = a < dim ? T(g->d[a]) : T(1);
+        r.n[a] = a < dim ? (int)g->n[a] : 1;
+    }
+    return r;
+}
+
+template <typename T> cc::Cols<T> cols_t(const cc_cache *c) {
+    cc::Cols<T> r;
+    r.kind = (int8_t *)c->col[CC_FIELD_KIND];
+    r.amb = (uint8_t *)c->col[CC_FIELD_AMBIGUOUS];
+    r.vf = (T *)c->col[CC_FIELD_VOLUME_FRACTION];
+    r.centroid = (T *)c->col[CC_FIELD_CENTROID];
+    r.ff = (T *)c->col[CC_FIELD_FACE_FRACTION];
+    r.fcl = (T *)c->col[CC_FIELD_FACE_CENTROID_LOCAL];
+    r.ic = (T *)c->col[CC_FIELD_INTERFACE_CENTROID];
+    return r;
+}
+
+int stage_geometry(cc_cache *c, const cc_geometry *geo, cc::Geo *out) {
+    if (!geo) return fail(CC_ERR_ARGUMENT, "geometry is NULL");
+    if (geo->kind < CC_GEO_EMPTY || geo->kind > CC_GEO_SPHERE_UNION)
+        return fail(CC_ERR_UNSUPPORTED, "unknown geometry kind %d", geo->kind);
+    if (geo->kind == CC_GEO_TORUS && c->dim != 3)
+        return fail(CC_ERR_UNSUPPORTED, "a torus is a 3D body");
+    out->kind = geo->kind;
+    out->nprims = 0;
+    std::memcpy(out->p, geo->p, sizeof out->p);
+    out->prims = nullptr;
+    if (geo->kind == CC_GEO_SPHERE_UNION) {
+        if (geo->n_prims < 1 || !geo->prims)
+            return fail(CC_ERR_ARGUMENT, "a sphere union needs at least one primitive");
+        if (!(geo->p[0] > 0.0)) return fail(CC_ERR_ARGUMENT, "the smoothing k must be positive");
+        const int64_t nd = 4 * (int64_t)geo->n_prims;
+        bool same = (int64_t)c->prims_host.size() == nd &&
+                    std::memcmp(c->prims_host.data(), geo->prims, nd * sizeof(double)) == 0;
+        if (!same) {
+            if (nd > c->prims_cap) {
+                if (c->prims_dev) HIPC(hipFree(c->prims_dev));
+                c->prims_dev = nullptr;
+                HIPC(hipMalloc(&c->prims_dev, nd * sizeof(double)));
+                c->prims_cap = nd;
+            }
+            HIPC(hipStreamSynchronize(c->stream));
+            HIPC(hipMemcpy(c->prims_dev, geo->prims, nd * sizeof(double), hipMemcpyHostToDevice));
+            c->prims_host.assign(geo->prims, geo->prims + nd);
+        }
+        out->nprims = geo->n_prims;
+        out->prims = c->prims_dev;
+    }
+    return CC_OK;
+}
+
+void mark(cc_cache *c, const char *name) {
+    if (!c->profiling || c->nev >= MAX_TIMED) return;
+    if (c->nev == 0) hipEventRecord(c->ev[0], c->stream);
+    c->ev_name[c->nev] = name;
+    hipEventRecord(c->ev[c->nev + 1], c->stream);
+    c->nev += 1;
+}
+
+template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void *phi_in, int on_dev,
+                                     const cc_grid *g) {
+    const cc::GridT<T> gt = grid_t<T>(g, c->dim);
+    const cc::Cols<T> cols = cols_t<T>(c);
+    c->nev = 0;
+    c->ev_ready = false;
+    if (c->profiling) hipEventRecord(c->ev[0], c->stream);
+    T *phi = (T *)c->col[CC_FIELD_PHI];
+    if (c->method == CC_MARCHING_SQUARES || c->method == CC_MARCHING_CUBES) {
+        if (geo) {
+            cc::launch_sample<T>(phi, gt, *geo, c->dim, c->stream);
+            mark(c, "sample_nodes");
+        } else if (phi_in) {
+            HIPC(hipMemcpyAsync(phi, phi_in, c->colbytes[CC_FIELD_PHI],
+                                on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+            mark(c, "copy_phi");
+        }
+        cc::launch_zero_u32(c->counter, c->stream);
+        if (c->method == CC_MARCHING_CUBES) {
+            cc::launch_mc_stream<T>(phi, gt, cols, c->cutlist, c->counter, c->stream);
+            mark(c, "mc_stream");
+            cc::launch_mc_cut<T>(phi, gt, cols, c->cutlist, c->counter, c->stream);
+            mark(c, "mc_cut");
+        } else {
+            cc::launch_ms_cells<T>(phi, gt, cols, c->counter, c->stream);
+            mark(c, "ms_cells");
+        }
+    } else {
+        cc::launch_plic_fit<T>(gt, *geo, c->dim, cols, (T *)c->col[CC_FIELD_NORMALS],
+                               (T *)c->col[CC_FIELD_INTERCEPTS], (uint8_t *)c->col[CC_FIELD_IS_VALID],
+                               c->stream);
+        mark(c, "plic_fit");
+        cc::launch_plic_faces<T>(gt, c->dim, cols, (const T *)c->col[CC_FIELD_NORMALS],
+                                 (const T *)c->col[CC_FIELD_INTERCEPTS],
+                                 (const uint8_t *)c->col[CC_FIELD_IS_VALID], c->stream);
+        mark(c, "plic_faces");
+    }
+    HIPC(hipGetLastError());
+    c->ev_ready = c->profiling;
+    return CC_OK;
+}
+
+int dispatch_update(cc_cache *c, const cc::Geo *geo, const void *phi, int on_dev, const cc_grid *g) {
+    HIPC(hipSetDevice(c->device));
+    return c->dtype == CC_FLOAT32 ? run_update<float>(c, geo, phi, on_dev, g)
+                                  : run_update<double>(c, geo, phi, on_dev, g);
+}
+
+int update_impl(cc_cache *c, const cc_geometry *geo, const cc_grid *g, bool sync) {
+    if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
+    int rc = check_grid(c, g);
+    if (rc) return rc;
+    cc::Geo gd;
+    rc = stage_geometry(c, geo, &gd);
+    if (rc) return rc;
+    rc = dispatch_update(c, &gd, nullptr, 0, g);
+    if (rc) return rc;
+    if (sync) HIPC(hipStreamSynchronize(c->stream));
+    return CC_OK;
+}
+
+void free_all(cc_cache *c) {
+    for (auto &p : c->col)
+        if (p) { hipFree(p); p = nullptr; }
+    if (c->cutlist) hipFree(c->cutlist);
+    if (c->counter) hipFree(c->counter);
+    if (c->prims_dev) hipFree(c->prims_dev);
+    for (auto &e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+}
+
+// ---- the marching-squares contour (src/marching_squares/surface.jl:59-177), on the host --------
+struct KeyHash {
+    size_t operator()(const std::pair<uint64_t, uint64_t> &k) const {
+        return std::hash<uint64_t>()(k.first * 0x9E3779B97F4A7C15ull ^ k.second);
+    }
+};
+
+template <typename T>
+void contour(const std::vector<T> &phi, const std::vector<int8_t> &kind, const cc_grid *g,
+             std::vector<double> &nodes, std::vector<int64_t> &lines) {
+    const int64_t n0 = g->n[0], n1 = g->n[1], sy = n0 + 1;
+    std::unordered_map<std::pair<uint64_t, uint64_t>, int64_t, KeyHash> ids;
+    auto vid = [&](T x, T y) -> int64_t {
+        // exact-equality weld; `+ 0` folds -0.0 onto 0.0 (surface.jl:170-177)
+        const double kx = (double)(x + T(0)), ky = (double)(y + T(0));
+        uint64_t bx, by;
+        std::memcpy(&bx, &kx, 8);
+        std::memcpy(&by, &ky, 8);
+        auto it = ids.find({bx, by});
+        if (it != ids.end()) return it->second;
+        nodes.push_back(kx);
+        nodes.push_back(ky);
+        const int64_t id = (int64_t)nodes.size() / 2;  // 1-based, as MeshLibrary numbers nodes
+        ids.emplace(std::make_pair(bx, by), id);
+        return id;
+    };
+    const T x0 = T(g->x0[0]), y0 = T(g->x0[1]), dx = T(g->d[0]), dy = T(g->d[1]);
+    for (int64_t c = 0; c < n0 * n1; ++c) {
+        if (kind[c] != CC_CELL_CUT) continue;  // the screen: corners not all of one sign
+        const int64_t i = c % n0, j = c / n0;
+        const T ph[4] = {phi[i + sy * j], phi[i + 1 + sy * j], phi[i + 1 + sy * (j + 1)],
+                         phi[i + sy * (j + 1)]};
+        const T xl = x0 + T(i) * dx, xh = x0 + T(i + 1) * dx;
+        const T yl = y0 + T(j) * dy, yh = y0 + T(j + 1) * dy;
+        const T nx[4] = {xl, xh, xh, xl}, ny[4] = {yl, yl, yh, yh};
+        T sax[4], say[4], sbx[4], sby[4];
+        bool pres[4];
+        for (int k = 0; k < 4; ++k) {  // cell_boundary_walk
+            const int k2 = (k + 1) & 3;
+            const T fa = ph[k], fb = ph[k2];
+            const bool oa = fa >= T(0), ob = fb >= T(0);
+            const T fr = (oa && ob) ? T(1) : ((!oa && !ob) ? T(0) : (oa ? fa / (fa - fb) : fb / (fb - fa)));
+            pres[k] = oa || ob;
+            if (oa && ob) { sax[k] = nx[k]; say[k] = ny[k]; sbx[k] = nx[k2]; sby[k] = ny[k2]; }
+            else if (oa) {
+                sax[k] = nx[k]; say[k] = ny[k];
+                sbx[k] = nx[k] + fr * (nx[k2] - nx[k]);
+                sby[k] = ny[k] + fr * (ny[k2] - ny[k]);
+            } else {
+                sax[k] = nx[k2] + fr * (nx[k] - nx[k2]);
+                say[k] = ny[k2] + fr * (ny[k] - ny[k2]);
+                sbx[k] = nx[k2]; sby[k] = ny[k2];
+            }
+        }
+        int nseg = 0;
+        for (int k = 0; k < 4 && nseg < 2; ++k) {  // cell_gap_segments, then cell_interface's swap
+            if (!pres[k]) continue;
+            int jn = k;
+            for (int s = 1; s <= 3; ++s)
+                if (pres[(k + s) & 3]) { jn = (k + s) & 3; break; }
+            const T ax = sbx[k], ay = sby[k], bx = sax[jn], by = say[jn];
+            if (ax != bx || ay != by) {
+                nseg += 1;
+                const int64_t i1 = vid(bx, by);
+                const int64_t i2 = vid(ax, ay);
+                lines.push_back(i1);
+                lines.push_back(i2);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cc_last_error(void) { return g_err.c_str(); }
+
+const char *cc_version(void) { return "libcutcell_hip 0.1.0 (gfx950)"; }
+
+int cc_cache_create(int method, int dtype, const cc_grid *grid, cc_cache **out) {
+    if (!out || !grid) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (dtype != CC_FLOAT64 && dtype != CC_FLOAT32)
+        return fail(CC_ERR_UNSUPPORTED, "element type must be CC_FLOAT64 or CC_FLOAT32");
+    const int dim = grid->dim;
+    if (method == CC_MARCHING_SQUARES && dim != 2)
+        return fail(CC_ERR_ARGUMENT, "MarchingSquaresCutCell needs a 2D grid");
+    if (method == CC_MARCHING_CUBES && dim != 3)
+        return fail(CC_ERR_ARGUMENT, "MarchingCubesCutCell needs a 3D grid");
+    if (method != CC_MARCHING_SQUARES && method != CC_MARCHING_CUBES && method != CC_PLIC)
+        return fail(CC_ERR_UNSUPPORTED, "unknown method %d", method);
+    if (dim != 2 && dim != 3) return fail(CC_ERR_ARGUMENT, "grid.dim must be 2 or 3");
+    int64_t ncell = 1, nnode = 1;
+    for (int a = 0; a < dim; ++a) {
+        if (grid->n[a] < 1) return fail(CC_ERR_ARGUMENT, "every axis needs at least one cell");
+        ncell *= grid->n[a];
+        nnode *= grid->n[a] + 1;
+    }
+    if (nnode >= (int64_t(1) << 32) || ncell >= (int64_t(1) << 32))
+        return fail(CC_ERR_UNSUPPORTED, "grids of 2^32 cells or nodes or more are not supported");
+
+    cc_cache *c = new cc_cache();
+    c->method = method;
+    c->dtype = dtype;
+    c->dim = dim;
+    for (int a = 0; a < dim; ++a) c->n[a] = grid->n[a];
+    c->ncell = ncell;
+    c->nnode = nnode;
+    hipGetDevice(&c->device);
+    int rc = check_grid(c, grid);
+    if (rc) { delete c; return rc; }
+
+    const size_t es = elem_size(dtype);
+    const int64_t D = dim;
+    int64_t bytes[CC_NUM_FIELDS] = {};
+    bytes[CC_FIELD_KIND] = ncell;
+    bytes[CC_FIELD_AMBIGUOUS] = ncell;
+    bytes[CC_FIELD_VOLUME_FRACTION] = ncell * es;
+    bytes[CC_FIELD_CENTROID] = ncell * D * es;
+    bytes[CC_FIELD_FACE_FRACTION] = ncell * 2 * D * es;
+    bytes[CC_FIELD_FACE_CENTROID_LOCAL] = ncell * 2 * D * (D - 1) * es;
+    bytes[CC_FIELD_INTERFACE_CENTROID] = ncell * D * es;
+    if (method == CC_PLIC) {
+        bytes[CC_FIELD_NORMALS] = ncell * D * es;
+        bytes[CC_FIELD_INTERCEPTS] = ncell * es;
+        bytes[CC_FIELD_IS_VALID] = ncell;
+    } else {
+        bytes[CC_FIELD_PHI] = nnode * es;
+    }
+#define CREATE_HIP(expr)                                                                        \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            int code = e_ == hipErrorOutOfMemory ? CC_ERR_OUT_OF_MEMORY : CC_ERR_HIP;           \
+            fail(code, "%s failed: %s", #expr, hipGetErrorString(e_));                          \
+            free_all(c);                                                                        \
+            delete c;                                                                           \
+            return code;                                                                        \
+        }                                                                                       \
+    } while (0)
+    CREATE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int f = 0; f < CC_NUM_FIELDS; ++f) {
+        c->colbytes[f] = bytes[f];
+        if (bytes[f] > 0) CREATE_HIP(hipMalloc(&c->col[f], bytes[f]));
+    }
+    CREATE_HIP(hipMalloc(&c->counter, 64));
+    if (method == CC_MARCHING_CUBES) CREATE_HIP(hipMalloc(&c->cutlist, ncell * sizeof(uint32_t)));
+    for (int e = 0; e <= MAX_TIMED; ++e) CREATE_HIP(hipEventCreate(&c->ev[e]));
+    if (method == CC_PLIC) {
+        // What a centroid plane fit has no value for, written once (src/plic/cache.jl:65-69).
+        const int fields[3] = {CC_FIELD_CENTROID, CC_FIELD_FACE_CENTROID_LOCAL,
+                               CC_FIELD_INTERFACE_CENTROID};
+        for (int f : fields) {
+            const size_t cnt = bytes[f] / es;
+            if (dtype == CC_FLOAT32)
+                hipLaunchKernelGGL(k_fill<float>, dim3(1024), dim3(256), 0, c->stream,
+                                   (float *)c->col[f], cnt, std::nanf(""));
+            else
+                hipLaunchKernelGGL(k_fill<double>, dim3(1024), dim3(256), 0, c->stream,
+                                   (double *)c->col[f], cnt, std::nan(""));
+        }
+        hipLaunchKernelGGL(k_fill<uint8_t>, dim3(1024), dim3(256), 0, c->stream,
+                           (uint8_t *)c->col[CC_FIELD_AMBIGUOUS], (size_t)ncell, (uint8_t)0);
+        CREATE_HIP(hipGetLastError());
+    }
+#undef CREATE_HIP
+    // Seeded as no body: one update against EmptyGeo(one(T)) (src/cache.jl:22-25).
+    cc_geometry empty{};
+    empty.kind = CC_GEO_EMPTY;
+    empty.p[0] = 1.0;
+    rc = update_impl(c, &empty, grid, true);
+    if (rc) {
+        std::string msg = g_err;
+        free_all(c);
+        delete c;
+        g_err = msg;
+        return rc;
+    }
+    *out = c;
+    return CC_OK;
+}
+
+int cc_cache_destroy(cc_cache *c) {
+    if (!c) return CC_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return CC_OK;
+}
+
+int cc_update(cc_cache *c, const cc_geometry *geo, const cc_grid *grid) {
+    return update_impl(c, geo, grid, true);
+}
+
+int cc_update_async(cc_cache *c, const cc_geometry *geo, const cc_grid *grid) {
+    return update_impl(c, geo, grid, false);
+}
+
+int cc_synchronize(cc_cache *c) {
+    if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
+    HIPC(hipStreamSynchronize(c->stream));
+    return CC_OK;
+}
+
+int cc_update_from_phi(cc_cache *c, const void *phi, int on_dev, const cc_grid *grid) {
+    if (!c || !phi) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    if (c->method == CC_PLIC)
+        return fail(CC_ERR_UNSUPPORTED, "PLIC fits planes from get_sdf, not from nodal values");
+    int rc = check_grid(c, grid);
+    if (rc) return rc;
+    rc = dispatch_update(c, nullptr, phi, on_dev, grid);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(c->stream));
+    return CC_OK;
+}
+
+int cc_column(cc_cache *c, int field, void **ptr, int64_t *nbytes) {
+    if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
+    if (field < 0 || field >= CC_NUM_FIELDS || !c->col[field])
+        return fail(CC_ERR_ARGUMENT, "the cache has no column %d", field);
+    if (ptr) *ptr = c->col[field];
+    if (nbytes) *nbytes = c->colbytes[field];
+    return CC_OK;
+}
+
+int cc_copy_to_host(cc_cache *c, int field, int64_t offset, int64_t nbytes, void *dst) {
+    if (!c || !dst) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    if (field < 0 || field >= CC_NUM_FIELDS || !c->col[field])
+        return fail(CC_ERR_ARGUMENT, "the cache has no column %d", field);
+    if (offset < 0 || nbytes < 0 || offset + nbytes > c->colbytes[field])
+        return fail(CC_ERR_ARGUMENT, "byte range [%lld, %lld) is outside the column (%lld bytes)",
+                    (long long)offset, (long long)(offset + nbytes), (long long)c->colbytes[field]);
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(hipMemcpy(dst, (char *)c->col[field] + offset, nbytes, hipMemcpyDeviceToHost));
+    return CC_OK;
+}
+
+int cc_cut_count(cc_cache *c, int64_t *count) {
+    if (!c || !count) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    if (c->method == CC_PLIC) return fail(CC_ERR_UNSUPPORTED, "PLIC keeps no cut-cell list");
+    uint32_t v = 0;
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->stream));
+    HIPC(hipMemcpy(&v, c->counter, sizeof v, hipMemcpyDeviceToHost));
+    *count = v;
+    return CC_OK;
+}
+
+int cc_stream(cc_cache *c, void **stream) {
+    if (!c || !stream) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    *stream = (void *)c->stream;
+    return CC_OK;
+}
+
+int cc_set_profiling(cc_cache *c, int on) {
+    if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
+    c->profiling = on != 0;
+    return CC_OK;
+}
+
+int cc_kernel_times(cc_cache *c, int max, float *ms, const char **names, int *count) {
+    if (!c || !count) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    *count = 0;
+    if (!c->ev_ready) return CC_OK;
+    HIPC(hipEventSynchronize(c->ev[c->nev]));
+    for (int k = 0; k < c->nev && k < max; ++k) {
+        float t = 0.0f;
+        HIPC(hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]));
+        if (ms) ms[k] = t;
+        if (names) names[k] = c->ev_name[k];
+        *count = k + 1;
+    }
+    return CC_OK;
+}
+
+int cc_generate_mesh_ms(cc_cache *c, const cc_grid *grid, double *nodes, int64_t *nnodes,
+                        int64_t *lines, int64_t *nlines) {
+    if (!c || !nnodes || !nlines) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    if (c->method != CC_MARCHING_SQUARES)
+        return fail(CC_ERR_UNSUPPORTED, "generate_mesh here is the marching-squares contour");
+    int rc = check_grid(c, grid);
+    if (rc) return rc;
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->stream));
+    std::vector<int8_t> kind(c->ncell);
+    HIPC(hipMemcpy(kind.data(), c->col[CC_FIELD_KIND], c->ncell, hipMemcpyDeviceToHost));
+    std::vector<double> vn;
+    std::vector<int64_t> vl;
+    if (c->dtype == CC_FLOAT32) {
+        std::vector<float> phi(c->nnode);
+        HIPC(hipMemcpy(phi.data(), c->col[CC_FIELD_PHI], c->colbytes[CC_FIELD_PHI], hipMemcpyDeviceToHost));
+        contour<float>(phi, kind, grid, vn, vl);
+    } else {
+        std::vector<double> phi(c->nnode);
+        HIPC(hipMemcpy(phi.data(), c->col[CC_FIELD_PHI], c->colbytes[CC_FIELD_PHI], hipMemcpyDeviceToHost));
+        contour<double>(phi, kind, grid, vn, vl);
+    }
+    const int64_t nn = (int64_t)vn.size() / 2, nl = (int64_t)vl.size() / 2;
+    if (nodes && lines) {
+        if (*nnodes < nn || *nlines < nl)
+            return fail(CC_ERR_ARGUMENT, "buffers hold %lld nodes / %lld lines, need %lld / %lld",
+                        (long long)*nnodes, (long long)*nlines, (long long)nn, (long long)nl);
+        std::memcpy(nodes, vn.data(), vn.size() * sizeof(double));
+        std::memcpy(lines, vl.data(), vl.size() * sizeof(int64_t));
+    }
+    *nnodes = nn;
+    *nlines = nl;
+    return CC_OK;
+}
+
+}  // extern "C"

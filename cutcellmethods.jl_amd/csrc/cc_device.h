@@ -1,0 +1,200 @@
+// Device-side building blocks shared by the cut-cell kernels: element-type limits, the body
+// descriptor and its signed-distance functions, and the lattice helpers.
+//
+// Compiled with -ffp-contract=off: the reference's Julia arithmetic fuses nothing (no @fastmath in
+// src/), and the watertightness argument (src/marching_squares/marching_squares.jl:17-31) needs the
+// same expression, rounded the same way, in both cells that share a face.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cc {
+
+template <typename T> struct Eps;
+template <> struct Eps<double> { static constexpr double v = 2.220446049250313e-16; };
+template <> struct Eps<float> { static constexpr float v = 1.1920928955078125e-07f; };
+
+enum : int8_t { CELL_INSIDE = 0, CELL_OUTSIDE = 1, CELL_CUT = 2 };  // src/CutCellMethods.jl:56-58
+
+enum { GEO_EMPTY = 0, GEO_SPHERE = 1, GEO_TORUS = 2, GEO_BOX = 3, GEO_PLANE = 4,
+       GEO_SPHERE_UNION = 5 };
+
+// The body as the kernels see it: parameters by value, the primitive table in device memory.
+struct Geo {
+    int kind;
+    int nprims;
+    double p[24];
+    const double *prims;  // device pointer, nprims x (cx, cy, cz, r)
+};
+
+// The grid as the kernels see it, converted to the cache's element type once on the host
+// (update_cache! converts the grid to CartesianGrid{D,T}, src/marching_squares/cache.jl:58).
+template <typename T> struct GridT {
+    T x0[3];
+    T d[3];
+    int n[3];
+};
+
+template <typename T> __device__ __forceinline__ T sgn(T v) {
+    return v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0));
+}
+
+// _open_fraction, src/marching_squares/marching_squares.jl:192-198: anchored on the outside end.
+template <typename T> __device__ __forceinline__ T open_fraction(T a, T b) {
+    const bool fa = a >= T(0), fb = b >= T(0);
+    if (fa && fb) return T(1);
+    if (!fa && !fb) return T(0);
+    return fa ? a / (a - b) : b / (b - a);
+}
+
+// ||x - c|| - r with the norm accumulated ((dx^2 + dy^2) + dz^2).
+template <typename T, int D>
+__device__ __forceinline__ T sphere_dist(T cx, T cy, T cz, T r, T x, T y, T z, T *nr_out = nullptr,
+                                         T *v = nullptr) {
+    T dx = x - cx, dy = y - cy, dz = z - cz;
+    T s = dx * dx + dy * dy;
+    if (D == 3) s = s + dz * dz;
+    T nr = sqrt(s);
+    if (nr_out) *nr_out = nr;
+    if (v) { v[0] = dx; v[1] = dy; v[2] = dz; }
+    return nr - r;
+}
+
+// Polynomial smooth minimum: h = max(k - |a-b|, 0)/k; min(a,b) - h*h*k/4.
+template <typename T> __device__ __forceinline__ T smin_poly(T a, T b, T k) {
+    T h = k - fabs(a - b);
+    h = (h > T(0) ? h : T(0)) / k;
+    T m = a < b ? a : b;
+    return m - ((h * h) * k) / T(4);
+}
+
+// p = R^T (x - c), R row-major 3x3.
+template <typename T, int D>
+__device__ __forceinline__ void to_local(const double *r, const double *c, const T *x, T *p) {
+    T v[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int a = 0; a < D; ++a) v[a] = x[a] - T(c[a]);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        T s = T(r[0 * 3 + i]) * v[0];
+#pragma unroll
+        for (int j = 1; j < D; ++j) s = s + T(r[j * 3 + i]) * v[j];
+        p[i] = s;
+    }
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void from_local(const double *r, const T *gl, T *n) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        T s = T(r[i * 3 + 0]) * gl[0];
+#pragma unroll
+        for (int j = 1; j < D; ++j) s = s + T(r[i * 3 + j]) * gl[j];
+        n[i] = s;
+    }
+}
+
+// The sphere union folded left with the polynomial smooth minimum.
+template <typename T, int D>
+__device__ __forceinline__ T union_dist(const Geo &g, const T *x) {
+    const T z = D == 3 ? x[2] : T(0);
+    const double *s = g.prims;
+    const T k = T(g.p[0]);
+    T acc = sphere_dist<T, D>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), x[0], x[1], z);
+    for (int i = 1; i < g.nprims; ++i) {
+        const double *si = s + 4 * i;
+        acc = smin_poly(acc, sphere_dist<T, D>(T(si[0]), T(si[1]), T(si[2]), T(si[3]), x[0], x[1], z),
+                        k);
+    }
+    return acc;
+}
+
+// get_sdf(geo, x) -> (distance, normal) (SDFLibrary, un-vendored: the build's own bodies).  The
+// normal is the analytic gradient; it is NaN or zero where the distance is not differentiable,
+// which the PLIC fit reports as a degenerate plane (src/plic/plic.jl:82-85).
+template <typename T, int D>
+__device__ __forceinline__ T get_sdf(const Geo &g, const T *x, T *normal) {
+    T v[3] = {T(0), T(0), T(0)}, p[3] = {T(0), T(0), T(0)}, gl[3] = {T(0), T(0), T(0)};
+    switch (g.kind) {
+    case GEO_SPHERE: {
+        T nr;
+        T dist = sphere_dist<T, D>(T(g.p[0]), T(g.p[1]), T(g.p[2]), T(g.p[3]), x[0], x[1],
+                                   D == 3 ? x[2] : T(0), &nr, v);
+#pragma unroll
+        for (int a = 0; a < D; ++a) normal[a] = v[a] / nr;
+        return dist;
+    }
+    case GEO_TORUS: {  // 3D only
+        const double *r = g.p + 5;
+        to_local<T, 3>(r, g.p, x, p);
+        T rho = sqrt(p[0] * p[0] + p[1] * p[1]);
+        T qx = rho - T(g.p[3]), qz = p[2];
+        T nq = sqrt(qx * qx + qz * qz);
+        T u = qx / nq;
+        gl[0] = u * (p[0] / rho);
+        gl[1] = u * (p[1] / rho);
+        gl[2] = qz / nq;
+        from_local<T, 3>(r, gl, normal);
+        return nq - T(g.p[4]);
+    }
+    case GEO_BOX: {
+        const double *r = g.p + 6;
+        T q[3] = {T(0), T(0), T(0)}, mx[3] = {T(0), T(0), T(0)};
+        to_local<T, D>(r, g.p, x, p);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            q[a] = fabs(p[a]) - T(g.p[3 + a]);
+            mx[a] = q[a] > T(0) ? q[a] : T(0);
+        }
+        T o = mx[0] * mx[0];
+#pragma unroll
+        for (int a = 1; a < D; ++a) o = o + mx[a] * mx[a];
+        o = sqrt(o);
+        T qm = q[0];
+        int am = 0;
+#pragma unroll
+        for (int a = 1; a < D; ++a)
+            if (q[a] > qm) { qm = q[a]; am = a; }
+        T inner = qm < T(0) ? qm : T(0);
+        if (qm > T(0)) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) gl[a] = (mx[a] / o) * sgn(p[a]);
+        } else {
+#pragma unroll
+            for (int a = 0; a < D; ++a) gl[a] = a == am ? sgn(p[a]) : T(0);
+        }
+        from_local<T, D>(r, gl, normal);
+        return o + inner;
+    }
+    case GEO_PLANE: {
+        T s = T(g.p[0]) * (x[0] - T(g.p[3]));
+#pragma unroll
+        for (int a = 1; a < D; ++a) s = s + T(g.p[a]) * (x[a] - T(g.p[3 + a]));
+#pragma unroll
+        for (int a = 0; a < D; ++a) normal[a] = T(g.p[a]);
+        return s;
+    }
+    default: break;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) normal[a] = T(0);
+    if (g.kind == GEO_EMPTY) return T(g.p[0]);
+    return union_dist<T, D>(g, x);  // the sphere union offers no analytic normal
+}
+
+// sdf_value(geo, x) (SDFLibrary, un-vendored: the build's own bodies).
+template <typename T, int D>
+__device__ __forceinline__ T sdf_value(const Geo &g, const T *x) {
+    const T z = D == 3 ? x[2] : T(0);
+    switch (g.kind) {
+    case GEO_EMPTY: return T(g.p[0]);
+    case GEO_SPHERE:
+        return sphere_dist<T, D>(T(g.p[0]), T(g.p[1]), T(g.p[2]), T(g.p[3]), x[0], x[1], z);
+    case GEO_SPHERE_UNION: return union_dist<T, D>(g, x);
+    default: break;
+    }
+    T nrm[3];
+    return get_sdf<T, D>(g, x, nrm);
+}
+
+}  // namespace cc

@@ -1,0 +1,49 @@
+// Host-side declarations shared by the kernel translation units and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cc_device.h"
+
+namespace cc {
+
+// One device pointer per CutCellData column (src/cut_cell.jl:112-120).
+template <typename T> struct Cols {
+    int8_t *kind;
+    uint8_t *amb;
+    T *vf;
+    T *centroid;
+    T *ff;
+    T *fcl;
+    T *ic;
+};
+
+// Tile width of the marching-cubes stream pass: cells per workgroup, one per thread.
+constexpr int MC_TILE = 256;
+// Workgroups of the cut-cell pass (a grid-stride loop over the device-side count).
+constexpr int MC_CUT_BLOCKS = 2048;
+
+template <typename T>
+void launch_sample(T *phi, const GridT<T> &g, const Geo &geo, int dim, hipStream_t s);
+
+template <typename T>
+void launch_mc_stream(const T *phi, const GridT<T> &g, const Cols<T> &cols, uint32_t *cutlist,
+                      uint32_t *counter, hipStream_t s);
+template <typename T>
+void launch_mc_cut(const T *phi, const GridT<T> &g, const Cols<T> &cols, const uint32_t *cutlist,
+                   const uint32_t *counter, hipStream_t s);
+
+template <typename T>
+void launch_ms_cells(const T *phi, const GridT<T> &g, const Cols<T> &cols, uint32_t *counter,
+                     hipStream_t s);
+
+template <typename T>
+void launch_plic_fit(const GridT<T> &g, const Geo &geo, int dim, const Cols<T> &cols, T *normals,
+                     T *intercepts, uint8_t *valid, hipStream_t s);
+template <typename T>
+void launch_plic_faces(const GridT<T> &g, int dim, const Cols<T> &cols, const T *normals,
+                       const T *intercepts, const uint8_t *valid, hipStream_t s);
+
+void launch_zero_u32(uint32_t *p, hipStream_t s);
+
+}  // namespace cc

@@ -1,0 +1,132 @@
+"""ctypes binding of libcutcell_hip.so (the C ABI declared in include/cutcell_hip.h).
+
+The library is built in-tree by ``cutcellmethods.jl_amd/build.py``; nothing here falls back to a
+CPU path: if the library is missing or no HIP device is present, calls raise.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libcutcell_hip.so")
+
+CC_OK = 0
+CC_ERR_DIMENSION_MISMATCH = 1
+CC_ERR_ARGUMENT = 2
+CC_ERR_HIP = 3
+CC_ERR_UNSUPPORTED = 4
+CC_ERR_OUT_OF_MEMORY = 5
+
+CC_MARCHING_SQUARES, CC_MARCHING_CUBES, CC_PLIC = 1, 2, 3
+CC_FLOAT64, CC_FLOAT32 = 0, 1
+
+FIELDS = {
+    "kind": 0, "ambiguous": 1, "volume_fraction": 2, "centroid": 3, "face_fraction": 4,
+    "face_centroid_local": 5, "interface_centroid": 6, "phi": 7, "normals": 8, "intercepts": 9,
+    "is_valid": 10,
+}
+
+EXPORTS = ("cc_cache_create", "cc_cache_destroy", "cc_update", "cc_update_from_phi",
+           "cc_update_async", "cc_synchronize", "cc_column", "cc_copy_to_host", "cc_cut_count",
+           "cc_stream", "cc_set_profiling", "cc_kernel_times", "cc_generate_mesh_ms",
+           "cc_last_error", "cc_version")
+
+
+class CCGeometry(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("n_prims", ctypes.c_int32),
+                ("p", ctypes.c_double * 24), ("prims", ctypes.c_void_p)]
+
+
+class CCGrid(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("n", ctypes.c_int64 * 3), ("x0", ctypes.c_double * 3), ("d", ctypes.c_double * 3)]
+
+
+class DimensionMismatch(ValueError):
+    """The reference's DimensionMismatch (src/cache.jl:107-121)."""
+
+
+class ArgumentError(ValueError):
+    """The reference's ArgumentError (src/plic/surface.jl:41-45)."""
+
+
+class HIPError(RuntimeError):
+    """A HIP runtime failure inside libcutcell_hip."""
+
+
+class Unsupported(NotImplementedError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libcutcell_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run cutcellmethods.jl_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)
+        L.cc_cache_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(CCGrid),
+                                      ctypes.POINTER(vp)]
+        L.cc_cache_destroy.argtypes = [vp]
+        L.cc_update.argtypes = [vp, ctypes.POINTER(CCGeometry), ctypes.POINTER(CCGrid)]
+        L.cc_update_async.argtypes = [vp, ctypes.POINTER(CCGeometry), ctypes.POINTER(CCGrid)]
+        L.cc_update_from_phi.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(CCGrid)]
+        L.cc_synchronize.argtypes = [vp]
+        L.cc_column.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp), i64p]
+        L.cc_copy_to_host.argtypes = [vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, vp]
+        L.cc_cut_count.argtypes = [vp, i64p]
+        L.cc_stream.argtypes = [vp, ctypes.POINTER(vp)]
+        L.cc_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.cc_kernel_times.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                      ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_int)]
+        L.cc_generate_mesh_ms.argtypes = [vp, ctypes.POINTER(CCGrid), vp, i64p, vp, i64p]
+        L.cc_last_error.restype = ctypes.c_char_p
+        L.cc_version.restype = ctypes.c_char_p
+        for name in EXPORTS:
+            if name not in ("cc_last_error", "cc_version"):
+                getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc == CC_OK:
+        return
+    msg = lib().cc_last_error().decode(errors="replace")
+    if rc == CC_ERR_DIMENSION_MISMATCH:
+        raise DimensionMismatch(msg)
+    if rc == CC_ERR_ARGUMENT:
+        raise ArgumentError(msg)
+    if rc == CC_ERR_UNSUPPORTED:
+        raise Unsupported(msg)
+    if rc == CC_ERR_OUT_OF_MEMORY:
+        raise MemoryError(msg)
+    raise HIPError(msg)
+
+
+def make_grid(grid):
+    g = CCGrid()
+    g.dim = grid.dim
+    for a in range(3):
+        g.n[a] = grid.n[a] if a < grid.dim else 1
+        g.x0[a] = grid.x0[a] if a < grid.dim else 0.0
+        g.d[a] = grid.d[a] if a < grid.dim else 1.0
+    return g
+
+
+def make_geometry(geo):
+    import numpy as np
+    g = CCGeometry()
+    g.kind = int(geo.kind)
+    for i, v in enumerate(list(geo.params)[:24]):
+        g.p[i] = float(v)
+    prims = getattr(geo, "prims", None)
+    if prims is not None:
+        arr = np.ascontiguousarray(np.asarray(prims, dtype=np.float64).reshape(-1, 4))
+        g.n_prims = arr.shape[0]
+        g.prims = arr.ctypes.data
+        g._keep = arr
+    return g

@@ -1,0 +1,257 @@
+"""The reference's cache API over libcutcell_hip -- ``allocate_cache`` / ``update_cache`` /
+``generate_mesh`` and the uniform readers (``src/cache.jl:15-121``).
+
+* ``allocate_cache(grid, method, dtype=np.float64)`` allocates every column on the device and seeds
+  the cache as "no body" (``src/marching_squares/cache.jl:41-47``, ``src/marching_cubes/cache.jl:30-36``,
+  ``src/plic/cache.jl:60-75``).
+* ``update_cache(cache, geo, grid)`` is ``update_cache!``: same ``n``, any origin; blocks until the
+  device has finished (``src/cache.jl:31-45``).  ``geo`` is a ``cutcell.geometry`` descriptor, or a
+  nodal array / callable for the nodal methods (the reference's "any callable ``x -> phi``").
+* ``cache.cells`` reads columns back as numpy arrays (``Adapt.adapt(Array, cache.cells)``): flat
+  over the cell linear index (i fastest), with the per-cell components as trailing axes.
+* ``generate_mesh(cache, grid)`` is the marching-squares contour (``src/marching_squares/surface.jl``).
+
+Errors keep the reference's types: ``DimensionMismatch`` for a wrong ``n``, ``ArgumentError`` for an
+anisotropic PLIC grid.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError, DimensionMismatch, check, lib, make_geometry, make_grid
+from .grid import CartesianGrid
+
+
+class MarchingSquaresCutCell:
+    """src/marching_squares/marching_squares.jl:57"""
+    code = _lib.CC_MARCHING_SQUARES
+
+
+class MarchingCubesCutCell:
+    """src/marching_cubes/marching_cubes.jl:54"""
+    code = _lib.CC_MARCHING_CUBES
+
+
+class PLICCutCell:
+    """src/plic/plic.jl:30"""
+    code = _lib.CC_PLIC
+
+
+def _method_code(method):
+    if isinstance(method, type):
+        return method.code
+    return method.code
+
+
+class Cells:
+    """Host views of a cache's CutCellData columns, read on access."""
+
+    def __init__(self, cache):
+        self._c = cache
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        if name in ("kind", "ambiguous", "volume_fraction", "centroid", "face_fraction",
+                    "face_centroid_local", "interface_centroid"):
+            return self._c.read(name)
+        raise AttributeError(name)
+
+    def __len__(self):
+        return self._c.grid.ncells
+
+
+class Cache:
+    """A device cache of one method over one grid's ``n`` (MarchingSquaresCutCellCache,
+    MarchingCubesCutCellCache or PLICCutCellCache)."""
+
+    def __init__(self, handle, grid, method, dtype):
+        self._h = ctypes.c_void_p(handle)
+        self.grid = grid
+        self.method = method
+        self.dtype = np.dtype(dtype)
+        self.cells = Cells(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().cc_cache_destroy(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- columns -----------------------------------------------------------------------------
+    def _shape(self, name):
+        D = self.grid.dim
+        N = self.grid.ncells
+        return {
+            "kind": (N,), "ambiguous": (N,), "volume_fraction": (N,), "centroid": (N, D),
+            "face_fraction": (N, 2 * D), "face_centroid_local": (N, 2 * D, D - 1),
+            "interface_centroid": (N, D), "phi": (int(np.prod(self.grid.nodegrid_size())),),
+            "normals": (N, D), "intercepts": (N,), "is_valid": (N,),
+        }[name]
+
+    def _dtype(self, name):
+        if name == "kind":
+            return np.dtype(np.int8)
+        if name in ("ambiguous", "is_valid"):
+            return np.dtype(np.uint8)
+        return self.dtype
+
+    def column_ptr(self, name):
+        """(device pointer, byte size) of one column: the cache's own storage."""
+        p = ctypes.c_void_p()
+        nb = ctypes.c_int64()
+        check(lib().cc_column(self._h, _lib.FIELDS[name], ctypes.byref(p), ctypes.byref(nb)))
+        return p.value, nb.value
+
+    def read(self, name, start=0, stop=None):
+        """Column ``name`` copied to the host, optionally only leading-axis rows [start, stop)."""
+        shape = self._shape(name)
+        dt = self._dtype(name)
+        stop = shape[0] if stop is None else stop
+        per = int(np.prod(shape[1:])) * dt.itemsize
+        out = np.empty((stop - start,) + shape[1:], dtype=dt)
+        check(lib().cc_copy_to_host(self._h, _lib.FIELDS[name], start * per, (stop - start) * per,
+                                    out.ctypes.data))
+        return out
+
+    @property
+    def phi(self):
+        return self.read("phi")
+
+    @property
+    def normals(self):
+        return self.read("normals")
+
+    @property
+    def intercepts(self):
+        return self.read("intercepts")
+
+    @property
+    def is_valid(self):
+        return self.read("is_valid")
+
+    def cut_count(self):
+        v = ctypes.c_int64()
+        check(lib().cc_cut_count(self._h, ctypes.byref(v)))
+        return v.value
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        check(lib().cc_stream(self._h, ctypes.byref(s)))
+        return s.value
+
+    def set_profiling(self, on=True):
+        check(lib().cc_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        """[(kernel name, ms)] of the last update (profiling must be on)."""
+        ms = (ctypes.c_float * 8)()
+        names = (ctypes.c_char_p * 8)()
+        n = ctypes.c_int()
+        check(lib().cc_kernel_times(self._h, 8, ms, names, ctypes.byref(n)))
+        return [(names[i].decode(), float(ms[i])) for i in range(n.value)]
+
+
+def allocate_cache(grid, method, dtype=np.float64):
+    """allocate_cache(grid, method; backend) -- seeded as no body."""
+    dt = np.dtype(dtype)
+    if dt not in (np.dtype(np.float64), np.dtype(np.float32)):
+        raise ArgumentError("dtype must be float64 or float32")
+    h = ctypes.c_void_p()
+    g = make_grid(grid)
+    check(lib().cc_cache_create(_method_code(method),
+                                _lib.CC_FLOAT32 if dt == np.float32 else _lib.CC_FLOAT64,
+                                ctypes.byref(g), ctypes.byref(h)))
+    return Cache(h.value, grid, method, dt)
+
+
+def _check_same_n(cache, grid):
+    if tuple(grid.n) != tuple(cache.grid.n):
+        raise DimensionMismatch(
+            f"the cache holds {cache.grid.n} cells, but `grid` has {grid.n}; a cache is allocated "
+            "for one grid's `n`, and only its origin may move between updates")
+
+
+def update_cache(cache, geo, grid, sync=True):
+    """update_cache!(cache, geo, grid) -> cache.
+
+    ``geo`` is a geometry descriptor, a nodal array (one value per node, column-major or shaped as
+    the node grid), or a callable ``f(x) -> phi`` sampled at every node on the host.
+    """
+    _check_same_n(cache, grid)
+    g = make_grid(grid)
+    if hasattr(geo, "kind") and hasattr(geo, "params"):
+        gg = make_geometry(geo)
+        fn = lib().cc_update if sync else lib().cc_update_async
+        check(fn(cache.handle, ctypes.byref(gg), ctypes.byref(g)))
+        cache.grid = grid
+        return cache
+    nn = grid.nodegrid_size()
+    if callable(geo):
+        from itertools import product
+        pts = product(*[range(m) for m in reversed(nn)])
+        vals = np.array([float(geo(grid.get_node(tuple(reversed(ix))))) for ix in pts])
+    else:
+        vals = np.asarray(geo)
+        if vals.ndim > 1:
+            if vals.shape != tuple(nn):
+                raise DimensionMismatch(
+                    f"`vals` is {vals.shape}, but `grid` has {nn} nodes ({grid.n} elements per "
+                    "axis); the nodal construction reads one value per grid node")
+            vals = vals.reshape(-1, order="F")
+    vals = np.ascontiguousarray(vals, dtype=cache.dtype)
+    if vals.size != int(np.prod(nn)):
+        raise DimensionMismatch(f"`vals` has {vals.size} values, but `grid` has {nn} nodes")
+    check(lib().cc_update_from_phi(cache.handle, vals.ctypes.data, 0, ctypes.byref(g)))
+    cache.grid = grid
+    return cache
+
+
+def update_cache_from_device_phi(cache, phi_ptr, grid):
+    """update_cache! from a field already in device memory (a HIP pointer to one value per node)."""
+    _check_same_n(cache, grid)
+    g = make_grid(grid)
+    check(lib().cc_update_from_phi(cache.handle, ctypes.c_void_p(phi_ptr), 1, ctypes.byref(g)))
+    cache.grid = grid
+    return cache
+
+
+def generate_mesh(cache, grid):
+    """The marching-squares contour of the cache's phi as (nodes (m, 2), lines (k, 2) 1-based)."""
+    g = make_grid(grid)
+    nn = ctypes.c_int64()
+    nl = ctypes.c_int64()
+    check(lib().cc_generate_mesh_ms(cache.handle, ctypes.byref(g), None, ctypes.byref(nn), None,
+                                    ctypes.byref(nl)))
+    nodes = np.empty((nn.value, 2))
+    lines = np.empty((nl.value, 2), dtype=np.int64)
+    check(lib().cc_generate_mesh_ms(cache.handle, ctypes.byref(g), nodes.ctypes.data,
+                                    ctypes.byref(nn), lines.ctypes.data, ctypes.byref(nl)))
+    return nodes, lines
+
+
+def volume_fractions(cache):
+    return cache.read("volume_fraction")
+
+
+def kinds(cache):
+    return cache.read("kind")
+
+
+def face_fractions(cache):
+    return cache.read("face_fraction")
+
+
+def is_cell_open(cache):
+    """Per cell: whether any face is open (src/cache.jl:95-96), read off face_fraction alone."""
+    return np.any(face_fractions(cache) != 0, axis=1)

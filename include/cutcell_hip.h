@@ -1,0 +1,144 @@
+/* libcutcell_hip -- the C ABI of an MI355X (gfx950) implementation of CutCellMethods.jl's
+ * update_cache! hot path: nodal marching squares (2D), nodal marching cubes (3D) and PLIC.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns a status code and never
+ * throws across the ABI.  A failing call leaves a message for cc_last_error().  Device buffers are
+ * owned by the library and live until cc_cache_destroy.  Calls on different caches are thread-safe;
+ * calls on one cache must be serialised by the caller (as in the reference, which has no locking).
+ *
+ * Layout (src/cut_cell.jl:112-120, src/cache.jl:101-104): one device column per CutCellData field,
+ * column-major over the grid (i fastest), vector fields interleaved per cell:
+ *   kind int8[N] | ambiguous uint8[N] | volume_fraction T[N] | centroid T[N][D] |
+ *   face_fraction T[N][2D] | face_centroid_local T[N][2D][D-1] | interface_centroid T[N][D]
+ * plus phi T[(n0+1)(n1+1)(n2+1)] for the nodal methods, and normals T[N][D], intercepts T[N],
+ * is_valid uint8[N] for PLIC.  Directions are 0 = -x, 1 = +x, 2 = -y, ... (the reference's 1..2D).
+ */
+#ifndef CUTCELL_HIP_H
+#define CUTCELL_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------------------------- */
+#define CC_OK 0
+#define CC_ERR_DIMENSION_MISMATCH 1 /* the reference's DimensionMismatch (src/cache.jl:107-121) */
+#define CC_ERR_ARGUMENT 2           /* the reference's ArgumentError (src/plic/surface.jl:41-45) */
+#define CC_ERR_HIP 3                /* a HIP runtime failure                                    */
+#define CC_ERR_UNSUPPORTED 4        /* method/geometry/precision combination not offered         */
+#define CC_ERR_OUT_OF_MEMORY 5
+
+/* ---- method tags (src/CutCellMethods.jl:4-5) and element types ------------------------------ */
+#define CC_MARCHING_SQUARES 1 /* MarchingSquaresCutCell, 2D  (src/marching_squares/marching_squares.jl:57) */
+#define CC_MARCHING_CUBES 2   /* MarchingCubesCutCell, 3D    (src/marching_cubes/marching_cubes.jl:54)    */
+#define CC_PLIC 3             /* PLICCutCell, 2D or 3D       (src/plic/plic.jl:30)                        */
+#define CC_FLOAT64 0
+#define CC_FLOAT32 1
+
+/* ---- cell kinds (src/CutCellMethods.jl:56-58) ------------------------------------------------ */
+#define CC_CELL_INSIDE 0
+#define CC_CELL_OUTSIDE 1
+#define CC_CELL_CUT 2
+
+/* ---- columns --------------------------------------------------------------------------------- */
+#define CC_FIELD_KIND 0
+#define CC_FIELD_AMBIGUOUS 1
+#define CC_FIELD_VOLUME_FRACTION 2
+#define CC_FIELD_CENTROID 3
+#define CC_FIELD_FACE_FRACTION 4
+#define CC_FIELD_FACE_CENTROID_LOCAL 5
+#define CC_FIELD_INTERFACE_CENTROID 6
+#define CC_FIELD_PHI 7        /* nodal methods (MarchingSquaresCutCellCache.phi, src/marching_squares/cache.jl:32-35) */
+#define CC_FIELD_NORMALS 8    /* PLIC (src/plic/cache.jl:48-53) */
+#define CC_FIELD_INTERCEPTS 9 /* PLIC */
+#define CC_FIELD_IS_VALID 10  /* PLIC */
+#define CC_NUM_FIELDS 11
+
+/* ---- geometry (POD form of an SDFLibrary body) -------------------------------- */
+#define CC_GEO_EMPTY 0        /* p[0] constant value; EmptyGeo(one(T)) */
+#define CC_GEO_SPHERE 1       /* p[0..2] centre, p[3] radius (SDFSphere; SDFCircle in 2D) */
+#define CC_GEO_TORUS 2        /* p[0..2] centre, p[3] major R, p[4] minor r, p[5..13] rotation (row-major) */
+#define CC_GEO_BOX 3          /* p[0..2] centre, p[3..5] half extents, p[6..14] rotation */
+#define CC_GEO_PLANE 4        /* p[0..2] unit normal, p[3..5] point on the plane */
+#define CC_GEO_SPHERE_UNION 5 /* p[0] smoothing k; prims = n_prims x (cx, cy, cz, r), host memory */
+
+typedef struct cc_geometry {
+    int32_t kind;
+    int32_t n_prims;
+    double p[24];
+    const double *prims;
+} cc_geometry;
+
+/* A CartesianGrid{D}: node i (0-based) of axis a sits at x0[a] + i*d[a] (CartesianMeshes.get_node). */
+typedef struct cc_grid {
+    int32_t dim;
+    int32_t reserved;
+    int64_t n[3];
+    double x0[3];
+    double d[3];
+} cc_grid;
+
+typedef struct cc_cache cc_cache;
+
+/* allocate_cache(grid, method; backend) -- src/marching_squares/cache.jl:41-47,
+ * src/marching_cubes/cache.jl:30-36, src/plic/cache.jl:60-75.  Allocates every column on the current
+ * device and seeds the cache as "no body" with one update against EmptyGeo(1) (src/cache.jl:22-25);
+ * PLIC additionally fills the centroid columns with NaN once.  Returns CC_ERR_ARGUMENT for an
+ * anisotropic PLIC grid (src/plic/surface.jl:41-45) or a method/dimension mismatch. */
+int cc_cache_create(int method, int dtype, const cc_grid *grid, cc_cache **out);
+
+/* Releases every device buffer of the cache (the Julia finalizer's job). */
+int cc_cache_destroy(cc_cache *cache);
+
+/* update_cache!(cache, geo, grid) -- src/marching_squares/cache.jl:49-64, src/marching_cubes/cache.jl:38-39,
+ * src/plic/cache.jl:77-92.  `grid` must have the cache's n (CC_ERR_DIMENSION_MISMATCH otherwise,
+ * src/cache.jl:107-112); its origin may move.  Blocks until the device has finished (src/cache.jl:40). */
+int cc_update(cc_cache *cache, const cc_geometry *geo, const cc_grid *grid);
+
+/* update_cache! with a field the caller sampled: the reference's "any callable x -> phi"
+ * (src/marching_squares/marching_squares.jl:138-141) for the nodal methods.  `phi` holds one value
+ * per node in the cache's element type, column-major; `phi_on_device` != 0 for a device pointer.
+ * Checks the node count (src/cache.jl:115-121).  Blocks until done. */
+int cc_update_from_phi(cc_cache *cache, const void *phi, int phi_on_device, const cc_grid *grid);
+
+/* The same two updates without the final synchronize, for callers that order work on the cache's
+ * stream themselves (benchmarks, graph capture).  Not a reference entry point. */
+int cc_update_async(cc_cache *cache, const cc_geometry *geo, const cc_grid *grid);
+int cc_synchronize(cc_cache *cache);
+
+/* The cache's own storage (volume_fractions/kinds/face_fractions return storage, src/cache.jl:47-82):
+ * device pointer and byte size of one column. */
+int cc_column(cc_cache *cache, int field, void **device_ptr, int64_t *nbytes);
+
+/* Copies bytes [offset, offset + nbytes) of one column to host memory (Adapt.adapt(Array, ...)). */
+int cc_copy_to_host(cc_cache *cache, int field, int64_t offset, int64_t nbytes, void *dst);
+
+/* Number of cut cells found by the last update (the length of its cut-cell work list). */
+int cc_cut_count(cc_cache *cache, int64_t *count);
+
+/* The cache's HIP stream (hipStream_t), for callers that time or order work on it. */
+int cc_stream(cc_cache *cache, void **stream);
+
+/* Per-kernel device times (milliseconds) of the last update, in launch order, recorded with HIP
+ * events on the cache's stream when profiling is on.  `names` receives static strings. */
+int cc_set_profiling(cc_cache *cache, int on);
+int cc_kernel_times(cc_cache *cache, int max, float *ms, const char **names, int *count);
+
+/* generate_mesh(cache, grid) for a marching-squares cache -- src/marching_squares/surface.jl:59-86:
+ * the contour of the cache's phi, welded by exact coordinate equality.  Two passes: call with
+ * nodes/lines NULL to get the counts, then again with buffers of that size.  nodes: 2*nnodes
+ * doubles; lines: 2*nlines int64 (1-based node ids, as MeshLibrary stores them). */
+int cc_generate_mesh_ms(cc_cache *cache, const cc_grid *grid, double *nodes, int64_t *nnodes,
+                        int64_t *lines, int64_t *nlines);
+
+/* The last error message of this thread ("" if none). */
+const char *cc_last_error(void);
+
+/* Library version string. */
+const char *cc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

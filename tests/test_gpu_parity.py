@@ -1,0 +1,227 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle, on the same seeded inputs.
+
+Bar: bit-exact for every column of the nodal methods and of PLIC (the kernels keep the reference's
+expression order and are compiled without contraction), plus the reference's own invariants on the
+device output: exactly-zero closure, bitwise single-valued shared faces, a moved grid bitwise equal
+to a fresh cache, the error types of the host checks.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from cutcell import (ArgumentError, CartesianGrid, CartesianIsoGrid, DimensionMismatch, EmptyGeo,
+                     MarchingCubesCutCell, MarchingSquaresCutCell, PLICCutCell, SDFBox, SDFPlane,
+                     SDFSphere, SDFSphereUnion, SDFTorus, allocate_cache, closure_residual,
+                     generate_mesh, random_rotation, update_cache)
+from helpers import MC_BITS, fortran, sample_vec, shared_face_mismatches
+from refrng import XorShift, randn_field
+
+pytestmark = pytest.mark.gpu
+
+COLS = ("kind", "ambiguous", "volume_fraction", "centroid", "face_fraction",
+        "face_centroid_local", "interface_centroid")
+
+
+def same_bits(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.shape != b.shape:
+        return False
+    if a.dtype.kind == "f":
+        ua = a.view(np.uint64 if a.dtype.itemsize == 8 else np.uint32)
+        ub = b.view(np.uint64 if b.dtype.itemsize == 8 else np.uint32)
+        return bool(np.all((ua == ub) | (np.isnan(a) & np.isnan(b))))
+    return bool(np.array_equal(a, b))
+
+
+def assert_cells_equal(cache, ref, start=0, stop=None):
+    for name in COLS:
+        got = cache.read(name, start, stop)
+        want = ref[name]
+        if not same_bits(got, want):
+            bad = np.flatnonzero(~np.all((got == want).reshape(got.shape[0], -1), axis=1))
+            raise AssertionError(f"{name}: {bad.size} cells differ, first {bad[:5]}: "
+                                 f"got {got[bad[:2]]} want {want[bad[:2]]}")
+
+
+def test_fresh_cache_reads_as_no_body():
+    # test/test_cache.jl:65-74
+    g = CartesianGrid((-1.5,) * 3, (24,) * 3, (0.125,) * 3)
+    c = allocate_cache(g, MarchingCubesCutCell)
+    assert np.all(c.cells.kind == 1)
+    assert np.all(c.cells.volume_fraction == 1.0)
+    assert np.all(c.cells.face_fraction == 1.0)
+    assert np.all(c.phi == 1.0)
+
+
+def test_mc_sphere_bitwise_and_invariants():
+    # test/test_cache.jl:164-191 (CACHE_G3, CACHE_SPHERE)
+    g = CartesianGrid((-1.5,) * 3, (24,) * 3, (0.125,) * 3)
+    geo = SDFSphere((0.01, 0.02, -0.03), 1.0)
+    c = update_cache(allocate_cache(g, MarchingCubesCutCell), geo, g)
+    phi_ref = orc.sample_nodes(geo, 3, g.x0, g.d, g.n)
+    assert same_bits(c.phi, phi_ref)  # the sampled field, bit for bit (sqrt correctly rounded)
+    ref = orc.mc_cells(phi_ref, g.x0, g.d, g.n)
+    assert_cells_equal(c, ref)
+    assert (c.cells.kind == 2).sum() > 1000
+    assert c.cut_count() == int((ref["kind"] == 2).sum())
+    ff = c.cells.face_fraction
+    assert shared_face_mismatches(ff, g.n, 3) == 0
+    assert shared_face_mismatches(c.cells.face_centroid_local, g.n, 3) == 0
+    assert np.count_nonzero(closure_residual(ff, g.d)) == 0
+    # a moved grid rewrites every cell: bitwise a fresh cache's
+    moved = g.moved((0.013, -0.007, 0.021))
+    update_cache(c, geo, moved)
+    fresh = update_cache(allocate_cache(moved, MarchingCubesCutCell), geo, moved)
+    for name in COLS:
+        assert same_bits(c.read(name), fresh.read(name))
+    with pytest.raises(DimensionMismatch):
+        update_cache(c, geo, CartesianGrid((0.0,) * 3, (10,) * 3, (0.1,) * 3))
+
+
+def test_mc_random_field_bitwise():
+    # test/test_mc_moments.jl:421-495: every Lewiner case, through update_cache! from nodal values
+    rng = XorShift(31337)
+    vals = randn_field(rng, (36, 36, 36))
+    g = CartesianGrid((0.0,) * 3, (35,) * 3, (1.0,) * 3)
+    c = update_cache(allocate_cache(g, MarchingCubesCutCell), vals, g)
+    assert same_bits(c.phi, vals)
+    ref = orc.mc_cells(vals, g.x0, g.d, g.n)
+    assert_cells_equal(c, ref)
+    assert shared_face_mismatches(c.cells.face_fraction, g.n, 3) == 0
+    assert shared_face_mismatches(c.cells.face_centroid_local, g.n, 3) == 0
+    assert np.count_nonzero(closure_residual(c.cells.face_fraction, g.d)) == 0
+
+
+def test_mc_every_sign_pattern_single_cubes():
+    # test/test_mc_moments.jl:219-244 on the device: 256 masks x 4 magnitude draws
+    rng = XorShift(20260919)
+    g = CartesianGrid((0.0,) * 3, (1, 1, 1), (1.0,) * 3)
+    c = allocate_cache(g, MarchingCubesCutCell)
+    nodes = MC_BITS.astype(float)
+    ncut = 0
+    for mask in range(256):
+        for _ in range(4):
+            phi = [(1 if (mask >> k) & 1 else -1) * (0.05 + rng.rand()) for k in range(8)]
+            vals = np.zeros((2, 2, 2))
+            for k in range(8):
+                b = MC_BITS[k]
+                vals[b[0], b[1], b[2]] = phi[k]
+            update_cache(c, vals, g)
+            m = orc.mc_cell(nodes, phi, (1.0, 1.0, 1.0))
+            ncut += m["kind"] == 2
+            for name in COLS:
+                got = c.read(name)[0]
+                want = np.asarray(m[name]).reshape(got.shape)
+                assert same_bits(got, want), (mask, name, got, want)
+    assert ncut == 1016
+
+
+def test_mc_stretched_and_offset_grid():
+    # anisotropic cells off the lattice origin: every face's own full measure
+    g = CartesianGrid((-1.3, 0.7, -0.45), (33, 17, 21), (0.11, 0.23, 0.07))
+    f = lambda x, y, z: np.sin(2.7 * x) * np.sin(3.1 * y) * np.sin(2.3 * z) - 0.12
+    vals = sample_vec(f, g.x0, g.d, g.n)
+    c = update_cache(allocate_cache(g, MarchingCubesCutCell), vals, g)
+    assert_cells_equal(c, orc.mc_cells(vals, g.x0, g.d, g.n))
+
+
+def test_mc_torus_box_union_geometries():
+    rng = np.random.default_rng(5)
+    R = random_rotation(rng)
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=40)
+    centers = rng.uniform(-0.5, 0.5, (24, 3))
+    radii = rng.uniform(0.05, 0.2, 24)
+    for geo in (SDFTorus((0.02, -0.03, 0.01), 0.5, 0.15, R), SDFBox((0.05, 0, 0), (0.4, 0.3, 0.5), R),
+                SDFSphereUnion(centers, radii, 0.02), SDFPlane((0.6, 0.0, 0.8), (0.1, 0.1, 0.1))):
+        c = update_cache(allocate_cache(g, MarchingCubesCutCell), geo, g)
+        phi_ref = orc.sample_nodes(geo, 3, g.x0, g.d, g.n)
+        assert same_bits(c.phi, phi_ref), type(geo).__name__
+        assert_cells_equal(c, orc.mc_cells(phi_ref, g.x0, g.d, g.n))
+
+
+def test_ms_circle_config_bitwise_and_contour():
+    # MarchingSquaresCutCell on [-1,1]^2, 200 x 200, circle r = 0.5, then generate_mesh
+    g = CartesianIsoGrid((-1.0, -1.0), (1.0, 1.0), sz=0.01)
+    assert g.n == (200, 200)
+    geo = SDFSphere((0.0, 0.0), 0.5)
+    c = update_cache(allocate_cache(g, MarchingSquaresCutCell), geo, g)
+    phi_ref = orc.sample_nodes(geo, 2, g.x0, g.d, g.n)
+    assert same_bits(c.phi, phi_ref)
+    ref = orc.ms_cells(phi_ref, g.x0, g.d, g.n)
+    assert_cells_equal(c, ref)
+    ff = c.cells.face_fraction
+    assert shared_face_mismatches(ff, g.n, 2) == 0
+    assert np.count_nonzero(closure_residual(ff, g.d)) == 0
+    nodes, lines = generate_mesh(c, g)
+    ncut = int((ref["kind"] == 2).sum())
+    assert lines.shape[0] == ncut  # one segment per cut cell: nothing ambiguous on a circle
+    counts = np.bincount(lines.reshape(-1), minlength=nodes.shape[0] + 1)[1:]
+    assert np.all(counts == 2) and nodes.shape[0] == lines.shape[0]  # one closed loop, bitwise
+    r = np.hypot(nodes[:, 0], nodes[:, 1])
+    assert np.all(np.abs(r - 0.5) < 2e-3 * 0.5)
+    # winding: the right-hand normal points out of the body
+    a, b = nodes[lines[:, 0] - 1], nodes[lines[:, 1] - 1]
+    v = b - a
+    nrm = np.stack([v[:, 1], -v[:, 0]], axis=1)
+    assert np.all(np.sum(nrm * (0.5 * (a + b)), axis=1) > 0)
+
+
+def test_ms_callable_field_and_errors():
+    # test/test_cache.jl:62-107
+    g = CartesianGrid((-1.3, -1.25), (48, 44), (0.055, 0.055))
+    c = allocate_cache(g, MarchingSquaresCutCell)
+    assert np.all(c.cells.kind == 1)
+    update_cache(c, lambda x: x[1] - 0.3, g)
+    assert int((c.cells.kind == 2).sum()) == g.n[0]
+    with pytest.raises(DimensionMismatch):
+        update_cache(c, SDFSphere((0, 0), 0.5), CartesianGrid((0.0, 0.0), (10, 10), (0.1, 0.1)))
+
+
+def test_plic_2d_golden_and_bitwise():
+    # test/test_plic.jl:45-124: 84 cut cells
+    g = CartesianIsoGrid((-1.5, -1.5), (1.5, 1.5), sz=0.1)
+    geo = SDFSphere((0.0, 0.0), 1.0)
+    c = update_cache(allocate_cache(g, PLICCutCell), geo, g)
+    ref, normals, intercepts, valid = orc.plic(geo, 2, g.x0, g.d, g.n)
+    assert int((c.cells.kind == 2).sum()) == 84
+    assert_cells_equal(c, ref)
+    assert same_bits(c.normals, normals) and same_bits(c.intercepts, intercepts)
+    assert np.array_equal(c.is_valid, valid)
+    assert shared_face_mismatches(c.cells.face_fraction, g.n, 2) == 0
+    with pytest.raises(ArgumentError):
+        allocate_cache(CartesianGrid((-1.5, -1.5), (30, 15), (0.1, 0.2)), PLICCutCell)
+    iso = CartesianGrid((-1.5, -1.5), (30, 15), (0.1, 0.1))
+    ci = allocate_cache(iso, PLICCutCell)
+    assert np.all(ci.cells.kind == 1) and np.all(ci.cells.face_fraction == 1)
+    with pytest.raises(ArgumentError):
+        update_cache(ci, geo, CartesianGrid((-1.5, -1.5), (30, 15), (0.1, 0.2)))
+
+
+def test_plic_3d_torus_box_degenerate_bitwise():
+    rng = np.random.default_rng(11)
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=48)
+    for geo in (SDFTorus((0.0, 0.0, 0.0), 0.5, 0.15, random_rotation(rng)),
+                SDFBox((0.0, 0.0, 0.0), (0.4, 0.4, 0.4)), SDFSphere((0.0, 0.0, 0.0), 0.6)):
+        c = update_cache(allocate_cache(g, PLICCutCell), geo, g)
+        ref, normals, intercepts, valid = orc.plic(geo, 3, g.x0, g.d, g.n)
+        assert_cells_equal(c, ref)
+        assert same_bits(c.normals, normals) and np.array_equal(c.is_valid, valid)
+        ff = c.cells.face_fraction
+        assert shared_face_mismatches(ff, g.n, 3) == 0
+        assert np.count_nonzero(closure_residual(ff, g.d)) == 0
+
+
+def test_float32_variant_against_the_float64_oracle():
+    # test/test_cache.jl:290-312 / test_mc_moments.jl:555-568: Float32 end to end
+    g = CartesianGrid((-1.5,) * 3, (24,) * 3, (0.125,) * 3)
+    geo = SDFSphere((0.01, 0.02, -0.03), 1.0)
+    c = update_cache(allocate_cache(g, MarchingCubesCutCell, np.float32), geo, g)
+    assert c.cells.volume_fraction.dtype == np.float32
+    phi64 = orc.sample_nodes(geo, 3, g.x0, g.d, g.n)
+    ref = orc.mc_cells(phi64, g.x0, g.d, g.n)
+    assert np.array_equal(c.cells.kind, ref["kind"])
+    assert np.max(np.abs(c.cells.volume_fraction - ref["volume_fraction"])) < 100 * np.finfo(np.float32).eps
+    assert np.max(np.abs(c.cells.face_fraction - ref["face_fraction"])) < 1e-4
+    assert shared_face_mismatches(c.cells.face_fraction, g.n, 3) == 0
+    assert np.count_nonzero(closure_residual(c.cells.face_fraction, np.float32(g.d))) == 0
