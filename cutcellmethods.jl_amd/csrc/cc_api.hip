@@ -61,6 +61,7 @@ struct cc_cache {
     const char *ev_name[MAX_TIMED] = {};
     int nev = 0;
     bool ev_ready = false;
+    cc::McTuning tune;
 };
 
 namespace {
@@ -167,7 +168,11 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
     if (c->profiling) hipEventRecord(c->ev[0], c->stream);
     T *phi = (T *)c->col[CC_FIELD_PHI];
     if (c->method == CC_MARCHING_SQUARES || c->method == CC_MARCHING_CUBES) {
-        if (geo) {
+        // The marching-cubes stream pass samples an analytic body itself (phi written once, never
+        // re-read); a sphere union is sampled once by its own pass and then read.
+        const bool fuse = c->method == CC_MARCHING_CUBES && geo && c->tune.variant == cc::MC_MARCH_FUSED &&
+                          geo->kind != cc::GEO_SPHERE_UNION;
+        if (geo && !fuse) {
             cc::launch_sample<T>(phi, gt, *geo, c->dim, c->stream);
             mark(c, "sample_nodes");
         } else if (phi_in) {
@@ -177,7 +182,11 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
         }
         cc::launch_zero_u32(c->counter, c->stream);
         if (c->method == CC_MARCHING_CUBES) {
-            cc::launch_mc_stream<T>(phi, gt, cols, c->cutlist, c->counter, c->stream);
+            cc::McTuning tn = c->tune;
+            if (!fuse && tn.variant == cc::MC_MARCH_FUSED) tn.variant = cc::MC_MARCH;
+            if (tn.variant != cc::MC_FLAT && gt.n[0] < 128) tn.variant = cc::MC_FLAT;
+            cc::launch_mc_stream<T>(phi, gt, fuse ? geo : nullptr, cols, c->cutlist, c->counter, tn,
+                                    c->stream);
             mark(c, "mc_stream");
             cc::launch_mc_cut<T>(phi, gt, cols, c->cutlist, c->counter, c->stream);
             mark(c, "mc_cut");
@@ -484,6 +493,25 @@ int cc_cut_count(cc_cache *c, int64_t *count) {
 int cc_stream(cc_cache *c, void **stream) {
     if (!c || !stream) return fail(CC_ERR_ARGUMENT, "NULL argument");
     *stream = (void *)c->stream;
+    return CC_OK;
+}
+
+int cc_set_tuning(cc_cache *c, int variant, int kz, int nontemporal) {
+    if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
+    if (variant < cc::MC_FLAT || variant > cc::MC_MARCH_FUSED)
+        return fail(CC_ERR_ARGUMENT, "unknown stream-pass variant %d", variant);
+    if (kz < 1 || kz > 4096) return fail(CC_ERR_ARGUMENT, "kz must be in [1, 4096]");
+    c->tune.variant = variant;
+    c->tune.kz = kz;
+    c->tune.nontemporal = nontemporal != 0;
+    return CC_OK;
+}
+
+int cc_get_tuning(cc_cache *c, int *variant, int *kz, int *nontemporal) {
+    if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
+    if (variant) *variant = c->tune.variant;
+    if (kz) *kz = c->tune.kz;
+    if (nontemporal) *nontemporal = c->tune.nontemporal;
     return CC_OK;
 }
 

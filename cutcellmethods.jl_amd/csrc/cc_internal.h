@@ -26,9 +26,19 @@ constexpr int MC_CUT_BLOCKS = 2048;
 template <typename T>
 void launch_sample(T *phi, const GridT<T> &g, const Geo &geo, int dim, hipStream_t s);
 
+// Variants of the marching-cubes stream pass (picked per cache; the autotuner sweeps them).
+enum McVariant { MC_FLAT = 0, MC_MARCH = 1, MC_MARCH_FUSED = 2 };
+struct McTuning {
+    int variant = MC_MARCH_FUSED;  // fused needs an analytic body; falls back to MC_FLAT otherwise
+    int kz = 16;                   // z-march depth (layers per workgroup)
+    int nontemporal = 0;           // streaming (non-temporal) stores for the column writes
+};
+
+// The stream pass.  `geo` is the body when the variant samples it in-kernel (MC_MARCH_FUSED, phi is
+// then written by this pass); otherwise phi must already hold the nodal field.
 template <typename T>
-void launch_mc_stream(const T *phi, const GridT<T> &g, const Cols<T> &cols, uint32_t *cutlist,
-                      uint32_t *counter, hipStream_t s);
+void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &cols,
+                      uint32_t *cutlist, uint32_t *counter, const McTuning &tune, hipStream_t s);
 template <typename T>
 void launch_mc_cut(const T *phi, const GridT<T> &g, const Cols<T> &cols, const uint32_t *cutlist,
                    const uint32_t *counter, hipStream_t s);

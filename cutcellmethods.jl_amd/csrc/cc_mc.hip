@@ -418,9 +418,16 @@ template <typename T> struct Vec16;
 template <> struct Vec16<double> { typedef double __attribute__((ext_vector_type(2))) type; };
 template <> struct Vec16<float> { typedef float __attribute__((ext_vector_type(4))) type; };
 
+typedef int __attribute__((ext_vector_type(4))) i32x4;
+
+template <bool NT, typename V> __device__ __forceinline__ void put(V *p, V v) {
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 // One column of the tile, [tc cells x M components], written from `f(q)` (q = flat element index
 // within the tile) with 16-byte lane-contiguous stores, then a scalar tail.
-template <typename T, int M, typename F>
+template <typename T, int M, bool NT, typename F>
 __device__ __forceinline__ void stream_column(T *__restrict__ dst, int tc, F f) {
     constexpr int V = 16 / sizeof(T);
     typedef typename Vec16<T>::type vec;
@@ -430,12 +437,86 @@ __device__ __forceinline__ void stream_column(T *__restrict__ dst, int tc, F f) 
         vec v;
 #pragma unroll
         for (int u = 0; u < V; ++u) v[u] = f(e * V + u);
-        *reinterpret_cast<vec *>(dst + (size_t)e * V) = v;
+        put<NT>(reinterpret_cast<vec *>(dst + (size_t)e * V), v);
     }
-    for (int q = nfull * V + threadIdx.x; q < total; q += blockDim.x) dst[q] = f(q);
+    for (int q = nfull * V + threadIdx.x; q < total; q += blockDim.x) put<NT>(dst + q, f(q));
 }
 
+// Per-tile staging in LDS: each cell's kind, centre, and half cell-local extents.
+template <typename T> struct TileLDS {
+    __attribute__((aligned(16))) int8_t kind[MC_TILE];
+    T cen[MC_TILE * 3];
+    T hl[MC_TILE * 3];
+};
+
+// Classify one cell from its nudged corner mask and stage its closed-form uncut moments.
 template <typename T>
+__device__ __forceinline__ bool stage_cell(TileLDS<T> &s, int t, int lut, uint32_t i, uint32_t j,
+                                           uint32_t k, const GridT<T> &g) {
+    const int8_t kind = lut == 255 ? CELL_OUTSIDE : (lut == 0 ? CELL_INSIDE : CELL_CUT);
+    s.kind[t] = kind;
+    const uint32_t ijk[3] = {i, j, k};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T xl = g.x0[a] + T(ijk[a]) * g.d[a];
+        const T xh = g.x0[a] + T(ijk[a] + 1) * g.d[a];
+        const T L = xh - xl;
+        s.cen[3 * t + a] = xl + T(0.5) * g.d[a];
+        // 0.25 * (((0 + L) + L) + 0): the uncut face's corner average in its (p, q) frame
+        s.hl[3 * t + a] = T(0.25) * (L + L);
+    }
+    return kind == CELL_CUT;
+}
+
+// Append this wave's cut cells to the work list: one atomic per wave.  Called by every lane.
+__device__ __forceinline__ void append_cut(bool cut, uint32_t c, uint32_t *cutlist, uint32_t *counter) {
+    const uint64_t m = __ballot(cut);
+    if (m) {
+        const int lane = __lane_id();
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t b = 0;
+        if (lane == leader) b = atomicAdd(counter, (uint32_t)__popcll(m));
+        b = __shfl(b, leader);
+        if (cut) cutlist[b + __popcll(m & ((1ull << lane) - 1))] = c;
+    }
+}
+
+// Every column of cells [c0, c0 + tc) from the staged tile.  Cut cells get placeholder values
+// here; k_mc_cut overwrites them afterwards on the same stream.
+template <typename T, bool NT>
+__device__ __forceinline__ void stream_tile(const Cols<T> &cols, uint32_t c0, int tc,
+                                            const TileLDS<T> &s) {
+    const int t = threadIdx.x;
+    {
+        int8_t *dk = cols.kind + c0;
+        uint8_t *da = cols.amb + c0;
+        const int nfull = tc / 16;
+        for (int e = t; e < nfull; e += blockDim.x) {
+            put<NT>(reinterpret_cast<i32x4 *>(dk + 16 * e), *reinterpret_cast<const i32x4 *>(s.kind + 16 * e));
+            put<NT>(reinterpret_cast<i32x4 *>(da + 16 * e), i32x4{0, 0, 0, 0});
+        }
+        for (int q = nfull * 16 + t; q < tc; q += blockDim.x) {
+            dk[q] = s.kind[q];
+            da[q] = 0;
+        }
+    }
+    stream_column<T, 1, NT>(cols.vf + c0, tc, [&](int q) { return s.kind[q] == CELL_OUTSIDE ? T(1) : T(0); });
+    stream_column<T, 3, NT>(cols.centroid + 3 * (size_t)c0, tc, [&](int q) { return s.cen[q]; });
+    stream_column<T, 6, NT>(cols.ff + 6 * (size_t)c0, tc,
+                            [&](int q) { return s.kind[q / 6] == CELL_OUTSIDE ? T(1) : T(0); });
+    // face_centroid_local: per direction the (p, q) half extents; axes (y,z),(x,z),(x,y).
+    stream_column<T, 12, NT>(cols.fcl + 12 * (size_t)c0, tc, [&](int q) {
+        const int cell = q / 12, r = q - 12 * cell, dir = r >> 1, w = r & 1;
+        const int ax = dir >> 1;
+        const int a = w == 0 ? (ax == 0 ? 1 : 0) : (ax == 2 ? 1 : 2);
+        return s.hl[3 * cell + a];
+    });
+    stream_column<T, 3, NT>(cols.ic + 3 * (size_t)c0, tc, [&](int q) { return s.cen[q]; });
+}
+
+// Flat variant: a workgroup owns MC_TILE consecutive cells of the whole grid (any shape), reading
+// the 8 corners of each from phi.  Used for narrow grids and fields handed in by the caller.
+template <typename T, bool NT>
 __global__ __launch_bounds__(MC_TILE) void k_mc_stream(const T *__restrict__ phi, GridT<T> g,
                                                        Cols<T> cols, uint32_t *__restrict__ cutlist,
                                                        uint32_t *__restrict__ counter,
@@ -449,71 +530,88 @@ __global__ __launch_bounds__(MC_TILE) void k_mc_stream(const T *__restrict__ phi
     const int t = threadIdx.x;
     const uint32_t c = c0 + t;
     const uint32_t n0 = g.n[0], n1 = g.n[1];
-
-    __shared__ __attribute__((aligned(16))) int8_t kind_s[MC_TILE];
-    __shared__ T cen_s[MC_TILE * 3];
-    __shared__ T hl_s[MC_TILE * 3];
-
+    __shared__ TileLDS<T> s;
     bool cut = false;
     if (c < ncell) {
         const uint32_t i = c % n0, r = c / n0, j = r % n1, k = r / n1;
         const size_t sy = n0 + 1, sz = sy * (n1 + 1);
-        const size_t base = i + sy * j + sz * k;
         T ph[8];
-        const int lut = load_corners(phi, base, sy, sz, ph);
-        const int8_t kind = lut == 255 ? CELL_OUTSIDE : (lut == 0 ? CELL_INSIDE : CELL_CUT);
-        cut = kind == CELL_CUT;
-        kind_s[t] = kind;
-        const uint32_t ijk[3] = {i, j, k};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const T xl = g.x0[a] + T(ijk[a]) * g.d[a];
-            const T xh = g.x0[a] + T(ijk[a] + 1) * g.d[a];
-            const T L = xh - xl;
-            cen_s[3 * t + a] = xl + T(0.5) * g.d[a];
-            // 0.25 * (((0 + L) + L) + 0), the uncut face's corner average in its (p, q) frame
-            hl_s[3 * t + a] = T(0.25) * (L + L);
-        }
+        const int lut = load_corners(phi, i + sy * j + sz * k, sy, sz, ph);
+        cut = stage_cell(s, t, lut, i, j, k, g);
     }
-    // Append this wave's cut cells to the work list: one atomic per wave.
-    const uint64_t m = __ballot(cut);
-    if (m) {
-        const int lane = __lane_id();
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t b = 0;
-        if (lane == leader) b = atomicAdd(counter, (uint32_t)__popcll(m));
-        b = __shfl(b, leader);
-        if (cut) cutlist[b + __popcll(m & ((1ull << lane) - 1))] = c;
-    }
+    append_cut(cut, c, cutlist, counter);
     __syncthreads();
+    stream_tile<T, NT>(cols, c0, (int)min((uint32_t)MC_TILE, ncell - c0), s);
+}
 
-    const int tc = (int)min((uint32_t)MC_TILE, ncell - c0);
-    // kind and ambiguous: bytes
-    {
-        int8_t *dk = cols.kind + c0;
-        uint8_t *da = cols.amb + c0;
-        const int nfull = tc / 16;
-        for (int e = t; e < nfull; e += MC_TILE) {
-            *reinterpret_cast<int4 *>(dk + 16 * e) = *reinterpret_cast<const int4 *>(kind_s + 16 * e);
-            *reinterpret_cast<int4 *>(da + 16 * e) = make_int4(0, 0, 0, 0);
+// z-marching variant: a workgroup owns a 256-cell run of one x-row, row j, and marches kz layers
+// in z.  The two node rows (j, k) and (j+1, k) of the layer below stay in LDS, so each step brings
+// in only the layer above: computed from the body (FUSED, owned nodes written to phi exactly once,
+// phi never re-read) or read from phi.  Cells of one step are one contiguous range per column.
+template <typename T, bool FUSED, bool NT>
+__global__ __launch_bounds__(MC_TILE) void k_mc_march(T *__restrict__ phi, GridT<T> g, Geo geo,
+                                                      Cols<T> cols, uint32_t *__restrict__ cutlist,
+                                                      uint32_t *__restrict__ counter, uint32_t nseg,
+                                                      uint32_t nkb, uint32_t kz, uint32_t nwork) {
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (w >= nwork) return;
+    const uint32_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    // work item order: segment fastest, then j, then the k block
+    const uint32_t seg = w % nseg, rest = w / nseg, j = rest % n1, kb = rest / n1;
+    const uint32_t i0 = seg * MC_TILE;
+    const int tc = (int)min((uint32_t)MC_TILE, n0 - i0);
+    const uint32_t k0 = kb * kz, k1 = min(k0 + kz, n2);
+    const int t = threadIdx.x;
+    const size_t sy = n0 + 1, sz = sy * (n1 + 1);
+
+    __shared__ TileLDS<T> s;
+    __shared__ T rows[2][2][MC_TILE + 1];  // [layer parity][row j / j+1][node]
+
+    // Node row pair (j, j+1) of layer k into rows[par].
+    auto fill = [&](int par, uint32_t k) {
+        const bool own_layer = k < k1 || k == n2;
+        for (int v = t; v < 2 * (tc + 1); v += MC_TILE) {
+            const int r = v >= tc + 1 ? 1 : 0;
+            const int ii = v - r * (tc + 1);
+            const uint32_t gi = i0 + ii, gj = j + r;
+            const size_t idx = gi + sy * gj + sz * k;
+            T val;
+            if (FUSED) {
+                const T x[3] = {g.x0[0] + T(gi) * g.d[0], g.x0[1] + T(gj) * g.d[1],
+                                g.x0[2] + T(k) * g.d[2]};
+                val = sdf_value<T, 3>(geo, x);
+                const bool own = own_layer && (r == 0 || gj == n1) && (ii < tc || gi == n0);
+                if (own) phi[idx] = val;
+            } else {
+                val = phi[idx];
+            }
+            rows[par][r][ii] = nudge(val);
         }
-        for (int q = nfull * 16 + t; q < tc; q += MC_TILE) {
-            dk[q] = kind_s[q];
-            da[q] = 0;
+    };
+    fill(0, k0);
+    int par = 0;
+    for (uint32_t k = k0; k < k1; ++k) {
+        fill(par ^ 1, k + 1);
+        __syncthreads();
+        bool cut = false;
+        const uint32_t c0 = i0 + n0 * (j + n1 * k);
+        if (t < tc) {
+            const T *lo0 = rows[par][0], *lo1 = rows[par][1];
+            const T *hi0 = rows[par ^ 1][0], *hi1 = rows[par ^ 1][1];
+            // MC_NODE_BITS order
+            const T ph[8] = {lo0[t], lo0[t + 1], lo1[t + 1], lo1[t], hi0[t], hi0[t + 1], hi1[t + 1], hi1[t]};
+            int lut = 0;
+#pragma unroll
+            for (int p = 0; p < 8; ++p) lut |= (ph[p] >= T(0) ? 1 : 0) << p;
+            cut = stage_cell(s, t, lut, i0 + t, j, k, g);
         }
+        append_cut(cut, c0 + t, cutlist, counter);
+        __syncthreads();
+        stream_tile<T, NT>(cols, c0, tc, s);
+        __syncthreads();
+        par ^= 1;
     }
-    stream_column<T, 1>(cols.vf + c0, tc, [&](int q) { return kind_s[q] == CELL_OUTSIDE ? T(1) : T(0); });
-    stream_column<T, 3>(cols.centroid + 3 * (size_t)c0, tc, [&](int q) { return cen_s[q]; });
-    stream_column<T, 6>(cols.ff + 6 * (size_t)c0, tc,
-                        [&](int q) { return kind_s[q / 6] == CELL_OUTSIDE ? T(1) : T(0); });
-    // face_centroid_local: per direction the (p, q) half lengths; axes (y,z),(x,z),(x,y).
-    stream_column<T, 12>(cols.fcl + 12 * (size_t)c0, tc, [&](int q) {
-        const int cell = q / 12, r = q - 12 * cell, dir = r >> 1, w = r & 1;
-        const int ax = dir >> 1;
-        const int a = w == 0 ? (ax == 0 ? 1 : 0) : (ax == 2 ? 1 : 2);
-        return hl_s[3 * cell + a];
-    });
-    stream_column<T, 3>(cols.ic + 3 * (size_t)c0, tc, [&](int q) { return cen_s[q]; });
 }
 
 template <typename T>
@@ -571,13 +669,35 @@ void launch_sample(T *phi, const GridT<T> &g, const Geo &geo, int dim, hipStream
 }
 
 template <typename T>
-void launch_mc_stream(const T *phi, const GridT<T> &g, const Cols<T> &cols, uint32_t *cutlist,
-                      uint32_t *counter, hipStream_t s) {
+void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &cols,
+                      uint32_t *cutlist, uint32_t *counter, const McTuning &tune, hipStream_t s) {
     const uint32_t ncell = (uint32_t)((uint64_t)g.n[0] * g.n[1] * g.n[2]);
-    const uint32_t ntiles = (ncell + MC_TILE - 1) / MC_TILE;
-    const uint32_t blocks = ((ntiles + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_mc_stream<T>), dim3(blocks), dim3(MC_TILE), 0, s, phi, g, cols, cutlist,
-                       counter, ncell, ntiles);
+    if (tune.variant == MC_FLAT || (geo == nullptr && tune.variant == MC_MARCH_FUSED)) {
+        const uint32_t ntiles = (ncell + MC_TILE - 1) / MC_TILE;
+        const uint32_t blocks = ((ntiles + 7) / 8) * 8;
+        if (tune.nontemporal)
+            hipLaunchKernelGGL((k_mc_stream<T, true>), dim3(blocks), dim3(MC_TILE), 0, s, phi, g, cols,
+                               cutlist, counter, ncell, ntiles);
+        else
+            hipLaunchKernelGGL((k_mc_stream<T, false>), dim3(blocks), dim3(MC_TILE), 0, s, phi, g,
+                               cols, cutlist, counter, ncell, ntiles);
+        return;
+    }
+    const uint32_t nseg = (g.n[0] + MC_TILE - 1) / MC_TILE;
+    const uint32_t kz = tune.kz > 0 ? tune.kz : 16;
+    const uint32_t nkb = (g.n[2] + kz - 1) / kz;
+    const uint32_t nwork = nseg * g.n[1] * nkb;
+    const uint32_t blocks = ((nwork + 7) / 8) * 8;
+    const Geo gg = geo ? *geo : Geo{};
+    const bool fused = tune.variant == MC_MARCH_FUSED;
+#define CC_LAUNCH_MARCH(F, N)                                                                     \
+    hipLaunchKernelGGL((k_mc_march<T, F, N>), dim3(blocks), dim3(MC_TILE), 0, s, phi, g, gg, cols, \
+                       cutlist, counter, nseg, nkb, kz, nwork)
+    if (fused && tune.nontemporal) CC_LAUNCH_MARCH(true, true);
+    else if (fused) CC_LAUNCH_MARCH(true, false);
+    else if (tune.nontemporal) CC_LAUNCH_MARCH(false, true);
+    else CC_LAUNCH_MARCH(false, false);
+#undef CC_LAUNCH_MARCH
 }
 
 template <typename T>
@@ -593,10 +713,11 @@ void launch_zero_u32(uint32_t *p, hipStream_t s) {
 
 template void launch_sample<double>(double *, const GridT<double> &, const Geo &, int, hipStream_t);
 template void launch_sample<float>(float *, const GridT<float> &, const Geo &, int, hipStream_t);
-template void launch_mc_stream<double>(const double *, const GridT<double> &, const Cols<double> &,
-                                       uint32_t *, uint32_t *, hipStream_t);
-template void launch_mc_stream<float>(const float *, const GridT<float> &, const Cols<float> &,
-                                      uint32_t *, uint32_t *, hipStream_t);
+template void launch_mc_stream<double>(double *, const GridT<double> &, const Geo *,
+                                       const Cols<double> &, uint32_t *, uint32_t *,
+                                       const McTuning &, hipStream_t);
+template void launch_mc_stream<float>(float *, const GridT<float> &, const Geo *, const Cols<float> &,
+                                      uint32_t *, uint32_t *, const McTuning &, hipStream_t);
 template void launch_mc_cut<double>(const double *, const GridT<double> &, const Cols<double> &,
                                     const uint32_t *, const uint32_t *, hipStream_t);
 template void launch_mc_cut<float>(const float *, const GridT<float> &, const Cols<float> &,

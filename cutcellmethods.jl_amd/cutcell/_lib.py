@@ -27,7 +27,7 @@ FIELDS = {
 
 EXPORTS = ("cc_cache_create", "cc_cache_destroy", "cc_update", "cc_update_from_phi",
            "cc_update_async", "cc_synchronize", "cc_column", "cc_copy_to_host", "cc_cut_count",
-           "cc_stream", "cc_set_profiling", "cc_kernel_times", "cc_generate_mesh_ms",
+           "cc_stream", "cc_set_tuning", "cc_get_tuning", "cc_set_profiling", "cc_kernel_times", "cc_generate_mesh_ms",
            "cc_last_error", "cc_version")
 
 
@@ -80,6 +80,9 @@ def lib():
         L.cc_cut_count.argtypes = [vp, i64p]
         L.cc_stream.argtypes = [vp, ctypes.POINTER(vp)]
         L.cc_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.cc_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.cc_get_tuning.argtypes = [vp, ip, ip, ip]
         L.cc_kernel_times.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_int)]
         L.cc_generate_mesh_ms.argtypes = [vp, ctypes.POINTER(CCGrid), vp, i64p, vp, i64p]

@@ -150,6 +150,15 @@ class Cache:
         check(lib().cc_stream(self._h, ctypes.byref(s)))
         return s.value
 
+    def set_tuning(self, variant=2, kz=16, nontemporal=False):
+        """Launch shape of the marching-cubes stream pass (0 flat, 1 z-march, 2 fused z-march)."""
+        check(lib().cc_set_tuning(self._h, int(variant), int(kz), 1 if nontemporal else 0))
+
+    def get_tuning(self):
+        v, k, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().cc_get_tuning(self._h, ctypes.byref(v), ctypes.byref(k), ctypes.byref(nt)))
+        return v.value, k.value, bool(nt.value)
+
     def set_profiling(self, on=True):
         check(lib().cc_set_profiling(self._h, 1 if on else 0))
 

@@ -120,6 +120,19 @@ int cc_cut_count(cc_cache *cache, int64_t *count);
 /* The cache's HIP stream (hipStream_t), for callers that time or order work on it. */
 int cc_stream(cc_cache *cache, void **stream);
 
+/* Launch shape of the marching-cubes stream pass, chosen per cache (the autotuner's knob; no
+ * counterpart in the reference, whose KernelAbstractions launch is fixed at workgroup 64):
+ *   variant 0 = flat tiles of 256 consecutive cells reading phi,
+ *           1 = z-march of 256-cell row runs over kz layers reading phi,
+ *           2 = z-march sampling an analytic body in-kernel (phi written once, never re-read);
+ *   nontemporal != 0 streams the column stores past the caches.  Results are identical for every
+ *   setting; only the speed differs. */
+#define CC_MC_FLAT 0
+#define CC_MC_MARCH 1
+#define CC_MC_MARCH_FUSED 2
+int cc_set_tuning(cc_cache *cache, int variant, int kz, int nontemporal);
+int cc_get_tuning(cc_cache *cache, int *variant, int *kz, int *nontemporal);
+
 /* Per-kernel device times (milliseconds) of the last update, in launch order, recorded with HIP
  * events on the cache's stream when profiling is on.  `names` receives static strings. */
 int cc_set_profiling(cc_cache *cache, int on);

@@ -225,3 +225,22 @@ def test_float32_variant_against_the_float64_oracle():
     assert np.max(np.abs(c.cells.face_fraction - ref["face_fraction"])) < 1e-4
     assert shared_face_mismatches(c.cells.face_fraction, g.n, 3) == 0
     assert np.count_nonzero(closure_residual(c.cells.face_fraction, np.float32(g.d))) == 0
+
+
+@pytest.mark.parametrize("variant,kz,nt", [(0, 16, False), (1, 7, False), (2, 16, False),
+                                           (2, 5, True), (1, 64, True), (0, 16, True)])
+def test_mc_stream_variants_bitwise(variant, kz, nt):
+    # every launch shape of the stream pass gives the same bits, on a grid whose x extent is not a
+    # multiple of the 256-cell run and whose z extent is not a multiple of kz
+    g = CartesianGrid((-1.1, -0.9, -1.05), (300, 13, 23), (0.0071, 0.13, 0.091))
+    geo = SDFSphere((0.05, 0.1, -0.02), 0.7)
+    c = allocate_cache(g, MarchingCubesCutCell)
+    c.set_tuning(variant, kz, nt)
+    assert c.get_tuning() == (variant, kz, nt)
+    update_cache(c, geo, g)
+    phi_ref = orc.sample_nodes(geo, 3, g.x0, g.d, g.n)
+    assert same_bits(c.phi, phi_ref)
+    assert_cells_equal(c, orc.mc_cells(phi_ref, g.x0, g.d, g.n))
+    # and from a caller-sampled field (no body for the fused pass to sample)
+    update_cache(c, phi_ref, g)
+    assert_cells_equal(c, orc.mc_cells(phi_ref, g.x0, g.d, g.n))
