@@ -34,9 +34,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", default="2,16,0",
+                    help="stream-pass launch shape: variant,kz,nontemporal")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also time every stream-pass launch shape and report them")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "mc_stream_512_traffic.json"))
     return ap.parse_args()
 
@@ -50,15 +54,14 @@ def workload(n, seed):
 
 
 def cpu_baseline(grid, geo, budget_s):
-    """The oracle (all host cores) on k-slabs of the same workload, starting at the sphere's middle
-    and walking outward in chunks of 8 layers until about budget_s seconds have been spent."""
+    """The oracle (all host cores, OpenMP) over the same workload in slabs of 8 k-layers, from k = 0
+    upward (wrapping), until about budget_s seconds of CPU update time have been spent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     n = grid.n
     chunk = 8
-    k0 = max(0, n[2] // 2 - chunk // 2)
-    spent, cells, k = 0.0, 0, k0
-    while spent < budget_s and k < n[2]:
+    spent, cells, k, slabs = 0.0, 0, 0, 0
+    while spent < budget_s:
         k1 = min(n[2], k + chunk)
         x0 = (grid.x0[0], grid.x0[1], grid.x0[2] + k * grid.d[2])
         sub = (n[0], n[1], k1 - k)
@@ -67,10 +70,12 @@ def cpu_baseline(grid, geo, budget_s):
         orc.mc_cells(phi, x0, grid.d, sub)
         spent += time.perf_counter() - t
         cells += n[0] * n[1] * (k1 - k)
-        k = k1
+        slabs += 1
+        k = 0 if k1 >= n[2] else k1
     return {"value": cells / spent, "unit": "cells/s", "cores": orc.num_threads(), "kind": "port",
-            "sample": f"oracle update (sample_nodes + cell moments) over k-layers [{k0},{k}) of the "
-                      f"{n[0]}x{n[1]}x{n[2]} workload = {cells} cells in {spent:.2f} s"}
+            "sample": f"oracle update (sample_nodes + cell moments) of {slabs} slabs of 8 k-layers "
+                      f"of the {n[0]}x{n[1]}x{n[2]} workload from k=0 up = {cells} cells in "
+                      f"{spent:.2f} s"}
 
 
 def main():
@@ -84,6 +89,8 @@ def main():
     grid, geo, centre = workload(a.n, a.seed)
     cache = allocate_cache(grid, MarchingCubesCutCell)
     cache.set_profiling(True)
+    tv, tk, tn = (int(x) for x in a.tune.split(","))
+    cache.set_tuning(tv, tk, bool(tn))
     d = grid.d[0]
 
     def grid_at(step):
@@ -106,7 +113,8 @@ def main():
     avg = {k: float(np.mean(v)) for k, v in per_kernel.items()}
     dom = max(avg, key=avg.get)
     if dom == "mc_stream":
-        alg = BYTES_PER_CELL * ncell + BYTES_PER_NODE * nnode  # cells written + phi read once
+        # cells written + phi once (written when the pass samples the body, read otherwise)
+        alg = BYTES_PER_CELL * ncell + BYTES_PER_NODE * nnode
     elif dom == "sample_nodes":
         alg = BYTES_PER_NODE * nnode
     else:
@@ -123,6 +131,26 @@ def main():
                 "traffic": traffic, "algorithmic_bytes": alg,
                 "kernel_ms": {k: round(v, 4) for k, v in avg.items()}}
     update_bytes = BYTES_PER_CELL * ncell + BYTES_PER_NODE * nnode
+    from cutcell.cache import probe_bandwidth
+    fill_gbs, copy_gbs = probe_bandwidth(8 << 30, 5)
+    roofline["probe_fill_GBps"] = round(fill_gbs, 1)
+    roofline["probe_copy_GBps"] = round(copy_gbs, 1)
+    sweep = None
+    if a.sweep:
+        sweep = []
+        for v, kz, nt in [(0, 16, 0), (0, 16, 1), (1, 8, 0), (1, 16, 0), (1, 16, 1), (2, 4, 0),
+                          (2, 8, 0), (2, 16, 0), (2, 32, 0), (2, 16, 1), (2, 64, 1)]:
+            cache.set_tuning(v, kz, bool(nt))
+            update_cache(cache, geo, grid_at(0))
+            ts = []
+            t1 = time.perf_counter()
+            for s in range(4):
+                update_cache(cache, geo, grid_at(s + 1))
+                ts.append(sum(ms for _, ms in cache.kernel_times()))
+            wall = (time.perf_counter() - t1) / 4 * 1e3
+            sweep.append({"variant": v, "kz": kz, "nontemporal": nt, "device_ms": round(min(ts), 4),
+                          "wall_ms": round(wall, 3), "kernels": cache.kernel_times()})
+        cache.set_tuning(tv, tk, bool(tn))
     cpu = None if a.no_cpu else cpu_baseline(grid, geo, a.cpu_seconds)
     out = {
         "metric": "update_cache! cells/s @512^3 MC FP64",
@@ -136,6 +164,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if sweep is not None:
+        out["tuning_sweep"] = sweep
     print(json.dumps(out))
 
 

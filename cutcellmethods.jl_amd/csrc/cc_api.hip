@@ -42,6 +42,20 @@ template <typename T> __global__ void k_fill(T *p, size_t n, T v) {
 
 constexpr int MAX_TIMED = 8;
 
+// Calibration kernels for the bandwidth probe: a write-only fill and a read+write copy, both with
+// 16-byte lane-contiguous accesses in a grid-stride loop.
+typedef double __attribute__((ext_vector_type(2))) f64x2;
+__global__ __launch_bounds__(256) void k_probe_fill(f64x2 *dst, size_t n, double v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = f64x2{v, v};
+}
+__global__ __launch_bounds__(256) void k_probe_copy(f64x2 *dst, const f64x2 *src, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 struct cc_cache {
@@ -153,7 +167,6 @@ int stage_geometry(cc_cache *c, const cc_geometry *geo, cc::Geo *out) {
 
 void mark(cc_cache *c, const char *name) {
     if (!c->profiling || c->nev >= MAX_TIMED) return;
-    if (c->nev == 0) hipEventRecord(c->ev[0], c->stream);
     c->ev_name[c->nev] = name;
     hipEventRecord(c->ev[c->nev + 1], c->stream);
     c->nev += 1;
@@ -170,8 +183,12 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
     if (c->method == CC_MARCHING_SQUARES || c->method == CC_MARCHING_CUBES) {
         // The marching-cubes stream pass samples an analytic body itself (phi written once, never
         // re-read); a sphere union is sampled once by its own pass and then read.
-        const bool fuse = c->method == CC_MARCHING_CUBES && geo && c->tune.variant == cc::MC_MARCH_FUSED &&
+        // Narrow grids (under 128 cells along x) use the flat tiles, which read phi.
+        cc::McTuning tn = c->tune;
+        if (tn.variant != cc::MC_FLAT && gt.n[0] < 128) tn.variant = cc::MC_FLAT;
+        const bool fuse = c->method == CC_MARCHING_CUBES && geo && tn.variant == cc::MC_MARCH_FUSED &&
                           geo->kind != cc::GEO_SPHERE_UNION;
+        if (!fuse && tn.variant == cc::MC_MARCH_FUSED) tn.variant = cc::MC_MARCH;
         if (geo && !fuse) {
             cc::launch_sample<T>(phi, gt, *geo, c->dim, c->stream);
             mark(c, "sample_nodes");
@@ -182,9 +199,6 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
         }
         cc::launch_zero_u32(c->counter, c->stream);
         if (c->method == CC_MARCHING_CUBES) {
-            cc::McTuning tn = c->tune;
-            if (!fuse && tn.variant == cc::MC_MARCH_FUSED) tn.variant = cc::MC_MARCH;
-            if (tn.variant != cc::MC_FLAT && gt.n[0] < 128) tn.variant = cc::MC_FLAT;
             cc::launch_mc_stream<T>(phi, gt, fuse ? geo : nullptr, cols, c->cutlist, c->counter, tn,
                                     c->stream);
             mark(c, "mc_stream");
@@ -534,6 +548,51 @@ int cc_kernel_times(cc_cache *c, int max, float *ms, const char **names, int *co
         *count = k + 1;
     }
     return CC_OK;
+}
+
+int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms) {
+    if (nbytes < (1 << 20) || iters < 1) return fail(CC_ERR_ARGUMENT, "need nbytes >= 1 MiB, iters >= 1");
+    nbytes &= ~int64_t(4095);
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = CC_OK;
+    float best_fill = 1e30f, best_copy = 1e30f;
+    const size_t n = (size_t)nbytes / 16;
+    const int blocks = 256 * 64;
+    if (hipMalloc(&a, nbytes) != hipSuccess || hipMalloc(&b, nbytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(CC_ERR_HIP, "probe allocation failed: %s", hipGetErrorString(hipGetLastError()));
+    } else {
+        hipLaunchKernelGGL(k_probe_fill, dim3(blocks), dim3(256), 0, st, (f64x2 *)b, n, 2.0);
+        for (int it = 0; it < iters + 1; ++it) {
+            float t = 0.0f;
+            hipEventRecord(e0, st);
+            hipLaunchKernelGGL(k_probe_fill, dim3(blocks), dim3(256), 0, st, (f64x2 *)a, n, 1.0);
+            hipEventRecord(e1, st);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&t, e0, e1);
+            if (it > 0 && t < best_fill) best_fill = t;
+            hipEventRecord(e0, st);
+            hipLaunchKernelGGL(k_probe_copy, dim3(blocks), dim3(256), 0, st, (f64x2 *)a,
+                               (const f64x2 *)b, n);
+            hipEventRecord(e1, st);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&t, e0, e1);
+            if (it > 0 && t < best_copy) best_copy = t;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(CC_ERR_HIP, "probe failed: %s", hipGetErrorString(e));
+    }
+    if (fill_ms) *fill_ms = best_fill;
+    if (copy_ms) *copy_ms = best_copy;
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return rc;
 }
 
 int cc_generate_mesh_ms(cc_cache *c, const cc_grid *grid, double *nodes, int64_t *nnodes,

@@ -264,3 +264,13 @@ def face_fractions(cache):
 def is_cell_open(cache):
     """Per cell: whether any face is open (src/cache.jl:95-96), read off face_fraction alone."""
     return np.any(face_fractions(cache) != 0, axis=1)
+
+
+def probe_bandwidth(nbytes, iters=5):
+    """(fill GB/s, copy GB/s counting read + write) of a plain 16-byte-per-lane fill and copy of
+    ``nbytes`` on the current device: the achievable-bandwidth yardstick."""
+    f = ctypes.c_float()
+    c = ctypes.c_float()
+    check(lib().cc_probe_bandwidth(int(nbytes), int(iters), ctypes.byref(f), ctypes.byref(c)))
+    nb = int(nbytes) & ~4095
+    return nb / (f.value * 1e-3) / 1e9, 2 * nb / (c.value * 1e-3) / 1e9

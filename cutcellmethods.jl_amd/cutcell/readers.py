@@ -17,21 +17,23 @@ import numpy as np
 CELL_INSIDE, CELL_OUTSIDE, CELL_CUT = 0, 1, 2
 
 
-def full_face_area(cellsize, direction):
-    """0-based ``direction`` (0 = -x, 1 = +x, 2 = -y, ...)."""
-    cs = [float(v) for v in cellsize]
+def full_face_area(cellsize, direction, dtype=np.float64):
+    """0-based ``direction`` (0 = -x, 1 = +x, 2 = -y, ...), in the element type ``dtype``."""
+    T = np.dtype(dtype).type
+    cs = [T(v) for v in cellsize]
     ax = direction >> 1
     if len(cs) == 2:
         return cs[1 - ax]
-    p = 1.0
+    p = T(1)
     for a in range(len(cs)):
         if a != ax:
-            p *= cs[a]
+            p = T(p * cs[a])
     return p
 
 
 def face_area_of(face_fraction, direction, cellsize):
-    return face_fraction[..., direction] * full_face_area(cellsize, direction)
+    ff = np.asarray(face_fraction)
+    return ff[..., direction] * full_face_area(cellsize, direction, ff.dtype)
 
 
 def interface_normal_area(face_fraction, cellsize):
@@ -42,15 +44,16 @@ def interface_normal_area(face_fraction, cellsize):
 
 def face_vector_area(face_fraction, direction, cellsize):
     D = len(cellsize)
-    s = 1.0 if direction & 1 else -1.0
-    out = np.zeros(face_fraction.shape[:-1] + (D,))
+    T = np.asarray(face_fraction).dtype.type
+    s = T(1) if direction & 1 else T(-1)
+    out = np.zeros(face_fraction.shape[:-1] + (D,), dtype=T)
     out[..., direction >> 1] = s * face_area_of(face_fraction, direction, cellsize)
     return out
 
 
 def closure_residual(face_fraction, cellsize):
     D = len(cellsize)
-    acc = np.zeros(face_fraction.shape[:-1] + (D,))
+    acc = np.zeros(face_fraction.shape[:-1] + (D,), dtype=np.asarray(face_fraction).dtype)
     for k in range(2 * D):
         acc = acc + face_vector_area(face_fraction, k, cellsize)
     return acc - interface_normal_area(face_fraction, cellsize)

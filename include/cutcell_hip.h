@@ -145,6 +145,12 @@ int cc_kernel_times(cc_cache *cache, int max, float *ms, const char **names, int
 int cc_generate_mesh_ms(cc_cache *cache, const cc_grid *grid, double *nodes, int64_t *nnodes,
                         int64_t *lines, int64_t *nlines);
 
+/* Measurement aid (no counterpart in the reference): the best time over `iters` of a write-only
+ * fill of `nbytes` and of a copy of `nbytes` (read + write) on the current device, with 16-byte
+ * lane-contiguous accesses -- the achievable-bandwidth yardstick the cut-cell passes are read
+ * against, and the plain copy kernel that scales rocprof's FETCH_SIZE/WRITE_SIZE counters. */
+int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms);
+
 /* The last error message of this thread ("" if none). */
 const char *cc_last_error(void);
 
