@@ -56,6 +56,32 @@ __global__ __launch_bounds__(256) void k_probe_copy(f64x2 *dst, const f64x2 *src
         dst[i] = src[i];
 }
 
+// Write-pattern variants of the fill probe: 0 grid-stride, 1 grid-stride non-temporal,
+// 2 one contiguous 64 KiB run per workgroup iteration, 3 grid-stride with 4 stores in flight.
+__global__ __launch_bounds__(256) void k_probe_fill_v(f64x2 *dst, size_t n, double v, int variant) {
+    const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const f64x2 val = f64x2{v, v};
+    if (variant == 1) {
+        for (size_t i = tid; i < n; i += stride) __builtin_nontemporal_store(val, dst + i);
+    } else if (variant == 2) {
+        const size_t run = 4096;  // 64 KiB of 16-byte elements
+        for (size_t b = blockIdx.x * run; b < n; b += (size_t)gridDim.x * run)
+            for (size_t i = b + threadIdx.x; i < b + run && i < n; i += blockDim.x) dst[i] = val;
+    } else if (variant == 3) {
+        size_t i = tid;
+        for (; i + 3 * stride < n; i += 4 * stride) {
+            dst[i] = val;
+            dst[i + stride] = val;
+            dst[i + 2 * stride] = val;
+            dst[i + 3 * stride] = val;
+        }
+        for (; i < n; i += stride) dst[i] = val;
+    } else {
+        for (size_t i = tid; i < n; i += stride) dst[i] = val;
+    }
+}
+
 }  // namespace
 
 struct cc_cache {
@@ -183,9 +209,7 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
     if (c->method == CC_MARCHING_SQUARES || c->method == CC_MARCHING_CUBES) {
         // The marching-cubes stream pass samples an analytic body itself (phi written once, never
         // re-read); a sphere union is sampled once by its own pass and then read.
-        // Narrow grids (under 128 cells along x) use the flat tiles, which read phi.
         cc::McTuning tn = c->tune;
-        if (tn.variant != cc::MC_FLAT && gt.n[0] < 128) tn.variant = cc::MC_FLAT;
         const bool fuse = c->method == CC_MARCHING_CUBES && geo && tn.variant == cc::MC_MARCH_FUSED &&
                           geo->kind != cc::GEO_SPHERE_UNION;
         if (!fuse && tn.variant == cc::MC_MARCH_FUSED) tn.variant = cc::MC_MARCH;
@@ -330,6 +354,21 @@ void contour(const std::vector<T> &phi, const std::vector<int8_t> &kind, const c
 extern "C" {
 
 const char *cc_last_error(void) { return g_err.c_str(); }
+
+void cc_internal_set_error(const char *msg) { g_err = msg ? msg : ""; }
+
+int cc_internal_body(cc_cache *c, int *method, int *dtype, int64_t *n, void **cols, void **cutlist,
+                     void **counter, int *kz) {
+    if (!c) return CC_ERR_ARGUMENT;
+    *method = c->method;
+    *dtype = c->dtype;
+    for (int a = 0; a < 3; ++a) n[a] = c->dim == 2 && a == 2 ? 0 : c->n[a];
+    for (int f = 0; f < CC_NUM_FIELDS; ++f) cols[f] = c->col[f];
+    *cutlist = c->cutlist;
+    *counter = c->counter;
+    *kz = c->tune.kz;
+    return CC_OK;
+}
 
 const char *cc_version(void) { return "libcutcell_hip 0.1.0 (gfx950)"; }
 
@@ -592,6 +631,39 @@ int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms
     if (st) hipStreamDestroy(st);
     if (a) hipFree(a);
     if (b) hipFree(b);
+    return rc;
+}
+
+int cc_probe_write(int64_t nbytes, int iters, int variant, int blocks, float *ms) {
+    if (nbytes < (1 << 20) || iters < 1 || blocks < 1) return fail(CC_ERR_ARGUMENT, "bad probe size");
+    nbytes &= ~int64_t(4095);
+    void *a = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t st = nullptr;
+    int rc = CC_OK;
+    float best = 1e30f;
+    if (hipMalloc(&a, nbytes) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(CC_ERR_HIP, "probe allocation failed");
+    } else {
+        for (int it = 0; it < iters + 1; ++it) {
+            float t = 0.0f;
+            hipEventRecord(e0, st);
+            hipLaunchKernelGGL(k_probe_fill_v, dim3(blocks), dim3(256), 0, st, (f64x2 *)a,
+                               (size_t)nbytes / 16, 1.0, variant);
+            hipEventRecord(e1, st);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&t, e0, e1);
+            if (it > 0 && t < best) best = t;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(CC_ERR_HIP, "probe failed: %s", hipGetErrorString(e));
+    }
+    if (ms) *ms = best;
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    if (a) hipFree(a);
     return rc;
 }
 

@@ -56,4 +56,9 @@ void launch_plic_faces(const GridT<T> &g, int dim, const Cols<T> &cols, const T 
 
 void launch_zero_u32(uint32_t *p, hipStream_t s);
 
+// Tile shape of the z-marching stream pass for a grid of n cells: W cells along x (a 256-cell run,
+// or the whole row when narrower), R whole rows per tile when rows are short, kz layers per item.
+void mc_tile_shape(const int *n, uint32_t kz, uint32_t *W, uint32_t *R, uint32_t *KZ, uint32_t *nseg,
+                   uint32_t *nrg, uint32_t *nkb);
+
 }  // namespace cc

@@ -14,505 +14,10 @@
 // The arithmetic follows the reference expression by expression (left folds, outside-anchored
 // interpolation, cell-local coordinates) and is compiled without contraction, so the results are
 // the oracle's bit for bit.
-#include "cc_internal.h"
-
-#define CC_LUT_QUAL __constant__
-#include "lewiner_tables.h"
+#include "cc_mc_dev.h"
 
 namespace cc {
-
 namespace {
-
-__device__ __forceinline__ int nx4(int k) { return (k + 1) & 3; }  // _ms_next
-__device__ __forceinline__ int pv4(int k) { return (k + 3) & 3; }  // _ms_prev
-
-// MC_NODE_BITS (src/marching_cubes/marching_cubes.jl:106-107), 0-based corner c.
-__device__ __forceinline__ int bitx(int c) { return (c ^ (c >> 1)) & 1; }
-__device__ __forceinline__ int bity(int c) { return (c >> 1) & 1; }
-__device__ __forceinline__ int bitz(int c) { return (c >> 2) & 1; }
-
-// MC_FACE_NODES (:169-180) and MC_FACE_TEST_CODE (:385-388), 0-based.
-__constant__ int8_t FACE_NODE[6][4] = {{0, 3, 7, 4}, {1, 2, 6, 5}, {0, 1, 5, 4},
-                                      {3, 2, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
-__constant__ int8_t FACE_TEST_CODE[6] = {4, 2, 1, 3, 5, 6};
-// MC_TEST_FACE_QUADS (:372-373), 0-based.
-__constant__ int8_t TEST_QUAD[6][4] = {{0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3},
-                                      {3, 7, 4, 0}, {0, 3, 2, 1}, {4, 7, 6, 5}};
-// MC_EDGE_NODES (:135-137), 0-based.
-__constant__ int8_t EDGE_NODE[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
-                                       {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
-
-template <typename T> __device__ __forceinline__ T pick4(const T *a, int j) {
-    T r = a[0];
-    r = j == 1 ? a[1] : r;
-    r = j == 2 ? a[2] : r;
-    r = j == 3 ? a[3] : r;
-    return r;
-}
-
-// _test_face (:401-406): the asymptotic decider on one face's four corner values.
-template <typename T> __device__ __forceinline__ bool test_face(const T *cb, int face) {
-    const int f = face < 0 ? -face : face;
-    const T A = cb[TEST_QUAD[f - 1][0]], B = cb[TEST_QUAD[f - 1][1]];
-    const T C = cb[TEST_QUAD[f - 1][2]], D = cb[TEST_QUAD[f - 1][3]];
-    if (fabs(A * C - B * D) < Eps<T>::v) return face >= 0;
-    return T(face) * A * (A * C - B * D) >= T(0);
-}
-
-// _test_interior (:427-504); `case1` is the reference's 1-based case, `subcfg` 1-based.
-template <typename T>
-__device__ bool test_interior(int case1, const T *c, int cfg, int subcfg, int s) {
-    T At = T(0), Bt = T(0), Ct = T(0), Dt = T(0), t;
-    if (case1 == 5 || case1 == 11) {
-        T a = (c[4] - c[0]) * (c[6] - c[2]) - (c[7] - c[3]) * (c[5] - c[1]);
-        T b = c[2] * (c[4] - c[0]) + c[0] * (c[6] - c[2]) - c[1] * (c[7] - c[3]) -
-              c[3] * (c[5] - c[1]);
-        t = -b / (T(2) * a);
-        if (t < T(0) || t > T(1)) return s > 0;
-        At = c[0] + (c[4] - c[0]) * t;
-        Bt = c[3] + (c[7] - c[3]) * t;
-        Ct = c[2] + (c[6] - c[2]) * t;
-        Dt = c[1] + (c[5] - c[1]) * t;
-    } else {
-        int edge;
-        if (case1 == 7) edge = LW_TEST6[cfg][2];
-        else if (case1 == 8) edge = LW_TEST7[cfg][4];
-        else if (case1 == 13) edge = LW_TEST12[cfg][3];
-        else edge = LW_TILING13_5_1[cfg][subcfg - 1][0];
-        // Reference edge e: t on that edge, then B/C/D on the three parallel edges.
-        int e0, e1, b0, b1, c0, c1, d0, d1;
-        switch (edge) {
-        case 0:  e0 = 0; e1 = 1; b0 = 3; b1 = 2; c0 = 7; c1 = 6; d0 = 4; d1 = 5; break;
-        case 1:  e0 = 1; e1 = 2; b0 = 0; b1 = 3; c0 = 4; c1 = 7; d0 = 5; d1 = 6; break;
-        case 2:  e0 = 2; e1 = 3; b0 = 1; b1 = 0; c0 = 5; c1 = 4; d0 = 6; d1 = 7; break;
-        case 3:  e0 = 3; e1 = 0; b0 = 2; b1 = 1; c0 = 6; c1 = 5; d0 = 7; d1 = 4; break;
-        case 4:  e0 = 4; e1 = 5; b0 = 7; b1 = 6; c0 = 3; c1 = 2; d0 = 0; d1 = 1; break;
-        case 5:  e0 = 5; e1 = 6; b0 = 4; b1 = 7; c0 = 0; c1 = 3; d0 = 1; d1 = 2; break;
-        case 6:  e0 = 6; e1 = 7; b0 = 5; b1 = 4; c0 = 1; c1 = 0; d0 = 2; d1 = 3; break;
-        case 7:  e0 = 7; e1 = 4; b0 = 6; b1 = 5; c0 = 2; c1 = 1; d0 = 3; d1 = 0; break;
-        case 8:  e0 = 0; e1 = 4; b0 = 3; b1 = 7; c0 = 2; c1 = 6; d0 = 1; d1 = 5; break;
-        case 9:  e0 = 1; e1 = 5; b0 = 0; b1 = 4; c0 = 3; c1 = 7; d0 = 2; d1 = 6; break;
-        case 10: e0 = 2; e1 = 6; b0 = 1; b1 = 5; c0 = 0; c1 = 4; d0 = 3; d1 = 7; break;
-        default: e0 = 3; e1 = 7; b0 = 2; b1 = 6; c0 = 1; c1 = 5; d0 = 0; d1 = 4; break;
-        }
-        t = c[e0] / (c[e0] - c[e1]);
-        Bt = c[b0] + (c[b1] - c[b0]) * t;
-        Ct = c[c0] + (c[c1] - c[c0]) * t;
-        Dt = c[d0] + (c[d1] - c[d0]) * t;
-    }
-    int test = 0;
-    if (At >= T(0)) test += 1;
-    if (Bt >= T(0)) test += 2;
-    if (Ct >= T(0)) test += 4;
-    if (Dt >= T(0)) test += 8;
-    if (test == 6 || test == 8 || test == 9 || test == 12 || (test >= 0 && test <= 4)) return s > 0;
-    if (test == 5) return (At * Ct - Bt * Dt < Eps<T>::v) ? s > 0 : s < 0;
-    if (test == 10) return (At * Ct - Bt * Dt >= Eps<T>::v) ? s > 0 : s < 0;
-    return s < 0;
-}
-
-// cell_triangles (:538-651): the Lewiner dispatch.  Returns the triangle count (-1 for a
-// SUBCONFIG13 entry with no tiling) and points *tile at the flat run of 0-based edge codes.
-template <typename T>
-__device__ int cell_triangles(const T *phi, int lut, const signed char **tile) {
-    const int cs = LW_CASES[lut][0], cfg = LW_CASES[lut][1];
-    int subcfg = 1;
-    switch (cs) {
-    case 0: *tile = nullptr; return 0;
-    case 1: *tile = LW_TILING1[cfg]; return 1;
-    case 2: *tile = LW_TILING2[cfg]; return 2;
-    case 3:
-        if (test_face(phi, LW_TEST3[cfg])) { *tile = LW_TILING3_2[cfg]; return 4; }
-        *tile = LW_TILING3_1[cfg]; return 2;
-    case 4:
-        if (test_interior(5, phi, cfg, subcfg, LW_TEST4[cfg])) { *tile = LW_TILING4_1[cfg]; return 2; }
-        *tile = LW_TILING4_2[cfg]; return 6;
-    case 5: *tile = LW_TILING5[cfg]; return 3;
-    case 6:
-        if (test_face(phi, LW_TEST6[cfg][0])) { *tile = LW_TILING6_2[cfg]; return 5; }
-        if (test_interior(7, phi, cfg, subcfg, LW_TEST6[cfg][1])) { *tile = LW_TILING6_1_1[cfg]; return 3; }
-        *tile = LW_TILING6_1_2[cfg]; return 9;
-    case 7:
-        if (test_face(phi, LW_TEST7[cfg][0])) subcfg += 1;
-        if (test_face(phi, LW_TEST7[cfg][1])) subcfg += 2;
-        if (test_face(phi, LW_TEST7[cfg][2])) subcfg += 4;
-        switch (subcfg) {
-        case 1: *tile = LW_TILING7_1[cfg]; return 3;
-        case 2: case 3: *tile = LW_TILING7_2[cfg][subcfg - 2]; return 5;
-        case 4: *tile = LW_TILING7_3[cfg][0]; return 9;
-        case 5: *tile = LW_TILING7_2[cfg][2]; return 5;
-        case 6: case 7: *tile = LW_TILING7_3[cfg][subcfg - 5]; return 9;
-        default:
-            if (test_interior(8, phi, cfg, subcfg, LW_TEST7[cfg][3])) { *tile = LW_TILING7_4_2[cfg]; return 9; }
-            *tile = LW_TILING7_4_1[cfg]; return 5;
-        }
-    case 8: *tile = LW_TILING8[cfg]; return 2;
-    case 9: *tile = LW_TILING9[cfg]; return 4;
-    case 10:
-        if (test_face(phi, LW_TEST10[cfg][0])) {
-            if (test_face(phi, LW_TEST10[cfg][1])) { *tile = LW_TILING10_1_1_[cfg]; return 4; }
-            *tile = LW_TILING10_2[cfg]; return 8;
-        }
-        if (test_face(phi, LW_TEST10[cfg][1])) { *tile = LW_TILING10_2_[cfg]; return 8; }
-        if (test_interior(11, phi, cfg, subcfg, LW_TEST10[cfg][2])) { *tile = LW_TILING10_1_1[cfg]; return 4; }
-        *tile = LW_TILING10_1_2[cfg]; return 8;
-    case 11: *tile = LW_TILING11[cfg]; return 4;
-    case 12:
-        if (test_face(phi, LW_TEST12[cfg][0])) {
-            if (test_face(phi, LW_TEST12[cfg][1])) { *tile = LW_TILING12_1_1_[cfg]; return 4; }
-            *tile = LW_TILING12_2[cfg]; return 8;
-        }
-        if (test_face(phi, LW_TEST12[cfg][1])) { *tile = LW_TILING12_2_[cfg]; return 8; }
-        if (test_interior(13, phi, cfg, subcfg, LW_TEST12[cfg][2])) { *tile = LW_TILING12_1_1[cfg]; return 4; }
-        *tile = LW_TILING12_1_2[cfg]; return 8;
-    case 13: {
-        for (int f = 0; f < 6; ++f)
-            if (test_face(phi, LW_TEST13[cfg][f])) subcfg += 1 << f;
-        const int sc = LW_SUBCONFIG13[subcfg - 1];
-        if (sc < 0) { *tile = nullptr; return -1; }
-        if (sc == 0) { *tile = LW_TILING13_1[cfg]; return 4; }
-        if (sc <= 6) { *tile = LW_TILING13_2[cfg][sc - 1]; return 6; }
-        if (sc <= 18) { *tile = LW_TILING13_3[cfg][sc - 7]; return 10; }
-        if (sc <= 22) { *tile = LW_TILING13_4[cfg][sc - 19]; return 12; }
-        if (sc <= 26) {
-            const int sub = sc - 22;
-            // test13[cfg][6] in the reference's 1-based spelling (:631-637)
-            if (test_interior(14, phi, cfg, sub, LW_TEST13[cfg][5])) { *tile = LW_TILING13_5_1[cfg][sub - 1]; return 6; }
-            *tile = LW_TILING13_5_2[cfg][sub - 1]; return 10;
-        }
-        if (sc <= 38) { *tile = LW_TILING13_3_[cfg][sc - 27]; return 10; }
-        if (sc <= 44) { *tile = LW_TILING13_2_[cfg][sc - 39]; return 6; }
-        *tile = LW_TILING13_1_[cfg]; return 4;
-    }
-    default: *tile = LW_TILING14[cfg]; return 4;
-    }
-}
-
-// _face_polygon (src/marching_cubes/moments.jl:111-139) on a face's (p, q) frame: the boundary
-// walk (src/marching_squares/marching_squares.jl:240-251) with outside-anchored open segments,
-// then the chords (_chord_target, moments.jl:91-98), summed as directed edges.
-template <typename T>
-__device__ __forceinline__ void face_polygon(const T *fx, const T *fy, const T *fp, bool connect,
-                                             T &area2, T &c0, T &c1) {
-    T sax[4], say[4], sbx[4], sby[4];
-    bool pres[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int j = nx4(k);
-        const T fa = fp[k], fb = fp[j];
-        const T frac = open_fraction(fa, fb);
-        pres[k] = fa >= T(0) || fb >= T(0);
-        if (fa >= T(0) && fb >= T(0)) {
-            sax[k] = fx[k]; say[k] = fy[k]; sbx[k] = fx[j]; sby[k] = fy[j];
-        } else if (fa >= T(0)) {
-            sax[k] = fx[k]; say[k] = fy[k];
-            sbx[k] = fx[k] + frac * (fx[j] - fx[k]);
-            sby[k] = fy[k] + frac * (fy[j] - fy[k]);
-        } else {
-            sax[k] = fx[j] + frac * (fx[k] - fx[j]);
-            say[k] = fy[j] + frac * (fy[k] - fy[j]);
-            sbx[k] = fx[j]; sby[k] = fy[j];
-        }
-    }
-    T a2 = T(0), m0 = T(0), m1 = T(0);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (!pres[k]) continue;
-        const T cr = sax[k] * sby[k] - say[k] * sbx[k];
-        a2 += cr;
-        m0 += (sax[k] + sbx[k]) * cr;
-        m1 += (say[k] + sby[k]) * cr;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        if (!(fp[k] >= T(0) && fp[nx4(k)] < T(0))) continue;
-        int j = k, hit = -1;
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            j = connect ? nx4(j) : pv4(j);
-            if (hit < 0 && pick4(fp, j) < T(0) && pick4(fp, nx4(j)) >= T(0)) hit = j;
-        }
-        if (hit < 0) continue;
-        const T lax = sbx[k], lay = sby[k];
-        const T lbx = pick4(sax, hit), lby = pick4(say, hit);
-        const T cr = lax * lby - lay * lbx;
-        a2 += cr;
-        m0 += (lax + lbx) * cr;
-        m1 += (lay + lby) * cr;
-    }
-    area2 = a2;
-    c0 = m0;
-    c1 = m1;
-}
-
-template <typename T> __device__ __forceinline__ T full_face_area3(const T *cs, int dir) {
-    const int ax = dir >> 1;
-    T p = T(1);
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-        if (a != ax) p *= cs[a];
-    return p;
-}
-
-// The whole construction for one cut cell: cut_cell_moments (moments.jl:275-346) with
-// cut_cell_faces (:224-257) and edge_crossings (marching_cubes.jl:306-330).  `phi` is nudged;
-// `L` the cell-local upper corner (node(i+1) - node(i) per axis), `xl` the lower lattice corner.
-template <typename T>
-__device__ void mc_cut_cell(const T *phi, int lut, const T *xl, const T *L, const T *cs,
-                            const Cols<T> &cols, uint32_t c) {
-    // ---- six face clips ----
-    T ff[6], fcl[12];
-    bool amb_face = false;
-#pragma unroll
-    for (int dir = 0; dir < 6; ++dir) {
-        const int ax = dir >> 1;
-        const int p = ax == 0 ? 1 : 0, q = ax == 2 ? 1 : 2;
-        T fx[4], fy[4], fp[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int nd = FACE_NODE[dir][k];
-            const int bp = p == 0 ? bitx(nd) : bity(nd);
-            const int bq = q == 1 ? bity(nd) : bitz(nd);
-            fx[k] = bp ? L[p] : T(0);
-            fy[k] = bq ? L[q] : T(0);
-            fp[k] = phi[nd];
-        }
-        int nout = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) nout += fp[k] >= T(0) ? 1 : 0;
-        amb_face |= nout == 2 && ((fp[0] >= T(0)) == (fp[2] >= T(0)));
-        const T avg0 = T(0.25) * (((fx[0] + fx[1]) + fx[2]) + fx[3]);
-        const T avg1 = T(0.25) * (((fy[0] + fy[1]) + fy[2]) + fy[3]);
-        if (nout == 4 || nout == 0) {
-            ff[dir] = nout == 4 ? T(1) : T(0);
-            fcl[2 * dir] = avg0;
-            fcl[2 * dir + 1] = avg1;
-        } else {
-            T a2, m0, m1;
-            face_polygon(fx, fy, fp, test_face(phi, FACE_TEST_CODE[dir]), a2, m0, m1);
-            ff[dir] = T(0.5) * a2 / full_face_area3(cs, dir);
-            if (a2 != T(0)) {
-                const T den = T(3) * a2;
-                fcl[2 * dir] = m0 / den;
-                fcl[2 * dir + 1] = m1 / den;
-            } else {
-                fcl[2 * dir] = avg0;
-                fcl[2 * dir + 1] = avg1;
-            }
-        }
-    }
-
-    // ---- edge crossings on cell-local corners ----
-    T pts[13][3];
-    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-    int ncr = 0;
-#pragma unroll
-    for (int e = 0; e < 12; ++e) {
-        const int a = EDGE_NODE[e][0], b = EDGE_NODE[e][1];
-        const bool crossed = (phi[a] >= T(0)) != (phi[b] >= T(0));
-        const T pa[3] = {bitx(a) ? L[0] : T(0), bity(a) ? L[1] : T(0), bitz(a) ? L[2] : T(0)};
-        const T pb[3] = {bitx(b) ? L[0] : T(0), bity(b) ? L[1] : T(0), bitz(b) ? L[2] : T(0)};
-        if (crossed) {
-            const T s = open_fraction(phi[a], phi[b]);
-#pragma unroll
-            for (int x = 0; x < 3; ++x)
-                pts[e][x] = phi[a] >= T(0) ? pa[x] + s * (pb[x] - pa[x]) : pb[x] + s * (pa[x] - pb[x]);
-            acc0 += pts[e][0];
-            acc1 += pts[e][1];
-            acc2 += pts[e][2];
-            ncr += 1;
-        } else {
-            pts[e][0] = pts[e][1] = pts[e][2] = T(0);
-        }
-    }
-    pts[12][0] = ncr > 0 ? acc0 / T(ncr) : T(0);
-    pts[12][1] = ncr > 0 ? acc1 / T(ncr) : T(0);
-    pts[12][2] = ncr > 0 ? acc2 / T(ncr) : T(0);
-
-    // ---- the closed polyhedron, by the divergence theorem ----
-    T vol3 = T(0), cm[3] = {T(0), T(0), T(0)};
-#pragma unroll
-    for (int dir = 0; dir < 6; ++dir) {
-        const int ax = dir >> 1;
-        const T s = (dir & 1) ? T(1) : T(-1);
-        const T x_ax = (dir & 1) ? L[ax] : T(0);  // local coordinate of the face's first corner
-        const T a_open = ff[dir] * full_face_area3(cs, dir);
-        vol3 += s * x_ax * a_open;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) cm[a] += a == ax ? s * x_ax * x_ax * a_open : T(0);
-    }
-    const signed char *tile;
-    int ntri = cell_triangles(phi, lut, &tile);
-    if (ntri < 0) ntri = 0;
-    T iarea = T(0), icm[3] = {T(0), T(0), T(0)};
-    for (int t = 0; t < ntri; ++t) {
-        const int ia = tile[3 * t], ib = tile[3 * t + 1], id = tile[3 * t + 2];
-        const T a[3] = {pts[ia][0], pts[ia][1], pts[ia][2]};
-        const T b[3] = {pts[ib][0], pts[ib][1], pts[ib][2]};
-        const T d[3] = {pts[id][0], pts[id][1], pts[id][2]};
-        T u[3], v[3], w[3], xc[3];
-#pragma unroll
-        for (int x = 0; x < 3; ++x) { u[x] = b[x] - a[x]; v[x] = d[x] - a[x]; }
-        w[0] = T(0.5) * (u[1] * v[2] - u[2] * v[1]);
-        w[1] = T(0.5) * (u[2] * v[0] - u[0] * v[2]);
-        w[2] = T(0.5) * (u[0] * v[1] - u[1] * v[0]);
-#pragma unroll
-        for (int x = 0; x < 3; ++x) xc[x] = (a[x] + b[x] + d[x]) / T(3);
-        vol3 -= w[0] * xc[0] + w[1] * xc[1] + w[2] * xc[2];
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-            const T m2 = (a[x] * a[x] + b[x] * b[x] + d[x] * d[x] + a[x] * b[x] + b[x] * d[x] +
-                          d[x] * a[x]) / T(6);
-            cm[x] -= w[x] * m2;
-        }
-        const T len = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-        iarea += len;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) icm[x] += len * xc[x];
-    }
-    const T volume = vol3 / T(3);
-    const int cs0 = LW_CASES[lut][0];
-    const bool amb = amb_face || cs0 == 4 || cs0 == 6 || cs0 == 7 || cs0 == 10 || cs0 == 12 ||
-                     cs0 == 13;
-    const T vol_cell = (cs[0] * cs[1]) * cs[2];
-
-    cols.kind[c] = CELL_CUT;
-    cols.amb[c] = amb ? 1 : 0;
-    cols.vf[c] = volume / vol_cell;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const T centre = xl[a] + T(0.5) * cs[a];
-        cols.centroid[3 * (size_t)c + a] = volume > T(0) ? xl[a] + cm[a] / (T(2) * volume) : centre;
-        cols.ic[3 * (size_t)c + a] = iarea > T(0) ? xl[a] + icm[a] / iarea : centre;
-    }
-#pragma unroll
-    for (int k = 0; k < 6; ++k) cols.ff[6 * (size_t)c + k] = ff[k];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) cols.fcl[12 * (size_t)c + k] = fcl[k];
-}
-
-template <typename T> __device__ __forceinline__ T nudge(T v) {
-    return fabs(v) < Eps<T>::v ? Eps<T>::v : v;  // nudge_zeros, marching_cubes.jl:240-243
-}
-
-// The 8 corner values of cell (i, j, k) in MC_NODE_BITS order, nudged, and their sign mask
-// (cell_case's lut - 1, :343-349).
-template <typename T>
-__device__ __forceinline__ int load_corners(const T *__restrict__ phi, size_t base, size_t sy,
-                                            size_t sz, T *ph) {
-    ph[0] = nudge(phi[base]);
-    ph[1] = nudge(phi[base + 1]);
-    ph[2] = nudge(phi[base + sy + 1]);
-    ph[3] = nudge(phi[base + sy]);
-    ph[4] = nudge(phi[base + sz]);
-    ph[5] = nudge(phi[base + sz + 1]);
-    ph[6] = nudge(phi[base + sz + sy + 1]);
-    ph[7] = nudge(phi[base + sz + sy]);
-    int lut = 0;
-#pragma unroll
-    for (int p = 0; p < 8; ++p) lut |= (ph[p] >= T(0) ? 1 : 0) << p;
-    return lut;
-}
-
-template <typename T> struct Vec16;
-template <> struct Vec16<double> { typedef double __attribute__((ext_vector_type(2))) type; };
-template <> struct Vec16<float> { typedef float __attribute__((ext_vector_type(4))) type; };
-
-typedef int __attribute__((ext_vector_type(4))) i32x4;
-
-template <bool NT, typename V> __device__ __forceinline__ void put(V *p, V v) {
-    if (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
-}
-
-// One column of the tile, [tc cells x M components], written from `f(q)` (q = flat element index
-// within the tile) with 16-byte lane-contiguous stores, then a scalar tail.
-template <typename T, int M, bool NT, typename F>
-__device__ __forceinline__ void stream_column(T *__restrict__ dst, int tc, F f) {
-    constexpr int V = 16 / sizeof(T);
-    typedef typename Vec16<T>::type vec;
-    const int total = tc * M;
-    const int nfull = total / V;
-    for (int e = threadIdx.x; e < nfull; e += blockDim.x) {
-        vec v;
-#pragma unroll
-        for (int u = 0; u < V; ++u) v[u] = f(e * V + u);
-        put<NT>(reinterpret_cast<vec *>(dst + (size_t)e * V), v);
-    }
-    for (int q = nfull * V + threadIdx.x; q < total; q += blockDim.x) put<NT>(dst + q, f(q));
-}
-
-// Per-tile staging in LDS: each cell's kind, centre, and half cell-local extents.
-template <typename T> struct TileLDS {
-    __attribute__((aligned(16))) int8_t kind[MC_TILE];
-    T cen[MC_TILE * 3];
-    T hl[MC_TILE * 3];
-};
-
-// Classify one cell from its nudged corner mask and stage its closed-form uncut moments.
-template <typename T>
-__device__ __forceinline__ bool stage_cell(TileLDS<T> &s, int t, int lut, uint32_t i, uint32_t j,
-                                           uint32_t k, const GridT<T> &g) {
-    const int8_t kind = lut == 255 ? CELL_OUTSIDE : (lut == 0 ? CELL_INSIDE : CELL_CUT);
-    s.kind[t] = kind;
-    const uint32_t ijk[3] = {i, j, k};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const T xl = g.x0[a] + T(ijk[a]) * g.d[a];
-        const T xh = g.x0[a] + T(ijk[a] + 1) * g.d[a];
-        const T L = xh - xl;
-        s.cen[3 * t + a] = xl + T(0.5) * g.d[a];
-        // 0.25 * (((0 + L) + L) + 0): the uncut face's corner average in its (p, q) frame
-        s.hl[3 * t + a] = T(0.25) * (L + L);
-    }
-    return kind == CELL_CUT;
-}
-
-// Append this wave's cut cells to the work list: one atomic per wave.  Called by every lane.
-__device__ __forceinline__ void append_cut(bool cut, uint32_t c, uint32_t *cutlist, uint32_t *counter) {
-    const uint64_t m = __ballot(cut);
-    if (m) {
-        const int lane = __lane_id();
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t b = 0;
-        if (lane == leader) b = atomicAdd(counter, (uint32_t)__popcll(m));
-        b = __shfl(b, leader);
-        if (cut) cutlist[b + __popcll(m & ((1ull << lane) - 1))] = c;
-    }
-}
-
-// Every column of cells [c0, c0 + tc) from the staged tile.  Cut cells get placeholder values
-// here; k_mc_cut overwrites them afterwards on the same stream.
-template <typename T, bool NT>
-__device__ __forceinline__ void stream_tile(const Cols<T> &cols, uint32_t c0, int tc,
-                                            const TileLDS<T> &s) {
-    const int t = threadIdx.x;
-    {
-        int8_t *dk = cols.kind + c0;
-        uint8_t *da = cols.amb + c0;
-        const int nfull = tc / 16;
-        for (int e = t; e < nfull; e += blockDim.x) {
-            put<NT>(reinterpret_cast<i32x4 *>(dk + 16 * e), *reinterpret_cast<const i32x4 *>(s.kind + 16 * e));
-            put<NT>(reinterpret_cast<i32x4 *>(da + 16 * e), i32x4{0, 0, 0, 0});
-        }
-        for (int q = nfull * 16 + t; q < tc; q += blockDim.x) {
-            dk[q] = s.kind[q];
-            da[q] = 0;
-        }
-    }
-    stream_column<T, 1, NT>(cols.vf + c0, tc, [&](int q) { return s.kind[q] == CELL_OUTSIDE ? T(1) : T(0); });
-    stream_column<T, 3, NT>(cols.centroid + 3 * (size_t)c0, tc, [&](int q) { return s.cen[q]; });
-    stream_column<T, 6, NT>(cols.ff + 6 * (size_t)c0, tc,
-                            [&](int q) { return s.kind[q / 6] == CELL_OUTSIDE ? T(1) : T(0); });
-    // face_centroid_local: per direction the (p, q) half extents; axes (y,z),(x,z),(x,y).
-    stream_column<T, 12, NT>(cols.fcl + 12 * (size_t)c0, tc, [&](int q) {
-        const int cell = q / 12, r = q - 12 * cell, dir = r >> 1, w = r & 1;
-        const int ax = dir >> 1;
-        const int a = w == 0 ? (ax == 0 ? 1 : 0) : (ax == 2 ? 1 : 2);
-        return s.hl[3 * cell + a];
-    });
-    stream_column<T, 3, NT>(cols.ic + 3 * (size_t)c0, tc, [&](int q) { return s.cen[q]; });
-}
 
 // Flat variant: a workgroup owns MC_TILE consecutive cells of the whole grid (any shape), reading
 // the 8 corners of each from phi.  Used for narrow grids and fields handed in by the caller.
@@ -544,74 +49,15 @@ __global__ __launch_bounds__(MC_TILE) void k_mc_stream(const T *__restrict__ phi
     stream_tile<T, NT>(cols, c0, (int)min((uint32_t)MC_TILE, ncell - c0), s);
 }
 
-// z-marching variant: a workgroup owns a 256-cell run of one x-row, row j, and marches kz layers
-// in z.  The two node rows (j, k) and (j+1, k) of the layer below stay in LDS, so each step brings
-// in only the layer above: computed from the body (FUSED, owned nodes written to phi exactly once,
-// phi never re-read) or read from phi.  Cells of one step are one contiguous range per column.
 template <typename T, bool FUSED, bool NT>
-__global__ __launch_bounds__(MC_TILE) void k_mc_march(T *__restrict__ phi, GridT<T> g, Geo geo,
-                                                      Cols<T> cols, uint32_t *__restrict__ cutlist,
-                                                      uint32_t *__restrict__ counter, uint32_t nseg,
-                                                      uint32_t nkb, uint32_t kz, uint32_t nwork) {
+__global__ __launch_bounds__(MC_TILE) void k_mc_march(McBody<T> B, uint32_t nwork) {
+    // XCD-aware order: the hardware deals workgroups round-robin over the 8 XCDs, so give each XCD
+    // one contiguous run of work items; neighbouring rows then share node rows in that XCD's L2.
     const uint32_t per = gridDim.x >> 3;
     const uint32_t w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (w >= nwork) return;
-    const uint32_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
-    // work item order: segment fastest, then j, then the k block
-    const uint32_t seg = w % nseg, rest = w / nseg, j = rest % n1, kb = rest / n1;
-    const uint32_t i0 = seg * MC_TILE;
-    const int tc = (int)min((uint32_t)MC_TILE, n0 - i0);
-    const uint32_t k0 = kb * kz, k1 = min(k0 + kz, n2);
-    const int t = threadIdx.x;
-    const size_t sy = n0 + 1, sz = sy * (n1 + 1);
-
-    __shared__ TileLDS<T> s;
-    __shared__ T rows[2][2][MC_TILE + 1];  // [layer parity][row j / j+1][node]
-
-    // Node row pair (j, j+1) of layer k into rows[par].
-    auto fill = [&](int par, uint32_t k) {
-        const bool own_layer = k < k1 || k == n2;
-        for (int v = t; v < 2 * (tc + 1); v += MC_TILE) {
-            const int r = v >= tc + 1 ? 1 : 0;
-            const int ii = v - r * (tc + 1);
-            const uint32_t gi = i0 + ii, gj = j + r;
-            const size_t idx = gi + sy * gj + sz * k;
-            T val;
-            if (FUSED) {
-                const T x[3] = {g.x0[0] + T(gi) * g.d[0], g.x0[1] + T(gj) * g.d[1],
-                                g.x0[2] + T(k) * g.d[2]};
-                val = sdf_value<T, 3>(geo, x);
-                const bool own = own_layer && (r == 0 || gj == n1) && (ii < tc || gi == n0);
-                if (own) phi[idx] = val;
-            } else {
-                val = phi[idx];
-            }
-            rows[par][r][ii] = nudge(val);
-        }
-    };
-    fill(0, k0);
-    int par = 0;
-    for (uint32_t k = k0; k < k1; ++k) {
-        fill(par ^ 1, k + 1);
-        __syncthreads();
-        bool cut = false;
-        const uint32_t c0 = i0 + n0 * (j + n1 * k);
-        if (t < tc) {
-            const T *lo0 = rows[par][0], *lo1 = rows[par][1];
-            const T *hi0 = rows[par ^ 1][0], *hi1 = rows[par ^ 1][1];
-            // MC_NODE_BITS order
-            const T ph[8] = {lo0[t], lo0[t + 1], lo1[t + 1], lo1[t], hi0[t], hi0[t + 1], hi1[t + 1], hi1[t]};
-            int lut = 0;
-#pragma unroll
-            for (int p = 0; p < 8; ++p) lut |= (ph[p] >= T(0) ? 1 : 0) << p;
-            cut = stage_cell(s, t, lut, i0 + t, j, k, g);
-        }
-        append_cut(cut, c0 + t, cutlist, counter);
-        __syncthreads();
-        stream_tile<T, NT>(cols, c0, tc, s);
-        __syncthreads();
-        par ^= 1;
-    }
+    __shared__ MarchLDS<T> L;
+    mc_march_item<T, FUSED, NT>(B, w, L);
 }
 
 template <typename T>
@@ -619,22 +65,8 @@ __global__ __launch_bounds__(256) void k_mc_cut(const T *__restrict__ phi, GridT
                                                 const uint32_t *__restrict__ cutlist,
                                                 const uint32_t *__restrict__ counter) {
     const uint32_t count = *counter;
-    const uint32_t n0 = g.n[0], n1 = g.n[1];
-    const size_t sy = n0 + 1, sz = sy * (n1 + 1);
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < count; w += gridDim.x * blockDim.x) {
-        const uint32_t c = cutlist[w];
-        const uint32_t i = c % n0, r = c / n0, j = r % n1, k = r / n1;
-        T ph[8];
-        const int lut = load_corners(phi, i + sy * j + sz * k, sy, sz, ph);
-        const uint32_t ijk[3] = {i, j, k};
-        T xl[3], L[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            xl[a] = g.x0[a] + T(ijk[a]) * g.d[a];
-            L[a] = (g.x0[a] + T(ijk[a] + 1) * g.d[a]) - xl[a];
-        }
-        mc_cut_cell(ph, lut, xl, L, g.d, cols, c);
-    }
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < count; w += gridDim.x * blockDim.x)
+        mc_cut_item(phi, g, cols, cutlist[w]);
 }
 
 // _sample_nodes_kernel! (src/marching_squares/cache.jl:67-70): one thread per node, phi at the
@@ -683,18 +115,22 @@ void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &
                                cols, cutlist, counter, ncell, ntiles);
         return;
     }
-    const uint32_t nseg = (g.n[0] + MC_TILE - 1) / MC_TILE;
-    const uint32_t kz = tune.kz > 0 ? tune.kz : 16;
-    const uint32_t nkb = (g.n[2] + kz - 1) / kz;
-    const uint32_t nwork = nseg * g.n[1] * nkb;
+    McBody<T> B;
+    B.g = g;
+    B.geo = geo ? *geo : Geo{};
+    B.phi = phi;
+    B.cols = cols;
+    B.cutlist = cutlist;
+    B.counter = counter;
+    mc_tile_shape(g.n, tune.kz > 0 ? tune.kz : 16, &B.W, &B.R, &B.kz, &B.nseg, &B.nrg, &B.nkb);
+    B.fused = tune.variant == MC_MARCH_FUSED && geo != nullptr;
+    B.nontemporal = tune.nontemporal;
+    const uint32_t nwork = B.nseg * B.nrg * B.nkb;
     const uint32_t blocks = ((nwork + 7) / 8) * 8;
-    const Geo gg = geo ? *geo : Geo{};
-    const bool fused = tune.variant == MC_MARCH_FUSED;
-#define CC_LAUNCH_MARCH(F, N)                                                                     \
-    hipLaunchKernelGGL((k_mc_march<T, F, N>), dim3(blocks), dim3(MC_TILE), 0, s, phi, g, gg, cols, \
-                       cutlist, counter, nseg, nkb, kz, nwork)
-    if (fused && tune.nontemporal) CC_LAUNCH_MARCH(true, true);
-    else if (fused) CC_LAUNCH_MARCH(true, false);
+#define CC_LAUNCH_MARCH(F, N) \
+    hipLaunchKernelGGL((k_mc_march<T, F, N>), dim3(blocks), dim3(MC_TILE), 0, s, B, nwork)
+    if (B.fused && tune.nontemporal) CC_LAUNCH_MARCH(true, true);
+    else if (B.fused) CC_LAUNCH_MARCH(true, false);
     else if (tune.nontemporal) CC_LAUNCH_MARCH(false, true);
     else CC_LAUNCH_MARCH(false, false);
 #undef CC_LAUNCH_MARCH
@@ -705,6 +141,18 @@ void launch_mc_cut(const T *phi, const GridT<T> &g, const Cols<T> &cols, const u
                    const uint32_t *counter, hipStream_t s) {
     hipLaunchKernelGGL((k_mc_cut<T>), dim3(MC_CUT_BLOCKS), dim3(256), 0, s, phi, g, cols, cutlist,
                        counter);
+}
+
+void mc_tile_shape(const int *n, uint32_t kz, uint32_t *W, uint32_t *R, uint32_t *KZ, uint32_t *nseg,
+                   uint32_t *nrg, uint32_t *nkb) {
+    const uint32_t n0 = n[0], n1 = n[1], n2 = n[2];
+    *W = n0 < (uint32_t)MC_TILE ? n0 : (uint32_t)MC_TILE;
+    *R = n0 < (uint32_t)MC_TILE ? (uint32_t)MC_TILE / n0 : 1u;
+    if (*R > n1) *R = n1;
+    *KZ = kz;
+    *nseg = (n0 + *W - 1) / *W;
+    *nrg = (n1 + *R - 1) / *R;
+    *nkb = (n2 + kz - 1) / kz;
 }
 
 void launch_zero_u32(uint32_t *p, hipStream_t s) {

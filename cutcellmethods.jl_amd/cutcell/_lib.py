@@ -28,7 +28,7 @@ FIELDS = {
 EXPORTS = ("cc_cache_create", "cc_cache_destroy", "cc_update", "cc_update_from_phi",
            "cc_update_async", "cc_synchronize", "cc_column", "cc_copy_to_host", "cc_cut_count",
            "cc_stream", "cc_set_tuning", "cc_get_tuning", "cc_set_profiling", "cc_kernel_times", "cc_generate_mesh_ms",
-           "cc_probe_bandwidth", "cc_last_error", "cc_version")
+           "cc_probe_bandwidth", "cc_probe_write", "cc_last_error", "cc_version")
 
 
 class CCGeometry(ctypes.Structure):
@@ -88,6 +88,8 @@ def lib():
         L.cc_generate_mesh_ms.argtypes = [vp, ctypes.POINTER(CCGrid), vp, i64p, vp, i64p]
         L.cc_probe_bandwidth.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(ctypes.c_float)]
+        L.cc_probe_write.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_float)]
         L.cc_last_error.restype = ctypes.c_char_p
         L.cc_version.restype = ctypes.c_char_p
         for name in EXPORTS:

@@ -274,3 +274,10 @@ def probe_bandwidth(nbytes, iters=5):
     check(lib().cc_probe_bandwidth(int(nbytes), int(iters), ctypes.byref(f), ctypes.byref(c)))
     nb = int(nbytes) & ~4095
     return nb / (f.value * 1e-3) / 1e9, 2 * nb / (c.value * 1e-3) / 1e9
+
+
+def probe_write(nbytes, variant=0, blocks=16384, iters=5):
+    """GB/s of a fill of ``nbytes`` with one of the probe's store patterns."""
+    t = ctypes.c_float()
+    check(lib().cc_probe_write(int(nbytes), int(iters), int(variant), int(blocks), ctypes.byref(t)))
+    return (int(nbytes) & ~4095) / (t.value * 1e-3) / 1e9

@@ -145,11 +145,32 @@ int cc_kernel_times(cc_cache *cache, int max, float *ms, const char **names, int
 int cc_generate_mesh_ms(cc_cache *cache, const cc_grid *grid, double *nodes, int64_t *nnodes,
                         int64_t *lines, int64_t *nlines);
 
+/* ---- batched bodies (no counterpart in the reference, which updates one cache per call) --------
+ * A batch references B existing 3D caches (marching cubes and PLIC mixed, one element type) and
+ * updates all of them with ONE hipGraph launch per step: a counter-reset kernel, one stream/fit pass
+ * over every body's work items, one cut-cell/face pass.  Bodies must be analytic (sphere, torus,
+ * box, plane).  Each step's grids (origins may move) and bodies are read from `grids[b]`/`geos[b]`;
+ * every body's results land in its own cache's columns, identical to cc_update on that cache.
+ * The caches must outlive the batch and must not be updated on their own while it runs. */
+typedef struct cc_batch cc_batch;
+int cc_batch_create(int nbodies, cc_cache *const *caches, cc_batch **out);
+int cc_batch_destroy(cc_batch *batch);
+/* One step: every body updated; blocks.  `device_ms` (optional) receives the graph's device time. */
+int cc_batch_step(cc_batch *batch, const cc_geometry *geos, const cc_grid *grids, float *device_ms);
+/* Tile depth (z layers per work item) of one marching-cubes body: the autotuner's per-body knob. */
+int cc_batch_set_kz(cc_batch *batch, int body, int kz);
+/* Work items of the two passes of the captured graph. */
+int cc_batch_work_items(cc_batch *batch, int64_t *pass_a, int64_t *pass_b);
+
 /* Measurement aid (no counterpart in the reference): the best time over `iters` of a write-only
  * fill of `nbytes` and of a copy of `nbytes` (read + write) on the current device, with 16-byte
  * lane-contiguous accesses -- the achievable-bandwidth yardstick the cut-cell passes are read
  * against, and the plain copy kernel that scales rocprof's FETCH_SIZE/WRITE_SIZE counters. */
 int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms);
+/* Write-pattern study for the same yardstick: best time of a fill of `nbytes` with `blocks`
+ * workgroups of 256 and store pattern `variant` (0 grid-stride, 1 non-temporal, 2 one contiguous
+ * 64 KiB run per workgroup iteration, 3 four stores in flight per lane). */
+int cc_probe_write(int64_t nbytes, int iters, int variant, int blocks, float *ms);
 
 /* The last error message of this thread ("" if none). */
 const char *cc_last_error(void);

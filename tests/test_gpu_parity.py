@@ -294,3 +294,52 @@ def test_mc_stream_variants_bitwise(variant, kz, nt):
     # and from a caller-sampled field (no body for the fused pass to sample)
     update_cache(c, phi_ref, g)
     assert_cells_equal(c, orc.mc_cells(phi_ref, g.x0, g.d, g.n))
+
+
+def test_batched_bodies_match_single_updates_and_oracle():
+    # mixed marching-cubes and PLIC bodies on uneven grids, moving: one graph launch per step gives
+    # every body exactly what cc_update gives it alone
+    from cutcell.batch import Batch
+    from cutcell.workloads import MovingBodies
+    wl = MovingBodies(seed=7, nbodies=6, nmin=12, nmax=70)
+    meth = {"mc": MarchingCubesCutCell, "plic": PLICCutCell}
+    caches = [allocate_cache(wl.grid(b, 0), meth[B["method"]]) for b, B in enumerate(wl.bodies)]
+    solo = [allocate_cache(wl.grid(b, 0), meth[B["method"]]) for b, B in enumerate(wl.bodies)]
+    batch = Batch(caches)
+    assert {B["method"] for B in wl.bodies} == {"mc", "plic"}
+    for step in range(3):
+        geos = [wl.geometry(b, step) for b in range(6)]
+        grids = [wl.grid(b, step) for b in range(6)]
+        batch.step(geos, grids)
+        for b in range(6):
+            update_cache(solo[b], geos[b], grids[b])
+            for name in COLS:
+                assert same_bits(caches[b].read(name), solo[b].read(name)), (step, b, name)
+    b = next(i for i, B in enumerate(wl.bodies) if B["method"] == "mc")
+    g = wl.grid(b, 2)
+    phi_ref = orc.sample_nodes(wl.geometry(b, 2), 3, g.x0, g.d, g.n)
+    assert_cells_equal(caches[b], orc.mc_cells(phi_ref, g.x0, g.d, g.n))
+    p = next(i for i, B in enumerate(wl.bodies) if B["method"] == "plic")
+    g = wl.grid(p, 2)
+    ref, *_ = orc.plic(wl.geometry(p, 2), 3, g.x0, g.d, g.n)
+    assert_cells_equal(caches[p], ref)
+    # a different slab depth rebuilds the graph; results unchanged
+    batch.set_kz(b, 3)
+    batch.step([wl.geometry(i, 2) for i in range(6)], [wl.grid(i, 2) for i in range(6)])
+    assert_cells_equal(caches[b], orc.mc_cells(orc.sample_nodes(wl.geometry(b, 2), 3, wl.grid(b, 2).x0,
+                                                                wl.grid(b, 2).d, wl.grid(b, 2).n),
+                                               wl.grid(b, 2).x0, wl.grid(b, 2).d, wl.grid(b, 2).n))
+
+
+def test_autotuner_picks_a_shape_and_results_stay_identical(tmp_path, monkeypatch):
+    from cutcell.autotune import tune_stream
+    monkeypatch.setenv("CUTCELL_TUNING", str(tmp_path / "t.json"))
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=96)
+    geo = SDFTorus((0.0, 0.0, 0.0), 0.5, 0.15, random_rotation(np.random.default_rng(3)))
+    c = allocate_cache(g, MarchingCubesCutCell)
+    rec = tune_stream(c, geo, g, repeats=1)
+    assert c.get_tuning() == (rec["variant"], rec["kz"], rec["nontemporal"])
+    update_cache(c, geo, g)
+    phi_ref = orc.sample_nodes(geo, 3, g.x0, g.d, g.n)
+    assert_cells_equal(c, orc.mc_cells(phi_ref, g.x0, g.d, g.n))
+    assert (tmp_path / "t.json").exists()
