@@ -31,14 +31,16 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune", default="2,16,0",
+    ap.add_argument("--tune", default="2,8,0",
                     help="stream-pass launch shape: variant,kz,nontemporal")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the headline workload (skip the secondary workloads)")
     ap.add_argument("--sweep", action="store_true",
                     help="also time every stream-pass launch shape and report them")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "mc_stream_512_traffic.json"))
@@ -78,6 +80,114 @@ def cpu_baseline(grid, geo, budget_s):
                       f"{spent:.2f} s"}
 
 
+def _timed(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), ts
+
+
+def secondary(seed):
+    """The other workloads, each at its stated size: returns a dict of measurements."""
+    from cutcell import (CartesianIsoGrid, MarchingCubesCutCell, MarchingSquaresCutCell, PLICCutCell,
+                         SDFSphere, SDFTorus, allocate_cache, generate_mesh, random_rotation,
+                         update_cache)
+    out = {}
+    # marching squares, circle r=0.5 on [-1,1]^2 at sz=0.01 (200x200), then the contour
+    g2 = CartesianIsoGrid((-1.0, -1.0), (1.0, 1.0), sz=0.01)
+    c2 = allocate_cache(g2, MarchingSquaresCutCell)
+    geo2 = SDFSphere((0.0, 0.0), 0.5)
+    for _ in range(5):
+        update_cache(c2, geo2, g2)
+    t_up, _ = _timed(lambda: update_cache(c2, geo2, g2), 50)
+    t_mesh, _ = _timed(lambda: generate_mesh(c2, g2), 10)
+    nodes, lines = generate_mesh(c2, g2)
+    out["ms_200"] = {"update_us": round(t_up * 1e6, 1), "generate_mesh_ms": round(t_mesh * 1e3, 3),
+                     "contour_segments": int(lines.shape[0]), "cells": g2.ncells}
+    del c2
+    # PLIC, torus R=0.5 r=0.15, seeded tilt, 384^3
+    rng = np.random.default_rng(seed)
+    g3 = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=384)
+    tor = SDFTorus((0.0, 0.0, 0.0), 0.5, 0.15, random_rotation(rng))
+    c3 = allocate_cache(g3, PLICCutCell)
+    c3.set_profiling(True)
+    for _ in range(3):
+        update_cache(c3, tor, g3)
+    t_p, _ = _timed(lambda: update_cache(c3, tor, g3), 10)
+    kt = dict(c3.kernel_times())
+    n3 = g3.ncells
+    two_pass = 131 * n3
+    out["plic_384"] = {"update_ms": round(t_p * 1e3, 3), "cells_per_s": n3 / t_p,
+                       "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+                       "algorithmic_bytes_two_pass": two_pass,
+                       "hbm_frac_two_pass": round(two_pass / t_p / 1e9 / HBM_PEAK_GBS, 4),
+                       "cut_cells": int((c3.read("kind") == 2).sum())}
+    del c3
+    return out
+
+
+def fp32_error(grid, geo, cache64):
+    """The Float32 variant: its update time and its error against the Float64 cache (bitwise the
+    oracle) on a slab through the body."""
+    from cutcell import MarchingCubesCutCell, allocate_cache, update_cache
+    c32 = allocate_cache(grid, MarchingCubesCutCell, np.float32)
+    for _ in range(3):
+        update_cache(c32, geo, grid)
+    t32, _ = _timed(lambda: update_cache(c32, geo, grid), 8)
+    update_cache(cache64, geo, grid)
+    n = grid.n
+    k0 = n[2] // 2 - 8
+    lo, hi = n[0] * n[1] * k0, n[0] * n[1] * (k0 + 16)
+    err = {}
+    k64 = cache64.read("kind", lo, hi)
+    k32 = c32.read("kind", lo, hi)
+    err["kind_mismatch"] = int((k64 != k32).sum())
+    for name in ("volume_fraction", "face_fraction", "centroid", "interface_centroid"):
+        a = cache64.read(name, lo, hi)
+        b = c32.read(name, lo, hi).astype(np.float64)
+        err[name + "_max_abs"] = float(np.max(np.abs(a - b)))
+    del c32
+    return {"update_ms": round(t32 * 1e3, 3), "cells_per_s": grid.ncells / t32,
+            "error_vs_fp64_on_16_layers": err}
+
+
+def batched(seed, steps=32):
+    from cutcell import MarchingCubesCutCell, PLICCutCell, allocate_cache
+    from cutcell.autotune import tune_batch
+    from cutcell.batch import Batch
+    from cutcell.workloads import MovingBodies
+    wl = MovingBodies(seed=seed, nbodies=64)
+    meth = {"mc": MarchingCubesCutCell, "plic": PLICCutCell}
+    caches = [allocate_cache(wl.grid(b, 0), meth[B["method"]]) for b, B in enumerate(wl.bodies)]
+    batch = Batch(caches)
+    nb = len(caches)
+    geos0 = [wl.geometry(b, 0) for b in range(nb)]
+    grids0 = [wl.grid(b, 0) for b in range(nb)]
+    for _ in range(3):
+        batch.step(geos0, grids0)
+    kz = tune_batch(batch, geos0, grids0)
+    lat, dev = [], []
+    for s in range(steps):
+        geos = [wl.geometry(b, s) for b in range(nb)]
+        grids = [wl.grid(b, s) for b in range(nb)]
+        t = time.perf_counter()
+        dev.append(batch.step(geos, grids))
+        lat.append(time.perf_counter() - t)
+    lat = np.array(lat) * 1e3
+    ncell = wl.ncells()
+    mc_cells = sum(int(np.prod(B["n"])) for B in wl.bodies if B["method"] == "mc")
+    alg = BYTES_PER_CELL * mc_cells + 131 * (ncell - mc_cells)
+    return {"bodies": nb, "cells_per_step": ncell, "mc_cells": mc_cells,
+            "cells_per_s": ncell / (np.mean(lat) * 1e-3),
+            "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                "p99": round(float(np.percentile(lat, 99)), 3)},
+            "graph_device_ms_p50": round(float(np.percentile(dev, 50)), 3),
+            "hbm_frac": round(alg / (float(np.percentile(dev, 50)) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "kz_by_body_class": sorted(set(kz.values()))}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -96,6 +206,8 @@ def main():
     def grid_at(step):
         return grid.moved((-0.5 * d * step, 0.0, 0.0))
 
+    from cutcell.cache import probe_bandwidth
+    fill_gbs, copy_gbs = probe_bandwidth(8 << 30, 5)  # also brings the memory clocks up
     for s in range(a.warmup):
         update_cache(cache, geo, grid_at(s))
     per_kernel = {}
@@ -131,8 +243,6 @@ def main():
                 "traffic": traffic, "algorithmic_bytes": alg,
                 "kernel_ms": {k: round(v, 4) for k, v in avg.items()}}
     update_bytes = BYTES_PER_CELL * ncell + BYTES_PER_NODE * nnode
-    from cutcell.cache import probe_bandwidth
-    fill_gbs, copy_gbs = probe_bandwidth(8 << 30, 5)
     roofline["probe_fill_GBps"] = round(fill_gbs, 1)
     roofline["probe_copy_GBps"] = round(copy_gbs, 1)
     sweep = None
@@ -166,6 +276,11 @@ def main():
     }
     if sweep is not None:
         out["tuning_sweep"] = sweep
+    if not a.quick:
+        out["fp32"] = fp32_error(grid, geo, cache)
+        del cache
+        out.update(secondary(a.seed))
+        out["batched_64_bodies"] = batched(a.seed)
     print(json.dumps(out))
 
 

@@ -232,6 +232,12 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
             cc::launch_ms_cells<T>(phi, gt, cols, c->counter, c->stream);
             mark(c, "ms_cells");
         }
+    } else if (c->dim == 3 && c->tune.variant != 0) {
+        // the fused 3D pass: fit and resolved faces in one sweep
+        cc::launch_plic_fused<T>(gt, *geo, cols, (T *)c->col[CC_FIELD_NORMALS],
+                                 (T *)c->col[CC_FIELD_INTERCEPTS], (uint8_t *)c->col[CC_FIELD_IS_VALID],
+                                 (uint32_t)c->tune.kz, c->stream);
+        mark(c, "plic_fused");
     } else {
         cc::launch_plic_fit<T>(gt, *geo, c->dim, cols, (T *)c->col[CC_FIELD_NORMALS],
                                (T *)c->col[CC_FIELD_INTERCEPTS], (uint8_t *)c->col[CC_FIELD_IS_VALID],

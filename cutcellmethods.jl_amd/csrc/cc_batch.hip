@@ -5,10 +5,11 @@
 // src/plic/cache.jl:77-92), two or three launches each.  Here a step over B bodies is one graph of
 // three kernels, whatever B:
 //   reset  -- zero every marching-cubes body's cut-cell counter (a kernel, not a memset node);
-//   pass A -- every marching-cubes body's fused z-marching stream items and every PLIC body's fit
-//             items, in one grid: work item w belongs to the body whose prefix range holds w;
-//   pass B -- every marching-cubes body's cut-cell items (a grid-stride share of its work list) and
-//             every PLIC body's face items.
+//   pass A -- every marching-cubes body's fused z-marching stream items and every PLIC body's fused
+//             fit-and-faces items, in one grid: work item w belongs to the body whose prefix range
+//             holds w;
+//   pass B -- every marching-cubes body's cut-cell items (a grid-stride share of its work list).
+// A PLIC body is done entirely in pass A by the fused sweep (fit and resolved faces together).
 // The per-step inputs (grid origin, body parameters) live in a device descriptor array that the
 // host refreshes with one copy before each replay; the graph itself never changes.
 #include <algorithm>
@@ -29,6 +30,7 @@ template <typename T> struct BatchBody {
     int method;
     uint32_t ncell;
     uint32_t cut_items;  // pass-B work items of a marching-cubes body
+    uint32_t pnx, pny, pkz;  // fused PLIC tiling: tiles along x and y, layers per item
     McBody<T> mc;        // grid, body, columns, work list and tile shape (PLIC uses g, geo, cols)
     T *normals;
     T *intercepts;
@@ -64,14 +66,16 @@ __global__ __launch_bounds__(MC_TILE) void k_batch_a(const BatchBody<T> *__restr
     if (w >= total) return;
     const int b = find_body(begin, nb, w);
     const uint32_t local = w - begin[b];
-    __shared__ MarchLDS<T> L;
+    constexpr size_t lds_bytes = sizeof(MarchLDS<T>) > sizeof(PlicLDS<T>) ? sizeof(MarchLDS<T>)
+                                                                           : sizeof(PlicLDS<T>);
+    __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes];
     if (bodies[b].method == BODY_MC) {
         const McBody<T> B = bodies[b].mc;
-        mc_march_item<T, true, false>(B, local, L);
+        mc_march_item<T, true, false>(B, local, *reinterpret_cast<MarchLDS<T> *>(lds));
     } else {
         const BatchBody<T> &P = bodies[b];
-        const uint32_t c = local * MC_TILE + threadIdx.x;
-        if (c < P.ncell) plic_fit_cell<T, 3>(P.mc.g, P.mc.geo, P.mc.cols, P.normals, P.intercepts, P.valid, c);
+        plic_fused_item<T>(P.mc.g, P.mc.geo, P.mc.cols, P.normals, P.intercepts, P.valid, local, P.pnx,
+                           P.pny, P.pkz, *reinterpret_cast<PlicLDS<T> *>(lds));
     }
 }
 
@@ -89,9 +93,6 @@ __global__ __launch_bounds__(256) void k_batch_b(const BatchBody<T> *__restrict_
         const uint32_t stride = P.cut_items * 256u;
         for (uint32_t e = local * 256u + threadIdx.x; e < count; e += stride)
             mc_cut_item(P.mc.phi, P.mc.g, P.mc.cols, P.mc.cutlist[e]);
-    } else {
-        const uint32_t c = local * 256u + threadIdx.x;
-        if (c < P.ncell) plic_faces_cell<T, 3>(P.mc.g, P.mc.cols, P.normals, P.intercepts, P.valid, c);
     }
 }
 
@@ -182,8 +183,12 @@ template <typename T> int build_desc(cc_batch *bt, const cc_geometry *geos, cons
             B.cut_items = std::max(1u, B.ncell / 65536u);
             nbb = B.cut_items;
         } else {
-            na = (B.ncell + 255) / 256;
-            nbb = na;
+            // the fused PLIC sweep does the whole update in pass A
+            B.pnx = (B.mc.g.n[0] + cc::PF_W - 1) / cc::PF_W;
+            B.pny = (B.mc.g.n[1] + cc::PF_R - 1) / cc::PF_R;
+            B.pkz = (uint32_t)bt->kz[b];
+            na = B.pnx * B.pny * ((B.mc.g.n[2] + B.pkz - 1) / B.pkz);
+            nbb = 0;
         }
         if (ia) {
             ia->push_back(ia->back() + na);
@@ -202,8 +207,9 @@ template <typename T> int capture(cc_batch *bt) {
     const uint32_t ga = ((bt->total_a + 7) / 8) * 8, gb = ((bt->total_b + 7) / 8) * 8;
     hipLaunchKernelGGL((cc::k_batch_a<T>), dim3(ga), dim3(cc::MC_TILE), 0, bt->stream, dd,
                        bt->begin_a_dev, nb, bt->total_a);
-    hipLaunchKernelGGL((cc::k_batch_b<T>), dim3(gb), dim3(256), 0, bt->stream, dd, bt->begin_b_dev, nb,
-                       bt->total_b);
+    if (bt->total_b > 0)
+        hipLaunchKernelGGL((cc::k_batch_b<T>), dim3(gb), dim3(256), 0, bt->stream, dd, bt->begin_b_dev,
+                           nb, bt->total_b);
     if (hipStreamEndCapture(bt->stream, &bt->graph) != hipSuccess)
         return bfail(CC_ERR_HIP, "hipStreamEndCapture failed");
     if (hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0) != hipSuccess)

@@ -54,6 +54,11 @@ template <typename T>
 void launch_plic_faces(const GridT<T> &g, int dim, const Cols<T> &cols, const T *normals,
                        const T *intercepts, const uint8_t *valid, hipStream_t s);
 
+// The fused 3D PLIC pass (fit + resolved faces in one sweep, no read-back).
+template <typename T>
+void launch_plic_fused(const GridT<T> &g, const Geo &geo, const Cols<T> &cols, T *normals,
+                       T *intercepts, uint8_t *valid, uint32_t kz, hipStream_t s);
+
 void launch_zero_u32(uint32_t *p, hipStream_t s);
 
 // Tile shape of the z-marching stream pass for a grid of n cells: W cells along x (a 256-cell run,

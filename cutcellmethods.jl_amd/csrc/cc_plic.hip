@@ -27,7 +27,29 @@ __global__ __launch_bounds__(256) void k_plic_faces(GridT<T> g, Cols<T> cols,
     if (c < ncell) plic_faces_cell<T, D>(g, cols, normals, intercepts, valid, c);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_plic_fused(GridT<T> g, Geo geo, Cols<T> cols, T *normals,
+                                                    T *intercepts, uint8_t *valid, uint32_t nx,
+                                                    uint32_t ny, uint32_t kz, uint32_t nwork) {
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (w >= nwork) return;
+    __shared__ PlicLDS<T> L;
+    plic_fused_item<T>(g, geo, cols, normals, intercepts, valid, w, nx, ny, kz, L);
+}
+
 }  // namespace
+
+template <typename T>
+void launch_plic_fused(const GridT<T> &g, const Geo &geo, const Cols<T> &cols, T *normals,
+                       T *intercepts, uint8_t *valid, uint32_t kz, hipStream_t s) {
+    const uint32_t nx = (g.n[0] + PF_W - 1) / PF_W, ny = (g.n[1] + PF_R - 1) / PF_R;
+    const uint32_t nkb = (g.n[2] + kz - 1) / kz;
+    const uint32_t nwork = nx * ny * nkb;
+    const uint32_t blocks = ((nwork + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_plic_fused<T>), dim3(blocks), dim3(256), 0, s, g, geo, cols, normals,
+                       intercepts, valid, nx, ny, kz, nwork);
+}
 
 template <typename T>
 void launch_plic_fit(const GridT<T> &g, const Geo &geo, int dim, const Cols<T> &cols, T *normals,
@@ -59,6 +81,10 @@ template void launch_plic_fit<double>(const GridT<double> &, const Geo &, int, c
                                       double *, double *, uint8_t *, hipStream_t);
 template void launch_plic_fit<float>(const GridT<float> &, const Geo &, int, const Cols<float> &,
                                      float *, float *, uint8_t *, hipStream_t);
+template void launch_plic_fused<double>(const GridT<double> &, const Geo &, const Cols<double> &,
+                                        double *, double *, uint8_t *, uint32_t, hipStream_t);
+template void launch_plic_fused<float>(const GridT<float> &, const Geo &, const Cols<float> &, float *,
+                                       float *, uint8_t *, uint32_t, hipStream_t);
 template void launch_plic_faces<double>(const GridT<double> &, int, const Cols<double> &,
                                         const double *, const double *, const uint8_t *, hipStream_t);
 template void launch_plic_faces<float>(const GridT<float> &, int, const Cols<float> &, const float *,

@@ -8,23 +8,21 @@ namespace cc {
 namespace {
 
 // CartesianMeshes.get_volume_fraction(n, a) (un-vendored): the measure of {xi in [0,1]^D : n.xi <= a}
-// by Scardovelli & Zaleski (2000), negative components reflected, normalised by sum|n|, the
-// alpha <= 1/2 half by symmetry.  In 3D the m1 factor of alpha^3 - (alpha - m1)^3 is cancelled by
-// hand so a nearly axis-aligned plane loses no digits.
+// by Scardovelli & Zaleski (2000): negative components reflected, m_i = |n_i| sorted, s = sum m_i.
+// Every formula is homogeneous of degree zero in (a, m), so there is no normalisation by s, and
+// one division per evaluation; the alpha <= s/2 half by symmetry.  In 3D the m1 factor of
+// a^3 - (a - m1)^3 = m1 * P1 is kept explicit so a nearly axis-aligned plane loses no digits.
 template <typename T> __device__ __forceinline__ T vfrac2(T n0, T n1, T a) {
     if (n0 < T(0)) a -= n0;
     if (n1 < T(0)) a -= n1;
-    T m0 = fabs(n0), m1 = fabs(n1);
+    const T m0 = fabs(n0), m1 = fabs(n1);
     const T s = m0 + m1;
     if (!(s > T(0))) return a >= T(0) ? T(1) : T(0);
-    m0 /= s;
-    m1 /= s;
-    a /= s;
     if (a <= T(0)) return T(0);
-    if (a >= T(1)) return T(1);
+    if (a >= s) return T(1);
     const T lo = m0 < m1 ? m0 : m1, hi = m0 < m1 ? m1 : m0;
-    const bool flip = a > T(0.5);
-    const T al = flip ? T(1) - a : a;
+    const bool flip = a > T(0.5) * s;
+    const T al = flip ? s - a : a;
     const T v = al < lo ? (al * al) / (T(2) * lo * hi) : (al - T(0.5) * lo) / hi;
     return flip ? T(1) - v : v;
 }
@@ -38,44 +36,41 @@ template <typename T> __device__ __forceinline__ T vfrac3(const T *n, T a) {
     }
     const T s = (m[0] + m[1]) + m[2];
     if (!(s > T(0))) return a >= T(0) ? T(1) : T(0);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) m[i] /= s;
-    a /= s;
     if (a <= T(0)) return T(0);
-    if (a >= T(1)) return T(1);
+    if (a >= s) return T(1);
     T t;
     if (m[0] > m[1]) { t = m[0]; m[0] = m[1]; m[1] = t; }
     if (m[1] > m[2]) { t = m[1]; m[1] = m[2]; m[2] = t; }
     if (m[0] > m[1]) { t = m[0]; m[0] = m[1]; m[1] = t; }
-    const bool flip = a > T(0.5);
-    const T al = flip ? T(1) - a : a;
+    const bool flip = a > T(0.5) * s;
+    const T al = flip ? s - a : a;
     const T m1 = m[0], m2 = m[1], m3 = m[2], m12 = m1 + m2;
     T v;
     if (al < m1) {
         v = (al * al * al) / (T(6) * m1 * m2 * m3);
-    } else {
-        const T P1 = T(3) * al * (al - m1) + m1 * m1;
-        if (al < m2) {
-            v = P1 / (T(6) * m2 * m3);
-        } else if (al < m12) {
-            const T u2 = al - m2;
-            T c = u2 * u2 * (u2 / m1);
-            if (al > m3) {
-                const T u3 = al - m3;
-                c = c + u3 * u3 * (u3 / m1);
-            }
-            v = (P1 - c) / (T(6) * m2 * m3);
-        } else {
-            v = (T(2) * al - m12) / (T(2) * m3);
+    } else if (al < m2) {
+        v = (T(3) * al * (al - m1) + m1 * m1) / (T(6) * m2 * m3);
+    } else if (al < m12) {
+        const T u2 = al - m2;
+        T c = u2 * u2 * u2;
+        if (al > m3) {
+            const T u3 = al - m3;
+            c = c + u3 * u3 * u3;
         }
+        v = (m1 * (T(3) * al * (al - m1) + m1 * m1) - c) / (T(6) * m1 * m2 * m3);
+    } else {
+        v = (T(2) * al - m12) / (T(2) * m3);
     }
     return flip ? T(1) - v : v;
 }
 
 // _face_area (apertures.jl:112-125) with _cell_wet (:73-97): one cell's own wet measure of its face
-// `k` (0-based direction) from its stored fit.  noinline: one compiled body for both sides of a face.
+// `k` (0-based direction) from its stored fit.  The reference marks this @noinline because the
+// plane-clip it calls is @fastmath and could round differently where inlined; here nothing is
+// contracted or reassociated (-ffp-contract=off, no fast-math), so every inlined copy of this one
+// expression rounds identically and both sides of a face get the same bits.
 template <typename T, int D>
-__device__ __noinline__ T face_area(const T *nv, T a, T frac, bool valid, const T *cs, int k) {
+__device__ __forceinline__ T face_area(const T *nv, T a, T frac, bool valid, const T *cs, int k) {
     const T s = (k & 1) ? T(1) : T(0);
     if (D == 2) {
         const int c = k < 2 ? 0 : 1, p = 1 - c;
@@ -96,40 +91,63 @@ __device__ __noinline__ T face_area(const T *nv, T a, T frac, bool valid, const 
     return U * V * (T(1) - solid);
 }
 
-// cell_plane (src/plic/plic.jl:72-91) and _plic_fit_kernel! (src/plic/cache.jl:96-113) for cell c.
+// One cell's fitted plane: cell_plane (src/plic/plic.jl:72-91) as the fit pass stores it.
+template <typename T> struct Plane {
+    T n[3];
+    T a;
+    T vf;  // the fluid fraction, 1 - solid
+    bool ok;
+};
+
+template <typename T, int D>
+__device__ __forceinline__ Plane<T> plic_plane(const GridT<T> &g, const Geo &geo, uint32_t i,
+                                               uint32_t j, uint32_t k) {
+    const uint32_t ijk[3] = {i, j, k};
+    T x[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = g.x0[a] + (T(ijk[a]) + T(0.5)) * g.d[a];  // get_elem_centroid
+    Plane<T> P;
+    P.n[0] = P.n[1] = P.n[2] = T(0);
+    const T dist = get_sdf<T, D>(geo, x, P.n);
+    T sum = P.n[0] + P.n[1];
+    if (D == 3) sum = sum + P.n[2];
+    T a = T(0.5) * sum - dist / g.d[0];
+    const bool zero = P.n[0] == T(0) && P.n[1] == T(0) && (D == 2 || P.n[2] == T(0));
+    T fraction;
+    if (isnan(a) || zero) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) P.n[q] = T(0);
+        a = T(0);
+        fraction = dist <= T(0) ? T(1) : T(0);
+        P.ok = false;
+    } else {
+        fraction = D == 2 ? vfrac2(P.n[0], P.n[1], a) : vfrac3(P.n, a);
+        P.ok = true;
+    }
+    P.a = a;
+    P.vf = T(1) - fraction;
+    return P;
+}
+
+// The fit pass's stores for cell c (src/plic/cache.jl:104-112): kind from the stored fraction.
+template <typename T, int D>
+__device__ __forceinline__ void plic_store_fit(const Plane<T> &P, const Cols<T> &cols, T *normals,
+                                               T *intercepts, uint8_t *valid, size_t c) {
+#pragma unroll
+    for (int q = 0; q < D; ++q) normals[D * c + q] = P.n[q];
+    intercepts[c] = P.a;
+    valid[c] = P.ok ? 1 : 0;
+    cols.vf[c] = P.vf;
+    cols.kind[c] = P.vf == T(0) ? CELL_INSIDE : (P.vf == T(1) ? CELL_OUTSIDE : CELL_CUT);
+}
+
+// _plic_fit_kernel! (src/plic/cache.jl:96-113) for cell c.
 template <typename T, int D>
 __device__ __forceinline__ void plic_fit_cell(const GridT<T> &g, const Geo &geo, const Cols<T> &cols,
                                               T *normals, T *intercepts, uint8_t *valid, uint32_t c) {
     const uint32_t n0 = g.n[0], n1 = g.n[1];
-    const uint32_t ijk[3] = {c % n0, (c / n0) % n1, c / (n0 * n1)};
-    T x[3] = {T(0), T(0), T(0)};
-#pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = g.x0[a] + (T(ijk[a]) + T(0.5)) * g.d[a];  // get_elem_centroid
-    T nrm[3] = {T(0), T(0), T(0)};
-    const T dist = get_sdf<T, D>(geo, x, nrm);
-    T sum = nrm[0] + nrm[1];
-    if (D == 3) sum = sum + nrm[2];
-    T a = T(0.5) * sum - dist / g.d[0];
-    bool zero = nrm[0] == T(0) && nrm[1] == T(0) && (D == 2 || nrm[2] == T(0));
-    T fraction;
-    bool ok;
-    if (isnan(a) || zero) {
-#pragma unroll
-        for (int q = 0; q < D; ++q) nrm[q] = T(0);
-        a = T(0);
-        fraction = dist <= T(0) ? T(1) : T(0);
-        ok = false;
-    } else {
-        fraction = D == 2 ? vfrac2(nrm[0], nrm[1], a) : vfrac3(nrm, a);
-        ok = true;
-    }
-    const T frac = T(1) - fraction;
-#pragma unroll
-    for (int q = 0; q < D; ++q) normals[D * (size_t)c + q] = nrm[q];
-    intercepts[c] = a;
-    valid[c] = ok ? 1 : 0;
-    cols.vf[c] = frac;
-    cols.kind[c] = frac == T(0) ? CELL_INSIDE : (frac == T(1) ? CELL_OUTSIDE : CELL_CUT);
+    const Plane<T> P = plic_plane<T, D>(g, geo, c % n0, (c / n0) % n1, c / (n0 * n1));
+    plic_store_fit<T, D>(P, cols, normals, intercepts, valid, c);
 }
 
 // _plic_face_fraction_kernel! and _resolved_slot (src/plic/cache.jl:118-162) for cell c.
@@ -172,6 +190,119 @@ __device__ __forceinline__ void plic_faces_cell(const GridT<T> &g, const Cols<T>
             r = has ? (an + ap) / T(2) : an;
         }
         cols.ff[2 * D * (size_t)c + slot] = r;
+    }
+}
+
+template <typename T> __device__ __forceinline__ T plane_face(const Plane<T> &P, const T *cs, int k) {
+    return face_area<T, 3>(P.n, P.a, P.vf, P.ok, cs, k);
+}
+
+// Tile of the fused 3D PLIC pass: PF_W cells along x by PF_R rows, marched in z.
+constexpr int PF_W = 32, PF_R = 8;
+
+template <typename T> struct PlicLDS {
+    T cand[PF_R + 2][PF_W + 2][6];  // one-sided wet areas of the tile's cells and of the x/y halo
+};
+
+// One work item of the fused 3D PLIC pass: the fit (cell_plane, src/plic/plic.jl:72-91) and the
+// face resolution (src/plic/cache.jl:118-162) of the cells [i0, i0+PF_W) x [j0, j0+PF_R) over the
+// layers [k0, k0+kz), with no intermediate read-back.  Each thread owns one (i, j) column and
+// carries its cell's planes up the z-march in registers, so the -z and +z neighbours' candidates
+// come from its own previous and next layer; the x and y neighbours' candidates come through LDS,
+// the ring of cells just outside the tile being fitted again by spare threads.  A cell's plane is
+// a pure function of its centroid, so a recomputed plane is bitwise the one its owner stores, and
+// every face is resolved from the same two numbers as the two-pass form.
+template <typename T>
+__device__ void plic_fused_item(const GridT<T> &g, const Geo &geo, const Cols<T> &cols, T *normals,
+                                T *intercepts, uint8_t *valid, uint32_t w, uint32_t nx, uint32_t ny,
+                                uint32_t kz, PlicLDS<T> &L) {
+    const uint32_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const uint32_t tx = w % nx, rest = w / nx, ty = rest % ny, kb = rest / ny;
+    const uint32_t i0 = tx * PF_W, j0 = ty * PF_R, k0 = kb * kz, k1 = min(k0 + kz, n2);
+    const int wc = (int)min((uint32_t)PF_W, n0 - i0), rc = (int)min((uint32_t)PF_R, n1 - j0);
+    const int t = threadIdx.x, ii = t % PF_W, r = t / PF_W;
+    const bool inside = ii < wc && r < rc;
+    const uint32_t i = i0 + ii, j = j0 + r;
+    T fullf[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        T f = T(1);
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            if (b != ax) f *= g.d[b];
+        fullf[ax] = f;
+    }
+    // the halo cell this thread fits, if any: bottom row, top row, left column, right column
+    int hr = -1, hc = -1, hslot = 0;
+    uint32_t hi_ = 0, hj = 0;
+    {
+        const int h = t;
+        if (h < wc) {
+            if (j0 >= 1) { hr = 0; hc = h + 1; hslot = 3; hi_ = i0 + h; hj = j0 - 1; }
+        } else if (h >= PF_W && h < PF_W + wc) {
+            if (j0 + rc < n1) { hr = rc + 1; hc = h - PF_W + 1; hslot = 2; hi_ = i0 + h - PF_W; hj = j0 + rc; }
+        } else if (h >= 2 * PF_W && h < 2 * PF_W + rc) {
+            if (i0 >= 1) { hr = h - 2 * PF_W + 1; hc = 0; hslot = 1; hi_ = i0 - 1; hj = j0 + h - 2 * PF_W; }
+        } else if (h >= 2 * PF_W + PF_R && h < 2 * PF_W + PF_R + rc) {
+            if (i0 + wc < n0) {
+                hr = h - 2 * PF_W - PF_R + 1; hc = wc + 1; hslot = 0; hi_ = i0 + wc; hj = j0 + h - 2 * PF_W - PF_R;
+            }
+        }
+    }
+    Plane<T> cur, nxt;
+    T cz_prev = T(0);
+    T cc[6], cn[6];
+    if (inside) {
+        if (k0 >= 1) cz_prev = plane_face(plic_plane<T, 3>(g, geo, i, j, k0 - 1), g.d, 5);
+        cur = plic_plane<T, 3>(g, geo, i, j, k0);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) cc[q] = plane_face(cur, g.d, q);
+    }
+    for (uint32_t k = k0; k < k1; ++k) {
+        if (inside && k + 1 < n2) {
+            nxt = plic_plane<T, 3>(g, geo, i, j, k + 1);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) cn[q] = plane_face(nxt, g.d, q);
+        }
+        if (inside) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) L.cand[r + 1][ii + 1][q] = cc[q];
+        }
+        if (hr >= 0) L.cand[hr][hc][hslot] = plane_face(plic_plane<T, 3>(g, geo, hi_, hj, k), g.d, hslot);
+        __syncthreads();
+        if (inside) {
+            T ff[6];
+            // _resolved_slot: low faces take the neighbour below's high candidate first
+            {
+                const T ap = cc[0] / fullf[0];
+                ff[0] = i >= 1 ? (L.cand[r + 1][ii][1] / fullf[0] + ap) / T(2) : ap;
+                const T an = cc[1] / fullf[0];
+                ff[1] = i + 2 <= n0 ? (an + L.cand[r + 1][ii + 2][0] / fullf[0]) / T(2) : an;
+            }
+            {
+                const T ap = cc[2] / fullf[1];
+                ff[2] = j >= 1 ? (L.cand[r][ii + 1][3] / fullf[1] + ap) / T(2) : ap;
+                const T an = cc[3] / fullf[1];
+                ff[3] = j + 2 <= n1 ? (an + L.cand[r + 2][ii + 1][2] / fullf[1]) / T(2) : an;
+            }
+            {
+                const T ap = cc[4] / fullf[2];
+                ff[4] = k >= 1 ? (cz_prev / fullf[2] + ap) / T(2) : ap;
+                const T an = cc[5] / fullf[2];
+                ff[5] = k + 2 <= n2 ? (an + cn[4] / fullf[2]) / T(2) : an;
+            }
+            const size_t c = i + (size_t)n0 * (j + (size_t)n1 * k);
+            plic_store_fit<T, 3>(cur, cols, normals, intercepts, valid, c);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) cols.ff[6 * c + q] = ff[q];
+        }
+        __syncthreads();
+        if (inside) {
+            cz_prev = cc[5];
+            cur = nxt;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) cc[q] = cn[q];
+        }
     }
 }
 

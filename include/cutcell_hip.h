@@ -125,8 +125,10 @@ int cc_stream(cc_cache *cache, void **stream);
  *   variant 0 = flat tiles of 256 consecutive cells reading phi,
  *           1 = z-march of 256-cell row runs over kz layers reading phi,
  *           2 = z-march sampling an analytic body in-kernel (phi written once, never re-read);
- *   nontemporal != 0 streams the column stores past the caches.  Results are identical for every
- *   setting; only the speed differs. */
+ *   nontemporal != 0 streams the column stores past the caches.  For a 3D PLIC cache, variant 0
+ *   is the reference's two passes (fit, then faces) and any other value the fused single sweep
+ *   (fit + resolved faces, kz layers per work item).  Results are identical for every setting;
+ *   only the speed differs. */
 #define CC_MC_FLAT 0
 #define CC_MC_MARCH 1
 #define CC_MC_MARCH_FUSED 2

@@ -808,8 +808,11 @@ void orc_mc_cells(const double *phi, const double *x0, const double *d, const in
 /* =========================================================================================== */
 
 /* CartesianMeshes.get_volume_fraction(n, a): the measure of {xi in [0,1]^D : n.xi <= a}.
- * Scardovelli & Zaleski (2000): reflect negative components, normalise by sum|n|, use the
- * alpha <= 1/2 half by symmetry, then the piecewise closed form over sorted m1 <= m2 (<= m3). */
+ * Scardovelli & Zaleski (2000): reflect negative components (a -= n_i), then work with m_i = |n_i|
+ * sorted ascending, s = sum m_i.  Every formula is homogeneous of degree zero in (a, m), so no
+ * normalisation by s is needed; the alpha <= s/2 half is used by symmetry.  In 3D the m1 factor of
+ * a^3 - (a - m1)^3 = m1 * P1 is kept explicit (P1 = 3a(a - m1) + m1^2) so a nearly axis-aligned
+ * plane loses no digits; one division per evaluation. */
 static double vf2(double n0, double n1, double a)
 {
     if (n0 < 0.0) a -= n0;
@@ -817,12 +820,11 @@ static double vf2(double n0, double n1, double a)
     double m0 = fabs(n0), m1 = fabs(n1);
     double s = m0 + m1;
     if (!(s > 0.0)) return a >= 0.0 ? 1.0 : 0.0;
-    m0 /= s; m1 /= s; a /= s;
     if (a <= 0.0) return 0.0;
-    if (a >= 1.0) return 1.0;
+    if (a >= s) return 1.0;
     double lo = m0 < m1 ? m0 : m1, hi = m0 < m1 ? m1 : m0;
-    int flip = a > 0.5;
-    double al = flip ? 1.0 - a : a, v;
+    int flip = a > 0.5 * s;
+    double al = flip ? s - a : a, v;
     if (al < lo) v = (al * al) / (2.0 * lo * hi);
     else v = (al - 0.5 * lo) / hi;
     return flip ? 1.0 - v : v;
@@ -837,39 +839,30 @@ static double vf3(const double *n, double a)
     }
     double s = (m[0] + m[1]) + m[2];
     if (!(s > 0.0)) return a >= 0.0 ? 1.0 : 0.0;
-    for (int i = 0; i < 3; ++i) m[i] /= s;
-    a /= s;
     if (a <= 0.0) return 0.0;
-    if (a >= 1.0) return 1.0;
-    /* sort ascending */
+    if (a >= s) return 1.0;
     double t;
     if (m[0] > m[1]) { t = m[0]; m[0] = m[1]; m[1] = t; }
     if (m[1] > m[2]) { t = m[1]; m[1] = m[2]; m[2] = t; }
     if (m[0] > m[1]) { t = m[0]; m[0] = m[1]; m[1] = t; }
-    int flip = a > 0.5;
-    double al = flip ? 1.0 - a : a, v;
+    int flip = a > 0.5 * s;
+    double al = flip ? s - a : a, v;
     double m1 = m[0], m2 = m[1], m3 = m[2], m12 = m1 + m2;
-    /* alpha^3 - sum_i (alpha - m_i)^3_+ over 6 m1 m2 m3, with the m1 factor cancelled by hand:
-     * alpha^3 - (alpha - m1)^3 = m1 * P1, P1 = 3 alpha (alpha - m1) + m1^2.  Between m2 and m12
-     * every remaining (alpha - m_i) is below m1 (alpha <= 1/2 < m12 whenever alpha >= m3), so the
-     * subtraction below loses no digits even for a nearly axis-aligned plane. */
     if (al < m1) {
         v = (al * al * al) / (6.0 * m1 * m2 * m3);
-    } else {
-        double P1 = 3.0 * al * (al - m1) + m1 * m1;
-        if (al < m2) {
-            v = P1 / (6.0 * m2 * m3);
-        } else if (al < m12) {
-            double u2 = al - m2;
-            double c = u2 * u2 * (u2 / m1);
-            if (al > m3) {
-                double u3 = al - m3;
-                c = c + u3 * u3 * (u3 / m1);
-            }
-            v = (P1 - c) / (6.0 * m2 * m3);
-        } else {
-            v = (2.0 * al - m12) / (2.0 * m3);
+    } else if (al < m2) {
+        v = (3.0 * al * (al - m1) + m1 * m1) / (6.0 * m2 * m3);
+    } else if (al < m12) {
+        /* every remaining (al - m_i) is below m1 here (al <= s/2 < m12 whenever al >= m3) */
+        double u2 = al - m2;
+        double c = u2 * u2 * u2;
+        if (al > m3) {
+            double u3 = al - m3;
+            c = c + u3 * u3 * u3;
         }
+        v = (m1 * (3.0 * al * (al - m1) + m1 * m1) - c) / (6.0 * m1 * m2 * m3);
+    } else {
+        v = (2.0 * al - m12) / (2.0 * m3);
     }
     return flip ? 1.0 - v : v;
 }

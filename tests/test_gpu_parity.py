@@ -343,3 +343,21 @@ def test_autotuner_picks_a_shape_and_results_stay_identical(tmp_path, monkeypatc
     phi_ref = orc.sample_nodes(geo, 3, g.x0, g.d, g.n)
     assert_cells_equal(c, orc.mc_cells(phi_ref, g.x0, g.d, g.n))
     assert (tmp_path / "t.json").exists()
+
+
+@pytest.mark.parametrize("variant,kz", [(0, 16), (2, 16), (2, 5), (2, 1)])
+def test_plic_fused_and_two_pass_bitwise(variant, kz):
+    # the fused single sweep and the reference's two passes give the oracle's bits, on a grid whose
+    # extents are not multiples of the 32 x 8 tile or of kz; torus and box (degenerate fits) bodies
+    rng = np.random.default_rng(17)
+    g = CartesianGrid((-0.93, -1.07, -0.88), (70, 37, 29), (0.052,) * 3)
+    for geo in (SDFTorus((0.05, -0.02, 0.03), 0.5, 0.15, random_rotation(rng)),
+                SDFBox((0.0, 0.0, 0.0), (0.4, 0.3, 0.35), random_rotation(rng)),
+                SDFBox((0.026, 0.0, 0.0), (0.4, 0.3, 0.35))):
+        c = allocate_cache(g, PLICCutCell)
+        c.set_tuning(variant, kz, False)
+        update_cache(c, geo, g)
+        ref, normals, intercepts, valid = orc.plic(geo, 3, g.x0, g.d, g.n)
+        assert_cells_equal(c, ref)
+        assert same_bits(c.normals, normals) and same_bits(c.intercepts, intercepts)
+        assert np.array_equal(c.is_valid, valid)
