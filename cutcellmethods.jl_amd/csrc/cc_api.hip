@@ -102,6 +102,18 @@ struct cc_cache {
     int nev = 0;
     bool ev_ready = false;
     cc::McTuning tune;
+    // how a sphere union's nodal phi is produced (marching cubes)
+    int phi_source = CC_PHI_NAIVE;
+    cc_mlp *net = nullptr;
+    int coarse = 4;
+    double band_cells = 3.0;
+    uint16_t *cull_list = nullptr;
+    int64_t cull_cap = 0;  // entries
+    uint32_t *cull_cnt = nullptr;
+    int64_t cnt_cap = 0;
+    float *nn = nullptr;
+    int64_t nn_cap = 0;
+    unsigned long long *phi_stats = nullptr;
 };
 
 namespace {
@@ -198,6 +210,69 @@ void mark(cc_cache *c, const char *name) {
     c->nev += 1;
 }
 
+template <typename T> int ensure(T **p, int64_t *cap, int64_t want) {
+    if (want <= *cap) return CC_OK;
+    if (*p) HIPC(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc(p, sizeof(T) * want) != hipSuccess)
+        return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate %lld bytes", (long long)(sizeof(T) * want));
+    *cap = want;
+    return CC_OK;
+}
+
+// A sphere union's nodal phi through the cull pass and the brick pass (culled exact, or the
+// surrogate with certified signs).  Bitwise the plain fold at every node the surrogate does not keep.
+template <typename T> int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::GridT<T> &gt, T *phi) {
+    if (geo.nprims > 65535)
+        return fail(CC_ERR_UNSUPPORTED, "culled and surrogate phi take at most 65535 primitives");
+    int nb[3];
+    cc::phi_brick_counts(gt.n, nb);
+    const int64_t nbrick = (int64_t)nb[0] * nb[1] * nb[2];
+    int rc = ensure(&c->cull_list, &c->cull_cap, nbrick * geo.nprims);
+    if (!rc) rc = ensure(&c->cull_cnt, &c->cnt_cap, nbrick);
+    if (rc) return rc;
+    if (!c->phi_stats) HIPC(hipMalloc(&c->phi_stats, sizeof(unsigned long long) * cc::PHI_ST_COUNT));
+    // magnitude bound for the rounding margins: coordinates of primitives and lattice, radii
+    double scale = 1.0, pm = 0.0, gm = 0.0;
+    for (int64_t i = 0; i < geo.nprims; ++i) {
+        const double *q = c->prims_host.data() + 4 * i;
+        const double m = std::fmax(std::fabs(q[0]), std::fmax(std::fabs(q[1]), std::fabs(q[2]))) + std::fabs(q[3]);
+        pm = std::fmax(pm, m);
+    }
+    for (int a = 0; a < 3; ++a)
+        gm = std::fmax(gm, std::fmax(std::fabs((double)gt.x0[a]),
+                                     std::fabs((double)gt.x0[a] + (double)gt.n[a] * (double)gt.d[a])));
+    scale += 2.0 * (pm + gm) + std::fabs(geo.p[0]);
+    cc::SurrogateArgs sa{};
+    sa.scale = scale;
+    cc::launch_zero_u64(c->phi_stats, cc::PHI_ST_COUNT, c->stream);
+    if (c->phi_source == CC_PHI_SURROGATE) {
+        cc::Lattice lat{};
+        const int f = c->coarse;
+        for (int a = 0; a < 3; ++a) {
+            lat.n[a] = gt.n[a] / f + 2;
+            lat.x0[a] = (float)gt.x0[a];
+            lat.h[a] = (float)((double)f * (double)gt.d[a]);
+            sa.nc[a] = lat.n[a];
+        }
+        rc = ensure(&c->nn, &c->nn_cap, (int64_t)lat.n[0] * lat.n[1] * lat.n[2]);
+        if (rc) return rc;
+        if (cc::launch_mlp_lattice(c->net, lat, c->nn, c->stream) != CC_OK)
+            return fail(CC_ERR_ARGUMENT, "the surrogate lattice is too large");
+        mark(c, "nn_lattice");
+        sa.nn = c->nn;
+        sa.f = f;
+        const double h = std::fmax((double)gt.d[0], std::fmax((double)gt.d[1], (double)gt.d[2]));
+        sa.band = (float)(c->band_cells * h);
+    }
+    cc::launch_cull<T>(gt, geo, scale, c->cull_list, c->cull_cnt, c->phi_stats, c->stream);
+    mark(c, "cull");
+    cc::launch_phi_bricks<T>(phi, gt, geo, c->cull_list, c->cull_cnt, sa, c->phi_stats, c->stream);
+    mark(c, "phi_bricks");
+    return CC_OK;
+}
+
 template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void *phi_in, int on_dev,
                                      const cc_grid *g) {
     const cc::GridT<T> gt = grid_t<T>(g, c->dim);
@@ -213,7 +288,10 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
         const bool fuse = c->method == CC_MARCHING_CUBES && geo && tn.variant == cc::MC_MARCH_FUSED &&
                           geo->kind != cc::GEO_SPHERE_UNION;
         if (!fuse && tn.variant == cc::MC_MARCH_FUSED) tn.variant = cc::MC_MARCH;
-        if (geo && !fuse) {
+        if (geo && !fuse && geo->kind == cc::GEO_SPHERE_UNION && c->phi_source != CC_PHI_NAIVE) {
+            const int rc = phi_bricks<T>(c, *geo, gt, phi);
+            if (rc) return rc;
+        } else if (geo && !fuse) {
             cc::launch_sample<T>(phi, gt, *geo, c->dim, c->stream);
             mark(c, "sample_nodes");
         } else if (phi_in) {
@@ -278,6 +356,10 @@ void free_all(cc_cache *c) {
     if (c->cutlist) hipFree(c->cutlist);
     if (c->counter) hipFree(c->counter);
     if (c->prims_dev) hipFree(c->prims_dev);
+    if (c->cull_list) hipFree(c->cull_list);
+    if (c->cull_cnt) hipFree(c->cull_cnt);
+    if (c->nn) hipFree(c->nn);
+    if (c->phi_stats) hipFree(c->phi_stats);
     for (auto &e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -501,6 +583,39 @@ int cc_update_async(cc_cache *c, const cc_geometry *geo, const cc_grid *grid) {
 int cc_synchronize(cc_cache *c) {
     if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
     HIPC(hipStreamSynchronize(c->stream));
+    return CC_OK;
+}
+
+int cc_set_phi_source(cc_cache *c, int source, cc_mlp *net, int coarse, double band_cells) {
+    if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
+    if (source != CC_PHI_NAIVE && source != CC_PHI_CULLED && source != CC_PHI_SURROGATE)
+        return fail(CC_ERR_ARGUMENT, "unknown phi source %d", source);
+    if (source != CC_PHI_NAIVE && c->method != CC_MARCHING_CUBES)
+        return fail(CC_ERR_UNSUPPORTED, "culled and surrogate phi serve the marching-cubes cache");
+    if (source == CC_PHI_SURROGATE) {
+        if (!net) return fail(CC_ERR_ARGUMENT, "the surrogate source needs a network");
+        if (coarse < 1 || coarse > 64) return fail(CC_ERR_ARGUMENT, "coarse spacing must be 1..64 cells");
+        if (!(band_cells >= 0.0) || !std::isfinite(band_cells))
+            return fail(CC_ERR_ARGUMENT, "band must be a finite number of cells >= 0");
+    }
+    c->phi_source = source;
+    c->net = source == CC_PHI_SURROGATE ? net : nullptr;
+    if (source == CC_PHI_SURROGATE) {
+        c->coarse = coarse;
+        c->band_cells = band_cells;
+    }
+    return CC_OK;
+}
+
+int cc_phi_stats(cc_cache *c, int64_t *stats) {
+    if (!c || !stats) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    unsigned long long h[cc::PHI_ST_COUNT] = {};
+    if (c->phi_stats) {
+        HIPC(hipSetDevice(c->device));
+        HIPC(hipStreamSynchronize(c->stream));
+        HIPC(hipMemcpy(h, c->phi_stats, sizeof h, hipMemcpyDeviceToHost));
+    }
+    for (int i = 0; i < cc::PHI_ST_COUNT; ++i) stats[i] = (int64_t)h[i];
     return CC_OK;
 }
 

@@ -66,4 +66,63 @@ void launch_zero_u32(uint32_t *p, hipStream_t s);
 void mc_tile_shape(const int *n, uint32_t kz, uint32_t *W, uint32_t *R, uint32_t *KZ, uint32_t *nseg,
                    uint32_t *nrg, uint32_t *nkb);
 
+// ---- nodal phi of a sphere union, brick by brick ------------------------------------------------
+// The node lattice is cut into bricks of PHI_BRICK^3 nodes.  For each brick a cull pass keeps, in
+// fold order, the primitives that a bound over the brick's box cannot prove inert: primitive i is
+// dropped when its lower bound over the box exceeds min_{j<i} (upper bound of primitive j) by more
+// than the smoothing k (the polynomial smooth minimum of a and b with b - a >= k returns a exactly,
+// and the fold so far never exceeds any earlier primitive), so the culled fold is bitwise the full
+// one.  The phi pass then folds only the kept primitives, at every node (culled exact) or, with a
+// surrogate, only where the surrogate cannot be trusted.
+constexpr int PHI_BRICK = 16;  // nodes per brick edge
+constexpr int PHI_SUB = 4;     // nodes per certification sub-brick edge
+
+enum PhiStat {
+    PHI_ST_NODES = 0,       // nodes written
+    PHI_ST_EXACT = 1,       // nodes evaluated with the exact (culled) fold
+    PHI_ST_CERT = 2,        // exact evaluations at sub-brick centres (sign certificates)
+    PHI_ST_KEPT = 3,        // sum over bricks of the kept-primitive count
+    PHI_ST_UNCERT = 4,      // nodes in sub-bricks whose sign could not be certified
+    PHI_ST_BAND = 5,        // nodes with |phi_nn| under the band
+    PHI_ST_SIGN = 6,        // nodes whose network sign contradicts the certified sign
+    PHI_ST_BRICKS = 7,      // bricks
+    PHI_ST_COUNT = 8
+};
+
+// A lattice of points for the surrogate network: node (i, j, k) at x0 + (i, j, k) * h.
+struct Lattice {
+    float x0[3];
+    float h[3];
+    int n[3];
+};
+
+// The surrogate's inputs to the phi pass: network values on a coarse lattice of spacing f cells
+// whose node (I, J, K) sits on fine node (f I, f J, f K); nn == nullptr means exact everywhere.
+struct SurrogateArgs {
+    const float *nn;
+    int nc[3];
+    int f;
+    float band;    // nodes with |phi_nn| < band are evaluated exactly
+    double scale;  // bound on the magnitudes (coordinates, radii, distances) the fold rounds
+};
+
+inline void phi_brick_counts(const int *n, int *nb) {
+    for (int a = 0; a < 3; ++a) nb[a] = (n[a] + 1 + PHI_BRICK - 1) / PHI_BRICK;
+}
+
+template <typename T>
+void launch_cull(const GridT<T> &g, const Geo &geo, double scale, uint16_t *list, uint32_t *cnt,
+                 unsigned long long *stats, hipStream_t s);
+template <typename T>
+void launch_phi_bricks(T *phi, const GridT<T> &g, const Geo &geo, const uint16_t *list,
+                       const uint32_t *cnt, const SurrogateArgs &sa, unsigned long long *stats,
+                       hipStream_t s);
+void launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
+
+}  // namespace cc
+
+struct cc_mlp;
+namespace cc {
+// The network at every node of `lat` (n[0] n[1] n[2] values, i fastest) into `out`, on stream s.
+int launch_mlp_lattice(const cc_mlp *net, const Lattice &lat, float *out, hipStream_t s);
 }  // namespace cc

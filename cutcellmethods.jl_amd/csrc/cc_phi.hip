@@ -1,0 +1,307 @@
+// Nodal phi of a sphere union, brick by brick: the cull pass and the phi pass (culled exact, or
+// surrogate with certified signs and exact fallback).  No counterpart in the reference, whose
+// sample_nodes (src/marching_cubes/cache.jl:33-40) folds the whole body at every node; the folds
+// here are bitwise that fold (cc_device.h union_dist) restricted to primitives proven inert.
+//
+// Certification (surrogate mode).  Each brick is split into 4^3-node sub-bricks.  The exact phi at
+// a sub-brick's centre c, with every node of the sub-brick within rho of c, the cell diagonal diag
+// and the body 1-Lipschitz (spheres are, and the polynomial smooth minimum's gradient is a convex
+// combination of its arguments'), gives |phi(x)| > diag with the sign of phi(c) at every node x of
+// the sub-brick whenever |phi(c)| - rho - diag exceeds a rounding margin.  No cell with a corner in
+// such a sub-brick can be cut (its other corners lie within diag of x), so every corner of every
+// cut cell sits in an uncertified sub-brick, and all of those nodes are evaluated exactly.  Inside a
+// certified sub-brick a node keeps the network's value unless |phi_nn| is under the band or its
+// sign contradicts the certificate; then it is evaluated exactly too.  Every node's sign is thus
+// the exact one, `kind` is identical everywhere and every cut cell sees its exact corners.
+#include "cc_internal.h"
+
+namespace cc {
+namespace {
+
+constexpr int CULL_LDS = 512;  // primitives staged in LDS per brick (more are read from memory)
+constexpr int BRICK_NODES = PHI_BRICK * PHI_BRICK * PHI_BRICK;
+
+__device__ __forceinline__ void brick_of(int b, int nbx, int nby, int *bi) {
+    bi[0] = b % nbx;
+    bi[1] = (b / nbx) % nby;
+    bi[2] = b / (nbx * nby);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_cull(GridT<T> g, Geo geo, T delta, int nbx, int nby,
+                                              uint16_t *__restrict__ list, uint32_t *__restrict__ cnt,
+                                              unsigned long long *__restrict__ stats) {
+    __shared__ T wmin[4];
+    __shared__ uint32_t wcnt[4];
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int bi[3];
+    brick_of(b, nbx, nby, bi);
+    T lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int i0 = bi[a] * PHI_BRICK, i1 = min(i0 + PHI_BRICK - 1, g.n[a]);
+        lo[a] = g.x0[a] + T(i0) * g.d[a];
+        hi[a] = g.x0[a] + T(i1) * g.d[a];
+    }
+    const T k = T(geo.p[0]);
+    const T inf = T(INFINITY);
+    T run = inf;
+    uint32_t total = 0;
+    uint16_t *out = list + (size_t)b * geo.nprims;
+    for (int base = 0; base < geo.nprims; base += 256) {
+        const int i = base + t;
+        T lb = inf, ub = inf;
+        if (i < geo.nprims) {
+            const double *s = geo.prims + 4 * (size_t)i;
+            T sl = T(0), sh = T(0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const T c = T(s[a]);
+                T dl = lo[a] - c, dr = c - hi[a];
+                dl = dl > dr ? dl : dr;
+                dl = dl > T(0) ? dl : T(0);
+                const T el = fabs(c - lo[a]), er = fabs(c - hi[a]);
+                const T e = el > er ? el : er;
+                sl = sl + dl * dl;
+                sh = sh + e * e;
+            }
+            lb = sqrt(sl) - T(s[3]);
+            ub = sqrt(sh) - T(s[3]);
+        }
+        // inclusive prefix minimum of the upper bounds within the wave
+        T v = ub;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const T o = __shfl_up(v, off);
+            if (lane >= off) v = o < v ? o : v;
+        }
+        if (lane == 63) wmin[wave] = v;
+        T excl = __shfl_up(v, 1);
+        if (lane == 0) excl = inf;
+        __syncthreads();
+        T before = run;
+        for (int w = 0; w < wave; ++w) before = wmin[w] < before ? wmin[w] : before;
+        const T U = excl < before ? excl : before;  // min of the upper bounds of primitives < i
+        const bool keep = i < geo.nprims && (i == 0 || !(lb - U >= k + delta));
+        const unsigned long long m = __ballot(keep);
+        if (lane == 0) wcnt[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t off = total;
+        for (int w = 0; w < wave; ++w) off += wcnt[w];
+        if (keep) out[off + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)i;
+        total += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        for (int w = 0; w < 4; ++w) run = wmin[w] < run ? wmin[w] : run;
+        __syncthreads();
+    }
+    if (t == 0) {
+        cnt[b] = total;
+        atomicAdd(&stats[PHI_ST_KEPT], (unsigned long long)total);
+        atomicAdd(&stats[PHI_ST_BRICKS], 1ull);
+    }
+}
+
+// The sphere-union fold over a brick's kept primitives, in the order and with the arithmetic of
+// union_dist (cc_device.h).  Primitive 0 is always kept.
+template <typename T>
+__device__ __forceinline__ T culled_fold(const T *sph, uint32_t nk, const uint16_t *L, const Geo &geo,
+                                         T x, T y, T z) {
+    const T k = T(geo.p[0]);
+    T acc;
+    if (nk <= (uint32_t)CULL_LDS) {
+        acc = sphere_dist<T, 3>(sph[0], sph[1], sph[2], sph[3], x, y, z);
+        for (uint32_t e = 1; e < nk; ++e)
+            acc = smin_poly(acc, sphere_dist<T, 3>(sph[4 * e], sph[4 * e + 1], sph[4 * e + 2], sph[4 * e + 3],
+                                                   x, y, z),
+                            k);
+    } else {
+        const double *s = geo.prims + 4 * (size_t)L[0];
+        acc = sphere_dist<T, 3>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), x, y, z);
+        for (uint32_t e = 1; e < nk; ++e) {
+            s = geo.prims + 4 * (size_t)L[e];
+            acc = smin_poly(acc, sphere_dist<T, 3>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), x, y, z), k);
+        }
+    }
+    return acc;
+}
+
+__device__ __forceinline__ float nn_trilinear(const SurrogateArgs &sa, int gi, int gj, int gk) {
+    const int f = sa.f;
+    const int I = gi / f, J = gj / f, K = gk / f;
+    const float fx = (float)(gi - f * I) / (float)f, fy = (float)(gj - f * J) / (float)f,
+                fz = (float)(gk - f * K) / (float)f;
+    const size_t sx = 1, sy = (size_t)sa.nc[0], sz = (size_t)sa.nc[0] * sa.nc[1];
+    const float *p = sa.nn + I * sx + J * sy + K * sz;
+    const float c00 = p[0] + fx * (p[sx] - p[0]);
+    const float c10 = p[sy] + fx * (p[sy + sx] - p[sy]);
+    const float c01 = p[sz] + fx * (p[sz + sx] - p[sz]);
+    const float c11 = p[sz + sy] + fx * (p[sz + sy + sx] - p[sz + sy]);
+    const float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
+    return c0 + fz * (c1 - c0);
+}
+
+template <typename T, bool SURR>
+__global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T> g, Geo geo,
+                                                    const uint16_t *__restrict__ list,
+                                                    const uint32_t *__restrict__ cnt, SurrogateArgs sa,
+                                                    T delta_c, int nbx, int nby,
+                                                    unsigned long long *__restrict__ stats) {
+    __shared__ T sph[4 * CULL_LDS];
+    __shared__ uint16_t queue[BRICK_NODES];
+    __shared__ int8_t ssign[64];
+    __shared__ uint32_t qn, st_uncert, st_band, st_sign, st_nodes;
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63;
+    int bi[3];
+    brick_of(b, nbx, nby, bi);
+    const int i0 = bi[0] * PHI_BRICK, j0 = bi[1] * PHI_BRICK, k0 = bi[2] * PHI_BRICK;
+    const uint32_t nk = cnt[b];
+    const uint16_t *L = list + (size_t)b * geo.nprims;
+    const uint32_t nstage = nk < (uint32_t)CULL_LDS ? nk : (uint32_t)CULL_LDS;
+    for (uint32_t e = t; e < nstage; e += 256) {
+        const double *s = geo.prims + 4 * (size_t)L[e];
+        sph[4 * e] = T(s[0]);
+        sph[4 * e + 1] = T(s[1]);
+        sph[4 * e + 2] = T(s[2]);
+        sph[4 * e + 3] = T(s[3]);
+    }
+    if (t == 0) qn = st_uncert = st_band = st_sign = st_nodes = 0;
+    __syncthreads();
+    const size_t m0 = (size_t)g.n[0] + 1, m1 = (size_t)g.n[1] + 1;
+    if (SURR && t < 64) {
+        // sign certificate of sub-brick t
+        const int s3[3] = {t & 3, (t >> 2) & 3, t >> 4};
+        const int base[3] = {i0, j0, k0};
+        bool valid = true;
+        T c[3], e2 = T(0), dg = T(0);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int slo = base[a] + PHI_SUB * s3[a];
+            const int shi = min(slo + PHI_SUB - 1, g.n[a]);
+            valid = valid && slo <= g.n[a];
+            const T xl = g.x0[a] + T(slo) * g.d[a], xh = g.x0[a] + T(shi) * g.d[a];
+            c[a] = (xl + xh) / T(2);
+            const T ea = fabs(c[a] - xl) > fabs(xh - c[a]) ? fabs(c[a] - xl) : fabs(xh - c[a]);
+            e2 = e2 + ea * ea;
+            dg = dg + g.d[a] * g.d[a];
+        }
+        int8_t sg = 0;
+        if (valid) {
+            const T pc = culled_fold<T>(sph, nk, L, geo, c[0], c[1], c[2]);
+            if (fabs(pc) - sqrt(e2) - sqrt(dg) > delta_c) sg = pc > T(0) ? 1 : -1;
+            atomicAdd(&stats[PHI_ST_CERT], 1ull);
+        }
+        ssign[t] = sg;
+    }
+    if (SURR) __syncthreads();
+    // classify: nodes the surrogate may keep are written now; the rest are queued for the fold
+    uint32_t n_unc = 0, n_band = 0, n_sign = 0, n_nodes = 0;
+    for (int q = 0; q < BRICK_NODES / 256; ++q) {
+        const int idx = t + 256 * q;
+        const int li = idx & 15, lj = (idx >> 4) & 15, lk = idx >> 8;
+        const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
+        const bool in = gi <= g.n[0] && gj <= g.n[1] && gk <= g.n[2];
+        bool push = in;
+        if (SURR && in) {
+            const int8_t sg = ssign[(li >> 2) + 4 * (lj >> 2) + 16 * (lk >> 2)];
+            if (sg == 0) {
+                ++n_unc;
+            } else {
+                const float v = nn_trilinear(sa, gi, gj, gk);
+                if (fabsf(v) < sa.band) {
+                    ++n_band;
+                } else if ((v > 0.0f) != (sg > 0)) {
+                    ++n_sign;
+                } else {
+                    phi[gi + m0 * (gj + m1 * gk)] = T(v);
+                    push = false;
+                }
+            }
+        }
+        n_nodes += in ? 1u : 0u;
+        const unsigned long long m = __ballot(push);
+        uint32_t qb = 0;
+        if (lane == 0 && m) qb = atomicAdd(&qn, (uint32_t)__popcll(m));
+        qb = __shfl(qb, 0);
+        if (push) queue[qb + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)idx;
+    }
+    if (SURR) {
+        atomicAdd(&st_uncert, n_unc);
+        atomicAdd(&st_band, n_band);
+        atomicAdd(&st_sign, n_sign);
+    }
+    atomicAdd(&st_nodes, n_nodes);
+    __syncthreads();
+    const uint32_t nq = qn;
+    for (uint32_t e = t; e < nq; e += 256) {
+        const int idx = queue[e];
+        const int gi = i0 + (idx & 15), gj = j0 + ((idx >> 4) & 15), gk = k0 + (idx >> 8);
+        const T x = g.x0[0] + T(gi) * g.d[0], y = g.x0[1] + T(gj) * g.d[1], z = g.x0[2] + T(gk) * g.d[2];
+        phi[gi + m0 * (gj + m1 * gk)] = culled_fold<T>(sph, nk, L, geo, x, y, z);
+    }
+    if (t == 0) {
+        atomicAdd(&stats[PHI_ST_NODES], (unsigned long long)st_nodes);
+        atomicAdd(&stats[PHI_ST_EXACT], (unsigned long long)nq);
+        if (SURR) {
+            atomicAdd(&stats[PHI_ST_UNCERT], (unsigned long long)st_uncert);
+            atomicAdd(&stats[PHI_ST_BAND], (unsigned long long)st_band);
+            atomicAdd(&stats[PHI_ST_SIGN], (unsigned long long)st_sign);
+        }
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+}  // namespace
+
+void launch_zero_u64(unsigned long long *p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
+// Rounding margins.  Every quantity the fold or the bounds round (coordinate differences, squared
+// norms, square roots, radii, smooth minima) is within a few units in the last place of `scale`;
+// the cull margin covers three such errors, the certificate margin the fold's accumulated rounding
+// over every primitive as well.
+template <typename T> static T cull_margin(double scale) { return T(256.0 * Eps<T>::v * scale); }
+template <typename T> static T cert_margin(double scale, int nprims) {
+    return T((4.0 * nprims + 256.0) * Eps<T>::v * scale);
+}
+
+template <typename T>
+void launch_cull(const GridT<T> &g, const Geo &geo, double scale, uint16_t *list, uint32_t *cnt,
+                 unsigned long long *stats, hipStream_t s) {
+    int nb[3];
+    phi_brick_counts(g.n, nb);
+    hipLaunchKernelGGL((k_cull<T>), dim3(nb[0] * nb[1] * nb[2]), dim3(256), 0, s, g, geo,
+                       cull_margin<T>(scale), nb[0], nb[1], list, cnt, stats);
+}
+
+template <typename T>
+void launch_phi_bricks(T *phi, const GridT<T> &g, const Geo &geo, const uint16_t *list,
+                       const uint32_t *cnt, const SurrogateArgs &sa, unsigned long long *stats,
+                       hipStream_t s) {
+    int nb[3];
+    phi_brick_counts(g.n, nb);
+    const dim3 grid(nb[0] * nb[1] * nb[2]);
+    const T dc = cert_margin<T>(sa.scale, geo.nprims);
+    if (sa.nn)
+        hipLaunchKernelGGL((k_phi_bricks<T, true>), grid, dim3(256), 0, s, phi, g, geo, list, cnt, sa, dc,
+                           nb[0], nb[1], stats);
+    else
+        hipLaunchKernelGGL((k_phi_bricks<T, false>), grid, dim3(256), 0, s, phi, g, geo, list, cnt, sa, dc,
+                           nb[0], nb[1], stats);
+}
+
+template void launch_cull<double>(const GridT<double> &, const Geo &, double, uint16_t *, uint32_t *,
+                                  unsigned long long *, hipStream_t);
+template void launch_cull<float>(const GridT<float> &, const Geo &, double, uint16_t *, uint32_t *,
+                                 unsigned long long *, hipStream_t);
+template void launch_phi_bricks<double>(double *, const GridT<double> &, const Geo &, const uint16_t *,
+                                        const uint32_t *, const SurrogateArgs &, unsigned long long *,
+                                        hipStream_t);
+template void launch_phi_bricks<float>(float *, const GridT<float> &, const Geo &, const uint16_t *,
+                                       const uint32_t *, const SurrogateArgs &, unsigned long long *,
+                                       hipStream_t);
+
+}  // namespace cc

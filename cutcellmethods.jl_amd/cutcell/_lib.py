@@ -18,6 +18,8 @@ CC_ERR_OUT_OF_MEMORY = 5
 
 CC_MARCHING_SQUARES, CC_MARCHING_CUBES, CC_PLIC = 1, 2, 3
 CC_FLOAT64, CC_FLOAT32 = 0, 1
+CC_PHI_NAIVE, CC_PHI_CULLED, CC_PHI_SURROGATE = 0, 1, 2
+CC_PHI_STATS = 8
 
 FIELDS = {
     "kind": 0, "ambiguous": 1, "volume_fraction": 2, "centroid": 3, "face_fraction": 4,
@@ -28,7 +30,10 @@ FIELDS = {
 EXPORTS = ("cc_cache_create", "cc_cache_destroy", "cc_update", "cc_update_from_phi",
            "cc_update_async", "cc_synchronize", "cc_column", "cc_copy_to_host", "cc_cut_count",
            "cc_stream", "cc_set_tuning", "cc_get_tuning", "cc_set_profiling", "cc_kernel_times", "cc_generate_mesh_ms",
-           "cc_probe_bandwidth", "cc_probe_write", "cc_last_error", "cc_version")
+           "cc_probe_bandwidth", "cc_probe_write", "cc_last_error", "cc_version",
+           "cc_mlp_create", "cc_mlp_destroy", "cc_mlp_num_params", "cc_mlp_get_params",
+           "cc_mlp_set_params", "cc_mlp_eval", "cc_mlp_loss_grad", "cc_mlp_train",
+           "cc_set_phi_source", "cc_phi_stats")
 
 
 class CCGeometry(ctypes.Structure):
@@ -90,6 +95,20 @@ def lib():
                                          ctypes.POINTER(ctypes.c_float)]
         L.cc_probe_write.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_float)]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.cc_mlp_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.cc_mlp_destroy.argtypes = [vp]
+        L.cc_mlp_num_params.argtypes = [vp, i64p, i64p]
+        L.cc_mlp_get_params.argtypes = [vp, vp, vp]
+        L.cc_mlp_set_params.argtypes = [vp, vp]
+        L.cc_mlp_eval.argtypes = [vp, vp, ctypes.c_int64, vp]
+        L.cc_mlp_loss_grad.argtypes = [vp, vp, vp, ctypes.c_int64, fp, vp]
+        L.cc_mlp_train.argtypes = [vp, ctypes.POINTER(CCGeometry), ctypes.c_int64, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_float, ctypes.c_uint64, vp]
+        L.cc_set_phi_source.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double]
+        L.cc_phi_stats.argtypes = [vp, i64p]
         L.cc_last_error.restype = ctypes.c_char_p
         L.cc_version.restype = ctypes.c_char_p
         for name in EXPORTS:

@@ -159,6 +159,26 @@ class Cache:
         check(lib().cc_get_tuning(self._h, ctypes.byref(v), ctypes.byref(k), ctypes.byref(nt)))
         return v.value, k.value, bool(nt.value)
 
+    PHI_STAT_NAMES = ("nodes", "exact", "cert_evals", "kept_sum", "uncertified", "band",
+                      "sign_fail", "bricks")
+
+    def set_phi_source(self, source, surrogate=None, coarse=4, band_cells=3.0):
+        """How a sphere union's nodal phi is produced: ``"naive"`` (every node folds every
+        primitive), ``"culled"`` (each brick folds only the primitives it cannot exclude; bitwise
+        the same) or ``"surrogate"`` (the network on a lattice of spacing ``coarse`` cells, exact
+        wherever the sign is not certified, under ``band_cells`` cells, or contradicted)."""
+        code = {"naive": _lib.CC_PHI_NAIVE, "culled": _lib.CC_PHI_CULLED,
+                "surrogate": _lib.CC_PHI_SURROGATE}[source]
+        net = surrogate.handle if surrogate is not None else None
+        check(lib().cc_set_phi_source(self._h, code, net, int(coarse), float(band_cells)))
+        self._surrogate = surrogate  # keep the network alive while the cache uses it
+
+    def phi_stats(self):
+        """Counters of the last culled/surrogate update."""
+        a = (ctypes.c_int64 * _lib.CC_PHI_STATS)()
+        check(lib().cc_phi_stats(self._h, a))
+        return dict(zip(self.PHI_STAT_NAMES, list(a)))
+
     def set_profiling(self, on=True):
         check(lib().cc_set_profiling(self._h, 1 if on else 0))
 

@@ -55,7 +55,7 @@ extern "C" {
 #define CC_FIELD_IS_VALID 10  /* PLIC */
 #define CC_NUM_FIELDS 11
 
-/* ---- geometry (POD form of an SDFLibrary body) -------------------------------- */
+/* ---- geometry (POD form of an SDFLibrary body; the library's own bodies) --------------------- */
 #define CC_GEO_EMPTY 0        /* p[0] constant value; EmptyGeo(one(T)) */
 #define CC_GEO_SPHERE 1       /* p[0..2] centre, p[3] radius (SDFSphere; SDFCircle in 2D) */
 #define CC_GEO_TORUS 2        /* p[0..2] centre, p[3] major R, p[4] minor r, p[5..13] rotation (row-major) */
@@ -163,6 +163,52 @@ int cc_batch_step(cc_batch *batch, const cc_geometry *geos, const cc_grid *grids
 int cc_batch_set_kz(cc_batch *batch, int body, int kz);
 /* Work items of the two passes of the captured graph. */
 int cc_batch_work_items(cc_batch *batch, int64_t *pass_a, int64_t *pass_b);
+
+/* ---- the learned surrogate (no counterpart in the reference, whose bodies are SDFLibrary
+ * closures evaluated exactly at every node) ---------------------------------------------------------
+ * A Fourier-feature MLP for an expensive body's signed distance: 3 -> 2*nfreq features
+ * [sin(2 pi B x), cos(2 pi B x)] (B: nfreq x 3, N(0, sigma^2)) -> depth x width SiLU -> 1.  FP32
+ * master parameters; forward and backward run in BF16 on the matrix cores with FP32 accumulation.
+ * Parameters are packed W_0 (2 nfreq x width, row-major), W_1..W_{depth-1} (width x width), b_0..,
+ * w_out (width), b_out.  width must be 256, depth 1..8, nfreq 32/64/96/128. */
+typedef struct cc_mlp cc_mlp;
+int cc_mlp_create(int nfreq, int width, int depth, float sigma, uint64_t seed, cc_mlp **out);
+int cc_mlp_destroy(cc_mlp *net);
+/* Parameter count and 3*nfreq (the size of B). */
+int cc_mlp_num_params(cc_mlp *net, int64_t *nparams, int64_t *nfreq3);
+/* Copies the packed parameters and B to host memory (either pointer may be NULL). */
+int cc_mlp_get_params(cc_mlp *net, float *params, float *ffB);
+int cc_mlp_set_params(cc_mlp *net, const float *params);
+/* The network at n host points (x: n x 3) into y (host, n floats). */
+int cc_mlp_eval(cc_mlp *net, const float *x, int64_t n, float *y);
+/* Mean squared error over n host samples (n a multiple of 64) and its gradient (nparams floats). */
+int cc_mlp_loss_grad(cc_mlp *net, const float *x, const float *y, int64_t n, float *loss, float *grad);
+/* Fits the network to exact samples of `geo` (half uniform in [lo, hi]^3, half within `band` of
+ * the surface), Adam with a cosine-decayed rate; `history` (optional) gets each epoch's mean loss. */
+int cc_mlp_train(cc_mlp *net, const cc_geometry *geo, int64_t nsamples, float lo, float hi, float band,
+                 int epochs, int batch, float lr, uint64_t seed, float *history);
+
+/* How cc_update produces a sphere union's nodal phi in a marching-cubes cache (sample_nodes,
+ * src/marching_cubes/cache.jl:33-40, folds the whole body at every node):
+ *   CC_PHI_NAIVE     -- every node folds every primitive (the plain exact path; the default);
+ *   CC_PHI_CULLED    -- each 16^3-node brick folds only the primitives a bound over its box cannot
+ *                       prove inert: bitwise the same phi;
+ *   CC_PHI_SURROGATE -- the network `net` on a lattice of spacing `coarse` cells, interpolated;
+ *                       exact (culled) wherever a Lipschitz sign certificate is missing, where
+ *                       |phi_nn| < band_cells * h, or where the network's sign contradicts the
+ *                       certificate.  kind and every moment are bitwise the exact path's; phi
+ *                       differs only at nodes of certified sub-bricks, none of which is a corner
+ *                       of a cut cell.
+ * Other bodies always take the plain path.  The network must outlive its use by the cache. */
+#define CC_PHI_NAIVE 0
+#define CC_PHI_CULLED 1
+#define CC_PHI_SURROGATE 2
+int cc_set_phi_source(cc_cache *cache, int source, cc_mlp *net, int coarse, double band_cells);
+/* Counters of the last culled/surrogate update (blocks until it is done), CC_PHI_STATS entries:
+ * nodes, exact node evaluations, certificate evaluations, sum over bricks of kept primitives,
+ * nodes in uncertified sub-bricks, nodes under the band, nodes with a contradicted sign, bricks. */
+#define CC_PHI_STATS 8
+int cc_phi_stats(cc_cache *cache, int64_t *stats);
 
 /* Measurement aid (no counterpart in the reference): the best time over `iters` of a write-only
  * fill of `nbytes` and of a copy of `nbytes` (read + write) on the current device, with 16-byte
