@@ -41,6 +41,8 @@ def parse():
                     help="stream-pass launch shape: variant,kz,nontemporal")
     ap.add_argument("--quick", action="store_true",
                     help="only the headline workload (skip the secondary workloads)")
+    ap.add_argument("--only", default=None, choices=["csg", "batched", "secondary"],
+                    help="run one secondary workload alone and print its JSON")
     ap.add_argument("--sweep", action="store_true",
                     help="also time every stream-pass launch shape and report them")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "mc_stream_512_traffic.json"))
@@ -153,6 +155,126 @@ def fp32_error(grid, geo, cache64):
             "error_vs_fp64_on_16_layers": err}
 
 
+MLP_FLOP_PER_POINT = 2 * (128 * 256 + 3 * 256 * 256 + 256)  # 64 freqs -> 4 x 256 -> 1, one forward
+
+
+def _compare_caches(exact, sur, grid, layers=16):
+    """Slab by slab on the host: every CutCellData column bitwise equal, phi equal at every corner
+    of every cut cell, the sign of phi equal everywhere; returns counts."""
+    n0, n1, n2 = grid.n
+    m0, m1 = n0 + 1, n1 + 1
+    res = {"columns_bitwise_equal": True, "cut_corner_phi_equal": True, "sign_equal": True,
+           "phi_nodes_differing": 0, "cut_cells": 0}
+    cols = ("kind", "ambiguous", "volume_fraction", "centroid", "face_fraction",
+            "face_centroid_local", "interface_centroid")
+    for k0 in range(0, n2, layers):
+        k1 = min(n2, k0 + layers)
+        lo, hi = n0 * n1 * k0, n0 * n1 * k1
+        for name in cols:
+            a, b = exact.read(name, lo, hi), sur.read(name, lo, hi)
+            if not np.array_equal(a.view(np.uint8), b.view(np.uint8)):
+                res["columns_bitwise_equal"] = False
+        kind = exact.read("kind", lo, hi)
+        pe = exact.read("phi", m0 * m1 * k0, m0 * m1 * (k1 + 1))
+        ps = sur.read("phi", m0 * m1 * k0, m0 * m1 * (k1 + 1))
+        if not np.array_equal(pe > 0, ps > 0):
+            res["sign_equal"] = False
+        last = k1 == n2
+        body = pe.size if last else m0 * m1 * (k1 - k0)  # count each node layer once
+        res["phi_nodes_differing"] += int((pe[:body].view(np.uint64) != ps[:body].view(np.uint64)).sum())
+        cut = np.flatnonzero(kind == 2)
+        res["cut_cells"] += int(cut.size)
+        i, j, k = cut % n0, (cut // n0) % n1, cut // (n0 * n1)
+        for dk in (0, 1):
+            for dj in (0, 1):
+                for di in (0, 1):
+                    idx = (i + di) + m0 * ((j + dj) + m1 * (k + dk))
+                    if not np.array_equal(pe[idx].view(np.uint64), ps[idx].view(np.uint64)):
+                        res["cut_corner_phi_equal"] = False
+    return res
+
+
+def csg(seed, n=512, train_epochs=24, train_samples=1 << 22):
+    """The composite body (smooth union, k = 0.02, of 512 spheres, centres U[-0.7, 0.7]^3, radii
+    U[0.05, 0.15]) at n^3, moving 0.5 cell per step: the exact path (every node folds every sphere),
+    the culled exact path (bitwise the same phi) and the learned surrogate (network trained here on
+    exact samples; exact fallback where it cannot be trusted).  Update times are medians of
+    blocking updates; the network's training time is reported on its own."""
+    from cutcell import CartesianIsoGrid, MarchingCubesCutCell, Surrogate, allocate_cache, update_cache
+    from cutcell.workloads import csg_spheres
+    body = csg_spheres(seed, 512)
+    grid = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=n)
+    d = grid.d[0]
+
+    def at(step):
+        return grid.moved((-0.5 * d * step, 0.0, 0.0))
+
+    out = {"body": "smooth union k=0.02 of 512 spheres, centres U[-0.7,0.7]^3, radii U[0.05,0.15]",
+           "cells": grid.ncells}
+    net = Surrogate(nfreq=64, width=256, depth=4, sigma=8.0, seed=seed & 0xFFFF)
+    t = time.perf_counter()
+    hist = net.train(body, nsamples=train_samples, lo=-1.0, hi=1.0, band=0.05, epochs=train_epochs,
+                     batch=16384, lr=5e-3, seed=seed)
+    tt = time.perf_counter() - t
+    tflop = 3 * MLP_FLOP_PER_POINT * train_samples * train_epochs / 1e12
+    out["training"] = {"seconds": round(tt, 2), "samples": train_samples, "epochs": train_epochs,
+                       "batch": 16384, "final_loss": float(hist[-1]), "first_epoch_loss": float(hist[0]),
+                       "approx_tflop": round(tflop, 2), "approx_tflops": round(tflop / tt, 1)}
+    cache = allocate_cache(grid, MarchingCubesCutCell)
+    cache.set_profiling(True)
+    res = {}
+    for src, reps in (("naive", 3), ("culled", 6)):
+        cache.set_phi_source(src)
+        update_cache(cache, body, at(0))
+        ts, kts = [], []
+        for s in range(reps):
+            t = time.perf_counter()
+            update_cache(cache, body, at(1 + s))
+            ts.append(time.perf_counter() - t)
+            kts.append(dict(cache.kernel_times()))
+        res[src] = {"update_ms": round(1e3 * float(np.median(ts)), 3),
+                    "kernel_ms": {k: round(float(np.median([kt[k] for kt in kts])), 4) for k in kts[0]}}
+        if src == "culled":
+            st = cache.phi_stats()
+            res[src]["mean_kept_spheres_per_brick"] = round(st["kept_sum"] / st["bricks"], 2)
+    sur = allocate_cache(grid, MarchingCubesCutCell)
+    sur.set_profiling(True)
+    sur.set_phi_source("surrogate", net, coarse=8, band_cells=3.0)
+    update_cache(sur, body, at(0))
+    ts, kts = [], []
+    for s in range(8):
+        t = time.perf_counter()
+        update_cache(sur, body, at(1 + s))
+        ts.append(time.perf_counter() - t)
+        kts.append(dict(sur.kernel_times()))
+    st = sur.phi_stats()
+    km = {k: round(float(np.median([kt[k] for kt in kts])), 4) for k in kts[0]}
+    nc = n // 8 + 2
+    lat_points = nc ** 3
+    nn_tflops = lat_points * MLP_FLOP_PER_POINT / (km["nn_lattice"] * 1e-3) / 1e12
+    res["surrogate"] = {
+        "update_ms": round(1e3 * float(np.median(ts)), 3), "kernel_ms": km,
+        "coarse_lattice": f"{nc}^3 (8 cells)", "band": "3h",
+        "exact_node_fraction": round(st["exact"] / st["nodes"], 4),
+        "uncertified_fraction": round(st["uncertified"] / st["nodes"], 4),
+        "band_fraction": round(st["band"] / st["nodes"], 4),
+        "sign_contradicted_fraction": round(st["sign_fail"] / st["nodes"], 6),
+        "certificate_evals": st["cert_evals"],
+        "nn_lattice_tflops": round(nn_tflops, 1),
+        "mfma_utilisation_bf16": round(nn_tflops / 2500.0, 4)}
+    # identical results: the same grid position through the exact path and the surrogate
+    cache.set_phi_source("naive")
+    update_cache(cache, body, at(20))
+    update_cache(sur, body, at(20))
+    res["agreement"] = _compare_caches(cache, sur, grid)
+    res["speedup_vs_naive"] = round(res["naive"]["update_ms"] / res["surrogate"]["update_ms"], 2)
+    res["speedup_vs_culled"] = round(res["culled"]["update_ms"] / res["surrogate"]["update_ms"], 3)
+    out.update(res)
+    del cache, sur
+    net.close()
+    return out
+
+
 def batched(seed, steps=32):
     from cutcell import MarchingCubesCutCell, PLICCutCell, allocate_cache
     from cutcell.autotune import tune_batch
@@ -196,6 +318,11 @@ def main():
         sys.exit(2)
     from cutcell import MarchingCubesCutCell, allocate_cache, update_cache
 
+    if a.only:
+        fn = {"csg": lambda: csg(a.seed), "batched": lambda: batched(a.seed),
+              "secondary": lambda: secondary(a.seed)}[a.only]
+        print(json.dumps({a.only: fn()}))
+        return
     grid, geo, centre = workload(a.n, a.seed)
     cache = allocate_cache(grid, MarchingCubesCutCell)
     cache.set_profiling(True)
@@ -281,6 +408,7 @@ def main():
         del cache
         out.update(secondary(a.seed))
         out["batched_64_bodies"] = batched(a.seed)
+        out["csg_512_spheres"] = csg(a.seed)
     print(json.dumps(out))
 
 

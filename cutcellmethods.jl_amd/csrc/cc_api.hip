@@ -105,7 +105,7 @@ struct cc_cache {
     // how a sphere union's nodal phi is produced (marching cubes)
     int phi_source = CC_PHI_NAIVE;
     cc_mlp *net = nullptr;
-    int coarse = 4;
+    int coarse = 8;
     double band_cells = 3.0;
     uint16_t *cull_list = nullptr;
     int64_t cull_cap = 0;  // entries
@@ -113,7 +113,9 @@ struct cc_cache {
     int64_t cnt_cap = 0;
     float *nn = nullptr;
     int64_t nn_cap = 0;
-    unsigned long long *phi_stats = nullptr;
+    uint32_t *bstat = nullptr;  // per-brick counters of the last culled/surrogate update
+    int64_t bstat_cap = 0;
+    int64_t phi_nbricks = 0;
 };
 
 namespace {
@@ -231,8 +233,9 @@ template <typename T> int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::
     const int64_t nbrick = (int64_t)nb[0] * nb[1] * nb[2];
     int rc = ensure(&c->cull_list, &c->cull_cap, nbrick * geo.nprims);
     if (!rc) rc = ensure(&c->cull_cnt, &c->cnt_cap, nbrick);
+    if (!rc) rc = ensure(&c->bstat, &c->bstat_cap, nbrick * cc::PHI_BSTAT);
     if (rc) return rc;
-    if (!c->phi_stats) HIPC(hipMalloc(&c->phi_stats, sizeof(unsigned long long) * cc::PHI_ST_COUNT));
+    c->phi_nbricks = nbrick;
     // magnitude bound for the rounding margins: coordinates of primitives and lattice, radii
     double scale = 1.0, pm = 0.0, gm = 0.0;
     for (int64_t i = 0; i < geo.nprims; ++i) {
@@ -246,7 +249,6 @@ template <typename T> int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::
     scale += 2.0 * (pm + gm) + std::fabs(geo.p[0]);
     cc::SurrogateArgs sa{};
     sa.scale = scale;
-    cc::launch_zero_u64(c->phi_stats, cc::PHI_ST_COUNT, c->stream);
     if (c->phi_source == CC_PHI_SURROGATE) {
         cc::Lattice lat{};
         const int f = c->coarse;
@@ -266,9 +268,9 @@ template <typename T> int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::
         const double h = std::fmax((double)gt.d[0], std::fmax((double)gt.d[1], (double)gt.d[2]));
         sa.band = (float)(c->band_cells * h);
     }
-    cc::launch_cull<T>(gt, geo, scale, c->cull_list, c->cull_cnt, c->phi_stats, c->stream);
+    cc::launch_cull<T>(gt, geo, scale, c->cull_list, c->cull_cnt, c->stream);
     mark(c, "cull");
-    cc::launch_phi_bricks<T>(phi, gt, geo, c->cull_list, c->cull_cnt, sa, c->phi_stats, c->stream);
+    cc::launch_phi_bricks<T>(phi, gt, geo, c->cull_list, c->cull_cnt, sa, c->bstat, c->stream);
     mark(c, "phi_bricks");
     return CC_OK;
 }
@@ -359,7 +361,7 @@ void free_all(cc_cache *c) {
     if (c->cull_list) hipFree(c->cull_list);
     if (c->cull_cnt) hipFree(c->cull_cnt);
     if (c->nn) hipFree(c->nn);
-    if (c->phi_stats) hipFree(c->phi_stats);
+    if (c->bstat) hipFree(c->bstat);
     for (auto &e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -609,13 +611,26 @@ int cc_set_phi_source(cc_cache *c, int source, cc_mlp *net, int coarse, double b
 
 int cc_phi_stats(cc_cache *c, int64_t *stats) {
     if (!c || !stats) return fail(CC_ERR_ARGUMENT, "NULL argument");
-    unsigned long long h[cc::PHI_ST_COUNT] = {};
-    if (c->phi_stats) {
-        HIPC(hipSetDevice(c->device));
-        HIPC(hipStreamSynchronize(c->stream));
-        HIPC(hipMemcpy(h, c->phi_stats, sizeof h, hipMemcpyDeviceToHost));
+    for (int i = 0; i < CC_PHI_STATS; ++i) stats[i] = 0;
+    if (!c->phi_nbricks) return CC_OK;
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->stream));
+    const int64_t nb = c->phi_nbricks;
+    std::vector<uint32_t> cnt(nb), bs(nb * cc::PHI_BSTAT);
+    HIPC(hipMemcpy(cnt.data(), c->cull_cnt, 4 * nb, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(bs.data(), c->bstat, 4 * nb * cc::PHI_BSTAT, hipMemcpyDeviceToHost));
+    // nodes, exact, certificate evaluations, kept primitives, uncertified, band, contradicted, bricks
+    for (int64_t b = 0; b < nb; ++b) {
+        const uint32_t *v = bs.data() + b * cc::PHI_BSTAT;
+        stats[0] += v[0];
+        stats[1] += v[1];
+        stats[2] += v[2];
+        stats[3] += cnt[b];
+        stats[4] += v[3];
+        stats[5] += v[4];
+        stats[6] += v[5];
     }
-    for (int i = 0; i < cc::PHI_ST_COUNT; ++i) stats[i] = (int64_t)h[i];
+    stats[7] = nb;
     return CC_OK;
 }
 

@@ -77,17 +77,11 @@ void mc_tile_shape(const int *n, uint32_t kz, uint32_t *W, uint32_t *R, uint32_t
 constexpr int PHI_BRICK = 16;  // nodes per brick edge
 constexpr int PHI_SUB = 4;     // nodes per certification sub-brick edge
 
-enum PhiStat {
-    PHI_ST_NODES = 0,       // nodes written
-    PHI_ST_EXACT = 1,       // nodes evaluated with the exact (culled) fold
-    PHI_ST_CERT = 2,        // exact evaluations at sub-brick centres (sign certificates)
-    PHI_ST_KEPT = 3,        // sum over bricks of the kept-primitive count
-    PHI_ST_UNCERT = 4,      // nodes in sub-bricks whose sign could not be certified
-    PHI_ST_BAND = 5,        // nodes with |phi_nn| under the band
-    PHI_ST_SIGN = 6,        // nodes whose network sign contradicts the certified sign
-    PHI_ST_BRICKS = 7,      // bricks
-    PHI_ST_COUNT = 8
-};
+// Per-brick counters of the phi pass, PHI_BSTAT uint32 per brick: nodes written, nodes evaluated
+// with the exact (culled) fold, exact evaluations at sub-brick centres (sign certificates), nodes
+// in sub-bricks whose sign could not be certified, nodes with |phi_nn| under the band, nodes whose
+// network sign contradicts the certified sign.
+constexpr int PHI_BSTAT = 8;
 
 // A lattice of points for the surrogate network: node (i, j, k) at x0 + (i, j, k) * h.
 struct Lattice {
@@ -112,12 +106,10 @@ inline void phi_brick_counts(const int *n, int *nb) {
 
 template <typename T>
 void launch_cull(const GridT<T> &g, const Geo &geo, double scale, uint16_t *list, uint32_t *cnt,
-                 unsigned long long *stats, hipStream_t s);
+                 hipStream_t s);
 template <typename T>
 void launch_phi_bricks(T *phi, const GridT<T> &g, const Geo &geo, const uint16_t *list,
-                       const uint32_t *cnt, const SurrogateArgs &sa, unsigned long long *stats,
-                       hipStream_t s);
-void launch_zero_u64(unsigned long long *p, int n, hipStream_t s);
+                       const uint32_t *cnt, const SurrogateArgs &sa, uint32_t *bstat, hipStream_t s);
 
 }  // namespace cc
 

@@ -521,10 +521,48 @@ template <typename T> struct McBody {
 
 constexpr int MARCH_NODES = 2 * (MC_TILE + 1);  // max (R+1)(W+1) over R*W <= MC_TILE
 
+// Cut cells of up to CUT_STEPS z-steps of one work item, held as the waves' ballot masks and
+// appended to the work list with ONE atomic per flush: a single global counter taking one atomic
+// per wave and step serialises in L2 once bodies have millions of cut cells.
+constexpr int CUT_STEPS = 32;
+struct CutMasks {
+    uint64_t m[CUT_STEPS][MC_TILE / 64];   // ballot of each wave at each step
+    uint32_t off[CUT_STEPS][MC_TILE / 64]; // exclusive prefix of their popcounts
+    uint32_t c0[CUT_STEPS];                // first cell of each step's tile
+    uint32_t base;
+};
+
 template <typename T> struct MarchLDS {
     TileLDS<T> s;
     T rows[2][MARCH_NODES];  // [layer parity][(R+1) x (W+1) nodes, row-major]
+    CutMasks cm;
 };
+
+// Appends the cut cells recorded in cm (steps [0, ns)) to the work list.  Called by every thread
+// of the workgroup after a barrier; ends with one.  Cell of thread t at step s: c0[s] + (t / wc) n0
+// + t % wc.
+__device__ __forceinline__ void flush_cuts(CutMasks &cm, int ns, int wc, uint32_t n0, uint32_t *cutlist,
+                                           uint32_t *counter) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    constexpr int NW = MC_TILE / 64;
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int q = 0; q < ns; ++q)
+            for (int w = 0; w < NW; ++w) {
+                cm.off[q][w] = acc;
+                acc += (uint32_t)__popcll(cm.m[q][w]);
+            }
+        cm.base = acc ? atomicAdd(counter, acc) : 0u;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t rel = (uint32_t)(t / wc) * n0 + (uint32_t)(t % wc);
+    for (int q = 0; q < ns; ++q) {
+        const uint64_t m = cm.m[q][wave];
+        if ((m >> lane) & 1ull) cutlist[cm.base + cm.off[q][wave] + __popcll(m & below)] = cm.c0[q] + rel;
+    }
+    __syncthreads();
+}
 
 // One work item of the z-marching stream pass: the R x W cells of rows [j0, j0+R) and columns
 // [i0, i0+W), marched over the layers [k0, k0+kz).  The node rows of the layer below stay in LDS, so
@@ -563,7 +601,7 @@ __device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L) {
         }
     };
     fill(0, k0);
-    int par = 0;
+    int par = 0, ns = 0;
     for (uint32_t k = k0; k < k1; ++k) {
         fill(par ^ 1, k + 1);
         __syncthreads();
@@ -580,10 +618,17 @@ __device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L) {
             for (int p = 0; p < 8; ++p) lut |= (ph[p] >= T(0) ? 1 : 0) << p;
             cut = stage_cell(L.s, t, lut, i0 + ii, j0 + r, k, g);
         }
-        append_cut(cut, c0 + (uint32_t)(t / wc) * n0 + (uint32_t)(t % wc), B.cutlist, B.counter);
+        const uint64_t m = __ballot(cut);
+        if ((t & 63) == 0) L.cm.m[ns][t >> 6] = m;
+        if (t == 0) L.cm.c0[ns] = c0;
+        ++ns;
         __syncthreads();
         stream_tile<T, NT>(B.cols, c0, tc, L.s);
         __syncthreads();
+        if (ns == CUT_STEPS || k + 1 == k1) {
+            flush_cuts(L.cm, ns, wc, n0, B.cutlist, B.counter);
+            ns = 0;
+        }
         par ^= 1;
     }
 }

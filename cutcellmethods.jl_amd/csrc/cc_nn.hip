@@ -67,6 +67,7 @@ struct MlpDev {
     const float *bias[8];
     const float *wout;   // [width]
     const float *bout;   // [1], read on the device
+    float in_lo, in_scale;  // input map u = (x - in_lo) * in_scale applied before the features
     int F, width, depth;
 };
 
@@ -107,8 +108,10 @@ __global__ __launch_bounds__(256) void k_mlp_forward(MlpDev m, const float *__re
                 xq[1] = x[3 * (size_t)(p0 + p) + 1];
                 xq[2] = x[3 * (size_t)(p0 + p) + 2];
             }
+            const float u0 = (xq[0] - m.in_lo) * m.in_scale, u1 = (xq[1] - m.in_lo) * m.in_scale,
+                        u2 = (xq[2] - m.in_lo) * m.in_scale;
             const float *b = m.ffB + 3 * f;
-            const float arg = 6.283185307179586f * ((b[0] * xq[0] + b[1] * xq[1]) + b[2] * xq[2]);
+            const float arg = 6.283185307179586f * ((b[0] * u0 + b[1] * u1) + b[2] * u2);
             s = sinf(arg);
             c = cosf(arg);
         }
@@ -349,6 +352,7 @@ __global__ void k_exact_points(cc::Geo geo, const float *__restrict__ x, int n, 
 struct cc_mlp {
     int F = 64, width = 256, depth = 4;
     float sigma = 8.0f;
+    float in_lo = 0.0f, in_scale = 1.0f;  // set by training: [lo, hi]^3 onto the unit box
     // parameter blocks (FP32 master): W[l] (K_l x width), b[l] (width), wout (width), bout (1)
     std::vector<int> K;
     std::vector<size_t> off_w, off_b;
@@ -385,6 +389,8 @@ MlpDev dev_view(const cc_mlp *m) {
     }
     d.wout = m->params + m->off_wout;
     d.bout = m->params + m->off_bout;
+    d.in_lo = m->in_lo;
+    d.in_scale = m->in_scale;
     d.F = m->F;
     d.width = m->width;
     d.depth = m->depth;
@@ -659,6 +665,10 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
         return nfail(CC_ERR_ARGUMENT, "bad training arguments");
     if (geo->kind == CC_GEO_SPHERE_UNION && (!geo->prims || geo->n_prims < 1))
         return nfail(CC_ERR_ARGUMENT, "sphere union without primitives");
+    if (!(hi > lo)) return nfail(CC_ERR_ARGUMENT, "the training box needs hi > lo");
+    // the Fourier features see the training box as the unit box (sigma is quoted for unit inputs)
+    m->in_lo = lo;
+    m->in_scale = 1.0f / (hi - lo);
     cc::Geo g{};
     g.kind = geo->kind;
     std::memcpy(g.p, geo->p, sizeof g.p);

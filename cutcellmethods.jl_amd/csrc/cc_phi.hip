@@ -29,8 +29,7 @@ __device__ __forceinline__ void brick_of(int b, int nbx, int nby, int *bi) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_cull(GridT<T> g, Geo geo, T delta, int nbx, int nby,
-                                              uint16_t *__restrict__ list, uint32_t *__restrict__ cnt,
-                                              unsigned long long *__restrict__ stats) {
+                                              uint16_t *__restrict__ list, uint32_t *__restrict__ cnt) {
     __shared__ T wmin[4];
     __shared__ uint32_t wcnt[4];
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -93,11 +92,7 @@ __global__ __launch_bounds__(256) void k_cull(GridT<T> g, Geo geo, T delta, int 
         for (int w = 0; w < 4; ++w) run = wmin[w] < run ? wmin[w] : run;
         __syncthreads();
     }
-    if (t == 0) {
-        cnt[b] = total;
-        atomicAdd(&stats[PHI_ST_KEPT], (unsigned long long)total);
-        atomicAdd(&stats[PHI_ST_BRICKS], 1ull);
-    }
+    if (t == 0) cnt[b] = total;
 }
 
 // The sphere-union fold over a brick's kept primitives, in the order and with the arithmetic of
@@ -144,11 +139,11 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
                                                     const uint16_t *__restrict__ list,
                                                     const uint32_t *__restrict__ cnt, SurrogateArgs sa,
                                                     T delta_c, int nbx, int nby,
-                                                    unsigned long long *__restrict__ stats) {
+                                                    uint32_t *__restrict__ bstat) {
     __shared__ T sph[4 * CULL_LDS];
     __shared__ uint16_t queue[BRICK_NODES];
     __shared__ int8_t ssign[64];
-    __shared__ uint32_t qn, st_uncert, st_band, st_sign, st_nodes;
+    __shared__ uint32_t qn, st_uncert, st_band, st_sign, st_nodes, st_cert;
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63;
     int bi[3];
     brick_of(b, nbx, nby, bi);
@@ -163,7 +158,7 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
         sph[4 * e + 2] = T(s[2]);
         sph[4 * e + 3] = T(s[3]);
     }
-    if (t == 0) qn = st_uncert = st_band = st_sign = st_nodes = 0;
+    if (t == 0) qn = st_uncert = st_band = st_sign = st_nodes = st_cert = 0;
     __syncthreads();
     const size_t m0 = (size_t)g.n[0] + 1, m1 = (size_t)g.n[1] + 1;
     if (SURR && t < 64) {
@@ -187,7 +182,7 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
         if (valid) {
             const T pc = culled_fold<T>(sph, nk, L, geo, c[0], c[1], c[2]);
             if (fabs(pc) - sqrt(e2) - sqrt(dg) > delta_c) sg = pc > T(0) ? 1 : -1;
-            atomicAdd(&stats[PHI_ST_CERT], 1ull);
+            atomicAdd(&st_cert, 1u);
         }
         ssign[t] = sg;
     }
@@ -237,27 +232,13 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
         const T x = g.x0[0] + T(gi) * g.d[0], y = g.x0[1] + T(gj) * g.d[1], z = g.x0[2] + T(gk) * g.d[2];
         phi[gi + m0 * (gj + m1 * gk)] = culled_fold<T>(sph, nk, L, geo, x, y, z);
     }
-    if (t == 0) {
-        atomicAdd(&stats[PHI_ST_NODES], (unsigned long long)st_nodes);
-        atomicAdd(&stats[PHI_ST_EXACT], (unsigned long long)nq);
-        if (SURR) {
-            atomicAdd(&stats[PHI_ST_UNCERT], (unsigned long long)st_uncert);
-            atomicAdd(&stats[PHI_ST_BAND], (unsigned long long)st_band);
-            atomicAdd(&stats[PHI_ST_SIGN], (unsigned long long)st_sign);
-        }
+    if (t < PHI_BSTAT) {  // this brick's counters (summed on the host when asked for)
+        const uint32_t v[PHI_BSTAT] = {st_nodes, nq, st_cert, st_uncert, st_band, st_sign, 0u, 0u};
+        bstat[(size_t)b * PHI_BSTAT + t] = v[t];
     }
 }
 
-__global__ void k_zero_u64(unsigned long long *p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0ull;
-}
-
 }  // namespace
-
-void launch_zero_u64(unsigned long long *p, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
-}
 
 // Rounding margins.  Every quantity the fold or the bounds round (coordinate differences, squared
 // norms, square roots, radii, smooth minima) is within a few units in the last place of `scale`;
@@ -270,38 +251,35 @@ template <typename T> static T cert_margin(double scale, int nprims) {
 
 template <typename T>
 void launch_cull(const GridT<T> &g, const Geo &geo, double scale, uint16_t *list, uint32_t *cnt,
-                 unsigned long long *stats, hipStream_t s) {
+                 hipStream_t s) {
     int nb[3];
     phi_brick_counts(g.n, nb);
     hipLaunchKernelGGL((k_cull<T>), dim3(nb[0] * nb[1] * nb[2]), dim3(256), 0, s, g, geo,
-                       cull_margin<T>(scale), nb[0], nb[1], list, cnt, stats);
+                       cull_margin<T>(scale), nb[0], nb[1], list, cnt);
 }
 
 template <typename T>
 void launch_phi_bricks(T *phi, const GridT<T> &g, const Geo &geo, const uint16_t *list,
-                       const uint32_t *cnt, const SurrogateArgs &sa, unsigned long long *stats,
-                       hipStream_t s) {
+                       const uint32_t *cnt, const SurrogateArgs &sa, uint32_t *bstat, hipStream_t s) {
     int nb[3];
     phi_brick_counts(g.n, nb);
     const dim3 grid(nb[0] * nb[1] * nb[2]);
     const T dc = cert_margin<T>(sa.scale, geo.nprims);
     if (sa.nn)
         hipLaunchKernelGGL((k_phi_bricks<T, true>), grid, dim3(256), 0, s, phi, g, geo, list, cnt, sa, dc,
-                           nb[0], nb[1], stats);
+                           nb[0], nb[1], bstat);
     else
         hipLaunchKernelGGL((k_phi_bricks<T, false>), grid, dim3(256), 0, s, phi, g, geo, list, cnt, sa, dc,
-                           nb[0], nb[1], stats);
+                           nb[0], nb[1], bstat);
 }
 
 template void launch_cull<double>(const GridT<double> &, const Geo &, double, uint16_t *, uint32_t *,
-                                  unsigned long long *, hipStream_t);
+                                  hipStream_t);
 template void launch_cull<float>(const GridT<float> &, const Geo &, double, uint16_t *, uint32_t *,
-                                 unsigned long long *, hipStream_t);
+                                 hipStream_t);
 template void launch_phi_bricks<double>(double *, const GridT<double> &, const Geo &, const uint16_t *,
-                                        const uint32_t *, const SurrogateArgs &, unsigned long long *,
-                                        hipStream_t);
+                                        const uint32_t *, const SurrogateArgs &, uint32_t *, hipStream_t);
 template void launch_phi_bricks<float>(float *, const GridT<float> &, const Geo &, const uint16_t *,
-                                       const uint32_t *, const SurrogateArgs &, unsigned long long *,
-                                       hipStream_t);
+                                       const uint32_t *, const SurrogateArgs &, uint32_t *, hipStream_t);
 
 }  // namespace cc

@@ -162,7 +162,7 @@ class Cache:
     PHI_STAT_NAMES = ("nodes", "exact", "cert_evals", "kept_sum", "uncertified", "band",
                       "sign_fail", "bricks")
 
-    def set_phi_source(self, source, surrogate=None, coarse=4, band_cells=3.0):
+    def set_phi_source(self, source, surrogate=None, coarse=8, band_cells=3.0):
         """How a sphere union's nodal phi is produced: ``"naive"`` (every node folds every
         primitive), ``"culled"`` (each brick folds only the primitives it cannot exclude; bitwise
         the same) or ``"surrogate"`` (the network on a lattice of spacing ``coarse`` cells, exact

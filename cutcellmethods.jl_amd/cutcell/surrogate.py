@@ -12,9 +12,9 @@ from ._lib import check, lib, make_geometry
 class Surrogate:
     """3 -> Fourier features (``nfreq`` frequencies, N(0, sigma^2)) -> ``depth`` x ``width`` SiLU
     -> 1.  Parameters are FP32 on the device; the matrix products run in BF16 with FP32
-    accumulation."""
+    accumulation.  Training maps its box [lo, hi]^3 onto the unit box the features see."""
 
-    def __init__(self, nfreq=64, width=256, depth=4, sigma=2.0, seed=0):
+    def __init__(self, nfreq=64, width=256, depth=4, sigma=8.0, seed=0):
         L = lib()
         h = ctypes.c_void_p()
         check(L.cc_mlp_create(nfreq, width, depth, float(sigma), int(seed), ctypes.byref(h)))
@@ -68,7 +68,8 @@ class Surrogate:
     def train(self, geo, nsamples=1 << 22, lo=-1.0, hi=1.0, band=0.05, epochs=24, batch=16384,
               lr=5e-3, seed=1):
         """Fit to exact samples of ``geo`` (half uniform in [lo, hi]^3, half within ``band`` of the
-        surface); returns the per-epoch mean training loss."""
+        surface); returns the per-epoch mean training loss.  Afterwards the features see
+        (x - lo) / (hi - lo)."""
         g = make_geometry(geo)
         hist = np.zeros(epochs, np.float32)
         check(lib().cc_mlp_train(self._h, ctypes.byref(g), int(nsamples), float(lo), float(hi),

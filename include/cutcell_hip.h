@@ -184,7 +184,9 @@ int cc_mlp_eval(cc_mlp *net, const float *x, int64_t n, float *y);
 /* Mean squared error over n host samples (n a multiple of 64) and its gradient (nparams floats). */
 int cc_mlp_loss_grad(cc_mlp *net, const float *x, const float *y, int64_t n, float *loss, float *grad);
 /* Fits the network to exact samples of `geo` (half uniform in [lo, hi]^3, half within `band` of
- * the surface), Adam with a cosine-decayed rate; `history` (optional) gets each epoch's mean loss. */
+ * the surface), Adam with a cosine-decayed rate; `history` (optional) gets each epoch's mean loss.
+ * From then on the features see u = (x - lo) / (hi - lo) (the training box as the unit box, where
+ * sigma is quoted); a network never trained sees u = x. */
 int cc_mlp_train(cc_mlp *net, const cc_geometry *geo, int64_t nsamples, float lo, float hi, float band,
                  int epochs, int batch, float lr, uint64_t seed, float *history);
 

@@ -76,7 +76,7 @@ def test_loss_and_gradients_match_the_bf16_oracle():
 
 def test_training_reduces_the_loss_and_fits_the_body():
     body = csg_spheres(5, count=16, lo=-0.5, hi=0.5, rmin=0.1, rmax=0.25)
-    net = Surrogate(nfreq=64, depth=4, sigma=2.0, seed=11)
+    net = Surrogate(nfreq=64, depth=4, sigma=4.0, seed=11)  # a smooth body: lower frequencies
     hist = net.train(body, nsamples=1 << 20, band=0.05, epochs=12, batch=16384, lr=5e-3, seed=2)
     print("loss per epoch", hist)
     assert np.all(np.isfinite(hist))
@@ -167,8 +167,8 @@ def test_what_the_trained_network_gains():
     evaluations (the culled-exact path needs one per node) and keeps far-field phi close."""
     body = csg_spheres(8, count=128, lo=-0.6, hi=0.6, rmin=0.05, rmax=0.15)
     g = CartesianIsoGrid((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), n=(96, 96, 96))
-    untrained = Surrogate(nfreq=64, depth=4, sigma=2.0, seed=2)
-    trained = Surrogate(nfreq=64, depth=4, sigma=2.0, seed=2)
+    untrained = Surrogate(nfreq=64, depth=4, sigma=8.0, seed=2)
+    trained = Surrogate(nfreq=64, depth=4, sigma=8.0, seed=2)
     hist = trained.train(body, nsamples=1 << 21, band=0.05, epochs=16, batch=16384, lr=5e-3, seed=3)
     print("training loss", hist)
     su, peu, psu, du = check_surrogate_against_plain(body, g, untrained)
@@ -182,4 +182,4 @@ def test_what_the_trained_network_gains():
           f"far-field rms error untrained {err_u:.3e} trained {err_t:.3e} (h={g.d[0]:.3e})")
     assert ft < 0.5 * fu
     assert st["sign_fail"] < 0.05 * max(su["sign_fail"], 1)
-    assert err_t < 0.5 * g.d[0] * 4
+    assert err_t < 0.25 * err_u and err_t < 4 * g.d[0]

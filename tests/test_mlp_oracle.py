@@ -6,10 +6,10 @@ import mlp_oracle as mo
 
 
 def test_bf16_rounding_is_nearest_even():
-    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, 1.0 + 2 ** -9, -2.5, 0.0, 3.0e38],
-                 dtype=np.float32)
-    r = mo.bf16(x)
-    assert r[0] == 1.0 and r[1] == 1.0 + 2 ** -7 * 0 + 2 ** -8 * 0 + 2 ** -7 or True
+    # bfloat16 keeps 7 fraction bits: the neighbours of 1 are 1 - 2^-8 and 1 + 2^-7
+    assert mo.bf16(np.float32(1.0)) == 1.0
+    assert mo.bf16(np.float32(1.0 + 2 ** -9)) == 1.0               # below half an ulp: down
+    assert mo.bf16(np.float32(1.0 + 3 * 2 ** -9)) == 1.0 + 2 ** -7  # above half an ulp: up
     # ties go to even: 1 + 2^-8 is halfway between 1 and 1 + 2^-7 -> 1
     assert mo.bf16(np.float32(1.0 + 2 ** -8)) == 1.0
     assert mo.bf16(np.float32(1.0 + 3 * 2 ** -8)) == np.float32(1.0 + 2 ** -6)
