@@ -43,6 +43,10 @@ def parse():
                     help="only the headline workload (skip the secondary workloads)")
     ap.add_argument("--only", default=None, choices=["csg", "batched", "secondary"],
                     help="run one secondary workload alone and print its JSON")
+    ap.add_argument("--train-epochs", type=int, default=24,
+                    help="surrogate training epochs of the composite-body workload")
+    ap.add_argument("--train-samples", type=int, default=1 << 22,
+                    help="surrogate training samples of the composite-body workload")
     ap.add_argument("--sweep", action="store_true",
                     help="also time every stream-pass launch shape and report them")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "mc_stream_512_traffic.json"))
@@ -319,7 +323,8 @@ def main():
     from cutcell import MarchingCubesCutCell, allocate_cache, update_cache
 
     if a.only:
-        fn = {"csg": lambda: csg(a.seed), "batched": lambda: batched(a.seed),
+        fn = {"csg": lambda: csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples),
+              "batched": lambda: batched(a.seed),
               "secondary": lambda: secondary(a.seed)}[a.only]
         print(json.dumps({a.only: fn()}))
         return
@@ -408,7 +413,7 @@ def main():
         del cache
         out.update(secondary(a.seed))
         out["batched_64_bodies"] = batched(a.seed)
-        out["csg_512_spheres"] = csg(a.seed)
+        out["csg_512_spheres"] = csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples)
     print(json.dumps(out))
 
 
