@@ -80,9 +80,10 @@ __global__ __launch_bounds__(MC_TILE) void k_batch_a(const BatchBody<T> *__restr
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_batch_b(const BatchBody<T> *__restrict__ bodies,
-                                                 const uint32_t *__restrict__ begin, int nb,
-                                                 uint32_t total) {
+__global__ __launch_bounds__(MC_CUT_THREADS) void k_batch_b(const BatchBody<T> *__restrict__ bodies,
+                                                            const uint32_t *__restrict__ begin, int nb,
+                                                            uint32_t total) {
+    __shared__ T pts[39 * MC_CUT_THREADS];
     const uint32_t w = xcd_order(blockIdx.x);
     if (w >= total) return;
     const int b = find_body(begin, nb, w);
@@ -90,9 +91,9 @@ __global__ __launch_bounds__(256) void k_batch_b(const BatchBody<T> *__restrict_
     const BatchBody<T> &P = bodies[b];
     if (P.method == BODY_MC) {
         const uint32_t count = *P.mc.counter;
-        const uint32_t stride = P.cut_items * 256u;
-        for (uint32_t e = local * 256u + threadIdx.x; e < count; e += stride)
-            mc_cut_item(P.mc.phi, P.mc.g, P.mc.cols, P.mc.cutlist[e]);
+        const uint32_t stride = P.cut_items * (uint32_t)MC_CUT_THREADS;
+        for (uint32_t e = local * (uint32_t)MC_CUT_THREADS + threadIdx.x; e < count; e += stride)
+            mc_cut_item(P.mc.phi, P.mc.g, P.mc.cols, P.mc.cutlist[e], pts + threadIdx.x);
     }
 }
 
@@ -208,7 +209,7 @@ template <typename T> int capture(cc_batch *bt) {
     hipLaunchKernelGGL((cc::k_batch_a<T>), dim3(ga), dim3(cc::MC_TILE), 0, bt->stream, dd,
                        bt->begin_a_dev, nb, bt->total_a);
     if (bt->total_b > 0)
-        hipLaunchKernelGGL((cc::k_batch_b<T>), dim3(gb), dim3(256), 0, bt->stream, dd, bt->begin_b_dev,
+        hipLaunchKernelGGL((cc::k_batch_b<T>), dim3(gb), dim3(cc::MC_CUT_THREADS), 0, bt->stream, dd, bt->begin_b_dev,
                            nb, bt->total_b);
     if (hipStreamEndCapture(bt->stream, &bt->graph) != hipSuccess)
         return bfail(CC_ERR_HIP, "hipStreamEndCapture failed");

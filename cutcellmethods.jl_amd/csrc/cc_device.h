@@ -60,11 +60,14 @@ __device__ __forceinline__ T sphere_dist(T cx, T cy, T cz, T r, T x, T y, T z, T
     return nr - r;
 }
 
-// Polynomial smooth minimum: h = max(k - |a-b|, 0)/k; min(a,b) - h*h*k/4.
+// Polynomial smooth minimum: h = max(k - |a-b|, 0)/k; min(a,b) - h*h*k/4.  When k - |a-b| is not
+// positive the correction is exactly zero and min(a,b) - 0 is min(a,b) bit for bit (also for -0,
+// inf and NaN), so that case returns early and skips the division.
 template <typename T> __device__ __forceinline__ T smin_poly(T a, T b, T k) {
     T h = k - fabs(a - b);
-    h = (h > T(0) ? h : T(0)) / k;
-    T m = a < b ? a : b;
+    const T m = a < b ? a : b;
+    if (!(h > T(0))) return m;
+    h = h / k;
     return m - ((h * h) * k) / T(4);
 }
 
@@ -94,6 +97,23 @@ __device__ __forceinline__ void from_local(const double *r, const T *gl, T *n) {
     }
 }
 
+// One step acc <- smin_poly(acc, |x - c| - r, k) of the sphere-union fold.  s is the squared norm
+// sphere_dist forms.  When sqrt(s) >= acc + k + r plus a rounding margin (or acc + k + r is below
+// minus the margin, so that even the centre is k above acc), the computed distance d satisfies
+// d - acc > k, the smooth minimum returns acc bit for bit, and the square root and the smooth
+// minimum are skipped.  The margin, 16 eps times the magnitudes involved, covers the rounding of
+// t, of t*t, of the square root and of d and acc - d.
+template <typename T, int D>
+__device__ __forceinline__ T fold_sphere(T acc, T cx, T cy, T cz, T r, T x, T y, T z, T k) {
+    const T dx = x - cx, dy = y - cy, dz = z - cz;
+    T s = dx * dx + dy * dy;
+    if (D == 3) s = s + dz * dz;
+    const T t = (acc + k) + r;
+    const T tm = t + T(16) * Eps<T>::v * (((fabs(acc) + k) + r) + fabs(t));
+    if (tm <= T(0) || s >= tm * tm) return acc;
+    return smin_poly(acc, sqrt(s) - r, k);
+}
+
 // The sphere union folded left with the polynomial smooth minimum.
 template <typename T, int D>
 __device__ __forceinline__ T union_dist(const Geo &g, const T *x) {
@@ -103,8 +123,7 @@ __device__ __forceinline__ T union_dist(const Geo &g, const T *x) {
     T acc = sphere_dist<T, D>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), x[0], x[1], z);
     for (int i = 1; i < g.nprims; ++i) {
         const double *si = s + 4 * i;
-        acc = smin_poly(acc, sphere_dist<T, D>(T(si[0]), T(si[1]), T(si[2]), T(si[3]), x[0], x[1], z),
-                        k);
+        acc = fold_sphere<T, D>(acc, T(si[0]), T(si[1]), T(si[2]), T(si[3]), x[0], x[1], z, k);
     }
     return acc;
 }

@@ -20,8 +20,10 @@ template <typename T> struct Cols {
 
 // Tile width of the marching-cubes stream pass: cells per workgroup, one per thread.
 constexpr int MC_TILE = 256;
-// Workgroups of the cut-cell pass (a grid-stride loop over the device-side count).
-constexpr int MC_CUT_BLOCKS = 2048;
+// Workgroups of the cut-cell pass (a grid-stride loop over the device-side count) and their size
+// (each lane stages 13 crossing points, 39 values, in LDS).
+constexpr int MC_CUT_BLOCKS = 4096;
+constexpr int MC_CUT_THREADS = 128;
 
 template <typename T>
 void launch_sample(T *phi, const GridT<T> &g, const Geo &geo, int dim, hipStream_t s);

@@ -61,12 +61,13 @@ __global__ __launch_bounds__(MC_TILE) void k_mc_march(McBody<T> B, uint32_t nwor
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_mc_cut(const T *__restrict__ phi, GridT<T> g, Cols<T> cols,
-                                                const uint32_t *__restrict__ cutlist,
-                                                const uint32_t *__restrict__ counter) {
+__global__ __launch_bounds__(MC_CUT_THREADS) void k_mc_cut(const T *__restrict__ phi, GridT<T> g, Cols<T> cols,
+                                                           const uint32_t *__restrict__ cutlist,
+                                                           const uint32_t *__restrict__ counter) {
+    __shared__ T pts[39 * MC_CUT_THREADS];
     const uint32_t count = *counter;
     for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < count; w += gridDim.x * blockDim.x)
-        mc_cut_item(phi, g, cols, cutlist[w]);
+        mc_cut_item(phi, g, cols, cutlist[w], pts + threadIdx.x);
 }
 
 // _sample_nodes_kernel! (src/marching_squares/cache.jl:67-70): one thread per node, phi at the
@@ -139,7 +140,7 @@ void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &
 template <typename T>
 void launch_mc_cut(const T *phi, const GridT<T> &g, const Cols<T> &cols, const uint32_t *cutlist,
                    const uint32_t *counter, hipStream_t s) {
-    hipLaunchKernelGGL((k_mc_cut<T>), dim3(MC_CUT_BLOCKS), dim3(256), 0, s, phi, g, cols, cutlist,
+    hipLaunchKernelGGL((k_mc_cut<T>), dim3(MC_CUT_BLOCKS), dim3(MC_CUT_THREADS), 0, s, phi, g, cols, cutlist,
                        counter);
 }
 

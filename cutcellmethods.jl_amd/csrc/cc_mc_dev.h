@@ -17,41 +17,81 @@ __device__ __forceinline__ int nx4(int k) { return (k + 1) & 3; }  // _ms_next
 __device__ __forceinline__ int pv4(int k) { return (k + 3) & 3; }  // _ms_prev
 
 // MC_NODE_BITS (src/marching_cubes/marching_cubes.jl:106-107), 0-based corner c.
-__device__ __forceinline__ int bitx(int c) { return (c ^ (c >> 1)) & 1; }
-__device__ __forceinline__ int bity(int c) { return (c >> 1) & 1; }
-__device__ __forceinline__ int bitz(int c) { return (c >> 2) & 1; }
+__device__ __forceinline__ constexpr int bitx(int c) { return (c ^ (c >> 1)) & 1; }
+__device__ __forceinline__ constexpr int bity(int c) { return (c >> 1) & 1; }
+__device__ __forceinline__ constexpr int bitz(int c) { return (c >> 2) & 1; }
 
-// MC_FACE_NODES (:169-180) and MC_FACE_TEST_CODE (:385-388), 0-based.
-__constant__ int8_t FACE_NODE[6][4] = {{0, 3, 7, 4}, {1, 2, 6, 5}, {0, 1, 5, 4},
-                                      {3, 2, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
-__constant__ int8_t FACE_TEST_CODE[6] = {4, 2, 1, 3, 5, 6};
-// MC_TEST_FACE_QUADS (:372-373), 0-based.
-__constant__ int8_t TEST_QUAD[6][4] = {{0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3},
-                                      {3, 7, 4, 0}, {0, 3, 2, 1}, {4, 7, 6, 5}};
-// MC_EDGE_NODES (:135-137), 0-based.
-__constant__ int8_t EDGE_NODE[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
-                                       {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
-
-template <typename T> __device__ __forceinline__ T pick4(const T *a, int j) {
-    T r = a[0];
-    r = j == 1 ? a[1] : r;
-    r = j == 2 ? a[2] : r;
-    r = j == 3 ? a[3] : r;
-    return r;
+// MC_FACE_NODES (:169-180), MC_FACE_TEST_CODE (:385-388), MC_TEST_FACE_QUADS (:372-373) and
+// MC_EDGE_NODES (:135-137), 0-based.  constexpr look-ups: in the unrolled per-face and per-edge
+// loops every index folds to a constant, so the corner values stay in registers.
+__device__ __forceinline__ constexpr int face_node(int dir, int k) {
+    constexpr int8_t t[6][4] = {{0, 3, 7, 4}, {1, 2, 6, 5}, {0, 1, 5, 4},
+                                {3, 2, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
+    return t[dir][k];
+}
+__device__ __forceinline__ constexpr int face_test_code(int dir) {
+    constexpr int8_t t[6] = {4, 2, 1, 3, 5, 6};
+    return t[dir];
+}
+__device__ __forceinline__ constexpr int test_quad(int f, int k) {
+    constexpr int8_t t[6][4] = {{0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3},
+                                {3, 7, 4, 0}, {0, 3, 2, 1}, {4, 7, 6, 5}};
+    return t[f][k];
+}
+__device__ __forceinline__ constexpr int edge_node(int e, int end) {
+    constexpr int8_t t[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
+                                 {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    return t[e][end];
 }
 
-// _test_face (:401-406): the asymptotic decider on one face's four corner values.
-template <typename T> __device__ __forceinline__ bool test_face(const T *cb, int face) {
+// _test_face (:401-406), the asymptotic decider on one face's four corner values, for all six
+// faces and both signs of the face code at once: bit f-1 of `pos` / `neg` is the result for face
+// code +f / -f.  The table-driven tests of the dispatch then read a bit instead of indexing the
+// corner values at run time (which would move them out of registers).  Each bit is the reference's
+// expression: |AC - BD| < eps ? (code >= 0) : code * A * (AC - BD) >= 0.
+struct FaceTests {
+    unsigned pos, neg;
+};
+
+template <int F, typename T> __device__ __forceinline__ void face_test_bits(const T *c, FaceTests &ft) {
+    const T A = c[test_quad(F - 1, 0)], B = c[test_quad(F - 1, 1)];
+    const T C = c[test_quad(F - 1, 2)], D = c[test_quad(F - 1, 3)];
+    const T det = A * C - B * D;
+    const bool small = fabs(det) < Eps<T>::v;
+    const bool p = small || T(F) * A * det >= T(0);
+    const bool n = !small && T(-F) * A * det >= T(0);
+    ft.pos |= (p ? 1u : 0u) << (F - 1);
+    ft.neg |= (n ? 1u : 0u) << (F - 1);
+}
+
+template <typename T> __device__ __forceinline__ FaceTests face_tests(const T *c) {
+    FaceTests ft{0u, 0u};
+    face_test_bits<1>(c, ft);
+    face_test_bits<2>(c, ft);
+    face_test_bits<3>(c, ft);
+    face_test_bits<4>(c, ft);
+    face_test_bits<5>(c, ft);
+    face_test_bits<6>(c, ft);
+    return ft;
+}
+
+__device__ __forceinline__ bool test_face(const FaceTests &ft, int face) {
     const int f = face < 0 ? -face : face;
-    const T A = cb[TEST_QUAD[f - 1][0]], B = cb[TEST_QUAD[f - 1][1]];
-    const T C = cb[TEST_QUAD[f - 1][2]], D = cb[TEST_QUAD[f - 1][3]];
-    if (fabs(A * C - B * D) < Eps<T>::v) return face >= 0;
-    return T(face) * A * (A * C - B * D) >= T(0);
+    return (((face >= 0 ? ft.pos : ft.neg) >> (f - 1)) & 1u) != 0u;
+}
+
+// t on reference edge (E0, E1) and the values at t on its three parallel edges (:466-489).
+template <int E0, int E1, int B0, int B1, int C0, int C1, int D0, int D1, typename T>
+__device__ __forceinline__ void interior_edge(const T *c, T &t, T &Bt, T &Ct, T &Dt) {
+    t = c[E0] / (c[E0] - c[E1]);
+    Bt = c[B0] + (c[B1] - c[B0]) * t;
+    Ct = c[C0] + (c[C1] - c[C0]) * t;
+    Dt = c[D0] + (c[D1] - c[D0]) * t;
 }
 
 // _test_interior (:427-504); `case1` is the reference's 1-based case, `subcfg` 1-based.
 template <typename T>
-__device__ bool test_interior(int case1, const T *c, int cfg, int subcfg, int s) {
+__device__ __forceinline__ bool test_interior(int case1, const T *c, int cfg, int subcfg, int s) {
     T At = T(0), Bt = T(0), Ct = T(0), Dt = T(0), t;
     if (case1 == 5 || case1 == 11) {
         T a = (c[4] - c[0]) * (c[6] - c[2]) - (c[7] - c[3]) * (c[5] - c[1]);
@@ -70,25 +110,20 @@ __device__ bool test_interior(int case1, const T *c, int cfg, int subcfg, int s)
         else if (case1 == 13) edge = LW_TEST12[cfg][3];
         else edge = LW_TILING13_5_1[cfg][subcfg - 1][0];
         // Reference edge e: t on that edge, then B/C/D on the three parallel edges.
-        int e0, e1, b0, b1, c0, c1, d0, d1;
         switch (edge) {
-        case 0:  e0 = 0; e1 = 1; b0 = 3; b1 = 2; c0 = 7; c1 = 6; d0 = 4; d1 = 5; break;
-        case 1:  e0 = 1; e1 = 2; b0 = 0; b1 = 3; c0 = 4; c1 = 7; d0 = 5; d1 = 6; break;
-        case 2:  e0 = 2; e1 = 3; b0 = 1; b1 = 0; c0 = 5; c1 = 4; d0 = 6; d1 = 7; break;
-        case 3:  e0 = 3; e1 = 0; b0 = 2; b1 = 1; c0 = 6; c1 = 5; d0 = 7; d1 = 4; break;
-        case 4:  e0 = 4; e1 = 5; b0 = 7; b1 = 6; c0 = 3; c1 = 2; d0 = 0; d1 = 1; break;
-        case 5:  e0 = 5; e1 = 6; b0 = 4; b1 = 7; c0 = 0; c1 = 3; d0 = 1; d1 = 2; break;
-        case 6:  e0 = 6; e1 = 7; b0 = 5; b1 = 4; c0 = 1; c1 = 0; d0 = 2; d1 = 3; break;
-        case 7:  e0 = 7; e1 = 4; b0 = 6; b1 = 5; c0 = 2; c1 = 1; d0 = 3; d1 = 0; break;
-        case 8:  e0 = 0; e1 = 4; b0 = 3; b1 = 7; c0 = 2; c1 = 6; d0 = 1; d1 = 5; break;
-        case 9:  e0 = 1; e1 = 5; b0 = 0; b1 = 4; c0 = 3; c1 = 7; d0 = 2; d1 = 6; break;
-        case 10: e0 = 2; e1 = 6; b0 = 1; b1 = 5; c0 = 0; c1 = 4; d0 = 3; d1 = 7; break;
-        default: e0 = 3; e1 = 7; b0 = 2; b1 = 6; c0 = 1; c1 = 5; d0 = 0; d1 = 4; break;
+        case 0:  interior_edge<0, 1, 3, 2, 7, 6, 4, 5>(c, t, Bt, Ct, Dt); break;
+        case 1:  interior_edge<1, 2, 0, 3, 4, 7, 5, 6>(c, t, Bt, Ct, Dt); break;
+        case 2:  interior_edge<2, 3, 1, 0, 5, 4, 6, 7>(c, t, Bt, Ct, Dt); break;
+        case 3:  interior_edge<3, 0, 2, 1, 6, 5, 7, 4>(c, t, Bt, Ct, Dt); break;
+        case 4:  interior_edge<4, 5, 7, 6, 3, 2, 0, 1>(c, t, Bt, Ct, Dt); break;
+        case 5:  interior_edge<5, 6, 4, 7, 0, 3, 1, 2>(c, t, Bt, Ct, Dt); break;
+        case 6:  interior_edge<6, 7, 5, 4, 1, 0, 2, 3>(c, t, Bt, Ct, Dt); break;
+        case 7:  interior_edge<7, 4, 6, 5, 2, 1, 3, 0>(c, t, Bt, Ct, Dt); break;
+        case 8:  interior_edge<0, 4, 3, 7, 2, 6, 1, 5>(c, t, Bt, Ct, Dt); break;
+        case 9:  interior_edge<1, 5, 0, 4, 3, 7, 2, 6>(c, t, Bt, Ct, Dt); break;
+        case 10: interior_edge<2, 6, 1, 5, 0, 4, 3, 7>(c, t, Bt, Ct, Dt); break;
+        default: interior_edge<3, 7, 2, 6, 1, 5, 0, 4>(c, t, Bt, Ct, Dt); break;
         }
-        t = c[e0] / (c[e0] - c[e1]);
-        Bt = c[b0] + (c[b1] - c[b0]) * t;
-        Ct = c[c0] + (c[c1] - c[c0]) * t;
-        Dt = c[d0] + (c[d1] - c[d0]) * t;
     }
     int test = 0;
     if (At >= T(0)) test += 1;
@@ -104,7 +139,8 @@ __device__ bool test_interior(int case1, const T *c, int cfg, int subcfg, int s)
 // cell_triangles (:538-651): the Lewiner dispatch.  Returns the triangle count (-1 for a
 // SUBCONFIG13 entry with no tiling) and points *tile at the flat run of 0-based edge codes.
 template <typename T>
-__device__ int cell_triangles(const T *phi, int lut, const signed char **tile) {
+__device__ __forceinline__ int cell_triangles(const T *phi, const FaceTests &ft, int lut,
+                                              const signed char **tile) {
     const int cs = LW_CASES[lut][0], cfg = LW_CASES[lut][1];
     int subcfg = 1;
     switch (cs) {
@@ -112,20 +148,20 @@ __device__ int cell_triangles(const T *phi, int lut, const signed char **tile) {
     case 1: *tile = LW_TILING1[cfg]; return 1;
     case 2: *tile = LW_TILING2[cfg]; return 2;
     case 3:
-        if (test_face(phi, LW_TEST3[cfg])) { *tile = LW_TILING3_2[cfg]; return 4; }
+        if (test_face(ft, LW_TEST3[cfg])) { *tile = LW_TILING3_2[cfg]; return 4; }
         *tile = LW_TILING3_1[cfg]; return 2;
     case 4:
         if (test_interior(5, phi, cfg, subcfg, LW_TEST4[cfg])) { *tile = LW_TILING4_1[cfg]; return 2; }
         *tile = LW_TILING4_2[cfg]; return 6;
     case 5: *tile = LW_TILING5[cfg]; return 3;
     case 6:
-        if (test_face(phi, LW_TEST6[cfg][0])) { *tile = LW_TILING6_2[cfg]; return 5; }
+        if (test_face(ft, LW_TEST6[cfg][0])) { *tile = LW_TILING6_2[cfg]; return 5; }
         if (test_interior(7, phi, cfg, subcfg, LW_TEST6[cfg][1])) { *tile = LW_TILING6_1_1[cfg]; return 3; }
         *tile = LW_TILING6_1_2[cfg]; return 9;
     case 7:
-        if (test_face(phi, LW_TEST7[cfg][0])) subcfg += 1;
-        if (test_face(phi, LW_TEST7[cfg][1])) subcfg += 2;
-        if (test_face(phi, LW_TEST7[cfg][2])) subcfg += 4;
+        if (test_face(ft, LW_TEST7[cfg][0])) subcfg += 1;
+        if (test_face(ft, LW_TEST7[cfg][1])) subcfg += 2;
+        if (test_face(ft, LW_TEST7[cfg][2])) subcfg += 4;
         switch (subcfg) {
         case 1: *tile = LW_TILING7_1[cfg]; return 3;
         case 2: case 3: *tile = LW_TILING7_2[cfg][subcfg - 2]; return 5;
@@ -139,25 +175,25 @@ __device__ int cell_triangles(const T *phi, int lut, const signed char **tile) {
     case 8: *tile = LW_TILING8[cfg]; return 2;
     case 9: *tile = LW_TILING9[cfg]; return 4;
     case 10:
-        if (test_face(phi, LW_TEST10[cfg][0])) {
-            if (test_face(phi, LW_TEST10[cfg][1])) { *tile = LW_TILING10_1_1_[cfg]; return 4; }
+        if (test_face(ft, LW_TEST10[cfg][0])) {
+            if (test_face(ft, LW_TEST10[cfg][1])) { *tile = LW_TILING10_1_1_[cfg]; return 4; }
             *tile = LW_TILING10_2[cfg]; return 8;
         }
-        if (test_face(phi, LW_TEST10[cfg][1])) { *tile = LW_TILING10_2_[cfg]; return 8; }
+        if (test_face(ft, LW_TEST10[cfg][1])) { *tile = LW_TILING10_2_[cfg]; return 8; }
         if (test_interior(11, phi, cfg, subcfg, LW_TEST10[cfg][2])) { *tile = LW_TILING10_1_1[cfg]; return 4; }
         *tile = LW_TILING10_1_2[cfg]; return 8;
     case 11: *tile = LW_TILING11[cfg]; return 4;
     case 12:
-        if (test_face(phi, LW_TEST12[cfg][0])) {
-            if (test_face(phi, LW_TEST12[cfg][1])) { *tile = LW_TILING12_1_1_[cfg]; return 4; }
+        if (test_face(ft, LW_TEST12[cfg][0])) {
+            if (test_face(ft, LW_TEST12[cfg][1])) { *tile = LW_TILING12_1_1_[cfg]; return 4; }
             *tile = LW_TILING12_2[cfg]; return 8;
         }
-        if (test_face(phi, LW_TEST12[cfg][1])) { *tile = LW_TILING12_2_[cfg]; return 8; }
+        if (test_face(ft, LW_TEST12[cfg][1])) { *tile = LW_TILING12_2_[cfg]; return 8; }
         if (test_interior(13, phi, cfg, subcfg, LW_TEST12[cfg][2])) { *tile = LW_TILING12_1_1[cfg]; return 4; }
         *tile = LW_TILING12_1_2[cfg]; return 8;
     case 13: {
         for (int f = 0; f < 6; ++f)
-            if (test_face(phi, LW_TEST13[cfg][f])) subcfg += 1 << f;
+            if (test_face(ft, LW_TEST13[cfg][f])) subcfg += 1 << f;
         const int sc = LW_SUBCONFIG13[subcfg - 1];
         if (sc < 0) { *tile = nullptr; return -1; }
         if (sc == 0) { *tile = LW_TILING13_1[cfg]; return 4; }
@@ -216,15 +252,25 @@ __device__ __forceinline__ void face_polygon(const T *fx, const T *fy, const T *
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (!(fp[k] >= T(0) && fp[nx4(k)] < T(0))) continue;
-        int j = k, hit = -1;
+        // the chord target: the first entering edge met walking on (connect) or back from k; both
+        // walks are unrolled so that every index is a constant
+        bool found = false;
+        T lbx = T(0), lby = T(0);
+        if (connect) {
 #pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            j = connect ? nx4(j) : pv4(j);
-            if (hit < 0 && pick4(fp, j) < T(0) && pick4(fp, nx4(j)) >= T(0)) hit = j;
+            for (int it = 1; it <= 4; ++it) {
+                const int j = (k + it) & 3;
+                if (!found && fp[j] < T(0) && fp[nx4(j)] >= T(0)) { found = true; lbx = sax[j]; lby = say[j]; }
+            }
+        } else {
+#pragma unroll
+            for (int it = 1; it <= 4; ++it) {
+                const int j = (k + 4 - it) & 3;
+                if (!found && fp[j] < T(0) && fp[nx4(j)] >= T(0)) { found = true; lbx = sax[j]; lby = say[j]; }
+            }
         }
-        if (hit < 0) continue;
+        if (!found) continue;
         const T lax = sbx[k], lay = sby[k];
-        const T lbx = pick4(sax, hit), lby = pick4(say, hit);
         const T cr = lax * lby - lay * lbx;
         a2 += cr;
         m0 += (lax + lbx) * cr;
@@ -244,80 +290,110 @@ template <typename T> __device__ __forceinline__ T full_face_area3(const T *cs, 
     return p;
 }
 
+// One face of cut_cell_faces (moments.jl:224-257): its clip in the face's (p, q) frame, the face
+// fraction and the centroid offsets, and the saddle flag.  DIR is a template parameter so that
+// every corner and table index folds to a constant and the corner values stay in registers.
+template <int DIR, typename T>
+__device__ __forceinline__ void face_clip(const T *phi, const FaceTests &ft, const T *L, const T *cs, T *ff,
+                                         T *fcl, bool &amb_face) {
+    constexpr int ax = DIR >> 1;
+    constexpr int p = ax == 0 ? 1 : 0, q = ax == 2 ? 1 : 2;
+    T fx[4], fy[4], fp[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int nd = face_node(DIR, k);
+        const int bp = p == 0 ? bitx(nd) : bity(nd);
+        const int bq = q == 1 ? bity(nd) : bitz(nd);
+        fx[k] = bp ? L[p] : T(0);
+        fy[k] = bq ? L[q] : T(0);
+        fp[k] = phi[nd];
+    }
+    int nout = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) nout += fp[k] >= T(0) ? 1 : 0;
+    amb_face |= nout == 2 && ((fp[0] >= T(0)) == (fp[2] >= T(0)));
+    const T avg0 = T(0.25) * (((fx[0] + fx[1]) + fx[2]) + fx[3]);
+    const T avg1 = T(0.25) * (((fy[0] + fy[1]) + fy[2]) + fy[3]);
+    if (nout == 4 || nout == 0) {
+        ff[DIR] = nout == 4 ? T(1) : T(0);
+        fcl[2 * DIR] = avg0;
+        fcl[2 * DIR + 1] = avg1;
+    } else {
+        T a2, m0, m1;
+        face_polygon(fx, fy, fp, test_face(ft, face_test_code(DIR)), a2, m0, m1);
+        ff[DIR] = T(0.5) * a2 / full_face_area3(cs, DIR);
+        if (a2 != T(0)) {
+            const T den = T(3) * a2;
+            fcl[2 * DIR] = m0 / den;
+            fcl[2 * DIR + 1] = m1 / den;
+        } else {
+            fcl[2 * DIR] = avg0;
+            fcl[2 * DIR + 1] = avg1;
+        }
+    }
+}
+
+// Crossing of edge E (edge_crossings, marching_cubes.jl:306-330), outside-anchored, into the
+// lane's LDS slots; zero where the edge is not crossed.  acc and ncr gather the interior vertex.
+template <int E, typename T>
+__device__ __forceinline__ void edge_point(const T *phi, const T *L, T *P, T *acc, int &ncr) {
+    constexpr int a = edge_node(E, 0), b = edge_node(E, 1);
+    const bool crossed = (phi[a] >= T(0)) != (phi[b] >= T(0));
+    const T pa[3] = {bitx(a) ? L[0] : T(0), bity(a) ? L[1] : T(0), bitz(a) ? L[2] : T(0)};
+    const T pb[3] = {bitx(b) ? L[0] : T(0), bity(b) ? L[1] : T(0), bitz(b) ? L[2] : T(0)};
+    T q[3] = {T(0), T(0), T(0)};
+    if (crossed) {
+        const T s = open_fraction(phi[a], phi[b]);
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            q[x] = phi[a] >= T(0) ? pa[x] + s * (pb[x] - pa[x]) : pb[x] + s * (pa[x] - pb[x]);
+            acc[x] += q[x];
+        }
+        ncr += 1;
+    }
+#pragma unroll
+    for (int x = 0; x < 3; ++x) P[(3 * E + x) * MC_CUT_THREADS] = q[x];
+}
+
 // The whole construction for one cut cell: cut_cell_moments (moments.jl:275-346) with
 // cut_cell_faces (:224-257) and edge_crossings (marching_cubes.jl:306-330).  `phi` is nudged;
 // `L` the cell-local upper corner (node(i+1) - node(i) per axis), `xl` the lower lattice corner.
 template <typename T>
-__device__ void mc_cut_cell(const T *phi, int lut, const T *xl, const T *L, const T *cs,
-                            const Cols<T> &cols, uint32_t c) {
-    // ---- six face clips ----
+__device__ __forceinline__ void mc_cut_cell(const T *phi, int lut, const T *xl, const T *L, const T *cs,
+                            const Cols<T> &cols, uint32_t c, T *P) {
+    // ---- six face clips (one instantiation per face: every table index is a constant) ----
+    const FaceTests ft = face_tests(phi);
     T ff[6], fcl[12];
     bool amb_face = false;
-#pragma unroll
-    for (int dir = 0; dir < 6; ++dir) {
-        const int ax = dir >> 1;
-        const int p = ax == 0 ? 1 : 0, q = ax == 2 ? 1 : 2;
-        T fx[4], fy[4], fp[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int nd = FACE_NODE[dir][k];
-            const int bp = p == 0 ? bitx(nd) : bity(nd);
-            const int bq = q == 1 ? bity(nd) : bitz(nd);
-            fx[k] = bp ? L[p] : T(0);
-            fy[k] = bq ? L[q] : T(0);
-            fp[k] = phi[nd];
-        }
-        int nout = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) nout += fp[k] >= T(0) ? 1 : 0;
-        amb_face |= nout == 2 && ((fp[0] >= T(0)) == (fp[2] >= T(0)));
-        const T avg0 = T(0.25) * (((fx[0] + fx[1]) + fx[2]) + fx[3]);
-        const T avg1 = T(0.25) * (((fy[0] + fy[1]) + fy[2]) + fy[3]);
-        if (nout == 4 || nout == 0) {
-            ff[dir] = nout == 4 ? T(1) : T(0);
-            fcl[2 * dir] = avg0;
-            fcl[2 * dir + 1] = avg1;
-        } else {
-            T a2, m0, m1;
-            face_polygon(fx, fy, fp, test_face(phi, FACE_TEST_CODE[dir]), a2, m0, m1);
-            ff[dir] = T(0.5) * a2 / full_face_area3(cs, dir);
-            if (a2 != T(0)) {
-                const T den = T(3) * a2;
-                fcl[2 * dir] = m0 / den;
-                fcl[2 * dir + 1] = m1 / den;
-            } else {
-                fcl[2 * dir] = avg0;
-                fcl[2 * dir + 1] = avg1;
-            }
-        }
-    }
+    face_clip<0>(phi, ft, L, cs, ff, fcl, amb_face);
+    face_clip<1>(phi, ft, L, cs, ff, fcl, amb_face);
+    face_clip<2>(phi, ft, L, cs, ff, fcl, amb_face);
+    face_clip<3>(phi, ft, L, cs, ff, fcl, amb_face);
+    face_clip<4>(phi, ft, L, cs, ff, fcl, amb_face);
+    face_clip<5>(phi, ft, L, cs, ff, fcl, amb_face);
 
     // ---- edge crossings on cell-local corners ----
-    T pts[13][3];
-    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+    // The tiling codes index the 13 points at run time, so they live in this lane's LDS slots
+    // P[(3 e + x) * MC_CUT_THREADS] (lane-major: consecutive lanes, consecutive banks), not in a
+    // private array (which would go to scratch).
+    auto pts = [&](int e, int x) -> T & { return P[(3 * e + x) * MC_CUT_THREADS]; };
+    T acc[3] = {T(0), T(0), T(0)};
     int ncr = 0;
-#pragma unroll
-    for (int e = 0; e < 12; ++e) {
-        const int a = EDGE_NODE[e][0], b = EDGE_NODE[e][1];
-        const bool crossed = (phi[a] >= T(0)) != (phi[b] >= T(0));
-        const T pa[3] = {bitx(a) ? L[0] : T(0), bity(a) ? L[1] : T(0), bitz(a) ? L[2] : T(0)};
-        const T pb[3] = {bitx(b) ? L[0] : T(0), bity(b) ? L[1] : T(0), bitz(b) ? L[2] : T(0)};
-        if (crossed) {
-            const T s = open_fraction(phi[a], phi[b]);
-#pragma unroll
-            for (int x = 0; x < 3; ++x)
-                pts[e][x] = phi[a] >= T(0) ? pa[x] + s * (pb[x] - pa[x]) : pb[x] + s * (pa[x] - pb[x]);
-            acc0 += pts[e][0];
-            acc1 += pts[e][1];
-            acc2 += pts[e][2];
-            ncr += 1;
-        } else {
-            pts[e][0] = pts[e][1] = pts[e][2] = T(0);
-        }
-    }
-    pts[12][0] = ncr > 0 ? acc0 / T(ncr) : T(0);
-    pts[12][1] = ncr > 0 ? acc1 / T(ncr) : T(0);
-    pts[12][2] = ncr > 0 ? acc2 / T(ncr) : T(0);
+    edge_point<0>(phi, L, P, acc, ncr);
+    edge_point<1>(phi, L, P, acc, ncr);
+    edge_point<2>(phi, L, P, acc, ncr);
+    edge_point<3>(phi, L, P, acc, ncr);
+    edge_point<4>(phi, L, P, acc, ncr);
+    edge_point<5>(phi, L, P, acc, ncr);
+    edge_point<6>(phi, L, P, acc, ncr);
+    edge_point<7>(phi, L, P, acc, ncr);
+    edge_point<8>(phi, L, P, acc, ncr);
+    edge_point<9>(phi, L, P, acc, ncr);
+    edge_point<10>(phi, L, P, acc, ncr);
+    edge_point<11>(phi, L, P, acc, ncr);
+    pts(12, 0) = ncr > 0 ? acc[0] / T(ncr) : T(0);
+    pts(12, 1) = ncr > 0 ? acc[1] / T(ncr) : T(0);
+    pts(12, 2) = ncr > 0 ? acc[2] / T(ncr) : T(0);
 
     // ---- the closed polyhedron, by the divergence theorem ----
     T vol3 = T(0), cm[3] = {T(0), T(0), T(0)};
@@ -332,14 +408,14 @@ __device__ void mc_cut_cell(const T *phi, int lut, const T *xl, const T *L, cons
         for (int a = 0; a < 3; ++a) cm[a] += a == ax ? s * x_ax * x_ax * a_open : T(0);
     }
     const signed char *tile;
-    int ntri = cell_triangles(phi, lut, &tile);
+    int ntri = cell_triangles(phi, ft, lut, &tile);
     if (ntri < 0) ntri = 0;
     T iarea = T(0), icm[3] = {T(0), T(0), T(0)};
     for (int t = 0; t < ntri; ++t) {
         const int ia = tile[3 * t], ib = tile[3 * t + 1], id = tile[3 * t + 2];
-        const T a[3] = {pts[ia][0], pts[ia][1], pts[ia][2]};
-        const T b[3] = {pts[ib][0], pts[ib][1], pts[ib][2]};
-        const T d[3] = {pts[id][0], pts[id][1], pts[id][2]};
+        const T a[3] = {pts(ia, 0), pts(ia, 1), pts(ia, 2)};
+        const T b[3] = {pts(ib, 0), pts(ib, 1), pts(ib, 2)};
+        const T d[3] = {pts(id, 0), pts(id, 1), pts(id, 2)};
         T u[3], v[3], w[3], xc[3];
 #pragma unroll
         for (int x = 0; x < 3; ++x) { u[x] = b[x] - a[x]; v[x] = d[x] - a[x]; }
@@ -633,10 +709,11 @@ __device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L) {
     }
 }
 
-// One entry of the cut-cell work list (if it exists): the full construction for that cell.
+// One entry of the cut-cell work list (if it exists): the full construction for that cell.  P is
+// the calling lane's slot in a workgroup array of 39 * MC_CUT_THREADS T in LDS.
 template <typename T>
 __device__ __forceinline__ void mc_cut_item(const T *__restrict__ phi, const GridT<T> &g,
-                                            const Cols<T> &cols, uint32_t c) {
+                                            const Cols<T> &cols, uint32_t c, T *P) {
     const uint32_t n0 = g.n[0], n1 = g.n[1];
     const size_t sy = n0 + 1, sz = sy * (n1 + 1);
     const uint32_t i = c % n0, r = c / n0, j = r % n1, k = r / n1;
@@ -649,7 +726,7 @@ __device__ __forceinline__ void mc_cut_item(const T *__restrict__ phi, const Gri
         xl[a] = g.x0[a] + T(ijk[a]) * g.d[a];
         L[a] = (g.x0[a] + T(ijk[a] + 1) * g.d[a]) - xl[a];
     }
-    mc_cut_cell(ph, lut, xl, L, g.d, cols, c);
+    mc_cut_cell(ph, lut, xl, L, g.d, cols, c, P);
 }
 
 }  // namespace

@@ -296,19 +296,30 @@ __global__ __launch_bounds__(256) void k_out_dz(const float *__restrict__ dphi, 
     dz[o] = f2bf(dphi[p] * bf2f(f2bf(wout[k])) * dsilu(zlast[o]));
 }
 
-// db[col] += sum over rows of w[row] * dz[row][col] (w == nullptr: weight 1); one atomic per column
-// and 256-row block
-__global__ __launch_bounds__(256) void k_colsum(const u16 *__restrict__ dz, const float *__restrict__ w, int n,
-                                                int W, float *__restrict__ db) {
-    const int col = blockIdx.x * 256 + threadIdx.x;
-    if (col >= W) return;
-    const int r0 = blockIdx.y * 256, r1 = min(n, r0 + 256);
-    float s = 0.0f;
-    if (w)
-        for (int r = r0; r < r1; ++r) s += w[r] * bf2f(dz[(size_t)r * W + col]);
-    else
-        for (int r = r0; r < r1; ++r) s += bf2f(dz[(size_t)r * W + col]);
-    atomicAdd(&db[col], s);
+// db[col] += sum over rows of w[row] * dz[row][col] (w == nullptr: weight 1), W == WIDTH.  A
+// workgroup takes COLSUM_ROWS rows: each thread two adjacent columns (one 32-bit load) and every
+// fourth row, eight rows in flight; the four row phases meet in LDS; one atomic per column.
+constexpr int COLSUM_ROWS = 64;
+__global__ __launch_bounds__(512) void k_colsum(const u16 *__restrict__ dz, const float *__restrict__ w, int n,
+                                                float *__restrict__ db) {
+    __shared__ float part[4][WIDTH];
+    const int t = threadIdx.x, c2 = (t & 127) * 2, ph = t >> 7;
+    const int r0 = blockIdx.x * COLSUM_ROWS;
+    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll 8
+    for (int q = 0; q < COLSUM_ROWS / 4; ++q) {
+        const int r = r0 + ph + 4 * q;
+        if (r < n) {
+            const uint32_t v = *reinterpret_cast<const uint32_t *>(dz + (size_t)r * WIDTH + c2);
+            const float wr = w ? w[r] : 1.0f;
+            s0 += wr * bf2f((u16)(v & 0xffffu));
+            s1 += wr * bf2f((u16)(v >> 16));
+        }
+    }
+    part[ph][c2] = s0;
+    part[ph][c2 + 1] = s1;
+    __syncthreads();
+    if (t < WIDTH) atomicAdd(&db[t], (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]));
 }
 
 // Adam on the FP32 master parameters; refreshes the BF16 transposed copy of a weight matrix.
@@ -425,8 +436,9 @@ void enqueue_loss_grad(cc_mlp *m, int n) {
                        m->grads + m->off_bout, m->loss);
     hipLaunchKernelGGL(k_out_dz, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, s, m->dphi, n, W,
                        m->params + m->off_wout, m->zs + (D - 1) * L, m->dz0);
-    hipLaunchKernelGGL(k_colsum, dim3((W + 255) / 256, (n + 255) / 256), dim3(256), 0, s, m->as + (D - 1) * L,
-                       (const float *)m->dphi, n, W, m->grads + m->off_wout);
+    const dim3 gcs((n + COLSUM_ROWS - 1) / COLSUM_ROWS);
+    hipLaunchKernelGGL(k_colsum, gcs, dim3(512), 0, s, m->as + (D - 1) * L, (const float *)m->dphi, n,
+                       m->grads + m->off_wout);
     u16 *dz = m->dz0, *dzp = m->dz1;
     // split the batch of the dW products into equal chunks of whole K steps: split divides n / KS
     int split = std::max(1, std::min(64, n / 256));
@@ -439,8 +451,8 @@ void enqueue_loss_grad(cc_mlp *m, int n) {
         hipLaunchKernelGGL((k_gemm<true, false, 0>), dim3(W / TILE, K / TILE, split), dim3(256), 0, s, ain,
                            lda, dz, W, m->grads + m->off_w[l], (u16 *)nullptr, W, K, W, n,
                            (const float *)nullptr, 0);
-        hipLaunchKernelGGL(k_colsum, dim3((W + 255) / 256, (n + 255) / 256), dim3(256), 0, s, dz,
-                           (const float *)nullptr, n, W, m->grads + m->off_b[l]);
+        hipLaunchKernelGGL(k_colsum, gcs, dim3(512), 0, s, dz, (const float *)nullptr, n,
+                           m->grads + m->off_b[l]);
         if (l > 0) {
             // dz_prev (n x W) = (dz (n x W) . W[l]^T (W x W)) * silu'(z_prev); W[l]^T via wt[l] = [W][K]
             hipLaunchKernelGGL((k_gemm<false, false, 1>), dim3(W / TILE, n / TILE, 1), dim3(256), 0, s, dz,

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_cull(GridT<T> g, Geo geo, T delta, int 
 }
 
 // The sphere-union fold over a brick's kept primitives, in the order and with the arithmetic of
-// union_dist (cc_device.h).  Primitive 0 is always kept.
+// union_dist (cc_device.h, fold_sphere).  Primitive 0 is always kept.
 template <typename T>
 __device__ __forceinline__ T culled_fold(const T *sph, uint32_t nk, const uint16_t *L, const Geo &geo,
                                          T x, T y, T z) {
@@ -105,15 +105,13 @@ __device__ __forceinline__ T culled_fold(const T *sph, uint32_t nk, const uint16
     if (nk <= (uint32_t)CULL_LDS) {
         acc = sphere_dist<T, 3>(sph[0], sph[1], sph[2], sph[3], x, y, z);
         for (uint32_t e = 1; e < nk; ++e)
-            acc = smin_poly(acc, sphere_dist<T, 3>(sph[4 * e], sph[4 * e + 1], sph[4 * e + 2], sph[4 * e + 3],
-                                                   x, y, z),
-                            k);
+            acc = fold_sphere<T, 3>(acc, sph[4 * e], sph[4 * e + 1], sph[4 * e + 2], sph[4 * e + 3], x, y, z, k);
     } else {
         const double *s = geo.prims + 4 * (size_t)L[0];
         acc = sphere_dist<T, 3>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), x, y, z);
         for (uint32_t e = 1; e < nk; ++e) {
             s = geo.prims + 4 * (size_t)L[e];
-            acc = smin_poly(acc, sphere_dist<T, 3>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), x, y, z), k);
+            acc = fold_sphere<T, 3>(acc, T(s[0]), T(s[1]), T(s[2]), T(s[3]), x, y, z, k);
         }
     }
     return acc;

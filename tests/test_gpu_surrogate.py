@@ -183,3 +183,85 @@ def test_what_the_trained_network_gains():
     assert ft < 0.5 * fu
     assert st["sign_fail"] < 0.05 * max(su["sign_fail"], 1)
     assert err_t < 0.25 * err_u and err_t < 4 * g.d[0]
+
+
+def _plain_vs(source, body, g, dtype=np.float64, net=None):
+    plain = allocate_cache(g, MarchingCubesCutCell, dtype)
+    other = allocate_cache(g, MarchingCubesCutCell, dtype)
+    if source == "surrogate":
+        other.set_phi_source("surrogate", net, coarse=4)
+    else:
+        other.set_phi_source(source)
+    update_cache(plain, body, g)
+    update_cache(other, body, g)
+    return plain, other
+
+
+def test_culled_fold_with_more_kept_primitives_than_lds_holds():
+    # 700 nearly coincident spheres: no bound separates them, every brick near them keeps all 700
+    rng = np.random.default_rng(12)
+    cen = 0.1 + 1e-3 * rng.standard_normal((700, 3))
+    rad = 0.3 + 1e-3 * rng.standard_normal(700)
+    body = SDFSphereUnion(cen, rad, 0.02)
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=(24, 20, 18))
+    plain, culled = _plain_vs("culled", body, g)
+    st = culled.phi_stats()
+    assert st["kept_sum"] > 512 * 4  # several bricks over the 512-primitive LDS stage
+    assert np.array_equal(plain.phi.view(np.uint64), culled.phi.view(np.uint64))
+    for name in COLS:
+        assert np.array_equal(plain.read(name).view(np.uint8), culled.read(name).view(np.uint8)), name
+
+
+@pytest.mark.parametrize("n", [(1, 1, 1), (5, 3, 2), (17, 31, 15), (33, 16, 47)])
+def test_culled_and_surrogate_on_ragged_grids(n):
+    body = csg_spheres(6, count=40, lo=-0.6, hi=0.6, rmin=0.1, rmax=0.3)
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=n)
+    plain, culled = _plain_vs("culled", body, g)
+    assert np.array_equal(plain.phi.view(np.uint64), culled.phi.view(np.uint64))
+    net = Surrogate(seed=3)
+    st, pe, ps, diff = check_surrogate_against_plain(body, g, net, coarse=4)
+    assert st["nodes"] == int(np.prod(np.array(n) + 1))
+
+
+def test_single_primitive_union_and_moved_grid():
+    body = SDFSphereUnion(np.array([[0.05, -0.1, 0.2]]), np.array([0.45]), 0.02)
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=(40, 36, 44))
+    for shift in (0.0, 0.37 * g.d[0], -2.5 * g.d[0]):
+        gm = g.moved((shift, 0.5 * shift, -shift))
+        plain, culled = _plain_vs("culled", body, gm)
+        assert np.array_equal(plain.phi.view(np.uint64), culled.phi.view(np.uint64))
+        for name in COLS:
+            assert np.array_equal(plain.read(name).view(np.uint8), culled.read(name).view(np.uint8)), name
+
+
+def test_surrogate_on_a_float32_cache():
+    body, g = union_case(n=(48, 40, 36))
+    net = Surrogate(seed=5)
+    net.train(body, nsamples=1 << 18, epochs=4, batch=16384)
+    plain, sur = _plain_vs("surrogate", body, g, np.float32, net)
+    for name in COLS:
+        assert np.array_equal(plain.read(name).view(np.uint8), sur.read(name).view(np.uint8)), name
+    pe, ps = plain.phi, sur.phi
+    assert np.array_equal(pe > 0, ps > 0)
+    corners = cut_corner_nodes(plain.read("kind"), g.n)
+    assert np.array_equal(pe[corners].view(np.uint32), ps[corners].view(np.uint32))
+
+
+def test_phi_source_argument_errors():
+    from cutcell import ArgumentError, MarchingSquaresCutCell, PLICCutCell, Unsupported
+    g3 = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=(8, 8, 8))
+    c = allocate_cache(g3, MarchingCubesCutCell)
+    with pytest.raises(ArgumentError):
+        c.set_phi_source("surrogate", None)
+    net = Surrogate(seed=1)
+    with pytest.raises(ArgumentError):
+        c.set_phi_source("surrogate", net, coarse=0)
+    with pytest.raises(ArgumentError):
+        c.set_phi_source("surrogate", net, band_cells=-1.0)
+    p = allocate_cache(g3, PLICCutCell)
+    with pytest.raises(Unsupported):
+        p.set_phi_source("culled")
+    g2 = CartesianIsoGrid((-1.0, -1.0), (1.0, 1.0), n=(8, 8))
+    m = allocate_cache(g2, MarchingSquaresCutCell)
+    with pytest.raises(Unsupported):
+        m.set_phi_source("culled")
