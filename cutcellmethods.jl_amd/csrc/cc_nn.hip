@@ -111,9 +111,10 @@ __global__ __launch_bounds__(256) void k_mlp_forward(MlpDev m, const float *__re
             const float u0 = (xq[0] - m.in_lo) * m.in_scale, u1 = (xq[1] - m.in_lo) * m.in_scale,
                         u2 = (xq[2] - m.in_lo) * m.in_scale;
             const float *b = m.ffB + 3 * f;
-            const float arg = 6.283185307179586f * ((b[0] * u0 + b[1] * u1) + b[2] * u2);
-            s = sinf(arg);
-            c = cosf(arg);
+            // sin and cos of 2 pi (B . u): the hardware sine and cosine take revolutions
+            const float rev = (b[0] * u0 + b[1] * u1) + b[2] * u2;
+            s = __builtin_amdgcn_sinf(rev);
+            c = __builtin_amdgcn_cosf(rev);
         }
         buf[p * LDF + f] = f2bf(s);
         buf[p * LDF + m.F + f] = f2bf(c);
@@ -177,6 +178,111 @@ __global__ __launch_bounds__(256) void k_mlp_forward(MlpDev m, const float *__re
         if ((t & 3) == 0 && p0 + p < n) phi[p0 + p] = s + bo;
     }
 }
+
+// ---- inference forward --------------------------------------------------------------------------
+// The same network for evaluation only, on 128-point tiles, with the matrix-core operands swapped:
+// A = the transposed weights (rows = output features, 8 consecutive k per lane, read from L2 with
+// the next K step's fragments loaded while the current step's MFMAs run), B = the activation tile
+// (columns = points).  Each wave owns 64 output features for all 128 points (4 x 8 MFMA tiles), and
+// its accumulators hold 4 consecutive features of one point per lane, so the epilogue -- bias,
+// SiLU with the hardware reciprocal, one packed BF16 conversion -- ends in one 8-byte LDS store per
+// four values.  The activation tile is [128][LDW] BF16 (67.6 KB), updated in place between layers.
+constexpr int ITILE = 128;
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+template <bool LAT>
+__global__ __launch_bounds__(256, 2) void k_mlp_infer(MlpDev m, const float *__restrict__ x, int n,
+                                                     float *__restrict__ phi, cc::Lattice lat) {
+    extern __shared__ __attribute__((aligned(16))) u16 ibuf[];  // [ITILE][LDW], point-major
+    __shared__ float wo[WIDTH];                                  // BF16-rounded output weights
+    const int F2 = 2 * m.F, LDF = F2 + 8;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int p0 = blockIdx.x * ITILE;
+    for (int e = t; e < ITILE * m.F; e += 256) {
+        const int p = e / m.F, f = e - p * m.F;
+        float sv = 0.0f, cv = 1.0f;
+        if (p0 + p < n) {
+            float xq[3];
+            if (LAT) {
+                const int q = p0 + p, i = q % lat.n[0], r = q / lat.n[0];
+                const int j = r % lat.n[1], k = r / lat.n[1];
+                xq[0] = lat.x0[0] + (float)i * lat.h[0];
+                xq[1] = lat.x0[1] + (float)j * lat.h[1];
+                xq[2] = lat.x0[2] + (float)k * lat.h[2];
+            } else {
+                xq[0] = x[3 * (size_t)(p0 + p)];
+                xq[1] = x[3 * (size_t)(p0 + p) + 1];
+                xq[2] = x[3 * (size_t)(p0 + p) + 2];
+            }
+            const float u0 = (xq[0] - m.in_lo) * m.in_scale, u1 = (xq[1] - m.in_lo) * m.in_scale,
+                        u2 = (xq[2] - m.in_lo) * m.in_scale;
+            const float *b = m.ffB + 3 * f;
+            const float rev = (b[0] * u0 + b[1] * u1) + b[2] * u2;  // revolutions
+            sv = __builtin_amdgcn_sinf(rev);
+            cv = __builtin_amdgcn_cosf(rev);
+        }
+        ibuf[p * LDF + f] = f2bf(sv);
+        ibuf[p * LDF + m.F + f] = f2bf(cv);
+    }
+    wo[t] = bf2f(f2bf(m.wout[t]));
+    __syncthreads();
+    int K = F2, LDin = LDF;
+    const int arow = 64 * wave + (lane & 15), koff = 8 * (lane >> 4);
+    for (int l = 0; l < m.depth; ++l) {
+        f32x4 acc[4][8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const u16 *Wt = m.Wt[l];
+        bf16x8 a[4], an[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = load8(Wt + (size_t)(arow + 16 * i) * K + koff);
+        for (int k0 = 0; k0 < K; k0 += KS) {
+            const int kn = k0 + KS < K ? k0 + KS : k0;  // next step's fragments (the last reloads its own)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) an[i] = load8(Wt + (size_t)(arow + 16 * i) * K + kn + koff);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const bf16x8 b = load8(ibuf + (16 * j + (lane & 15)) * LDin + k0 + koff);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[i][j] = mfma(a[i], b, acc[i][j]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = an[i];
+        }
+        __syncthreads();  // every wave has read the whole input tile
+        const float *bias = m.bias[l];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int f0 = 64 * wave + 16 * i + 4 * (lane >> 4);
+            const f32x4 bv = *reinterpret_cast<const f32x4 *>(bias + f0);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int p = 16 * j + (lane & 15);
+                const bf16x4 pk = {(__bf16)silu_fast(acc[i][j][0] + bv[0]), (__bf16)silu_fast(acc[i][j][1] + bv[1]),
+                                   (__bf16)silu_fast(acc[i][j][2] + bv[2]), (__bf16)silu_fast(acc[i][j][3] + bv[3])};
+                *reinterpret_cast<bf16x4 *>(ibuf + p * LDW + f0) = pk;
+            }
+        }
+        __syncthreads();
+        K = WIDTH;
+        LDin = LDW;
+    }
+    // output layer: two lanes per point, 128 of the 256 k each
+    const float bo = *m.bout;
+    {
+        const int p = t >> 1, h = t & 1;
+        float sacc = 0.0f;
+        for (int k = h; k < WIDTH; k += 2) sacc += bf2f(ibuf[p * LDW + k]) * wo[k];
+        sacc += __shfl_xor(sacc, 1);
+        if (h == 0 && p0 + p < n) phi[p0 + p] = sacc + bo;
+    }
+}
+
+constexpr size_t INFER_LDS = (size_t)ITILE * LDW * sizeof(u16);
 
 // ---- generic GEMM for the backward pass ---------------------------------------------------------
 // C[M x N] = op(A)[M x K] op(B)[K x N]; op(A) = A (A stored M x K, row stride lda) or A^T (A stored
@@ -509,9 +615,8 @@ namespace cc {
 int launch_mlp_lattice(const cc_mlp *m, const Lattice &lat, float *out, hipStream_t s) {
     const long long n = (long long)lat.n[0] * lat.n[1] * lat.n[2];
     if (n < 1 || n > (1ll << 31) - TILE) return CC_ERR_ARGUMENT;
-    hipLaunchKernelGGL((k_mlp_forward<false, true>), dim3((unsigned)((n + TILE - 1) / TILE)), dim3(256), 0, s,
-                       dev_view(m), (const float *)nullptr, (int)n, out, (u16 *)nullptr, (float *)nullptr,
-                       (u16 *)nullptr, lat);
+    hipLaunchKernelGGL((k_mlp_infer<true>), dim3((unsigned)((n + ITILE - 1) / ITILE)), dim3(256), INFER_LDS, s,
+                       dev_view(m), (const float *)nullptr, (int)n, out, lat);
     return CC_OK;
 }
 
@@ -562,6 +667,11 @@ int cc_mlp_create(int nfreq, int width, int depth, float sigma, uint64_t seed, c
         std::uniform_real_distribution<float> ud(-a, a);
         for (int i = 0; i < width; ++i) p[m->off_wout + i] = ud(rng);
     }
+    // the inference kernel's activation tile exceeds the default 64 KB of dynamic LDS
+    hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_infer<true>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)INFER_LDS);
+    hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_infer<false>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)INFER_LDS);
     if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) || hipMalloc(&m->ffB, 12 * nfreq) ||
         hipMalloc(&m->params, 4 * m->nparam) || hipMalloc(&m->grads, 4 * m->nparam) ||
         hipMalloc(&m->m1, 4 * m->nparam) || hipMalloc(&m->m2, 4 * m->nparam)) {
@@ -640,9 +750,8 @@ int cc_mlp_eval(cc_mlp *m, const float *x, int64_t n, float *y) {
         m->ecap = np;
     }
     hipMemcpyAsync(m->ex, x, 12 * n, hipMemcpyHostToDevice, m->stream);
-    hipLaunchKernelGGL((k_mlp_forward<false, false>), dim3(np / TILE), dim3(256), 0, m->stream, dev_view(m),
-                       m->ex, (int)n, m->ephi, (u16 *)nullptr, (float *)nullptr, (u16 *)nullptr,
-                       cc::Lattice{});
+    hipLaunchKernelGGL((k_mlp_infer<false>), dim3((unsigned)((n + ITILE - 1) / ITILE)), dim3(256), INFER_LDS,
+                       m->stream, dev_view(m), m->ex, (int)n, m->ephi, cc::Lattice{});
     hipMemcpyAsync(y, m->ephi, 4 * n, hipMemcpyDeviceToHost, m->stream);
     if (hipStreamSynchronize(m->stream) != hipSuccess)
         return nfail(CC_ERR_HIP, hipGetErrorString(hipGetLastError()));

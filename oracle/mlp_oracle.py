@@ -45,9 +45,11 @@ def forward(p, B, x, F, width, depth, emulate=True):
     Ws, bs, wout, bout = unpack(p, F, width, depth)
     x = np.asarray(x, dtype=dt)
     B = np.asarray(B, dtype=dt)
-    arg = dt(2 * np.pi) * ((B[:, 0][None, :] * x[:, 0:1] + B[:, 1][None, :] * x[:, 1:2]) +
-                           B[:, 2][None, :] * x[:, 2:3])
-    a = r(np.concatenate([np.sin(arg), np.cos(arg)], axis=1))
+    # the device forms B . x in the working precision and takes sin and cos of 2 pi times it (its
+    # hardware sine works in revolutions): emulated as the float64 sine of the rounded dot product
+    rev = (B[:, 0][None, :] * x[:, 0:1] + B[:, 1][None, :] * x[:, 1:2]) + B[:, 2][None, :] * x[:, 2:3]
+    arg = 2 * np.pi * rev.astype(np.float64)
+    a = r(np.concatenate([np.sin(arg), np.cos(arg)], axis=1).astype(dt))
     acts, zs = [a], []
     for l in range(depth):
         z = (a.astype(dt) @ r(Ws[l]).astype(dt)).astype(dt) + bs[l].astype(dt)

@@ -265,3 +265,23 @@ def test_phi_source_argument_errors():
     m = allocate_cache(g2, MarchingSquaresCutCell)
     with pytest.raises(Unsupported):
         m.set_phi_source("culled")
+
+
+@pytest.mark.parametrize("tune", [(0, 16, 0), (1, 3, 0), (1, 16, 1), (2, 64, 0), (2, 100, 0)])
+def test_culled_and_surrogate_under_every_stream_shape(tune):
+    """Every launch shape of the stream pass reads the phi the brick pass wrote, update after update."""
+    body = csg_spheres(9, count=48, lo=-0.6, hi=0.6, rmin=0.08, rmax=0.25)
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=(40, 36, 150))  # several 64-layer slabs
+    plain = allocate_cache(g, MarchingCubesCutCell)
+    update_cache(plain, body, g)
+    net = Surrogate(seed=4)
+    for source in ("culled", "surrogate"):
+        c = allocate_cache(g, MarchingCubesCutCell)
+        c.set_tuning(*tune)
+        c.set_phi_source(source, net if source == "surrogate" else None, coarse=4)
+        for step in range(2):  # the second update reuses the streams and events
+            update_cache(c, body, g)
+        for name in COLS:
+            assert np.array_equal(plain.read(name).view(np.uint8), c.read(name).view(np.uint8)), (source, name)
+        if source == "culled":
+            assert np.array_equal(plain.phi.view(np.uint64), c.phi.view(np.uint64))
