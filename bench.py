@@ -37,8 +37,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget of the oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune", default="2,8,0",
-                    help="stream-pass launch shape: variant,kz,nontemporal")
+    ap.add_argument("--tune", default="auto",
+                    help="stream-pass launch shape: variant,kz,nontemporal, or 'auto' (the library's "
+                         "autotuner picks among the fused z-march shapes on this GPU before warm-up)")
     ap.add_argument("--quick", action="store_true",
                     help="only the headline workload (skip the secondary workloads)")
     ap.add_argument("--only", default=None, choices=["csg", "batched", "secondary"],
@@ -154,6 +155,14 @@ def fp32_error(grid, geo, cache64):
         a = cache64.read(name, lo, hi)
         b = c32.read(name, lo, hi).astype(np.float64)
         err[name + "_max_abs"] = float(np.max(np.abs(a - b)))
+    # the centroid is the first moment divided by the volume: ill-conditioned for slivers
+    vf = cache64.read("volume_fraction", lo, hi)
+    dc = np.max(np.abs(cache64.read("centroid", lo, hi) - c32.read("centroid", lo, hi).astype(np.float64)),
+                axis=1)
+    for lim in (1e-6, 1e-3):
+        sel = (k64 == 2) & (vf >= lim) & (vf <= 1.0 - lim)
+        err[f"centroid_max_abs_cut_vf_{lim:g}_to_1-{lim:g}"] = float(dc[sel].max()) if sel.any() else 0.0
+    err["cell_size"] = float(grid.d[0])
     del c32
     return {"update_ms": round(t32 * 1e3, 3), "cells_per_s": grid.ncells / t32,
             "error_vs_fp64_on_16_layers": err}
@@ -331,8 +340,6 @@ def main():
     grid, geo, centre = workload(a.n, a.seed)
     cache = allocate_cache(grid, MarchingCubesCutCell)
     cache.set_profiling(True)
-    tv, tk, tn = (int(x) for x in a.tune.split(","))
-    cache.set_tuning(tv, tk, bool(tn))
     d = grid.d[0]
 
     def grid_at(step):
@@ -340,6 +347,15 @@ def main():
 
     from cutcell.cache import probe_bandwidth
     fill_gbs, copy_gbs = probe_bandwidth(8 << 30, 5)  # also brings the memory clocks up
+    tuning = None
+    if a.tune == "auto":
+        from cutcell.autotune import tune_stream
+        cands = [(2, kz, nt) for kz in (4, 8, 16) for nt in (False, True)]
+        tuning = tune_stream(cache, geo, grid_at(0), candidates=cands, repeats=4, use_cache=False)
+        tv, tk, tn = tuning["variant"], tuning["kz"], int(tuning["nontemporal"])
+    else:
+        tv, tk, tn = (int(x) for x in a.tune.split(","))
+    cache.set_tuning(tv, tk, bool(tn))
     for s in range(a.warmup):
         update_cache(cache, geo, grid_at(s))
     per_kernel = {}
@@ -402,6 +418,8 @@ def main():
         "config": {"workload": f"MarchingCubesCutCell update_cache! {a.n}^3 Float64, sphere r=0.4 "
                                f"centre {tuple(round(c, 4) for c in centre)}, moving 0.5 cell/step",
                    "cells": ncell, "cut_cells": cache.cut_count(),
+                   "stream_tuning": {"variant": tv, "kz": tk, "nontemporal": bool(tn),
+                                     "chosen_by": "autotuner" if tuning else "--tune"},
                    "update_hbm_fraction": round(update_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
