@@ -326,8 +326,9 @@ def batched(seed, steps=32):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not offered (the nodal update shards "
-              "by z-slab with no exchange -- see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not offered (the nodal update would "
+              "shard by z-slab with no exchange, each GPU recomputing its boundary node layer)",
+              file=sys.stderr)
         sys.exit(2)
     from cutcell import MarchingCubesCutCell, allocate_cache, update_cache
 
