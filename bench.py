@@ -233,11 +233,15 @@ def csg(seed, n=512, train_epochs=24, train_samples=1 << 22):
     out["training"] = {"seconds": round(tt, 2), "samples": train_samples, "epochs": train_epochs,
                        "batch": 16384, "final_loss": float(hist[-1]), "first_epoch_loss": float(hist[0]),
                        "approx_tflop": round(tflop, 2), "approx_tflops": round(tflop / tt, 1)}
+    from cutcell.autotune import tune_stream
+    # the stream pass reads phi here: z-march depths that reuse node layers, or flat tiles
+    cands = [(1, 4, False), (1, 8, False), (1, 16, False), (1, 8, True), (0, 16, False)]
     cache = allocate_cache(grid, MarchingCubesCutCell)
     cache.set_profiling(True)
     res = {}
     for src, reps in (("naive", 3), ("culled", 6)):
         cache.set_phi_source(src)
+        tune = tune_stream(cache, body, at(0), candidates=cands, repeats=2, use_cache=False)
         update_cache(cache, body, at(0))
         ts, kts = [], []
         for s in range(reps):
@@ -246,13 +250,15 @@ def csg(seed, n=512, train_epochs=24, train_samples=1 << 22):
             ts.append(time.perf_counter() - t)
             kts.append(dict(cache.kernel_times()))
         res[src] = {"update_ms": round(1e3 * float(np.median(ts)), 3),
-                    "kernel_ms": {k: round(float(np.median([kt[k] for kt in kts])), 4) for k in kts[0]}}
+                    "kernel_ms": {k: round(float(np.median([kt[k] for kt in kts])), 4) for k in kts[0]},
+                    "stream_tuning": [tune["variant"], tune["kz"], tune["nontemporal"]]}
         if src == "culled":
             st = cache.phi_stats()
             res[src]["mean_kept_spheres_per_brick"] = round(st["kept_sum"] / st["bricks"], 2)
     sur = allocate_cache(grid, MarchingCubesCutCell)
     sur.set_profiling(True)
     sur.set_phi_source("surrogate", net, coarse=8, band_cells=3.0)
+    stune = tune_stream(sur, body, at(0), candidates=cands, repeats=2, use_cache=False)
     update_cache(sur, body, at(0))
     ts, kts = [], []
     for s in range(8):
@@ -267,6 +273,7 @@ def csg(seed, n=512, train_epochs=24, train_samples=1 << 22):
     nn_tflops = lat_points * MLP_FLOP_PER_POINT / (km["nn_lattice"] * 1e-3) / 1e12
     res["surrogate"] = {
         "update_ms": round(1e3 * float(np.median(ts)), 3), "kernel_ms": km,
+        "stream_tuning": [stune["variant"], stune["kz"], stune["nontemporal"]],
         "coarse_lattice": f"{nc}^3 (8 cells)", "band": "3h",
         "exact_node_fraction": round(st["exact"] / st["nodes"], 4),
         "uncertified_fraction": round(st["uncertified"] / st["nodes"], 4),
@@ -351,7 +358,7 @@ def main():
     tuning = None
     if a.tune == "auto":
         from cutcell.autotune import tune_stream
-        cands = [(2, kz, nt) for kz in (4, 8, 16) for nt in (False, True)]
+        cands = [(2, kz, nt) for kz in (1, 2, 4, 8) for nt in (False, True)]
         tuning = tune_stream(cache, geo, grid_at(0), candidates=cands, repeats=4, use_cache=False)
         tv, tk, tn = tuning["variant"], tuning["kz"], int(tuning["nontemporal"])
     else:

@@ -32,7 +32,7 @@ void launch_sample(T *phi, const GridT<T> &g, const Geo &geo, int dim, hipStream
 enum McVariant { MC_FLAT = 0, MC_MARCH = 1, MC_MARCH_FUSED = 2 };
 struct McTuning {
     int variant = MC_MARCH_FUSED;  // fused needs an analytic body; falls back to MC_FLAT otherwise
-    int kz = 16;                   // z-march depth (layers per workgroup)
+    int kz = 4;                    // z-march depth (layers per workgroup)
     int nontemporal = 0;           // streaming (non-temporal) stores for the column writes
 };
 

@@ -15,9 +15,9 @@ import os
 
 import numpy as np
 
-DEFAULT_STREAM_CANDIDATES = [(2, 4, False), (2, 8, False), (2, 16, False), (2, 32, False),
-                             (2, 8, True), (1, 8, False), (1, 16, False), (0, 16, False),
-                             (0, 16, True)]
+DEFAULT_STREAM_CANDIDATES = [(2, 1, False), (2, 2, False), (2, 4, False), (2, 8, False), (2, 16, False),
+                             (2, 1, True), (2, 4, True), (1, 4, False), (1, 8, False), (1, 16, False),
+                             (0, 16, False), (0, 16, True)]
 
 
 def _path():

@@ -150,7 +150,7 @@ class Cache:
         check(lib().cc_stream(self._h, ctypes.byref(s)))
         return s.value
 
-    def set_tuning(self, variant=2, kz=16, nontemporal=False):
+    def set_tuning(self, variant=2, kz=4, nontemporal=False):
         """Launch shape of the marching-cubes stream pass (0 flat, 1 z-march, 2 fused z-march)."""
         check(lib().cc_set_tuning(self._h, int(variant), int(kz), 1 if nontemporal else 0))
 
