@@ -596,6 +596,9 @@ int cc_set_phi_source(cc_cache *c, int source, cc_mlp *net, int coarse, double b
         return fail(CC_ERR_UNSUPPORTED, "culled and surrogate phi serve the marching-cubes cache");
     if (source == CC_PHI_SURROGATE) {
         if (!net) return fail(CC_ERR_ARGUMENT, "the surrogate source needs a network");
+        if (cc::mlp_device(net) != c->device)
+            return fail(CC_ERR_ARGUMENT, "the network lives on GPU %d, the cache on GPU %d",
+                        cc::mlp_device(net), c->device);
         if (coarse < 1 || coarse > 64) return fail(CC_ERR_ARGUMENT, "coarse spacing must be 1..64 cells");
         if (!(band_cells >= 0.0) || !std::isfinite(band_cells))
             return fail(CC_ERR_ARGUMENT, "band must be a finite number of cells >= 0");

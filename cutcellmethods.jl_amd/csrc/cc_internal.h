@@ -119,4 +119,6 @@ struct cc_mlp;
 namespace cc {
 // The network at every node of `lat` (n[0] n[1] n[2] values, i fastest) into `out`, on stream s.
 int launch_mlp_lattice(const cc_mlp *net, const Lattice &lat, float *out, hipStream_t s);
+// The GPU the network lives on.
+int mlp_device(const cc_mlp *net);
 }  // namespace cc

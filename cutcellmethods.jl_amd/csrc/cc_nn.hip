@@ -468,6 +468,7 @@ __global__ void k_exact_points(cc::Geo geo, const float *__restrict__ x, int n, 
 // ---- the model object -----------------------------------------------------------------------------
 struct cc_mlp {
     int F = 64, width = 256, depth = 4;
+    int device = 0;  // the GPU the model lives on (current when it was created)
     float sigma = 8.0f;
     float in_lo = 0.0f, in_scale = 1.0f;  // set by training: [lo, hi]^3 onto the unit box
     // parameter blocks (FP32 master): W[l] (K_l x width), b[l] (width), wout (width), bout (1)
@@ -514,10 +515,16 @@ MlpDev dev_view(const cc_mlp *m) {
     return d;
 }
 
+template <typename P> void release(P *&p) {
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
 int ensure_cap(cc_mlp *m, int n) {
     if (n <= m->cap) return CC_OK;
-    hipFree(m->x); hipFree(m->y); hipFree(m->phi); hipFree(m->zs); hipFree(m->feat); hipFree(m->as);
-    hipFree(m->dz0); hipFree(m->dz1); hipFree(m->loss); hipFree(m->dphi);
+    release(m->x); release(m->y); release(m->phi); release(m->zs); release(m->feat); release(m->as);
+    release(m->dz0); release(m->dz1); release(m->loss); release(m->dphi);
+    m->cap = 0;
     const size_t W = m->width, D = m->depth, F2 = 2 * m->F;
     if (hipMalloc(&m->x, 3 * sizeof(float) * n) || hipMalloc(&m->y, sizeof(float) * n) ||
         hipMalloc(&m->phi, sizeof(float) * n) || hipMalloc(&m->zs, sizeof(float) * D * n * W) ||
@@ -612,6 +619,8 @@ void adam_step(cc_mlp *m, float lr) {
 
 namespace cc {
 
+int mlp_device(const cc_mlp *m) { return m->device; }
+
 int launch_mlp_lattice(const cc_mlp *m, const Lattice &lat, float *out, hipStream_t s) {
     const long long n = (long long)lat.n[0] * lat.n[1] * lat.n[2];
     if (n < 1 || n > (1ll << 31) - TILE) return CC_ERR_ARGUMENT;
@@ -672,30 +681,42 @@ int cc_mlp_create(int nfreq, int width, int depth, float sigma, uint64_t seed, c
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)INFER_LDS);
     hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_infer<false>),
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)INFER_LDS);
-    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) || hipMalloc(&m->ffB, 12 * nfreq) ||
-        hipMalloc(&m->params, 4 * m->nparam) || hipMalloc(&m->grads, 4 * m->nparam) ||
-        hipMalloc(&m->m1, 4 * m->nparam) || hipMalloc(&m->m2, 4 * m->nparam)) {
+    hipGetDevice(&m->device);
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
         delete m;
+        return nfail(CC_ERR_HIP, "surrogate stream");
+    }
+    bool ok = hipMalloc(&m->ffB, 12 * nfreq) == hipSuccess && hipMalloc(&m->params, 4 * m->nparam) == hipSuccess &&
+              hipMalloc(&m->grads, 4 * m->nparam) == hipSuccess && hipMalloc(&m->m1, 4 * m->nparam) == hipSuccess &&
+              hipMalloc(&m->m2, 4 * m->nparam) == hipSuccess;
+    for (int l = 0; ok && l < depth; ++l) ok = hipMalloc(&m->wt[l], 2 * (size_t)m->K[l] * width) == hipSuccess;
+    if (!ok) {
+        cc_mlp_destroy(m);
         return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate parameters");
     }
-    hipMemcpy(m->ffB, m->ffB_host.data(), 12 * nfreq, hipMemcpyHostToDevice);
-    hipMemcpy(m->params, p.data(), 4 * m->nparam, hipMemcpyHostToDevice);
-    hipMemset(m->m1, 0, 4 * m->nparam);
-    hipMemset(m->m2, 0, 4 * m->nparam);
-    for (int l = 0; l < depth; ++l) {
-        hipMalloc(&m->wt[l], 2 * (size_t)m->K[l] * width);
+    // everything ordered on the model's (non-blocking) stream
+    ok = hipMemcpyAsync(m->ffB, m->ffB_host.data(), 12 * nfreq, hipMemcpyHostToDevice, m->stream) == hipSuccess &&
+         hipMemcpyAsync(m->params, p.data(), 4 * m->nparam, hipMemcpyHostToDevice, m->stream) == hipSuccess &&
+         hipMemsetAsync(m->m1, 0, 4 * m->nparam, m->stream) == hipSuccess &&
+         hipMemsetAsync(m->m2, 0, 4 * m->nparam, m->stream) == hipSuccess;
+    for (int l = 0; ok && l < depth; ++l) {
         const int cnt = m->K[l] * width;
         hipLaunchKernelGGL(k_to_bf16_t, dim3((cnt + 255) / 256), dim3(256), 0, m->stream,
                            m->params + m->off_w[l], m->wt[l], m->K[l], width);
     }
-    hipStreamSynchronize(m->stream);
+    if (!ok || hipStreamSynchronize(m->stream) != hipSuccess) {
+        const std::string msg = std::string("surrogate initialisation: ") + hipGetErrorString(hipGetLastError());
+        cc_mlp_destroy(m);
+        return nfail(CC_ERR_HIP, msg);
+    }
     *out = m;
     return CC_OK;
 }
 
 int cc_mlp_destroy(cc_mlp *m) {
     if (!m) return CC_OK;
-    hipStreamSynchronize(m->stream);
+    hipSetDevice(m->device);
+    if (m->stream) hipStreamSynchronize(m->stream);
     for (auto p : {(void *)m->ffB, (void *)m->params, (void *)m->grads, (void *)m->m1, (void *)m->m2,
                    (void *)m->x, (void *)m->y, (void *)m->phi, (void *)m->zs, (void *)m->feat,
                    (void *)m->as, (void *)m->dz0, (void *)m->dz1, (void *)m->loss, (void *)m->ex,
@@ -703,7 +724,7 @@ int cc_mlp_destroy(cc_mlp *m) {
         if (p) hipFree(p);
     for (auto w : m->wt)
         if (w) hipFree(w);
-    hipStreamDestroy(m->stream);
+    if (m->stream) hipStreamDestroy(m->stream);
     delete m;
     return CC_OK;
 }
@@ -717,6 +738,7 @@ int cc_mlp_num_params(cc_mlp *m, int64_t *n, int64_t *nfreq3) {
 
 int cc_mlp_get_params(cc_mlp *m, float *params, float *ffB) {
     if (!m) return nfail(CC_ERR_ARGUMENT, "NULL model");
+    hipSetDevice(m->device);
     if (params) {
         hipMemcpyAsync(params, m->params, 4 * m->nparam, hipMemcpyDeviceToHost, m->stream);
         if (hipStreamSynchronize(m->stream) != hipSuccess)
@@ -728,6 +750,7 @@ int cc_mlp_get_params(cc_mlp *m, float *params, float *ffB) {
 
 int cc_mlp_set_params(cc_mlp *m, const float *params) {
     if (!m || !params) return nfail(CC_ERR_ARGUMENT, "NULL argument");
+    hipSetDevice(m->device);
     hipMemcpyAsync(m->params, params, 4 * m->nparam, hipMemcpyHostToDevice, m->stream);
     for (int l = 0; l < m->depth; ++l) {
         const int cnt = m->K[l] * m->width;
@@ -740,11 +763,13 @@ int cc_mlp_set_params(cc_mlp *m, const float *params) {
 
 // phi_nn at n host points (x: n x 3 floats) into y (host).
 int cc_mlp_eval(cc_mlp *m, const float *x, int64_t n, float *y) {
-    if (!m || !x || !y || n < 1) return nfail(CC_ERR_ARGUMENT, "bad argument");
+    if (!m || !x || !y || n < 1 || n > (int64_t(1) << 30)) return nfail(CC_ERR_ARGUMENT, "bad argument");
+    hipSetDevice(m->device);
     const int np = (int)((n + TILE - 1) / TILE * TILE);
     if (np > m->ecap) {
-        hipFree(m->ex);
-        hipFree(m->ephi);
+        release(m->ex);
+        release(m->ephi);
+        m->ecap = 0;
         if (hipMalloc(&m->ex, 12 * (size_t)np) || hipMalloc(&m->ephi, 4 * (size_t)np))
             return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate evaluation buffers");
         m->ecap = np;
@@ -761,7 +786,9 @@ int cc_mlp_eval(cc_mlp *m, const float *x, int64_t n, float *y) {
 // Mean squared error and its gradient w.r.t. every parameter at n host samples (x, y); n a multiple
 // of 64.  For checking the backward kernels against an independent implementation.
 int cc_mlp_loss_grad(cc_mlp *m, const float *x, const float *y, int64_t n, float *loss, float *grad) {
-    if (!m || !x || !y || n < 64 || n % 64) return nfail(CC_ERR_ARGUMENT, "n must be a positive multiple of 64");
+    if (!m || !x || !y || n < 64 || n % 64 || n > (int64_t(1) << 24))
+        return nfail(CC_ERR_ARGUMENT, "n must be a multiple of 64 in [64, 2^24]");
+    hipSetDevice(m->device);
     int rc = ensure_cap(m, (int)n);
     if (rc) return rc;
     hipMemcpyAsync(m->x, x, 12 * n, hipMemcpyHostToDevice, m->stream);
@@ -782,8 +809,10 @@ int cc_mlp_loss_grad(cc_mlp *m, const float *x, const float *y, int64_t n, float
 // training loss of each epoch.
 int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, float hi, float band,
                  int epochs, int batch, float lr, uint64_t seed, float *history) {
-    if (!m || !geo || nsamples < batch || batch < 64 || batch % 64 || epochs < 1)
+    if (!m || !geo || nsamples < batch || batch < 64 || batch % 64 || batch > (1 << 20) || epochs < 1 ||
+        nsamples > (int64_t(1) << 30) || !(band > 0.0f))
         return nfail(CC_ERR_ARGUMENT, "bad training arguments");
+    hipSetDevice(m->device);
     if (geo->kind == CC_GEO_SPHERE_UNION && (!geo->prims || geo->n_prims < 1))
         return nfail(CC_ERR_ARGUMENT, "sphere union without primitives");
     if (!(hi > lo)) return nfail(CC_ERR_ARGUMENT, "the training box needs hi > lo");
@@ -795,8 +824,12 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
     std::memcpy(g.p, geo->p, sizeof g.p);
     double *prims = nullptr;
     if (geo->kind == CC_GEO_SPHERE_UNION) {
-        hipMalloc(&prims, 32 * (size_t)geo->n_prims);
-        hipMemcpy(prims, geo->prims, 32 * (size_t)geo->n_prims, hipMemcpyHostToDevice);
+        if (hipMalloc(&prims, 32 * (size_t)geo->n_prims) != hipSuccess)
+            return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate training: primitive table");
+        if (hipMemcpy(prims, geo->prims, 32 * (size_t)geo->n_prims, hipMemcpyHostToDevice) != hipSuccess) {
+            hipFree(prims);
+            return nfail(CC_ERR_HIP, "surrogate training: primitive table copy");
+        }
         g.nprims = geo->n_prims;
         g.prims = prims;
     }
@@ -808,8 +841,11 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
     const size_t chunk = 1 << 20;
     std::vector<float> cx(3 * chunk), cy(chunk);
     float *dcx = nullptr, *dcy = nullptr;
-    hipMalloc(&dcx, 12 * chunk);
-    hipMalloc(&dcy, 4 * chunk);
+    if (hipMalloc(&dcx, 12 * chunk) != hipSuccess || hipMalloc(&dcy, 4 * chunk) != hipSuccess) {
+        hipFree(dcx);
+        if (prims) hipFree(prims);
+        return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate training: sample buffers");
+    }
     size_t nu = 0, nn = 0, half = N / 2;
     int guard = 0;
     while ((nu < half || nn < N - half) && guard < 4096) {
