@@ -201,7 +201,9 @@ template <typename T> __device__ __forceinline__ T plane_face(const Plane<T> &P,
 constexpr int PF_W = 32, PF_R = 8;
 
 template <typename T> struct PlicLDS {
-    T cand[PF_R + 2][PF_W + 2][6];  // one-sided wet areas of the tile's cells and of the x/y halo
+    // one-sided wet areas of the tile's cells and of the x/y halo, each already divided by its
+    // face's full area (the quotient both cells of a face use, so it is formed once)
+    T cand[PF_R + 2][PF_W + 2][6];
 };
 
 // One work item of the fused 3D PLIC pass: the fit (cell_plane, src/plic/plic.jl:72-91) and the
@@ -250,47 +252,37 @@ __device__ void plic_fused_item(const GridT<T> &g, const Geo &geo, const Cols<T>
         }
     }
     Plane<T> cur, nxt;
+    // per face slot q of this cell's plane: its one-sided wet area / full face area (_stored_face_area)
     T cz_prev = T(0);
     T cc[6], cn[6];
     if (inside) {
-        if (k0 >= 1) cz_prev = plane_face(plic_plane<T, 3>(g, geo, i, j, k0 - 1), g.d, 5);
+        if (k0 >= 1) cz_prev = plane_face(plic_plane<T, 3>(g, geo, i, j, k0 - 1), g.d, 5) / fullf[2];
         cur = plic_plane<T, 3>(g, geo, i, j, k0);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) cc[q] = plane_face(cur, g.d, q);
+        for (int q = 0; q < 6; ++q) cc[q] = plane_face(cur, g.d, q) / fullf[q >> 1];
     }
     for (uint32_t k = k0; k < k1; ++k) {
         if (inside && k + 1 < n2) {
             nxt = plic_plane<T, 3>(g, geo, i, j, k + 1);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) cn[q] = plane_face(nxt, g.d, q);
+            for (int q = 0; q < 6; ++q) cn[q] = plane_face(nxt, g.d, q) / fullf[q >> 1];
         }
         if (inside) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) L.cand[r + 1][ii + 1][q] = cc[q];
         }
-        if (hr >= 0) L.cand[hr][hc][hslot] = plane_face(plic_plane<T, 3>(g, geo, hi_, hj, k), g.d, hslot);
+        if (hr >= 0)
+            L.cand[hr][hc][hslot] = plane_face(plic_plane<T, 3>(g, geo, hi_, hj, k), g.d, hslot) / fullf[hslot >> 1];
         __syncthreads();
         if (inside) {
             T ff[6];
             // _resolved_slot: low faces take the neighbour below's high candidate first
-            {
-                const T ap = cc[0] / fullf[0];
-                ff[0] = i >= 1 ? (L.cand[r + 1][ii][1] / fullf[0] + ap) / T(2) : ap;
-                const T an = cc[1] / fullf[0];
-                ff[1] = i + 2 <= n0 ? (an + L.cand[r + 1][ii + 2][0] / fullf[0]) / T(2) : an;
-            }
-            {
-                const T ap = cc[2] / fullf[1];
-                ff[2] = j >= 1 ? (L.cand[r][ii + 1][3] / fullf[1] + ap) / T(2) : ap;
-                const T an = cc[3] / fullf[1];
-                ff[3] = j + 2 <= n1 ? (an + L.cand[r + 2][ii + 1][2] / fullf[1]) / T(2) : an;
-            }
-            {
-                const T ap = cc[4] / fullf[2];
-                ff[4] = k >= 1 ? (cz_prev / fullf[2] + ap) / T(2) : ap;
-                const T an = cc[5] / fullf[2];
-                ff[5] = k + 2 <= n2 ? (an + cn[4] / fullf[2]) / T(2) : an;
-            }
+            ff[0] = i >= 1 ? (L.cand[r + 1][ii][1] + cc[0]) / T(2) : cc[0];
+            ff[1] = i + 2 <= n0 ? (cc[1] + L.cand[r + 1][ii + 2][0]) / T(2) : cc[1];
+            ff[2] = j >= 1 ? (L.cand[r][ii + 1][3] + cc[2]) / T(2) : cc[2];
+            ff[3] = j + 2 <= n1 ? (cc[3] + L.cand[r + 2][ii + 1][2]) / T(2) : cc[3];
+            ff[4] = k >= 1 ? (cz_prev + cc[4]) / T(2) : cc[4];
+            ff[5] = k + 2 <= n2 ? (cc[5] + cn[4]) / T(2) : cc[5];
             const size_t c = i + (size_t)n0 * (j + (size_t)n1 * k);
             plic_store_fit<T, 3>(cur, cols, normals, intercepts, valid, c);
 #pragma unroll
