@@ -208,7 +208,7 @@ int stage_geometry(cc_cache *c, const cc_geometry *geo, cc::Geo *out) {
 void mark(cc_cache *c, const char *name) {
     if (!c->profiling || c->nev >= MAX_TIMED) return;
     c->ev_name[c->nev] = name;
-    hipEventRecord(c->ev[c->nev + 1], c->stream);
+    (void)hipEventRecord(c->ev[c->nev + 1], c->stream);
     c->nev += 1;
 }
 
@@ -281,7 +281,7 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
     const cc::Cols<T> cols = cols_t<T>(c);
     c->nev = 0;
     c->ev_ready = false;
-    if (c->profiling) hipEventRecord(c->ev[0], c->stream);
+    if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
     T *phi = (T *)c->col[CC_FIELD_PHI];
     if (c->method == CC_MARCHING_SQUARES || c->method == CC_MARCHING_CUBES) {
         // The marching-cubes stream pass samples an analytic body itself (phi written once, never
@@ -354,17 +354,17 @@ int update_impl(cc_cache *c, const cc_geometry *geo, const cc_grid *g, bool sync
 
 void free_all(cc_cache *c) {
     for (auto &p : c->col)
-        if (p) { hipFree(p); p = nullptr; }
-    if (c->cutlist) hipFree(c->cutlist);
-    if (c->counter) hipFree(c->counter);
-    if (c->prims_dev) hipFree(c->prims_dev);
-    if (c->cull_list) hipFree(c->cull_list);
-    if (c->cull_cnt) hipFree(c->cull_cnt);
-    if (c->nn) hipFree(c->nn);
-    if (c->bstat) hipFree(c->bstat);
+        if (p) { (void)hipFree(p); p = nullptr; }
+    if (c->cutlist) (void)hipFree(c->cutlist);
+    if (c->counter) (void)hipFree(c->counter);
+    if (c->prims_dev) (void)hipFree(c->prims_dev);
+    if (c->cull_list) (void)hipFree(c->cull_list);
+    if (c->cull_cnt) (void)hipFree(c->cull_cnt);
+    if (c->nn) (void)hipFree(c->nn);
+    if (c->bstat) (void)hipFree(c->bstat);
     for (auto &e : c->ev)
-        if (e) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
 // ---- the marching-squares contour (src/marching_squares/surface.jl:59-177), on the host --------
@@ -491,7 +491,7 @@ int cc_cache_create(int method, int dtype, const cc_grid *grid, cc_cache **out) 
     for (int a = 0; a < dim; ++a) c->n[a] = grid->n[a];
     c->ncell = ncell;
     c->nnode = nnode;
-    hipGetDevice(&c->device);
+    (void)hipGetDevice(&c->device);
     int rc = check_grid(c, grid);
     if (rc) { delete c; return rc; }
 
@@ -567,8 +567,8 @@ int cc_cache_create(int method, int dtype, const cc_grid *grid, cc_cache **out) 
 
 int cc_cache_destroy(cc_cache *c) {
     if (!c) return CC_OK;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_all(c);
     delete c;
     return CC_OK;
@@ -746,18 +746,18 @@ int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms
         hipLaunchKernelGGL(k_probe_fill, dim3(blocks), dim3(256), 0, st, (f64x2 *)b, n, 2.0);
         for (int it = 0; it < iters + 1; ++it) {
             float t = 0.0f;
-            hipEventRecord(e0, st);
+            (void)hipEventRecord(e0, st);
             hipLaunchKernelGGL(k_probe_fill, dim3(blocks), dim3(256), 0, st, (f64x2 *)a, n, 1.0);
-            hipEventRecord(e1, st);
-            hipEventSynchronize(e1);
-            hipEventElapsedTime(&t, e0, e1);
+            (void)hipEventRecord(e1, st);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&t, e0, e1);
             if (it > 0 && t < best_fill) best_fill = t;
-            hipEventRecord(e0, st);
+            (void)hipEventRecord(e0, st);
             hipLaunchKernelGGL(k_probe_copy, dim3(blocks), dim3(256), 0, st, (f64x2 *)a,
                                (const f64x2 *)b, n);
-            hipEventRecord(e1, st);
-            hipEventSynchronize(e1);
-            hipEventElapsedTime(&t, e0, e1);
+            (void)hipEventRecord(e1, st);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&t, e0, e1);
             if (it > 0 && t < best_copy) best_copy = t;
         }
         hipError_t e = hipGetLastError();
@@ -765,11 +765,11 @@ int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms
     }
     if (fill_ms) *fill_ms = best_fill;
     if (copy_ms) *copy_ms = best_copy;
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (st) hipStreamDestroy(st);
-    if (a) hipFree(a);
-    if (b) hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
     return rc;
 }
 
@@ -787,22 +787,22 @@ int cc_probe_write(int64_t nbytes, int iters, int variant, int blocks, float *ms
     } else {
         for (int it = 0; it < iters + 1; ++it) {
             float t = 0.0f;
-            hipEventRecord(e0, st);
+            (void)hipEventRecord(e0, st);
             hipLaunchKernelGGL(k_probe_fill_v, dim3(blocks), dim3(256), 0, st, (f64x2 *)a,
                                (size_t)nbytes / 16, 1.0, variant);
-            hipEventRecord(e1, st);
-            hipEventSynchronize(e1);
-            hipEventElapsedTime(&t, e0, e1);
+            (void)hipEventRecord(e1, st);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&t, e0, e1);
             if (it > 0 && t < best) best = t;
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = fail(CC_ERR_HIP, "probe failed: %s", hipGetErrorString(e));
     }
     if (ms) *ms = best;
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (st) hipStreamDestroy(st);
-    if (a) hipFree(a);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
     return rc;
 }
 

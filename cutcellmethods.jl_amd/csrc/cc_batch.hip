@@ -224,8 +224,8 @@ template <typename T> int rebuild(cc_batch *bt, const cc_geometry *geos, const c
     if (rc) return rc;
     bt->total_a = ia.back();
     bt->total_b = ib.back();
-    if (bt->exec) { hipGraphExecDestroy(bt->exec); bt->exec = nullptr; }
-    if (bt->graph) { hipGraphDestroy(bt->graph); bt->graph = nullptr; }
+    if (bt->exec) { (void)hipGraphExecDestroy(bt->exec); bt->exec = nullptr; }
+    if (bt->graph) { (void)hipGraphDestroy(bt->graph); bt->graph = nullptr; }
     if (hipMemcpy(bt->begin_a_dev, ia.data(), ia.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(bt->begin_b_dev, ib.data(), ib.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
         return bfail(CC_ERR_HIP, "copying the work-item prefixes failed");
@@ -283,16 +283,16 @@ int cc_batch_create(int nbodies, cc_cache *const *caches, cc_batch **out) {
 
 int cc_batch_destroy(cc_batch *bt) {
     if (!bt) return CC_OK;
-    if (bt->stream) hipStreamSynchronize(bt->stream);
-    if (bt->exec) hipGraphExecDestroy(bt->exec);
-    if (bt->graph) hipGraphDestroy(bt->graph);
-    if (bt->desc_host) hipHostFree(bt->desc_host);
-    if (bt->desc_dev) hipFree(bt->desc_dev);
-    if (bt->begin_a_dev) hipFree(bt->begin_a_dev);
-    if (bt->begin_b_dev) hipFree(bt->begin_b_dev);
-    if (bt->e0) hipEventDestroy(bt->e0);
-    if (bt->e1) hipEventDestroy(bt->e1);
-    if (bt->stream) hipStreamDestroy(bt->stream);
+    if (bt->stream) (void)hipStreamSynchronize(bt->stream);
+    if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
+    if (bt->graph) (void)hipGraphDestroy(bt->graph);
+    if (bt->desc_host) (void)hipHostFree(bt->desc_host);
+    if (bt->desc_dev) (void)hipFree(bt->desc_dev);
+    if (bt->begin_a_dev) (void)hipFree(bt->begin_a_dev);
+    if (bt->begin_b_dev) (void)hipFree(bt->begin_b_dev);
+    if (bt->e0) (void)hipEventDestroy(bt->e0);
+    if (bt->e1) (void)hipEventDestroy(bt->e1);
+    if (bt->stream) (void)hipStreamDestroy(bt->stream);
     delete bt;
     return CC_OK;
 }
@@ -301,8 +301,8 @@ int cc_batch_set_kz(cc_batch *bt, int body, int kz) {
     if (!bt || body < 0 || body >= bt->nb || kz < 1 || kz > 4096)
         return bfail(CC_ERR_ARGUMENT, "bad batch body or kz");
     bt->kz[body] = kz;
-    if (bt->exec) { hipGraphExecDestroy(bt->exec); bt->exec = nullptr; }
-    if (bt->graph) { hipGraphDestroy(bt->graph); bt->graph = nullptr; }
+    if (bt->exec) { (void)hipGraphExecDestroy(bt->exec); bt->exec = nullptr; }
+    if (bt->graph) { (void)hipGraphDestroy(bt->graph); bt->graph = nullptr; }
     return CC_OK;
 }
 
@@ -324,7 +324,7 @@ int cc_batch_step(cc_batch *bt, const cc_geometry *geos, const cc_grid *grids, f
         hipStreamSynchronize(bt->stream) != hipSuccess)
         return bfail(CC_ERR_HIP, hipGetErrorString(hipGetLastError()));
     float ms = 0.0f;
-    hipEventElapsedTime(&ms, bt->e0, bt->e1);
+    (void)hipEventElapsedTime(&ms, bt->e0, bt->e1);
     bt->last_ms = ms;
     if (device_ms) *device_ms = ms;
     return CC_OK;

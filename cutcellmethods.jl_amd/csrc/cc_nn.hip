@@ -516,7 +516,7 @@ MlpDev dev_view(const cc_mlp *m) {
 }
 
 template <typename P> void release(P *&p) {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
 }
 
@@ -541,7 +541,7 @@ int ensure_cap(cc_mlp *m, int n) {
 void enqueue_loss_grad(cc_mlp *m, int n) {
     hipStream_t s = m->stream;
     const int W = m->width, D = m->depth, F2 = 2 * m->F;
-    hipMemsetAsync(m->grads, 0, sizeof(float) * m->nparam, s);
+    (void)hipMemsetAsync(m->grads, 0, sizeof(float) * m->nparam, s);
     hipLaunchKernelGGL((k_mlp_forward<true, false>), dim3(n / TILE), dim3(256), 0, s, dev_view(m), m->x, n,
                        m->phi, m->feat, m->zs, m->as, cc::Lattice{});
     const size_t L = (size_t)n * W;
@@ -577,10 +577,10 @@ void enqueue_loss_grad(cc_mlp *m, int n) {
 
 // One forward + backward with its own loss, synchronised.
 int loss_grad(cc_mlp *m, int n, float *loss_out) {
-    hipMemsetAsync(m->loss, 0, 4, m->stream);
+    (void)hipMemsetAsync(m->loss, 0, 4, m->stream);
     enqueue_loss_grad(m, n);
     float lh = 0.0f;
-    hipMemcpyAsync(&lh, m->loss, 4, hipMemcpyDeviceToHost, m->stream);
+    (void)hipMemcpyAsync(&lh, m->loss, 4, hipMemcpyDeviceToHost, m->stream);
     if (hipStreamSynchronize(m->stream) != hipSuccess)
         return nfail(CC_ERR_HIP, std::string("surrogate step: ") + hipGetErrorString(hipGetLastError()));
     if (loss_out) *loss_out = lh;
@@ -677,11 +677,14 @@ int cc_mlp_create(int nfreq, int width, int depth, float sigma, uint64_t seed, c
         for (int i = 0; i < width; ++i) p[m->off_wout + i] = ud(rng);
     }
     // the inference kernel's activation tile exceeds the default 64 KB of dynamic LDS
-    hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_infer<true>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)INFER_LDS);
-    hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_infer<false>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)INFER_LDS);
-    hipGetDevice(&m->device);
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_infer<true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)INFER_LDS) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_infer<false>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)INFER_LDS) != hipSuccess ||
+        hipGetDevice(&m->device) != hipSuccess) {
+        delete m;
+        return nfail(CC_ERR_HIP, std::string("surrogate kernel attributes: ") + hipGetErrorString(hipGetLastError()));
+    }
     if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
         delete m;
         return nfail(CC_ERR_HIP, "surrogate stream");
@@ -715,16 +718,16 @@ int cc_mlp_create(int nfreq, int width, int depth, float sigma, uint64_t seed, c
 
 int cc_mlp_destroy(cc_mlp *m) {
     if (!m) return CC_OK;
-    hipSetDevice(m->device);
-    if (m->stream) hipStreamSynchronize(m->stream);
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (auto p : {(void *)m->ffB, (void *)m->params, (void *)m->grads, (void *)m->m1, (void *)m->m2,
                    (void *)m->x, (void *)m->y, (void *)m->phi, (void *)m->zs, (void *)m->feat,
                    (void *)m->as, (void *)m->dz0, (void *)m->dz1, (void *)m->loss, (void *)m->ex,
                    (void *)m->ephi, (void *)m->dphi})
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (auto w : m->wt)
-        if (w) hipFree(w);
-    if (m->stream) hipStreamDestroy(m->stream);
+        if (w) (void)hipFree(w);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return CC_OK;
 }
@@ -738,9 +741,9 @@ int cc_mlp_num_params(cc_mlp *m, int64_t *n, int64_t *nfreq3) {
 
 int cc_mlp_get_params(cc_mlp *m, float *params, float *ffB) {
     if (!m) return nfail(CC_ERR_ARGUMENT, "NULL model");
-    hipSetDevice(m->device);
+    (void)hipSetDevice(m->device);
     if (params) {
-        hipMemcpyAsync(params, m->params, 4 * m->nparam, hipMemcpyDeviceToHost, m->stream);
+        (void)hipMemcpyAsync(params, m->params, 4 * m->nparam, hipMemcpyDeviceToHost, m->stream);
         if (hipStreamSynchronize(m->stream) != hipSuccess)
             return nfail(CC_ERR_HIP, hipGetErrorString(hipGetLastError()));
     }
@@ -750,21 +753,21 @@ int cc_mlp_get_params(cc_mlp *m, float *params, float *ffB) {
 
 int cc_mlp_set_params(cc_mlp *m, const float *params) {
     if (!m || !params) return nfail(CC_ERR_ARGUMENT, "NULL argument");
-    hipSetDevice(m->device);
-    hipMemcpyAsync(m->params, params, 4 * m->nparam, hipMemcpyHostToDevice, m->stream);
+    (void)hipSetDevice(m->device);
+    (void)hipMemcpyAsync(m->params, params, 4 * m->nparam, hipMemcpyHostToDevice, m->stream);
     for (int l = 0; l < m->depth; ++l) {
         const int cnt = m->K[l] * m->width;
         hipLaunchKernelGGL(k_to_bf16_t, dim3((cnt + 255) / 256), dim3(256), 0, m->stream,
                            m->params + m->off_w[l], m->wt[l], m->K[l], m->width);
     }
-    hipStreamSynchronize(m->stream);
+    (void)hipStreamSynchronize(m->stream);
     return CC_OK;
 }
 
 // phi_nn at n host points (x: n x 3 floats) into y (host).
 int cc_mlp_eval(cc_mlp *m, const float *x, int64_t n, float *y) {
     if (!m || !x || !y || n < 1 || n > (int64_t(1) << 30)) return nfail(CC_ERR_ARGUMENT, "bad argument");
-    hipSetDevice(m->device);
+    (void)hipSetDevice(m->device);
     const int np = (int)((n + TILE - 1) / TILE * TILE);
     if (np > m->ecap) {
         release(m->ex);
@@ -774,10 +777,10 @@ int cc_mlp_eval(cc_mlp *m, const float *x, int64_t n, float *y) {
             return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate evaluation buffers");
         m->ecap = np;
     }
-    hipMemcpyAsync(m->ex, x, 12 * n, hipMemcpyHostToDevice, m->stream);
+    (void)hipMemcpyAsync(m->ex, x, 12 * n, hipMemcpyHostToDevice, m->stream);
     hipLaunchKernelGGL((k_mlp_infer<false>), dim3((unsigned)((n + ITILE - 1) / ITILE)), dim3(256), INFER_LDS,
                        m->stream, dev_view(m), m->ex, (int)n, m->ephi, cc::Lattice{});
-    hipMemcpyAsync(y, m->ephi, 4 * n, hipMemcpyDeviceToHost, m->stream);
+    (void)hipMemcpyAsync(y, m->ephi, 4 * n, hipMemcpyDeviceToHost, m->stream);
     if (hipStreamSynchronize(m->stream) != hipSuccess)
         return nfail(CC_ERR_HIP, hipGetErrorString(hipGetLastError()));
     return CC_OK;
@@ -788,15 +791,15 @@ int cc_mlp_eval(cc_mlp *m, const float *x, int64_t n, float *y) {
 int cc_mlp_loss_grad(cc_mlp *m, const float *x, const float *y, int64_t n, float *loss, float *grad) {
     if (!m || !x || !y || n < 64 || n % 64 || n > (int64_t(1) << 24))
         return nfail(CC_ERR_ARGUMENT, "n must be a multiple of 64 in [64, 2^24]");
-    hipSetDevice(m->device);
+    (void)hipSetDevice(m->device);
     int rc = ensure_cap(m, (int)n);
     if (rc) return rc;
-    hipMemcpyAsync(m->x, x, 12 * n, hipMemcpyHostToDevice, m->stream);
-    hipMemcpyAsync(m->y, y, 4 * n, hipMemcpyHostToDevice, m->stream);
+    (void)hipMemcpyAsync(m->x, x, 12 * n, hipMemcpyHostToDevice, m->stream);
+    (void)hipMemcpyAsync(m->y, y, 4 * n, hipMemcpyHostToDevice, m->stream);
     rc = loss_grad(m, (int)n, loss);
     if (rc) return rc;
     if (grad) {
-        hipMemcpyAsync(grad, m->grads, 4 * m->nparam, hipMemcpyDeviceToHost, m->stream);
+        (void)hipMemcpyAsync(grad, m->grads, 4 * m->nparam, hipMemcpyDeviceToHost, m->stream);
         if (hipStreamSynchronize(m->stream) != hipSuccess)
             return nfail(CC_ERR_HIP, hipGetErrorString(hipGetLastError()));
     }
@@ -812,7 +815,7 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
     if (!m || !geo || nsamples < batch || batch < 64 || batch % 64 || batch > (1 << 20) || epochs < 1 ||
         nsamples > (int64_t(1) << 30) || !(band > 0.0f))
         return nfail(CC_ERR_ARGUMENT, "bad training arguments");
-    hipSetDevice(m->device);
+    (void)hipSetDevice(m->device);
     if (geo->kind == CC_GEO_SPHERE_UNION && (!geo->prims || geo->n_prims < 1))
         return nfail(CC_ERR_ARGUMENT, "sphere union without primitives");
     if (!(hi > lo)) return nfail(CC_ERR_ARGUMENT, "the training box needs hi > lo");
@@ -827,7 +830,7 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
         if (hipMalloc(&prims, 32 * (size_t)geo->n_prims) != hipSuccess)
             return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate training: primitive table");
         if (hipMemcpy(prims, geo->prims, 32 * (size_t)geo->n_prims, hipMemcpyHostToDevice) != hipSuccess) {
-            hipFree(prims);
+            (void)hipFree(prims);
             return nfail(CC_ERR_HIP, "surrogate training: primitive table copy");
         }
         g.nprims = geo->n_prims;
@@ -842,8 +845,8 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
     std::vector<float> cx(3 * chunk), cy(chunk);
     float *dcx = nullptr, *dcy = nullptr;
     if (hipMalloc(&dcx, 12 * chunk) != hipSuccess || hipMalloc(&dcy, 4 * chunk) != hipSuccess) {
-        hipFree(dcx);
-        if (prims) hipFree(prims);
+        (void)hipFree(dcx);
+        if (prims) (void)hipFree(prims);
         return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate training: sample buffers");
     }
     size_t nu = 0, nn = 0, half = N / 2;
@@ -852,13 +855,13 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
         ++guard;
         for (auto &v : cx) v = ud(rng);
         // every copy is ordered on the model's (non-blocking) stream with the kernel it feeds
-        hipMemcpyAsync(dcx, cx.data(), 12 * chunk, hipMemcpyHostToDevice, m->stream);
+        (void)hipMemcpyAsync(dcx, cx.data(), 12 * chunk, hipMemcpyHostToDevice, m->stream);
         hipLaunchKernelGGL(k_exact_points, dim3(chunk / 256), dim3(256), 0, m->stream, g, dcx, (int)chunk, dcy);
-        hipMemcpyAsync(cy.data(), dcy, 4 * chunk, hipMemcpyDeviceToHost, m->stream);
+        (void)hipMemcpyAsync(cy.data(), dcy, 4 * chunk, hipMemcpyDeviceToHost, m->stream);
         if (hipStreamSynchronize(m->stream) != hipSuccess) {
-            hipFree(dcx);
-            hipFree(dcy);
-            if (prims) hipFree(prims);
+            (void)hipFree(dcx);
+            (void)hipFree(dcy);
+            if (prims) (void)hipFree(prims);
             return nfail(CC_ERR_HIP, std::string("surrogate sampling: ") + hipGetErrorString(hipGetLastError()));
         }
         for (size_t i = 0; i < chunk; ++i) {
@@ -872,9 +875,9 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
             Y[slot] = cy[i];
         }
     }
-    hipFree(dcx);
-    hipFree(dcy);
-    if (prims) hipFree(prims);
+    (void)hipFree(dcx);
+    (void)hipFree(dcy);
+    if (prims) (void)hipFree(prims);
     if (nn < N - half) return nfail(CC_ERR_ARGUMENT, "could not find enough near-surface samples");
     int rc = ensure_cap(m, batch);
     if (rc) return rc;
@@ -882,12 +885,12 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
     float *dX = nullptr, *dY = nullptr;
     uint32_t *dperm = nullptr;
     if (hipMalloc(&dX, 12 * N) || hipMalloc(&dY, 4 * N) || hipMalloc(&dperm, 4 * N)) {
-        hipFree(dX);
-        hipFree(dY);
+        (void)hipFree(dX);
+        (void)hipFree(dY);
         return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate training samples");
     }
-    hipMemcpyAsync(dX, X.data(), 12 * N, hipMemcpyHostToDevice, m->stream);
-    hipMemcpyAsync(dY, Y.data(), 4 * N, hipMemcpyHostToDevice, m->stream);
+    (void)hipMemcpyAsync(dX, X.data(), 12 * N, hipMemcpyHostToDevice, m->stream);
+    (void)hipMemcpyAsync(dY, Y.data(), 4 * N, hipMemcpyHostToDevice, m->stream);
     std::vector<uint32_t> perm(N);
     for (size_t i = 0; i < N; ++i) perm[i] = (uint32_t)i;
     const size_t nbatch = N / batch;
@@ -895,8 +898,8 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
     size_t gstep = 0;
     for (int e = 0; e < epochs && rc == CC_OK; ++e) {
         std::shuffle(perm.begin(), perm.end(), rng);
-        hipMemcpyAsync(dperm, perm.data(), 4 * N, hipMemcpyHostToDevice, m->stream);
-        hipMemsetAsync(m->loss, 0, 4, m->stream);
+        (void)hipMemcpyAsync(dperm, perm.data(), 4 * N, hipMemcpyHostToDevice, m->stream);
+        (void)hipMemsetAsync(m->loss, 0, 4, m->stream);
         for (size_t b = 0; b < nbatch; ++b, ++gstep) {
             hipLaunchKernelGGL(k_gather, dim3((batch + 255) / 256), dim3(256), 0, m->stream, dX, dY,
                                dperm + b * batch, batch, m->x, m->y);
@@ -905,15 +908,15 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
             adam_step(m, lrt);
         }
         float l = 0.0f;
-        hipMemcpyAsync(&l, m->loss, 4, hipMemcpyDeviceToHost, m->stream);
+        (void)hipMemcpyAsync(&l, m->loss, 4, hipMemcpyDeviceToHost, m->stream);
         if (hipStreamSynchronize(m->stream) != hipSuccess)
             rc = nfail(CC_ERR_HIP, std::string("surrogate training: ") + hipGetErrorString(hipGetLastError()));
         else if (history)
             history[e] = l / (float)nbatch;
     }
-    hipFree(dX);
-    hipFree(dY);
-    hipFree(dperm);
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    (void)hipFree(dperm);
     if (rc) return rc;
     if (hipStreamSynchronize(m->stream) != hipSuccess)
         return nfail(CC_ERR_HIP, hipGetErrorString(hipGetLastError()));
