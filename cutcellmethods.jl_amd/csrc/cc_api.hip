@@ -116,6 +116,20 @@ struct cc_cache {
     uint32_t *bstat = nullptr;  // per-brick counters of the last culled/surrogate update
     int64_t bstat_cap = 0;
     int64_t phi_nbricks = 0;
+    // the whole-grid surface (marching cubes): work arrays, the last mesh, and what it was made from
+    uint64_t gen = 0;  // updates so far
+    cc::MeshBuffers mb = {};
+    std::vector<void *> mb_allocs;
+    int64_t *mesh_totals_host = nullptr;  // pinned, 3 values
+    void *mverts = nullptr;
+    int64_t mverts_cap = 0;  // bytes
+    int64_t *mtris = nullptr;
+    int64_t mtris_cap = 0;  // int64 entries
+    int64_t *mcells = nullptr;
+    int64_t mcells_cap = 0;
+    int64_t mesh_nv = -1, mesh_nt = -1, mesh_serial = 0;
+    uint64_t mesh_gen = 0;
+    double mesh_x0[3] = {}, mesh_d[3] = {};
 };
 
 namespace {
@@ -279,6 +293,7 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
                                      const cc_grid *g) {
     const cc::GridT<T> gt = grid_t<T>(g, c->dim);
     const cc::Cols<T> cols = cols_t<T>(c);
+    c->gen += 1;  // any surface marched before now describes an older field
     c->nev = 0;
     c->ev_ready = false;
     if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
@@ -352,7 +367,92 @@ int update_impl(cc_cache *c, const cc_geometry *geo, const cc_grid *g, bool sync
     return CC_OK;
 }
 
+// ---- the whole-grid marching-cubes surface (src/marching_cubes/surface.jl:82-141) ----------------
+int mesh_buffers(cc_cache *c) {
+    if (c->mb.ebits) return CC_OK;
+    const int n[3] = {(int)c->n[0], (int)c->n[1], (int)c->n[2]};
+    const cc::MeshSizes s = cc::mesh_sizes(n);
+    auto get = [&](void **p, size_t bytes) -> bool {
+        if (hipMalloc(p, bytes < 8 ? 8 : bytes) != hipSuccess) return false;
+        c->mb_allocs.push_back(*p);
+        return true;
+    };
+    cc::MeshBuffers &b = c->mb;
+    const bool ok = get(&b.ebits, 8 * s.ewords) && get(&b.cbits, 8 * s.cwords) &&
+                    get((void **)&b.wpre, 4 * s.ewords) && get((void **)&b.ecnt, 4 * s.node_rows) &&
+                    get((void **)&b.tcnt, 4 * s.cell_rows) && get((void **)&b.ccnt, 4 * s.cell_rows) &&
+                    get((void **)&b.epre, 8 * s.node_rows) && get((void **)&b.tpre, 8 * s.cell_rows) &&
+                    get((void **)&b.cpre, 8 * s.cell_rows) && get((void **)&b.part, 8 * 3 * s.scan_blocks) &&
+                    get((void **)&b.totals, 8 * 3) &&
+                    hipHostMalloc((void **)&c->mesh_totals_host, 8 * 3, hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        for (void *p : c->mb_allocs) (void)hipFree(p);
+        c->mb_allocs.clear();
+        c->mb = {};
+        if (c->mesh_totals_host) (void)hipHostFree(c->mesh_totals_host);
+        c->mesh_totals_host = nullptr;
+        return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the surface march's work arrays");
+    }
+    return CC_OK;
+}
+
+bool mesh_current(const cc_cache *c, const cc_grid *g) {
+    if (c->mesh_nv < 0 || c->mesh_gen != c->gen) return false;
+    for (int a = 0; a < 3; ++a)
+        if (std::memcmp(&c->mesh_x0[a], &g->x0[a], 8) != 0 || std::memcmp(&c->mesh_d[a], &g->d[a], 8) != 0)
+            return false;
+    return true;
+}
+
+// Marks, counts and ranks on the device; one host round trip for the three totals (the output
+// sizes); then every vertex and triangle.  Leaves the mesh in the cache's own device buffers.
+template <typename T> int mesh_march(cc_cache *c, const cc_grid *g) {
+    int rc = mesh_buffers(c);
+    if (rc) return rc;
+    const cc::GridT<T> gt = grid_t<T>(g, 3);
+    const T *phi = (const T *)c->col[CC_FIELD_PHI];
+    c->nev = 0;
+    c->ev_ready = false;
+    if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
+    cc::launch_mesh_count<T>(phi, gt, c->cutlist, c->counter, c->mb, c->stream);
+    mark(c, "mesh_count");
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(c->mesh_totals_host, c->mb.totals, 3 * sizeof(int64_t), hipMemcpyDeviceToHost,
+                        c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    const int64_t ne = c->mesh_totals_host[0], nt = c->mesh_totals_host[1], ni = c->mesh_totals_host[2];
+    const int64_t nv = ne + ni;
+    if (nv > INT32_MAX || nt > INT32_MAX)
+        return fail(CC_ERR_UNSUPPORTED, "a surface of %lld vertices / %lld triangles is beyond 2^31",
+                    (long long)nv, (long long)nt);
+    rc = ensure((char **)&c->mverts, &c->mverts_cap, 3 * sizeof(T) * (nv > 0 ? nv : 1));
+    if (rc) return rc;
+    rc = ensure(&c->mtris, &c->mtris_cap, 3 * (nt > 0 ? nt : 1));
+    if (rc) return rc;
+    mark(c, "mesh_host_sync");  // the round trip above: host time, not a kernel
+    cc::launch_mesh_emit<T>(phi, gt, c->mb, (T *)c->mverts, c->mtris, c->stream);
+    mark(c, "mesh_emit");
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->stream));
+    c->ev_ready = c->profiling;
+    c->mesh_nv = nv;
+    c->mesh_nt = nt;
+    c->mesh_gen = c->gen;
+    for (int a = 0; a < 3; ++a) {
+        c->mesh_x0[a] = g->x0[a];
+        c->mesh_d[a] = g->d[a];
+    }
+    c->mesh_serial += 1;
+    return CC_OK;
+}
+
 void free_all(cc_cache *c) {
+    for (void *p : c->mb_allocs) (void)hipFree(p);
+    c->mb_allocs.clear();
+    if (c->mesh_totals_host) (void)hipHostFree(c->mesh_totals_host);
+    if (c->mverts) (void)hipFree(c->mverts);
+    if (c->mtris) (void)hipFree(c->mtris);
+    if (c->mcells) (void)hipFree(c->mcells);
     for (auto &p : c->col)
         if (p) { (void)hipFree(p); p = nullptr; }
     if (c->cutlist) (void)hipFree(c->cutlist);
@@ -458,6 +558,11 @@ int cc_internal_body(cc_cache *c, int *method, int *dtype, int64_t *n, void **co
     *counter = c->counter;
     *kz = c->tune.kz;
     return CC_OK;
+}
+
+// A body of a batched step was updated: a surface marched before describes an older field.
+void cc_internal_touch(cc_cache *c) {
+    if (c) c->gen += 1;
 }
 
 const char *cc_version(void) { return "libcutcell_hip 0.1.0 (gfx950)"; }
@@ -838,6 +943,75 @@ int cc_generate_mesh_ms(cc_cache *c, const cc_grid *grid, double *nodes, int64_t
     }
     *nnodes = nn;
     *nlines = nl;
+    return CC_OK;
+}
+
+int cc_generate_mesh_mc(cc_cache *c, const cc_grid *grid, void *nodes, int64_t *nnodes, int64_t *tris,
+                        int64_t *ntris) {
+    if (!c || !nnodes || !ntris) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    if (c->method != CC_MARCHING_CUBES)
+        return fail(CC_ERR_UNSUPPORTED, "generate_mesh here is the marching-cubes surface");
+    int rc = check_grid(c, grid);
+    if (rc) return rc;
+    HIPC(hipSetDevice(c->device));
+    if (!mesh_current(c, grid)) {
+        rc = c->dtype == CC_FLOAT32 ? mesh_march<float>(c, grid) : mesh_march<double>(c, grid);
+        if (rc) return rc;
+    }
+    if (nodes && tris) {
+        if (*nnodes < c->mesh_nv || *ntris < c->mesh_nt)
+            return fail(CC_ERR_ARGUMENT, "buffers hold %lld nodes / %lld triangles, need %lld / %lld",
+                        (long long)*nnodes, (long long)*ntris, (long long)c->mesh_nv, (long long)c->mesh_nt);
+        HIPC(hipMemcpyAsync(nodes, c->mverts, 3 * elem_size(c->dtype) * c->mesh_nv, hipMemcpyDeviceToHost,
+                            c->stream));
+        HIPC(hipMemcpyAsync(tris, c->mtris, 3 * sizeof(int64_t) * c->mesh_nt, hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+    }
+    *nnodes = c->mesh_nv;
+    *ntris = c->mesh_nt;
+    return CC_OK;
+}
+
+int cc_mesh_device(cc_cache *c, const void **nodes, const int64_t **tris, int64_t *nnodes, int64_t *ntris,
+                   int64_t *serial) {
+    if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
+    if (c->mesh_nv < 0) return fail(CC_ERR_ARGUMENT, "no surface has been generated on this cache");
+    if (nodes) *nodes = c->mverts;
+    if (tris) *tris = c->mtris;
+    if (nnodes) *nnodes = c->mesh_nv;
+    if (ntris) *ntris = c->mesh_nt;
+    if (serial) *serial = c->mesh_serial;
+    return CC_OK;
+}
+
+int cc_mesh_cell_indices(cc_cache *c, const cc_grid *grid, int64_t *cells, int on_device) {
+    if (!c || !grid || !cells) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    if (c->mesh_nv < 0) return fail(CC_ERR_ARGUMENT, "no surface has been generated on this cache");
+    if (grid->dim != 3) return fail(CC_ERR_DIMENSION_MISMATCH, "cell_indices takes a 3D grid");
+    cc::MeshGrid mg;
+    for (int a = 0; a < 3; ++a) {
+        if (!(grid->d[a] > 0.0) || !std::isfinite(grid->d[a]) || !std::isfinite(grid->x0[a]) || grid->n[a] < 1)
+            return fail(CC_ERR_ARGUMENT, "grid.d must be positive and finite, x0 finite, n >= 1");
+        mg.x0[a] = grid->x0[a];
+        mg.d[a] = grid->d[a];
+        mg.n[a] = grid->n[a];
+    }
+    HIPC(hipSetDevice(c->device));
+    const int64_t nt = c->mesh_nt;
+    int64_t *out = cells;
+    if (!on_device) {
+        const int rc = ensure(&c->mcells, &c->mcells_cap, nt > 0 ? nt : 1);
+        if (rc) return rc;
+        out = c->mcells;
+    }
+    if (c->dtype == CC_FLOAT32)
+        cc::launch_cell_indices<float>((const float *)c->mverts, c->mtris, nt, mg, out, c->stream);
+    else
+        cc::launch_cell_indices<double>((const double *)c->mverts, c->mtris, nt, mg, out, c->stream);
+    HIPC(hipGetLastError());
+    if (!on_device && nt > 0)
+        HIPC(hipMemcpyAsync(cells, out, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
     return CC_OK;
 }
 

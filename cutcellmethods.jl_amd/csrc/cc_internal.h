@@ -113,6 +113,59 @@ template <typename T>
 void launch_phi_bricks(T *phi, const GridT<T> &g, const Geo &geo, const uint16_t *list,
                        const uint32_t *cnt, const SurrogateArgs &sa, uint32_t *bstat, hipStream_t s);
 
+// ---- the whole-grid marching-cubes surface (generate_mesh) ---------------------------------------
+// Sizes of the march's work arrays for a grid of n cells: node rows (j, k) of (n0+1) nodes and
+// cell rows of n0 cells, an edge bitmap of 3 bits per node, a cut bitmap of 1 bit per cell.
+struct MeshSizes {
+    size_t node_rows, cell_rows;
+    size_t ewords, cwords;  // 64-bit words of the edge and cut bitmaps
+    size_t scan_blocks;     // blocks of the row prefix sums
+};
+MeshSizes mesh_sizes(const int *n);
+
+// The march's device work arrays (owned by the cache, sized by mesh_sizes).
+struct MeshBuffers {
+    void *ebits;        // [node_rows][W] edge bits, bit 3 i + axis
+    void *cbits;        // [cell_rows][Wc] cut-cell bits
+    uint32_t *wpre;     // [node_rows][W] set bits before each word within its row
+    uint32_t *ecnt;     // [node_rows] edge vertices per node row
+    uint32_t *tcnt;     // [cell_rows] triangles per cell row
+    uint32_t *ccnt;     // [cell_rows] interior vertices per cell row
+    int64_t *epre, *tpre, *cpre;  // exclusive prefix sums of the three counts
+    unsigned long long *part;     // [3][scan_blocks] block sums of the prefix sums
+    int64_t *totals;              // [3] edge vertices, triangles, interior vertices
+};
+
+// Up to three exclusive prefix sums (uint32 in -> int64 out) in one set of launches.
+struct MeshScan {
+    struct Seg {
+        const uint32_t *in;
+        int64_t *out;
+        size_t n;
+    } seg[3];
+    unsigned long long *part;
+    size_t maxblocks;
+    int64_t *totals;
+};
+
+// The grid cell_indices floors into (Float64, as the reference converts it).
+struct MeshGrid {
+    double x0[3], d[3];
+    int64_t n[3];
+};
+
+// Marks, counts and ranks (leaves the three totals in b.totals on the device).
+template <typename T>
+void launch_mesh_count(const T *phi, const GridT<T> &g, const uint32_t *cutlist, const uint32_t *counter,
+                       const MeshBuffers &b, hipStream_t s);
+// Writes every vertex (3 T each) and triangle (3 int64, 1-based), sized from the totals.
+template <typename T>
+void launch_mesh_emit(const T *phi, const GridT<T> &g, const MeshBuffers &b, T *verts, int64_t *tris,
+                      hipStream_t s);
+template <typename T>
+void launch_cell_indices(const T *verts, const int64_t *tris, int64_t nt, const MeshGrid &g, int64_t *cells,
+                         hipStream_t s);
+
 }  // namespace cc
 
 struct cc_mlp;

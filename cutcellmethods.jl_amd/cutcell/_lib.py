@@ -30,6 +30,7 @@ FIELDS = {
 EXPORTS = ("cc_cache_create", "cc_cache_destroy", "cc_update", "cc_update_from_phi",
            "cc_update_async", "cc_synchronize", "cc_column", "cc_copy_to_host", "cc_cut_count",
            "cc_stream", "cc_set_tuning", "cc_get_tuning", "cc_set_profiling", "cc_kernel_times", "cc_generate_mesh_ms",
+           "cc_generate_mesh_mc", "cc_mesh_device", "cc_mesh_cell_indices",
            "cc_probe_bandwidth", "cc_probe_write", "cc_last_error", "cc_version",
            "cc_mlp_create", "cc_mlp_destroy", "cc_mlp_num_params", "cc_mlp_get_params",
            "cc_mlp_set_params", "cc_mlp_eval", "cc_mlp_loss_grad", "cc_mlp_train",
@@ -91,6 +92,9 @@ def lib():
         L.cc_kernel_times.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_int)]
         L.cc_generate_mesh_ms.argtypes = [vp, ctypes.POINTER(CCGrid), vp, i64p, vp, i64p]
+        L.cc_generate_mesh_mc.argtypes = [vp, ctypes.POINTER(CCGrid), vp, i64p, vp, i64p]
+        L.cc_mesh_device.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i64p, i64p, i64p]
+        L.cc_mesh_cell_indices.argtypes = [vp, ctypes.POINTER(CCGrid), vp, ctypes.c_int]
         L.cc_probe_bandwidth.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(ctypes.c_float)]
         L.cc_probe_write.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -9,12 +9,15 @@
   nodal array / callable for the nodal methods (the reference's "any callable ``x -> phi``").
 * ``cache.cells`` reads columns back as numpy arrays (``Adapt.adapt(Array, cache.cells)``): flat
   over the cell linear index (i fastest), with the per-cell components as trailing axes.
-* ``generate_mesh(cache, grid)`` is the marching-squares contour (``src/marching_squares/surface.jl``).
+* ``generate_mesh(cache, grid)`` is the marching-squares contour (``src/marching_squares/surface.jl``)
+  or the marching-cubes surface, marched on the device (``src/marching_cubes/surface.jl``);
+  ``cell_indices(mesh, grid)`` names each triangle's cell.
 
 Errors keep the reference's types: ``DimensionMismatch`` for a wrong ``n``, ``ArgumentError`` for an
 anisotropic PLIC grid.
 """
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -255,18 +258,80 @@ def update_cache_from_device_phi(cache, phi_ptr, grid):
     return cache
 
 
-def generate_mesh(cache, grid):
-    """The marching-squares contour of the cache's phi as (nodes (m, 2), lines (k, 2) 1-based)."""
+class Surface(tuple):
+    """A marching-cubes surface, ``(nodes (m, 3), triangles (k, 3) 1-based)`` -- the reference's
+    ``Mesh{3}`` of ``Tri`` -- that remembers which march of which cache made it, so that
+    ``cell_indices`` can run on the cache's device copy."""
+
+    def __new__(cls, nodes, triangles, cache, serial):
+        obj = super().__new__(cls, (nodes, triangles))
+        obj.cache = cache
+        obj.serial = serial
+        return obj
+
+    @property
+    def nodes(self):
+        return self[0]
+
+    @property
+    def elements(self):
+        return self[1]
+
+
+def _mesh_serial(cache):
+    serial = ctypes.c_int64()
+    check(lib().cc_mesh_device(cache.handle, None, None, None, None, ctypes.byref(serial)))
+    return serial.value
+
+
+def generate_mesh(cache, grid, warn=True):
+    """generate_mesh(cache, grid; warn=true).
+
+    Marching squares (``src/marching_squares/surface.jl:59-86``): the contour of the cache's phi as
+    ``(nodes (m, 2), lines (k, 2) 1-based)``.  Marching cubes (``src/marching_cubes/surface.jl:82-112``):
+    the cache's phi marched on the device into a ``Surface`` of shared vertices and outward-wound
+    triangles, in the cache's element type.  An empty result warns unless ``warn`` is false."""
+    _check_same_n(cache, grid)
     g = make_grid(grid)
     nn = ctypes.c_int64()
     nl = ctypes.c_int64()
+    if _method_code(cache.method) == _lib.CC_MARCHING_CUBES:
+        check(lib().cc_generate_mesh_mc(cache.handle, ctypes.byref(g), None, ctypes.byref(nn), None,
+                                        ctypes.byref(nl)))
+        nodes = np.empty((nn.value, 3), dtype=cache.dtype)
+        tris = np.empty((nl.value, 3), dtype=np.int64)
+        check(lib().cc_generate_mesh_mc(cache.handle, ctypes.byref(g), nodes.ctypes.data,
+                                        ctypes.byref(nn), tris.ctypes.data, ctypes.byref(nl)))
+        if warn and nl.value == 0:
+            warnings.warn("generate_mesh: the field has no sign change anywhere in `grid`, so the "
+                          "surface is empty (the grid misses the body, or lies wholly inside it)")
+        return Surface(nodes, tris, cache, _mesh_serial(cache))
     check(lib().cc_generate_mesh_ms(cache.handle, ctypes.byref(g), None, ctypes.byref(nn), None,
                                     ctypes.byref(nl)))
     nodes = np.empty((nn.value, 2))
     lines = np.empty((nl.value, 2), dtype=np.int64)
     check(lib().cc_generate_mesh_ms(cache.handle, ctypes.byref(g), nodes.ctypes.data,
                                     ctypes.byref(nn), lines.ctypes.data, ctypes.byref(nl)))
+    if warn and nl.value == 0:
+        warnings.warn("generate_mesh: the field has no sign change anywhere in `grid`, so the "
+                      "contour is empty (the grid misses the body, or lies wholly inside it)")
     return nodes, lines
+
+
+def cell_indices(mesh, grid):
+    """cell_indices(mesh, grid) (``src/marching_cubes/surface.jl:128-141``): for each triangle of a
+    marching-cubes surface, the 1-based linear index of the cell of ``grid`` its centroid floors
+    into, clamped to the grid.  Runs on the device copy of ``mesh``, which must be the last surface
+    its cache marched."""
+    cache = getattr(mesh, "cache", None)
+    if cache is None:
+        raise ArgumentError("cell_indices takes a surface returned by generate_mesh(cache, grid)")
+    if _mesh_serial(cache) != mesh.serial:
+        raise ArgumentError("the cache has marched a newer surface since this one; call "
+                            "generate_mesh again")
+    out = np.empty(len(mesh[1]), dtype=np.int64)
+    check(lib().cc_mesh_cell_indices(cache.handle, ctypes.byref(make_grid(grid)), out.ctypes.data, 0))
+    return out
 
 
 def volume_fractions(cache):

@@ -147,6 +147,27 @@ int cc_kernel_times(cc_cache *cache, int max, float *ms, const char **names, int
 int cc_generate_mesh_ms(cc_cache *cache, const cc_grid *grid, double *nodes, int64_t *nnodes,
                         int64_t *lines, int64_t *nlines);
 
+/* generate_mesh(cache, grid) for a marching-cubes cache -- src/marching_cubes/surface.jl:82-112:
+ * the cache's phi marched on the device (Lewiner cases, shared vertices, wound so each triangle's
+ * normal points out of the body) into a mesh the cache keeps until its next update.  Call with
+ * nodes/tris NULL to march and get the counts, then with buffers of that size to copy: nodes is
+ * 3*nnodes values of the cache's element type (double or float); tris is 3*ntris int64, 1-based
+ * node ids (MeshLibrary's Tri connectivity).  A call after an update, or with a moved grid, marches
+ * again.  An empty surface is not an error (counts 0); the reference's warning is the caller's. */
+int cc_generate_mesh_mc(cc_cache *cache, const cc_grid *grid, void *nodes, int64_t *nnodes, int64_t *tris,
+                        int64_t *ntris);
+
+/* The last marched surface in the cache's device memory (valid until the next march or destroy),
+ * and a serial number that increases with every march.  No counterpart in the reference, whose
+ * march is host-only; for device consumers of the mesh. */
+int cc_mesh_device(cc_cache *cache, const void **nodes, const int64_t **tris, int64_t *nnodes,
+                   int64_t *ntris, int64_t *serial);
+
+/* cell_indices(mesh, grid) -- src/marching_cubes/surface.jl:128-141 -- for the cache's last marched
+ * surface: for each triangle the 1-based linear index of the cell its centroid floors into, clamped
+ * to the grid.  cells: ntris int64, host memory, or device memory when on_device != 0. */
+int cc_mesh_cell_indices(cc_cache *cache, const cc_grid *grid, int64_t *cells, int on_device);
+
 /* ---- batched bodies (no counterpart in the reference, which updates one cache per call) --------
  * A batch references B existing 3D caches (marching cubes and PLIC mixed, one element type) and
  * updates all of them with ONE hipGraph launch per step: a counter-reset kernel, one stream/fit pass

@@ -984,3 +984,117 @@ int orc_num_threads(void)
     return 1;
 #endif
 }
+
+/* =========================================================================================== */
+/* The whole-grid surface: generate_mesh(cache, grid), src/marching_cubes/surface.jl:82-112.    */
+/* =========================================================================================== */
+/* The reference marches the cache's phi with MarchingCubes.jl 0.1.11 (un-vendored), the Lewiner
+ * et al. (2003) algorithm.  Restated from the published method (Lewiner's reference
+ * implementation, which that package ports):
+ *   1. one pass over the nodes, column-major (i fastest), puts a vertex on each of the node's +x,
+ *      +y, +z edges (in that order) whose two ends differ in sign;
+ *   2. one pass over the cubes in the same order runs the per-cube Lewiner dispatch (the one the
+ *      moments use, cell_triangles above); a tiling that needs the interior vertex creates it then,
+ *      as the mean of the cube's edge vertices in edge order 0..11, and the triangles are appended
+ *      in tiling order with 1-based vertex numbers.
+ * Values are nudged as the cells nudge them (nudge_zeros, marching_cubes.jl:240-243).  A crossing
+ * is anchored on the edge's LOW node, as the march does (surface.jl:4-7; the moments anchor on the
+ * outside node): the low node's lattice coordinate plus u (x_high - x_low) along the edge, with
+ * u = phi_low / (phi_low - phi_high); the other two coordinates are the low node's own.
+ * MarchingCubes.jl's own vertex numbering and the last bits of its index-to-world map cannot be
+ * compared here (the package is absent); this order and these coordinates are the method's.
+ *
+ * nodes == NULL counts only.  owner (optional) gets each triangle's 0-based cell index. */
+static const int MC_EDGE_AXIS[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+
+static void mesh_edge_point(const double *x0, const double *d, const int64_t *ijk, int a, double fl,
+                            double fh, double *p)
+{
+    double u = fl / (fl - fh);
+    for (int b = 0; b < 3; ++b) {
+        double xl = x0[b] + (double)ijk[b] * d[b];
+        if (b == a) {
+            double xh = x0[b] + (double)(ijk[b] + 1) * d[b];
+            p[b] = xl + u * (xh - xl);
+        } else {
+            p[b] = xl;
+        }
+    }
+}
+
+int orc_mc_mesh(const double *phi, const double *x0, const double *d, const int64_t *n,
+                int64_t *nverts, int64_t *ntris, double *nodes, int64_t *tris, int64_t *owner)
+{
+    const int64_t m[3] = {n[0] + 1, n[1] + 1, n[2] + 1};
+    const int64_t st[3] = {1, m[0], m[0] * m[1]};
+    const int64_t nn = m[0] * m[1] * m[2];
+    int32_t *vid = (int32_t *)malloc((size_t)(3 * nn) * sizeof(int32_t));
+    if (!vid) return -1;
+    int64_t nv = 0, nt = 0;
+    for (int64_t k = 0; k < m[2]; ++k)
+        for (int64_t j = 0; j < m[1]; ++j)
+            for (int64_t i = 0; i < m[0]; ++i) {
+                const int64_t p = i + m[0] * (j + m[1] * k), ijk[3] = {i, j, k};
+                const double f = fabs(phi[p]) < EPS64 ? EPS64 : phi[p];
+                for (int a = 0; a < 3; ++a) {
+                    vid[3 * p + a] = -1;
+                    if (ijk[a] + 1 >= m[a]) continue;
+                    const double g = fabs(phi[p + st[a]]) < EPS64 ? EPS64 : phi[p + st[a]];
+                    if ((f >= 0.0) == (g >= 0.0)) continue;
+                    if (nodes) mesh_edge_point(x0, d, ijk, a, f, g, nodes + 3 * nv);
+                    vid[3 * p + a] = (int32_t)nv++;
+                }
+            }
+    for (int64_t k = 0; k < n[2]; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t i = 0; i < n[0]; ++i) {
+                const int64_t base = i + m[0] * (j + m[1] * k);
+                double ph[8];
+                for (int c = 0; c < 8; ++c) {
+                    const double v = phi[base + MC_BITS[c][0] + st[1] * MC_BITS[c][1] +
+                                         st[2] * MC_BITS[c][2]];
+                    ph[c] = fabs(v) < EPS64 ? EPS64 : v;
+                }
+                const int lut = cell_mask(ph);
+                if (lut == 0 || lut == 255) continue;
+                int8_t codes[48];
+                int ntc = cell_triangles(ph, codes);
+                if (ntc < 0) ntc = 0;
+                int64_t ev[12];
+                for (int e = 0; e < 12; ++e) {
+                    const int lo = MC_EDGE[e][0];
+                    const int64_t q = base + MC_BITS[lo][0] + st[1] * MC_BITS[lo][1] +
+                                      st[2] * MC_BITS[lo][2];
+                    ev[e] = vid[3 * q + MC_EDGE_AXIS[e]];
+                }
+                int uses12 = 0;
+                for (int t = 0; t < 3 * ntc; ++t) uses12 |= codes[t] == 12;
+                int64_t v12 = -1;
+                if (uses12) {
+                    if (nodes) {
+                        double acc[3] = {0.0, 0.0, 0.0};
+                        int cnt = 0;
+                        for (int e = 0; e < 12; ++e) {
+                            if (ev[e] < 0) continue;
+                            for (int x = 0; x < 3; ++x) acc[x] += nodes[3 * ev[e] + x];
+                            cnt += 1;
+                        }
+                        for (int x = 0; x < 3; ++x) nodes[3 * nv + x] = acc[x] / (double)cnt;
+                    }
+                    v12 = nv++;
+                }
+                for (int t = 0; t < ntc; ++t) {
+                    if (tris)
+                        for (int v = 0; v < 3; ++v) {
+                            const int code = codes[3 * t + v];
+                            tris[3 * nt + v] = (code == 12 ? v12 : ev[code]) + 1;
+                        }
+                    if (owner) owner[nt] = i + n[0] * (j + n[1] * k);
+                    nt += 1;
+                }
+            }
+    free(vid);
+    *nverts = nv;
+    *ntris = nt;
+    return 0;
+}

@@ -81,6 +81,13 @@ void orc_mc_cell(const double *nodes24, const double *phi_raw8, const double *ce
                  int8_t *kind, uint8_t *amb, double *vf, double *centroid3, double *ff6,
                  double *fcl12, double *ic3);
 
+/* ---- the whole-grid marching-cubes surface: src/marching_cubes/surface.jl:82-112 ------------
+ * Vertices (3 per node, column-major node order, then interior vertices in cube order) and
+ * 1-based triangles in cube order; owner = each triangle's 0-based cell.  nodes == NULL counts.
+ * Returns 0, or -1 when the scratch allocation fails. */
+int  orc_mc_mesh(const double *phi, const double *x0, const double *d, const int64_t *n,
+                 int64_t *nverts, int64_t *ntris, double *nodes, int64_t *tris, int64_t *owner);
+
 /* ---- PLIC: src/plic/cache.jl:77-162, src/plic/plic.jl:72-91, src/plic/apertures.jl --------- */
 double orc_volume_fraction(int dim, const double *normal, double a);
 void orc_plic_fit(const orc_geometry *g, int dim, const double *x0, const double *d,

@@ -61,6 +61,8 @@ def lib():
         L.orc_mc_cell_triangles.restype = ctypes.c_int
         L.orc_mc_edge_crossings.argtypes = [vp, vp, vp]
         L.orc_mc_cell.argtypes = [vp] * 10
+        L.orc_mc_mesh.argtypes = [vp] * 9
+        L.orc_mc_mesh.restype = ctypes.c_int
         L.orc_volume_fraction.argtypes = [ctypes.c_int, vp, ctypes.c_double]
         L.orc_volume_fraction.restype = ctypes.c_double
         L.orc_plic_fit.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -205,6 +207,23 @@ def mc_edge_crossings(local_nodes, phi_raw):
     pts = np.zeros(39)
     lib().orc_mc_edge_crossings(_p(_f64(local_nodes)), _p(_f64(phi_raw)), _p(pts))
     return pts.reshape(13, 3)
+
+
+def mc_mesh(phi, x0, d, n):
+    """The whole-grid surface (src/marching_cubes/surface.jl:82-112): (nodes (nv, 3), triangles
+    (nt, 3) 1-based, owner (nt,) 0-based cell of each triangle)."""
+    phi = _f64(phi)
+    args = (_p(phi), _p(_f64(x0, 3)), _p(_f64(d, 3)), _p(_i64(n)))
+    nv, nt = ctypes.c_int64(0), ctypes.c_int64(0)
+    if lib().orc_mc_mesh(*args, ctypes.byref(nv), ctypes.byref(nt), None, None, None) != 0:
+        raise MemoryError("orc_mc_mesh")
+    nodes = np.zeros((nv.value, 3))
+    tris = np.zeros((nt.value, 3), np.int64)
+    owner = np.zeros(nt.value, np.int64)
+    if lib().orc_mc_mesh(*args, ctypes.byref(nv), ctypes.byref(nt), _p(nodes), _p(tris),
+                         _p(owner)) != 0:
+        raise MemoryError("orc_mc_mesh")
+    return nodes, tris, owner
 
 
 def volume_fraction(normal, a):

@@ -56,3 +56,61 @@ def fortran(col, n):
     D = len(n)
     A = col.reshape(tuple(reversed(n)) + col.shape[1:])
     return np.transpose(A, tuple(reversed(range(D))) + tuple(range(D, A.ndim)))
+
+
+# ---- marched surfaces --------------------------------------------------------------------------
+def cell_indices_ref(nodes, tris, x0, d, n):
+    """cell_indices (src/marching_cubes/surface.jl:128-141): the 1-based linear index of the cell
+    each triangle's centroid floors into, clamped; the centroid in the nodes' own precision, the
+    rest in Float64 as the reference converts the grid."""
+    a, b, c = (nodes[tris[:, k] - 1] for k in range(3))
+    cen = ((a + b) + c) / nodes.dtype.type(3)
+    idx = []
+    for ax in range(3):
+        f = np.floor((cen[:, ax].astype(np.float64) - x0[ax]) / d[ax]) + 1.0
+        idx.append(np.clip(f, 1.0, float(n[ax])).astype(np.int64))
+    return idx[0] + n[0] * ((idx[1] - 1) + n[1] * (idx[2] - 1))
+
+
+def mesh_topology(tris):
+    """(histogram of undirected edge uses, largest directed edge multiplicity, V - E + F)."""
+    e = np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]])
+    _, und = np.unique(np.sort(e, axis=1), axis=0, return_counts=True)
+    _, dire = np.unique(e, axis=0, return_counts=True)
+    V = len(np.unique(tris))
+    return np.bincount(und), int(dire.max()) if len(dire) else 0, V - len(und) + len(tris)
+
+
+def tri_vector_areas(nodes, tris):
+    a, b, c = (nodes[tris[:, k] - 1].astype(np.float64) for k in range(3))
+    return 0.5 * np.cross(b - a, c - a), a, b, c
+
+
+def surface_moments(nodes, tris, face_fraction, origin, h, face_area_of):
+    """What the triangles marched through one cell imply for its moments, closed against the cell's
+    own open faces (test/test_mc_moments.jl:107-143): vector area, interface centroid, outside
+    volume fraction and centroid, in the cell's coordinates."""
+    o = np.asarray(origin, dtype=np.float64)
+    ff = np.asarray(face_fraction)[None, :]
+    vol3 = 0.0
+    cm2 = np.zeros(3)
+    for ax in range(3):
+        a_open = float(face_area_of(ff, 2 * ax + 1, h)[0])
+        vol3 += h[ax] * a_open
+        cm2[ax] += h[ax] ** 2 * a_open
+    S = np.zeros(3)
+    A = 0.0
+    Ac = np.zeros(3)
+    for t in tris:
+        a, b, c = (nodes[t[k] - 1] - o for k in range(3))
+        w = 0.5 * np.cross(b - a, c - a)
+        xc = (a + b + c) / 3
+        S += w
+        ln = np.linalg.norm(w)
+        A += ln
+        Ac += ln * xc
+        vol3 -= w @ xc
+        cm2 -= w * ((a * a + b * b + c * c + a * b + b * c + c * a) / 6)
+    V = vol3 / 3
+    return dict(vector_area=S, interface_centroid=o + Ac / A if A > 0 else None,
+                volume_fraction=V / np.prod(h), centroid=o + cm2 / (2 * V) if V != 0 else None)
