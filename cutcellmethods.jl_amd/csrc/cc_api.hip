@@ -368,30 +368,53 @@ int update_impl(cc_cache *c, const cc_geometry *geo, const cc_grid *g, bool sync
 }
 
 // ---- the whole-grid marching-cubes surface (src/marching_cubes/surface.jl:82-141) ----------------
-int mesh_buffers(cc_cache *c) {
-    if (c->mb.ebits) return CC_OK;
+// The march's per-grid arrays (once per cache) and its per-cut-cell arrays (grown to ncut).
+int mesh_buffers(cc_cache *c, uint32_t ncut) {
     const int n[3] = {(int)c->n[0], (int)c->n[1], (int)c->n[2]};
     const cc::MeshSizes s = cc::mesh_sizes(n);
+    cc::MeshBuffers &b = c->mb;
     auto get = [&](void **p, size_t bytes) -> bool {
         if (hipMalloc(p, bytes < 8 ? 8 : bytes) != hipSuccess) return false;
         c->mb_allocs.push_back(*p);
         return true;
     };
-    cc::MeshBuffers &b = c->mb;
-    const bool ok = get(&b.ebits, 8 * s.ewords) && get(&b.cbits, 8 * s.cwords) &&
-                    get((void **)&b.wpre, 4 * s.ewords) && get((void **)&b.ecnt, 4 * s.node_rows) &&
-                    get((void **)&b.tcnt, 4 * s.cell_rows) && get((void **)&b.ccnt, 4 * s.cell_rows) &&
-                    get((void **)&b.epre, 8 * s.node_rows) && get((void **)&b.tpre, 8 * s.cell_rows) &&
-                    get((void **)&b.cpre, 8 * s.cell_rows) && get((void **)&b.part, 8 * 3 * s.scan_blocks) &&
-                    get((void **)&b.totals, 8 * 3) &&
-                    hipHostMalloc((void **)&c->mesh_totals_host, 8 * 3, hipHostMallocDefault) == hipSuccess;
-    if (!ok) {
-        for (void *p : c->mb_allocs) (void)hipFree(p);
-        c->mb_allocs.clear();
-        c->mb = {};
-        if (c->mesh_totals_host) (void)hipHostFree(c->mesh_totals_host);
-        c->mesh_totals_host = nullptr;
-        return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the surface march's work arrays");
+    if (!b.ebits) {
+        const bool ok = get(&b.ebits, 8 * s.ewords) && get(&b.cbits, 8 * s.cwords) &&
+                        get((void **)&b.erank, 4 * s.ewords) && get((void **)&b.crank, 4 * s.cwords) &&
+                        get((void **)&b.totals, 8 * 4) &&
+                        hipHostMalloc((void **)&c->mesh_totals_host, 8 * 4, hipHostMallocDefault) == hipSuccess;
+        if (!ok) {
+            for (void *p : c->mb_allocs) (void)hipFree(p);
+            c->mb_allocs.clear();
+            b = {};
+            if (c->mesh_totals_host) (void)hipHostFree(c->mesh_totals_host);
+            c->mesh_totals_host = nullptr;
+            return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the surface march's work arrays");
+        }
+    }
+    if (ncut > b.cut_cap || !b.order) {
+        for (void *p : {(void *)b.order, (void *)b.tn, (void *)b.cn, (void *)b.tpre, (void *)b.cpre})
+            if (p) (void)hipFree(p);
+        b.order = b.tn = b.cn = nullptr;
+        b.tpre = b.cpre = nullptr;
+        b.cut_cap = 0;
+        const size_t cap = ncut > 0 ? ncut : 1;
+        if (hipMalloc((void **)&b.order, 4 * cap) != hipSuccess || hipMalloc((void **)&b.tn, 4 * cap) != hipSuccess ||
+            hipMalloc((void **)&b.cn, 4 * cap) != hipSuccess || hipMalloc((void **)&b.tpre, 8 * cap) != hipSuccess ||
+            hipMalloc((void **)&b.cpre, 8 * cap) != hipSuccess)
+            return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the surface march's %u-cell arrays", ncut);
+        b.cut_cap = cap;
+    }
+    size_t most = s.ewords > s.cwords ? s.ewords : s.cwords;
+    if (ncut > most) most = ncut;
+    const size_t blocks = cc::mesh_scan_blocks(most);
+    if (blocks > b.part_cap || !b.part) {
+        if (b.part) (void)hipFree(b.part);
+        b.part = nullptr;
+        b.part_cap = 0;
+        if (hipMalloc((void **)&b.part, 8 * 2 * (blocks > 0 ? blocks : 1)) != hipSuccess)
+            return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the surface march's block sums");
+        b.part_cap = blocks > 0 ? blocks : 1;
     }
     return CC_OK;
 }
@@ -407,20 +430,27 @@ bool mesh_current(const cc_cache *c, const cc_grid *g) {
 // Marks, counts and ranks on the device; one host round trip for the three totals (the output
 // sizes); then every vertex and triangle.  Leaves the mesh in the cache's own device buffers.
 template <typename T> int mesh_march(cc_cache *c, const cc_grid *g) {
-    int rc = mesh_buffers(c);
+    // the cut list of the last update holds every cell the surface passes through
+    uint32_t ncut = 0;
+    HIPC(hipMemcpyAsync(&ncut, c->counter, sizeof ncut, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    int rc = mesh_buffers(c, ncut);
     if (rc) return rc;
     const cc::GridT<T> gt = grid_t<T>(g, 3);
     const T *phi = (const T *)c->col[CC_FIELD_PHI];
     c->nev = 0;
     c->ev_ready = false;
     if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
-    cc::launch_mesh_count<T>(phi, gt, c->cutlist, c->counter, c->mb, c->stream);
+    cc::launch_mesh_count<T>(phi, gt, c->cutlist, ncut, c->mb, c->stream);
     mark(c, "mesh_count");
     HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(c->mesh_totals_host, c->mb.totals, 3 * sizeof(int64_t), hipMemcpyDeviceToHost,
+    HIPC(hipMemcpyAsync(c->mesh_totals_host, c->mb.totals, 4 * sizeof(int64_t), hipMemcpyDeviceToHost,
                         c->stream));
     HIPC(hipStreamSynchronize(c->stream));
-    const int64_t ne = c->mesh_totals_host[0], nt = c->mesh_totals_host[1], ni = c->mesh_totals_host[2];
+    const int64_t ne = c->mesh_totals_host[0], nt = c->mesh_totals_host[2], ni = c->mesh_totals_host[3];
+    if (c->mesh_totals_host[1] != (int64_t)ncut)
+        return fail(CC_ERR_HIP, "surface march: %lld cut cells ranked, %u listed",
+                    (long long)c->mesh_totals_host[1], ncut);
     const int64_t nv = ne + ni;
     if (nv > INT32_MAX || nt > INT32_MAX)
         return fail(CC_ERR_UNSUPPORTED, "a surface of %lld vertices / %lld triangles is beyond 2^31",
@@ -430,7 +460,7 @@ template <typename T> int mesh_march(cc_cache *c, const cc_grid *g) {
     rc = ensure(&c->mtris, &c->mtris_cap, 3 * (nt > 0 ? nt : 1));
     if (rc) return rc;
     mark(c, "mesh_host_sync");  // the round trip above: host time, not a kernel
-    cc::launch_mesh_emit<T>(phi, gt, c->mb, (T *)c->mverts, c->mtris, c->stream);
+    cc::launch_mesh_emit<T>(phi, gt, ncut, c->mb, (T *)c->mverts, c->mtris, c->stream);
     mark(c, "mesh_emit");
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(c->stream));
@@ -449,6 +479,9 @@ template <typename T> int mesh_march(cc_cache *c, const cc_grid *g) {
 void free_all(cc_cache *c) {
     for (void *p : c->mb_allocs) (void)hipFree(p);
     c->mb_allocs.clear();
+    for (void *p : {(void *)c->mb.order, (void *)c->mb.tn, (void *)c->mb.cn, (void *)c->mb.tpre,
+                    (void *)c->mb.cpre, (void *)c->mb.part})
+        if (p) (void)hipFree(p);
     if (c->mesh_totals_host) (void)hipHostFree(c->mesh_totals_host);
     if (c->mverts) (void)hipFree(c->mverts);
     if (c->mtris) (void)hipFree(c->mtris);

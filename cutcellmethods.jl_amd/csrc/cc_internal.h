@@ -114,33 +114,40 @@ void launch_phi_bricks(T *phi, const GridT<T> &g, const Geo &geo, const uint16_t
                        const uint32_t *cnt, const SurrogateArgs &sa, uint32_t *bstat, hipStream_t s);
 
 // ---- the whole-grid marching-cubes surface (generate_mesh) ---------------------------------------
-// Sizes of the march's work arrays for a grid of n cells: node rows (j, k) of (n0+1) nodes and
-// cell rows of n0 cells, an edge bitmap of 3 bits per node, a cut bitmap of 1 bit per cell.
+// Sizes of the march's per-grid work arrays for a grid of n cells: node rows (j, k) of (n0+1)
+// nodes and cell rows of n0 cells, an edge bitmap of 3 bits per node, a cut bitmap of 1 bit per cell.
 struct MeshSizes {
     size_t node_rows, cell_rows;
     size_t ewords, cwords;  // 64-bit words of the edge and cut bitmaps
-    size_t scan_blocks;     // blocks of the row prefix sums
 };
 MeshSizes mesh_sizes(const int *n);
+// Blocks of a prefix sum over n values (the block-sum array holds this many per sum).
+size_t mesh_scan_blocks(size_t n);
 
-// The march's device work arrays (owned by the cache, sized by mesh_sizes).
+// The march's device work arrays (owned by the cache).
 struct MeshBuffers {
-    void *ebits;        // [node_rows][W] edge bits, bit 3 i + axis
-    void *cbits;        // [cell_rows][Wc] cut-cell bits
-    uint32_t *wpre;     // [node_rows][W] set bits before each word within its row
-    uint32_t *ecnt;     // [node_rows] edge vertices per node row
-    uint32_t *tcnt;     // [cell_rows] triangles per cell row
-    uint32_t *ccnt;     // [cell_rows] interior vertices per cell row
-    int64_t *epre, *tpre, *cpre;  // exclusive prefix sums of the three counts
-    unsigned long long *part;     // [3][scan_blocks] block sums of the prefix sums
-    int64_t *totals;              // [3] edge vertices, triangles, interior vertices
+    void *ebits;              // [node_rows][W] edge bits, bit 3 i + axis
+    void *cbits;              // [cell_rows][Wc] cut-cell bits
+    uint32_t *erank, *crank;  // per bitmap word: the set bits in all words before it
+    unsigned long long *part; // [2][part_cap] block sums of a pair of prefix sums
+    size_t part_cap;
+    int64_t *totals;          // [4] edge vertices, cut cells, triangles, interior vertices
+    // per cut cell in cube order, capacity cut_cap
+    uint32_t *order;          // the cell's linear index
+    uint32_t *tn, *cn;        // its triangle count, 1 if its tiling uses the interior vertex
+    int64_t *tpre, *cpre;     // their exclusive prefix sums
+    size_t cut_cap;
 };
 
-// Up to three exclusive prefix sums (uint32 in -> int64 out) in one set of launches.
+// Up to three exclusive prefix sums in one set of launches.  A segment sums `in` (uint32 counts of
+// at most 64 each) or, with `bits`, the popcounts of bitmap words; it writes int64 `out` or uint32
+// `out32`.
 struct MeshScan {
     struct Seg {
         const uint32_t *in;
+        const unsigned long long *bits;
         int64_t *out;
+        uint32_t *out32;
         size_t n;
     } seg[3];
     unsigned long long *part;
@@ -154,14 +161,14 @@ struct MeshGrid {
     int64_t n[3];
 };
 
-// Marks, counts and ranks (leaves the three totals in b.totals on the device).
+// Marks, ranks and orders the ncut listed cut cells; leaves the four totals in b.totals.
 template <typename T>
-void launch_mesh_count(const T *phi, const GridT<T> &g, const uint32_t *cutlist, const uint32_t *counter,
+void launch_mesh_count(const T *phi, const GridT<T> &g, const uint32_t *cutlist, uint32_t ncut,
                        const MeshBuffers &b, hipStream_t s);
-// Writes every vertex (3 T each) and triangle (3 int64, 1-based), sized from the totals.
+// Writes every vertex (3 T each) and triangle (3 int64, 1-based), into arrays sized from the totals.
 template <typename T>
-void launch_mesh_emit(const T *phi, const GridT<T> &g, const MeshBuffers &b, T *verts, int64_t *tris,
-                      hipStream_t s);
+void launch_mesh_emit(const T *phi, const GridT<T> &g, uint32_t ncut, const MeshBuffers &b, T *verts,
+                      int64_t *tris, hipStream_t s);
 template <typename T>
 void launch_cell_indices(const T *verts, const int64_t *tris, int64_t nt, const MeshGrid &g, int64_t *cells,
                          hipStream_t s);
