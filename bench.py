@@ -42,7 +42,7 @@ def parse():
                          "autotuner picks among the fused z-march shapes on this GPU before warm-up)")
     ap.add_argument("--quick", action="store_true",
                     help="only the headline workload (skip the secondary workloads)")
-    ap.add_argument("--only", default=None, choices=["csg", "batched", "secondary"],
+    ap.add_argument("--only", default=None, choices=["csg", "batched", "secondary", "surface"],
                     help="run one secondary workload alone and print its JSON")
     ap.add_argument("--train-epochs", type=int, default=24,
                     help="surrogate training epochs of the composite-body workload")
@@ -132,6 +132,55 @@ def secondary(seed):
                        "hbm_frac_two_pass": round(two_pass / t_p / 1e9 / HBM_PEAK_GBS, 4),
                        "cut_cells": int((c3.read("kind") == 2).sum())}
     del c3
+    return out
+
+
+def surface(seed, n=512, reps=8, cpu=True):
+    """generate_mesh on the marching-cubes workload (the 512^3 sphere, moving 0.5 cell per step):
+    the device march (HIP events: count + emit kernels, the host round trip for the output sizes
+    excluded), the whole call with the copy of the mesh to the host, and the oracle's serial march
+    of the same field on one host core (the reference's generate_mesh is a serial host march)."""
+    from cutcell import MarchingCubesCutCell, allocate_cache, cell_indices, generate_mesh, update_cache
+    grid, geo, _ = workload(n, seed)
+    c = allocate_cache(grid, MarchingCubesCutCell)
+    c.set_profiling(True)
+    d = grid.d[0]
+    dev, wall, ci = [], [], []
+    for s in range(reps + 1):
+        g = grid.moved((-0.5 * d * s, 0.0, 0.0))
+        update_cache(c, geo, g)
+        t = time.perf_counter()
+        mesh = generate_mesh(c, g, warn=False)
+        t1 = time.perf_counter()
+        kt = dict(c.kernel_times())
+        t2 = time.perf_counter()
+        cell_indices(mesh, g)
+        t3 = time.perf_counter()
+        if s == 0:
+            continue  # first march allocates the work arrays
+        dev.append(kt["mesh_count"] + kt["mesh_emit"])
+        wall.append((t1 - t) * 1e3)
+        ci.append((t3 - t2) * 1e3)
+    nv, nt = len(mesh.nodes), len(mesh.elements)
+    out = {"workload": f"generate_mesh of the {n}^3 marching-cubes sphere cache",
+           "vertices": nv, "triangles": nt, "cut_cells": c.cut_count(),
+           "device_ms": round(float(np.median(dev)), 4), "call_ms_with_host_copy": round(float(np.median(wall)), 3),
+           "cell_indices_ms": round(float(np.median(ci)), 3),
+           "triangles_per_s_device": nt / (float(np.median(dev)) * 1e-3),
+           "kernel_ms": {k: round(v, 4) for k, v in kt.items()}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        phi = c.phi
+        g = grid.moved((-0.5 * d * reps, 0.0, 0.0))
+        t = time.perf_counter()
+        nodes, tris, _ = orc.mc_mesh(phi, g.x0, g.d, g.n)
+        t_cpu = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(t_cpu * 1e3 / 2, 1), "unit": "ms per march", "cores": 1, "kind": "port",
+                               "sample": "the oracle's serial march of the same field (it marches twice, counting "
+                                         "then filling; half the time is reported)",
+                               "same_mesh": bool(len(nodes) == nv and len(tris) == nt)}
+    del c
     return out
 
 
@@ -342,7 +391,8 @@ def main():
     if a.only:
         fn = {"csg": lambda: csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples),
               "batched": lambda: batched(a.seed),
-              "secondary": lambda: secondary(a.seed)}[a.only]
+              "secondary": lambda: secondary(a.seed),
+              "surface": lambda: surface(a.seed, cpu=not a.no_cpu)}[a.only]
         print(json.dumps({a.only: fn()}))
         return
     grid, geo, centre = workload(a.n, a.seed)
@@ -438,6 +488,7 @@ def main():
         out["fp32"] = fp32_error(grid, geo, cache)
         del cache
         out.update(secondary(a.seed))
+        out["mc_surface_512"] = surface(a.seed, cpu=not a.no_cpu)
         out["batched_64_bodies"] = batched(a.seed)
         out["csg_512_spheres"] = csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples)
     print(json.dumps(out))

@@ -11,13 +11,16 @@
  *   - every array is column-major over the grid (i fastest), one column per CutCellData field
  *     (src/cut_cell.jl:112-120, src/cache.jl:101-104); vector fields are interleaved per cell.
  *
- * Pinning: tests/test_oracle_{ms,mc,plic,geometry}.py check every function here against the
- * reference tests' golden values (hand cells, the flush cube, the PLIC count 84), closed forms,
- * invariants (closure, watertightness, the 256 sign patterns) and brute force.  Parity unpinned,
- * because no reference output exists to compare with: the exact bits of SDFLibrary's bodies (the
- * geometry here is the library's own, pinned by mathematics), of CartesianMeshes'
- * get_volume_fraction (orc_volume_fraction: pinned to tolerance against exact rationals) and of
- * get_elem_centroid (restated as x0 + (i + 0.5) d).
+ * Pinning: tests/test_oracle_{ms,mc,plic,geometry,mesh}.py check every function here against the
+ * reference tests' golden values (hand cells, the flush cube, the PLIC count 84, the documented
+ * sphere area of the marched surface), closed forms, invariants (closure, watertightness, the 256
+ * sign patterns, the marched surface closing the face clips' apertures) and brute force.  Parity
+ * unpinned, because no reference output exists to compare with: the exact bits of SDFLibrary's
+ * bodies (the geometry here is the library's own, pinned by mathematics), of CartesianMeshes'
+ * get_volume_fraction (orc_volume_fraction: pinned to tolerance against exact rationals), of
+ * get_elem_centroid (restated as x0 + (i + 0.5) d), and MarchingCubes.jl's own vertex numbering and
+ * index-to-world bits for the whole-grid surface (orc_mc_mesh: pinned by closure to the ulp-level
+ * agreement the reference states, by topology and by the documented area).
  */
 #ifndef CUTCELL_ORACLE_H
 #define CUTCELL_ORACLE_H
