@@ -130,6 +130,9 @@ struct cc_cache {
     int64_t mesh_nv = -1, mesh_nt = -1, mesh_serial = 0;
     uint64_t mesh_gen = 0;
     double mesh_x0[3] = {}, mesh_d[3] = {};
+    // marching squares laid out on a quadtree's leaves (ncell = leaves, phi = 4 values per leaf)
+    bool leaves = false;
+    int32_t *leaf_level = nullptr, *leaf_coord = nullptr;  // device copies of a host leaf list
 };
 
 namespace {
@@ -137,6 +140,9 @@ namespace {
 size_t elem_size(int dtype) { return dtype == CC_FLOAT32 ? 4 : 8; }
 
 int check_grid(const cc_cache *c, const cc_grid *g) {
+    if (c->leaves)
+        return fail(CC_ERR_UNSUPPORTED, "this cache is laid out on a tree's leaves: update and contour "
+                                        "it over the tree (cc_update_leaves, cc_generate_mesh_leaves)");
     if (!g) return fail(CC_ERR_ARGUMENT, "grid is NULL");
     if (g->dim != c->dim)
         return fail(CC_ERR_DIMENSION_MISMATCH, "the cache is %dD but the grid is %dD", c->dim, g->dim);
@@ -348,6 +354,69 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
     return CC_OK;
 }
 
+void free_all(cc_cache *c);
+
+// Every column, the stream, counters and events of a cache whose method, dtype, dim, ncell and
+// nnode are set (phi holds nnode values).  On failure everything is freed (the caller deletes c).
+int allocate_columns(cc_cache *c) {
+    const int method = c->method, dtype = c->dtype, dim = c->dim;
+    const int64_t ncell = c->ncell, nnode = c->nnode;
+    const size_t es = elem_size(dtype);
+    const int64_t D = dim;
+    int64_t bytes[CC_NUM_FIELDS] = {};
+    bytes[CC_FIELD_KIND] = ncell;
+    bytes[CC_FIELD_AMBIGUOUS] = ncell;
+    bytes[CC_FIELD_VOLUME_FRACTION] = ncell * es;
+    bytes[CC_FIELD_CENTROID] = ncell * D * es;
+    bytes[CC_FIELD_FACE_FRACTION] = ncell * 2 * D * es;
+    bytes[CC_FIELD_FACE_CENTROID_LOCAL] = ncell * 2 * D * (D - 1) * es;
+    bytes[CC_FIELD_INTERFACE_CENTROID] = ncell * D * es;
+    if (method == CC_PLIC) {
+        bytes[CC_FIELD_NORMALS] = ncell * D * es;
+        bytes[CC_FIELD_INTERCEPTS] = ncell * es;
+        bytes[CC_FIELD_IS_VALID] = ncell;
+    } else {
+        bytes[CC_FIELD_PHI] = nnode * es;
+    }
+#define CREATE_HIP(expr)                                                                        \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            int code = e_ == hipErrorOutOfMemory ? CC_ERR_OUT_OF_MEMORY : CC_ERR_HIP;           \
+            fail(code, "%s failed: %s", #expr, hipGetErrorString(e_));                          \
+            free_all(c);                                                                        \
+            return code;                                                                        \
+        }                                                                                       \
+    } while (0)
+    CREATE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int f = 0; f < CC_NUM_FIELDS; ++f) {
+        c->colbytes[f] = bytes[f];
+        if (bytes[f] > 0) CREATE_HIP(hipMalloc(&c->col[f], bytes[f]));
+    }
+    CREATE_HIP(hipMalloc(&c->counter, 64));
+    if (method == CC_MARCHING_CUBES) CREATE_HIP(hipMalloc(&c->cutlist, ncell * sizeof(uint32_t)));
+    for (int e = 0; e <= MAX_TIMED; ++e) CREATE_HIP(hipEventCreate(&c->ev[e]));
+    if (method == CC_PLIC) {
+        // What a centroid plane fit has no value for, written once (src/plic/cache.jl:65-69).
+        const int fields[3] = {CC_FIELD_CENTROID, CC_FIELD_FACE_CENTROID_LOCAL,
+                               CC_FIELD_INTERFACE_CENTROID};
+        for (int f : fields) {
+            const size_t cnt = bytes[f] / es;
+            if (dtype == CC_FLOAT32)
+                hipLaunchKernelGGL(k_fill<float>, dim3(1024), dim3(256), 0, c->stream,
+                                   (float *)c->col[f], cnt, std::nanf(""));
+            else
+                hipLaunchKernelGGL(k_fill<double>, dim3(1024), dim3(256), 0, c->stream,
+                                   (double *)c->col[f], cnt, std::nan(""));
+        }
+        hipLaunchKernelGGL(k_fill<uint8_t>, dim3(1024), dim3(256), 0, c->stream,
+                           (uint8_t *)c->col[CC_FIELD_AMBIGUOUS], (size_t)ncell, (uint8_t)0);
+        CREATE_HIP(hipGetLastError());
+    }
+#undef CREATE_HIP
+    return CC_OK;
+}
+
 int dispatch_update(cc_cache *c, const cc::Geo *geo, const void *phi, int on_dev, const cc_grid *g) {
     HIPC(hipSetDevice(c->device));
     return c->dtype == CC_FLOAT32 ? run_update<float>(c, geo, phi, on_dev, g)
@@ -476,6 +545,68 @@ template <typename T> int mesh_march(cc_cache *c, const cc_grid *g) {
     return CC_OK;
 }
 
+// ---- marching squares on a quadtree's leaves (src/marching_squares/cache.jl:91-147) -------------
+int check_leaves(const cc_cache *c, const cc_leaves *t) {
+    if (!c->leaves)
+        return fail(CC_ERR_UNSUPPORTED, "this cache is laid out on a grid: update it over the grid (cc_update)");
+    if (!t || !t->level || !t->coord) return fail(CC_ERR_ARGUMENT, "tree or its leaf arrays are NULL");
+    if (t->n != c->ncell)
+        return fail(CC_ERR_DIMENSION_MISMATCH,
+                    "the cache holds %lld corner values and %lld cells, but `mesh` has %lld leaves; on an "
+                    "`AdaptiveMesh` both are one entry per leaf, in leaf order",
+                    (long long)c->ncell, (long long)c->ncell, (long long)t->n);
+    for (int a = 0; a < 2; ++a)
+        if (!(t->d[a] > 0.0) || !std::isfinite(t->d[a]) || !std::isfinite(t->x0[a]))
+            return fail(CC_ERR_ARGUMENT, "the tree's base cell size must be positive and finite, its origin finite");
+    return CC_OK;
+}
+
+template <typename T>
+int run_leaves(cc_cache *c, const cc::Geo *geo, const void *phi4, int on_dev, const cc_leaves *t) {
+    c->gen += 1;
+    c->nev = 0;
+    c->ev_ready = false;
+    if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
+    const int32_t *level = t->level, *coord = t->coord;
+    if (!t->on_device) {
+        // the host leaf list, copied in stream order into the cache's own buffers
+        HIPC(hipMemcpyAsync(c->leaf_level, t->level, sizeof(int32_t) * t->n, hipMemcpyHostToDevice, c->stream));
+        HIPC(hipMemcpyAsync(c->leaf_coord, t->coord, 2 * sizeof(int32_t) * t->n, hipMemcpyHostToDevice, c->stream));
+        level = c->leaf_level;
+        coord = c->leaf_coord;
+        mark(c, "copy_leaves");
+    }
+    if (phi4) {
+        HIPC(hipMemcpyAsync(c->col[CC_FIELD_PHI], phi4, c->colbytes[CC_FIELD_PHI],
+                            on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+        mark(c, "copy_phi");
+    }
+    cc::launch_zero_u32(c->counter, c->stream);
+    cc::launch_leaf_cells<T>(geo, level, coord, (uint32_t)t->n, t->x0, t->d, (T *)c->col[CC_FIELD_PHI],
+                             cols_t<T>(c), c->counter, c->stream);
+    mark(c, "leaf_cells");
+    HIPC(hipGetLastError());
+    c->ev_ready = c->profiling;
+    return CC_OK;
+}
+
+int update_leaves_impl(cc_cache *c, const cc_geometry *geo, const void *phi4, int on_dev, const cc_leaves *t) {
+    int rc = check_leaves(c, t);
+    if (rc) return rc;
+    cc::Geo gd;
+    if (geo) {
+        rc = stage_geometry(c, geo, &gd);
+        if (rc) return rc;
+    }
+    HIPC(hipSetDevice(c->device));
+    const cc::Geo *gp = geo ? &gd : nullptr;
+    rc = c->dtype == CC_FLOAT32 ? run_leaves<float>(c, gp, phi4, on_dev, t)
+                                : run_leaves<double>(c, gp, phi4, on_dev, t);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(c->stream));
+    return CC_OK;
+}
+
 void free_all(cc_cache *c) {
     for (void *p : c->mb_allocs) (void)hipFree(p);
     c->mb_allocs.clear();
@@ -486,6 +617,8 @@ void free_all(cc_cache *c) {
     if (c->mverts) (void)hipFree(c->mverts);
     if (c->mtris) (void)hipFree(c->mtris);
     if (c->mcells) (void)hipFree(c->mcells);
+    if (c->leaf_level) (void)hipFree(c->leaf_level);
+    if (c->leaf_coord) (void)hipFree(c->leaf_coord);
     for (auto &p : c->col)
         if (p) { (void)hipFree(p); p = nullptr; }
     if (c->cutlist) (void)hipFree(c->cutlist);
@@ -507,13 +640,13 @@ struct KeyHash {
     }
 };
 
-template <typename T>
-void contour(const std::vector<T> &phi, const std::vector<int8_t> &kind, const cc_grid *g,
-             std::vector<double> &nodes, std::vector<int64_t> &lines) {
-    const int64_t n0 = g->n[0], n1 = g->n[1], sy = n0 + 1;
+// Welds contour vertices by exact coordinate equality (surface.jl:170-177): `+ 0` folds -0.0 onto
+// 0.0; ids are 1-based, as MeshLibrary numbers nodes.
+struct Welder {
     std::unordered_map<std::pair<uint64_t, uint64_t>, int64_t, KeyHash> ids;
-    auto vid = [&](T x, T y) -> int64_t {
-        // exact-equality weld; `+ 0` folds -0.0 onto 0.0 (surface.jl:170-177)
+    std::vector<double> &nodes;
+    explicit Welder(std::vector<double> &n) : nodes(n) {}
+    template <typename T> int64_t operator()(T x, T y) {
         const double kx = (double)(x + T(0)), ky = (double)(y + T(0));
         uint64_t bx, by;
         std::memcpy(&bx, &kx, 8);
@@ -522,10 +655,59 @@ void contour(const std::vector<T> &phi, const std::vector<int8_t> &kind, const c
         if (it != ids.end()) return it->second;
         nodes.push_back(kx);
         nodes.push_back(ky);
-        const int64_t id = (int64_t)nodes.size() / 2;  // 1-based, as MeshLibrary numbers nodes
+        const int64_t id = (int64_t)nodes.size() / 2;
         ids.emplace(std::make_pair(bx, by), id);
         return id;
-    };
+    }
+};
+
+// One cut cell's interface segments (cell_boundary_walk, cell_gap_segments and cell_interface's
+// orientation, marching_squares.jl:240-351), corners nx/ny and values ph in MS_NODE_BITS order,
+// appended as welded lines.
+template <typename T>
+void cell_segments(const T (&nx)[4], const T (&ny)[4], const T (&ph)[4], Welder &vid, std::vector<int64_t> &lines) {
+    T sax[4], say[4], sbx[4], sby[4];
+    bool pres[4];
+    for (int k = 0; k < 4; ++k) {  // cell_boundary_walk
+        const int k2 = (k + 1) & 3;
+        const T fa = ph[k], fb = ph[k2];
+        const bool oa = fa >= T(0), ob = fb >= T(0);
+        const T fr = (oa && ob) ? T(1) : ((!oa && !ob) ? T(0) : (oa ? fa / (fa - fb) : fb / (fb - fa)));
+        pres[k] = oa || ob;
+        if (oa && ob) { sax[k] = nx[k]; say[k] = ny[k]; sbx[k] = nx[k2]; sby[k] = ny[k2]; }
+        else if (oa) {
+            sax[k] = nx[k]; say[k] = ny[k];
+            sbx[k] = nx[k] + fr * (nx[k2] - nx[k]);
+            sby[k] = ny[k] + fr * (ny[k2] - ny[k]);
+        } else {
+            sax[k] = nx[k2] + fr * (nx[k] - nx[k2]);
+            say[k] = ny[k2] + fr * (ny[k] - ny[k2]);
+            sbx[k] = nx[k2]; sby[k] = ny[k2];
+        }
+    }
+    int nseg = 0;
+    for (int k = 0; k < 4 && nseg < 2; ++k) {  // cell_gap_segments, then cell_interface's swap
+        if (!pres[k]) continue;
+        int jn = k;
+        for (int s = 1; s <= 3; ++s)
+            if (pres[(k + s) & 3]) { jn = (k + s) & 3; break; }
+        const T ax = sbx[k], ay = sby[k], bx = sax[jn], by = say[jn];
+        if (ax != bx || ay != by) {
+            nseg += 1;
+            const int64_t i1 = vid(bx, by);
+            const int64_t i2 = vid(ax, ay);
+            lines.push_back(i1);
+            lines.push_back(i2);
+        }
+    }
+}
+
+// The grid contour: cut cells in linear order (surface.jl:59-86).
+template <typename T>
+void contour(const std::vector<T> &phi, const std::vector<int8_t> &kind, const cc_grid *g,
+             std::vector<double> &nodes, std::vector<int64_t> &lines) {
+    const int64_t n0 = g->n[0], n1 = g->n[1], sy = n0 + 1;
+    Welder vid(nodes);
     const T x0 = T(g->x0[0]), y0 = T(g->x0[1]), dx = T(g->d[0]), dy = T(g->d[1]);
     for (int64_t c = 0; c < n0 * n1; ++c) {
         if (kind[c] != CC_CELL_CUT) continue;  // the screen: corners not all of one sign
@@ -535,40 +717,27 @@ void contour(const std::vector<T> &phi, const std::vector<int8_t> &kind, const c
         const T xl = x0 + T(i) * dx, xh = x0 + T(i + 1) * dx;
         const T yl = y0 + T(j) * dy, yh = y0 + T(j + 1) * dy;
         const T nx[4] = {xl, xh, xh, xl}, ny[4] = {yl, yl, yh, yh};
-        T sax[4], say[4], sbx[4], sby[4];
-        bool pres[4];
-        for (int k = 0; k < 4; ++k) {  // cell_boundary_walk
-            const int k2 = (k + 1) & 3;
-            const T fa = ph[k], fb = ph[k2];
-            const bool oa = fa >= T(0), ob = fb >= T(0);
-            const T fr = (oa && ob) ? T(1) : ((!oa && !ob) ? T(0) : (oa ? fa / (fa - fb) : fb / (fb - fa)));
-            pres[k] = oa || ob;
-            if (oa && ob) { sax[k] = nx[k]; say[k] = ny[k]; sbx[k] = nx[k2]; sby[k] = ny[k2]; }
-            else if (oa) {
-                sax[k] = nx[k]; say[k] = ny[k];
-                sbx[k] = nx[k] + fr * (nx[k2] - nx[k]);
-                sby[k] = ny[k] + fr * (ny[k2] - ny[k]);
-            } else {
-                sax[k] = nx[k2] + fr * (nx[k] - nx[k2]);
-                say[k] = ny[k2] + fr * (ny[k] - ny[k2]);
-                sbx[k] = nx[k2]; sby[k] = ny[k2];
-            }
-        }
-        int nseg = 0;
-        for (int k = 0; k < 4 && nseg < 2; ++k) {  // cell_gap_segments, then cell_interface's swap
-            if (!pres[k]) continue;
-            int jn = k;
-            for (int s = 1; s <= 3; ++s)
-                if (pres[(k + s) & 3]) { jn = (k + s) & 3; break; }
-            const T ax = sbx[k], ay = sby[k], bx = sax[jn], by = say[jn];
-            if (ax != bx || ay != by) {
-                nseg += 1;
-                const int64_t i1 = vid(bx, by);
-                const int64_t i2 = vid(ax, ay);
-                lines.push_back(i1);
-                lines.push_back(i2);
-            }
-        }
+        cell_segments(nx, ny, ph, vid, lines);
+    }
+}
+
+// The tree contour: cut leaves in leaf order, each from its own four stored corner values
+// (surface.jl:92-110), corners off the lattice as the leaf pass places them.
+template <typename T>
+void contour_leaves(const std::vector<T> &phi4, const std::vector<int8_t> &kind, const std::vector<int32_t> &level,
+                    const std::vector<int32_t> &coord, const cc_leaves *t, std::vector<double> &nodes,
+                    std::vector<int64_t> &lines) {
+    Welder vid(nodes);
+    const T x0 = T(t->x0[0]), y0 = T(t->x0[1]), dx = T(t->d[0]), dy = T(t->d[1]);
+    for (int64_t c = 0; c < t->n; ++c) {
+        if (kind[c] != CC_CELL_CUT) continue;
+        const T hx = std::ldexp(dx, -level[c]), hy = std::ldexp(dy, -level[c]);
+        const int cx = coord[2 * c], cy = coord[2 * c + 1];
+        const T xl = x0 + T(cx) * hx, xh = x0 + T(cx + 1) * hx;
+        const T yl = y0 + T(cy) * hy, yh = y0 + T(cy + 1) * hy;
+        const T nx[4] = {xl, xh, xh, xl}, ny[4] = {yl, yl, yh, yh};
+        const T ph[4] = {phi4[4 * c], phi4[4 * c + 1], phi4[4 * c + 2], phi4[4 * c + 3]};
+        cell_segments(nx, ny, ph, vid, lines);
     }
 }
 
@@ -633,60 +802,8 @@ int cc_cache_create(int method, int dtype, const cc_grid *grid, cc_cache **out) 
     int rc = check_grid(c, grid);
     if (rc) { delete c; return rc; }
 
-    const size_t es = elem_size(dtype);
-    const int64_t D = dim;
-    int64_t bytes[CC_NUM_FIELDS] = {};
-    bytes[CC_FIELD_KIND] = ncell;
-    bytes[CC_FIELD_AMBIGUOUS] = ncell;
-    bytes[CC_FIELD_VOLUME_FRACTION] = ncell * es;
-    bytes[CC_FIELD_CENTROID] = ncell * D * es;
-    bytes[CC_FIELD_FACE_FRACTION] = ncell * 2 * D * es;
-    bytes[CC_FIELD_FACE_CENTROID_LOCAL] = ncell * 2 * D * (D - 1) * es;
-    bytes[CC_FIELD_INTERFACE_CENTROID] = ncell * D * es;
-    if (method == CC_PLIC) {
-        bytes[CC_FIELD_NORMALS] = ncell * D * es;
-        bytes[CC_FIELD_INTERCEPTS] = ncell * es;
-        bytes[CC_FIELD_IS_VALID] = ncell;
-    } else {
-        bytes[CC_FIELD_PHI] = nnode * es;
-    }
-#define CREATE_HIP(expr)                                                                        \
-    do {                                                                                        \
-        hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess) {                                                                 \
-            int code = e_ == hipErrorOutOfMemory ? CC_ERR_OUT_OF_MEMORY : CC_ERR_HIP;           \
-            fail(code, "%s failed: %s", #expr, hipGetErrorString(e_));                          \
-            free_all(c);                                                                        \
-            delete c;                                                                           \
-            return code;                                                                        \
-        }                                                                                       \
-    } while (0)
-    CREATE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (int f = 0; f < CC_NUM_FIELDS; ++f) {
-        c->colbytes[f] = bytes[f];
-        if (bytes[f] > 0) CREATE_HIP(hipMalloc(&c->col[f], bytes[f]));
-    }
-    CREATE_HIP(hipMalloc(&c->counter, 64));
-    if (method == CC_MARCHING_CUBES) CREATE_HIP(hipMalloc(&c->cutlist, ncell * sizeof(uint32_t)));
-    for (int e = 0; e <= MAX_TIMED; ++e) CREATE_HIP(hipEventCreate(&c->ev[e]));
-    if (method == CC_PLIC) {
-        // What a centroid plane fit has no value for, written once (src/plic/cache.jl:65-69).
-        const int fields[3] = {CC_FIELD_CENTROID, CC_FIELD_FACE_CENTROID_LOCAL,
-                               CC_FIELD_INTERFACE_CENTROID};
-        for (int f : fields) {
-            const size_t cnt = bytes[f] / es;
-            if (dtype == CC_FLOAT32)
-                hipLaunchKernelGGL(k_fill<float>, dim3(1024), dim3(256), 0, c->stream,
-                                   (float *)c->col[f], cnt, std::nanf(""));
-            else
-                hipLaunchKernelGGL(k_fill<double>, dim3(1024), dim3(256), 0, c->stream,
-                                   (double *)c->col[f], cnt, std::nan(""));
-        }
-        hipLaunchKernelGGL(k_fill<uint8_t>, dim3(1024), dim3(256), 0, c->stream,
-                           (uint8_t *)c->col[CC_FIELD_AMBIGUOUS], (size_t)ncell, (uint8_t)0);
-        CREATE_HIP(hipGetLastError());
-    }
-#undef CREATE_HIP
+    rc = allocate_columns(c);
+    if (rc) { delete c; return rc; }
     // Seeded as no body: one update against EmptyGeo(one(T)) (src/cache.jl:22-25).
     cc_geometry empty{};
     empty.kind = CC_GEO_EMPTY;
@@ -965,6 +1082,100 @@ int cc_generate_mesh_ms(cc_cache *c, const cc_grid *grid, double *nodes, int64_t
         std::vector<double> phi(c->nnode);
         HIPC(hipMemcpy(phi.data(), c->col[CC_FIELD_PHI], c->colbytes[CC_FIELD_PHI], hipMemcpyDeviceToHost));
         contour<double>(phi, kind, grid, vn, vl);
+    }
+    const int64_t nn = (int64_t)vn.size() / 2, nl = (int64_t)vl.size() / 2;
+    if (nodes && lines) {
+        if (*nnodes < nn || *nlines < nl)
+            return fail(CC_ERR_ARGUMENT, "buffers hold %lld nodes / %lld lines, need %lld / %lld",
+                        (long long)*nnodes, (long long)*nlines, (long long)nn, (long long)nl);
+        std::memcpy(nodes, vn.data(), vn.size() * sizeof(double));
+        std::memcpy(lines, vl.data(), vl.size() * sizeof(int64_t));
+    }
+    *nnodes = nn;
+    *nlines = nl;
+    return CC_OK;
+}
+
+int cc_leaf_cache_create(int dtype, const cc_leaves *tree, cc_cache **out) {
+    if (!out || !tree) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (dtype != CC_FLOAT64 && dtype != CC_FLOAT32)
+        return fail(CC_ERR_UNSUPPORTED, "element type must be CC_FLOAT64 or CC_FLOAT32");
+    if (tree->n < 1 || tree->n >= (int64_t(1) << 30))
+        return fail(CC_ERR_ARGUMENT, "a tree of %lld leaves is not supported (1 to 2^30)", (long long)tree->n);
+    cc_cache *c = new cc_cache();
+    c->method = CC_MARCHING_SQUARES;
+    c->dtype = dtype;
+    c->dim = 2;
+    c->leaves = true;
+    c->n[0] = tree->n;
+    c->ncell = tree->n;
+    c->nnode = 4 * tree->n;  // four corner values per leaf
+    (void)hipGetDevice(&c->device);
+    int rc = check_leaves(c, tree);
+    if (rc) { delete c; return rc; }
+    rc = allocate_columns(c);
+    if (rc) { delete c; return rc; }
+    if (hipMalloc((void **)&c->leaf_level, sizeof(int32_t) * tree->n) != hipSuccess ||
+        hipMalloc((void **)&c->leaf_coord, 2 * sizeof(int32_t) * tree->n) != hipSuccess) {
+        free_all(c);
+        delete c;
+        return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the leaf list");
+    }
+    // seeded as no body: one update against EmptyGeo(1) over the tree (src/marching_squares/cache.jl:105)
+    cc_geometry empty{};
+    empty.kind = CC_GEO_EMPTY;
+    empty.p[0] = 1.0;
+    rc = update_leaves_impl(c, &empty, nullptr, 0, tree);
+    if (rc) {
+        std::string msg = g_err;
+        free_all(c);
+        delete c;
+        g_err = msg;
+        return rc;
+    }
+    *out = c;
+    return CC_OK;
+}
+
+int cc_update_leaves(cc_cache *c, const cc_geometry *geo, const cc_leaves *tree) {
+    if (!c || !geo) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    return update_leaves_impl(c, geo, nullptr, 0, tree);
+}
+
+int cc_update_leaves_from_phi(cc_cache *c, const void *phi4, int on_device, const cc_leaves *tree) {
+    if (!c || !phi4) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    return update_leaves_impl(c, nullptr, phi4, on_device, tree);
+}
+
+int cc_generate_mesh_leaves(cc_cache *c, const cc_leaves *tree, double *nodes, int64_t *nnodes, int64_t *lines,
+                            int64_t *nlines) {
+    if (!c || !nnodes || !nlines) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    int rc = check_leaves(c, tree);
+    if (rc) return rc;
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->stream));
+    const int64_t n = tree->n;
+    std::vector<int8_t> kind(n);
+    std::vector<int32_t> level(n), coord(2 * n);
+    HIPC(hipMemcpy(kind.data(), c->col[CC_FIELD_KIND], n, hipMemcpyDeviceToHost));
+    if (tree->on_device) {
+        HIPC(hipMemcpy(level.data(), tree->level, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(coord.data(), tree->coord, 2 * sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    } else {
+        std::memcpy(level.data(), tree->level, sizeof(int32_t) * n);
+        std::memcpy(coord.data(), tree->coord, 2 * sizeof(int32_t) * n);
+    }
+    std::vector<double> vn;
+    std::vector<int64_t> vl;
+    if (c->dtype == CC_FLOAT32) {
+        std::vector<float> phi4(4 * n);
+        HIPC(hipMemcpy(phi4.data(), c->col[CC_FIELD_PHI], c->colbytes[CC_FIELD_PHI], hipMemcpyDeviceToHost));
+        contour_leaves<float>(phi4, kind, level, coord, tree, vn, vl);
+    } else {
+        std::vector<double> phi4(4 * n);
+        HIPC(hipMemcpy(phi4.data(), c->col[CC_FIELD_PHI], c->colbytes[CC_FIELD_PHI], hipMemcpyDeviceToHost));
+        contour_leaves<double>(phi4, kind, level, coord, tree, vn, vl);
     }
     const int64_t nn = (int64_t)vn.size() / 2, nl = (int64_t)vl.size() / 2;
     if (nodes && lines) {

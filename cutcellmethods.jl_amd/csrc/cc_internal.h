@@ -49,6 +49,13 @@ template <typename T>
 void launch_ms_cells(const T *phi, const GridT<T> &g, const Cols<T> &cols, uint32_t *counter,
                      hipStream_t s);
 
+// The quadtree pass: leaf c at `level[c]`, lower corner `coord[2c..2c+1]` on its level's lattice
+// (spacing d / 2^level from x0); samples `geo` at each leaf's corners into phi4 (four per leaf) --
+// or, with geo == nullptr, reads them from phi4 -- and writes one record per leaf.
+template <typename T>
+void launch_leaf_cells(const Geo *geo, const int32_t *level, const int32_t *coord, uint32_t n, const double *x0,
+                       const double *d, T *phi4, const Cols<T> &cols, uint32_t *counter, hipStream_t s);
+
 template <typename T>
 void launch_plic_fit(const GridT<T> &g, const Geo &geo, int dim, const Cols<T> &cols, T *normals,
                      T *intercepts, uint8_t *valid, hipStream_t s);

@@ -12,6 +12,7 @@ from .geometry import (EmptyGeo, SDFBox, SDFCircle, SDFPlane, SDFSphere, SDFSphe
                        SDFTorus, random_rotation)
 from .grid import CartesianGrid, CartesianIsoGrid
 from .surrogate import Surrogate
+from .tree import AdaptiveMesh
 from .readers import (CELL_CUT, CELL_INSIDE, CELL_OUTSIDE, closure_residual, cut_cell_report,
                       face_area_of, face_centroid, full_face_area, interface_area,
                       interface_normal, interface_normal_area)

@@ -22,8 +22,9 @@ import warnings
 import numpy as np
 
 from . import _lib
-from ._lib import ArgumentError, DimensionMismatch, check, lib, make_geometry, make_grid
+from ._lib import ArgumentError, DimensionMismatch, Unsupported, check, lib, make_geometry, make_grid
 from .grid import CartesianGrid
+from .tree import AdaptiveMesh
 
 
 class MarchingSquaresCutCell:
@@ -92,14 +93,20 @@ class Cache:
             pass
 
     # -- columns -----------------------------------------------------------------------------
+    @property
+    def on_leaves(self):
+        """Whether the cache is laid out on a quadtree's leaves (allocated on an AdaptiveMesh)."""
+        return isinstance(self.grid, AdaptiveMesh)
+
     def _shape(self, name):
         D = self.grid.dim
         N = self.grid.ncells
+        if name == "phi":
+            return (N, 4) if self.on_leaves else (int(np.prod(self.grid.nodegrid_size())),)
         return {
             "kind": (N,), "ambiguous": (N,), "volume_fraction": (N,), "centroid": (N, D),
             "face_fraction": (N, 2 * D), "face_centroid_local": (N, 2 * D, D - 1),
-            "interface_centroid": (N, D), "phi": (int(np.prod(self.grid.nodegrid_size())),),
-            "normals": (N, D), "intercepts": (N,), "is_valid": (N,),
+            "interface_centroid": (N, D), "normals": (N, D), "intercepts": (N,), "is_valid": (N,),
         }[name]
 
     def _dtype(self, name):
@@ -195,11 +202,19 @@ class Cache:
 
 
 def allocate_cache(grid, method, dtype=np.float64):
-    """allocate_cache(grid, method; backend) -- seeded as no body."""
+    """allocate_cache(grid, method; backend) -- seeded as no body.  On an ``AdaptiveMesh`` (marching
+    squares only, src/marching_squares/cache.jl:100-106) the cache is laid out on the tree's leaves."""
     dt = np.dtype(dtype)
     if dt not in (np.dtype(np.float64), np.dtype(np.float32)):
         raise ArgumentError("dtype must be float64 or float32")
     h = ctypes.c_void_p()
+    if isinstance(grid, AdaptiveMesh):
+        if _method_code(method) != _lib.CC_MARCHING_SQUARES:
+            raise Unsupported("a quadtree cache is a marching-squares cache")
+        t = grid.leaves()
+        check(lib().cc_leaf_cache_create(_lib.CC_FLOAT32 if dt == np.float32 else _lib.CC_FLOAT64,
+                                         ctypes.byref(t), ctypes.byref(h)))
+        return Cache(h.value, grid, method, dt)
     g = make_grid(grid)
     check(lib().cc_cache_create(_method_code(method),
                                 _lib.CC_FLOAT32 if dt == np.float32 else _lib.CC_FLOAT64,
@@ -207,7 +222,35 @@ def allocate_cache(grid, method, dtype=np.float64):
     return Cache(h.value, grid, method, dt)
 
 
+def _update_leaves(cache, geo, mesh):
+    """update_cache!(cache, geo, mesh::AdaptiveMesh{2}) (src/marching_squares/cache.jl:115-137)."""
+    t = mesh.leaves()
+    if hasattr(geo, "kind") and hasattr(geo, "params"):
+        gg = make_geometry(geo)
+        check(lib().cc_update_leaves(cache.handle, ctypes.byref(gg), ctypes.byref(t)))
+    else:
+        if callable(geo):  # cell_values(field, mesh, c) with a callable field: each leaf's corners
+            X = mesh.cell_nodes().reshape(-1, 2)
+            vals = np.array([float(geo(x)) for x in X])
+        else:
+            vals = np.asarray(geo)
+        vals = np.ascontiguousarray(vals, dtype=cache.dtype).reshape(-1)
+        if vals.size != 4 * mesh.nleaves:
+            raise DimensionMismatch(f"`vals` has {vals.size} values, but `mesh` has {mesh.nleaves} "
+                                    "leaves of four corners each")
+        check(lib().cc_update_leaves_from_phi(cache.handle, vals.ctypes.data, 0, ctypes.byref(t)))
+    cache.grid = mesh
+    return cache
+
+
+def _check_layout(cache, domain):
+    if isinstance(domain, AdaptiveMesh) != cache.on_leaves:
+        raise Unsupported("a tree cache is updated and contoured over its tree, a grid cache over "
+                          "its grid")
+
+
 def _check_same_n(cache, grid):
+    _check_layout(cache, grid)
     if tuple(grid.n) != tuple(cache.grid.n):
         raise DimensionMismatch(
             f"the cache holds {cache.grid.n} cells, but `grid` has {grid.n}; a cache is allocated "
@@ -218,8 +261,12 @@ def update_cache(cache, geo, grid, sync=True):
     """update_cache!(cache, geo, grid) -> cache.
 
     ``geo`` is a geometry descriptor, a nodal array (one value per node, column-major or shaped as
-    the node grid), or a callable ``f(x) -> phi`` sampled at every node on the host.
+    the node grid), or a callable ``f(x) -> phi`` sampled at every node on the host.  ``grid`` may be
+    an ``AdaptiveMesh`` for a cache allocated on one (then an array holds four corner values per leaf).
     """
+    if isinstance(grid, AdaptiveMesh):
+        _check_layout(cache, grid)
+        return _update_leaves(cache, geo, grid)
     _check_same_n(cache, grid)
     g = make_grid(grid)
     if hasattr(geo, "kind") and hasattr(geo, "params"):
@@ -291,10 +338,23 @@ def generate_mesh(cache, grid, warn=True):
     ``(nodes (m, 2), lines (k, 2) 1-based)``.  Marching cubes (``src/marching_cubes/surface.jl:82-112``):
     the cache's phi marched on the device into a ``Surface`` of shared vertices and outward-wound
     triangles, in the cache's element type.  An empty result warns unless ``warn`` is false."""
-    _check_same_n(cache, grid)
-    g = make_grid(grid)
     nn = ctypes.c_int64()
     nl = ctypes.c_int64()
+    if isinstance(grid, AdaptiveMesh):  # src/marching_squares/surface.jl:92-110
+        _check_layout(cache, grid)
+        t = grid.leaves()
+        check(lib().cc_generate_mesh_leaves(cache.handle, ctypes.byref(t), None, ctypes.byref(nn), None,
+                                            ctypes.byref(nl)))
+        nodes = np.empty((nn.value, 2))
+        lines = np.empty((nl.value, 2), dtype=np.int64)
+        check(lib().cc_generate_mesh_leaves(cache.handle, ctypes.byref(t), nodes.ctypes.data,
+                                            ctypes.byref(nn), lines.ctypes.data, ctypes.byref(nl)))
+        if warn and nl.value == 0:
+            warnings.warn("generate_mesh: the field has no sign change on any leaf of `mesh`, so the "
+                          "contour is empty (the tree misses the body, or lies wholly inside it)")
+        return nodes, lines
+    _check_same_n(cache, grid)
+    g = make_grid(grid)
     if _method_code(cache.method) == _lib.CC_MARCHING_CUBES:
         check(lib().cc_generate_mesh_mc(cache.handle, ctypes.byref(g), None, ctypes.byref(nn), None,
                                         ctypes.byref(nl)))

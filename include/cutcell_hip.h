@@ -12,6 +12,8 @@
  *   face_fraction T[N][2D] | face_centroid_local T[N][2D][D-1] | interface_centroid T[N][D]
  * plus phi T[(n0+1)(n1+1)(n2+1)] for the nodal methods, and normals T[N][D], intercepts T[N],
  * is_valid uint8[N] for PLIC.  Directions are 0 = -x, 1 = +x, 2 = -y, ... (the reference's 1..2D).
+ * A marching-squares cache on a quadtree's leaves (cc_leaf_cache_create) has N = leaves, in leaf
+ * order, and phi T[N][4]: each leaf's four corner values. 
  */
 #ifndef CUTCELL_HIP_H
 #define CUTCELL_HIP_H
@@ -146,6 +148,43 @@ int cc_kernel_times(cc_cache *cache, int max, float *ms, const char **names, int
  * doubles; lines: 2*nlines int64 (1-based node ids, as MeshLibrary stores them). */
 int cc_generate_mesh_ms(cc_cache *cache, const cc_grid *grid, double *nodes, int64_t *nnodes,
                         int64_t *lines, int64_t *nlines);
+
+/* ---- marching squares on a quadtree (CartesianMeshes.AdaptiveMesh{2}) ----------------------------
+ * The tree crosses the ABI as its leaf list, in the tree's leaf order: leaf c has refinement level
+ * level[c] (0 = the base grid's cells) and its lower corner at integer lattice coordinates
+ * coord[2c], coord[2c+1] of its level, so its corners are x0 + (coord + bit) * d / 2^level
+ * (cell_nodes(mesh, c), src/marching_squares/marching_squares.jl:113-121). */
+typedef struct cc_leaves {
+    int64_t n;             /* leaves */
+    const int32_t *level;  /* [n] */
+    const int32_t *coord;  /* [n][2] */
+    double x0[2];          /* the base grid's origin */
+    double d[2];           /* the base grid's cell size */
+    int32_t on_device;     /* 1: level and coord are device pointers (the tree adapted to the device) */
+    int32_t reserved;
+} cc_leaves;
+
+/* allocate_cache(mesh::AdaptiveMesh{2}, MarchingSquaresCutCell()) -- src/marching_squares/cache.jl:100-106:
+ * a marching-squares cache laid out on the leaves (one CutCellData record per leaf, in leaf order;
+ * the phi column holds each leaf's four corner values in MS_NODE_BITS order), seeded as no body. */
+int cc_leaf_cache_create(int dtype, const cc_leaves *tree, cc_cache **out);
+
+/* update_cache!(cache, geo, mesh::AdaptiveMesh{2}) -- src/marching_squares/cache.jl:115-137: every
+ * leaf's corners sampled from geo and its moments built from exactly those values, one thread per
+ * leaf; blocks until done.  CC_ERR_DIMENSION_MISMATCH when the tree has another leaf count
+ * (_check_leaf_size, :140-147); CC_ERR_UNSUPPORTED on a grid cache. */
+int cc_update_leaves(cc_cache *cache, const cc_geometry *geo, const cc_leaves *tree);
+
+/* The same from corner values the caller sampled (cell_values(field, mesh, c) with a callable field,
+ * src/marching_squares/marching_squares.jl:152-155): phi4 holds four values per leaf in
+ * MS_NODE_BITS order, in the cache's element type, in host memory or (on_device != 0) device memory. */
+int cc_update_leaves_from_phi(cc_cache *cache, const void *phi4, int on_device, const cc_leaves *tree);
+
+/* generate_mesh(cache, mesh::AdaptiveMesh{2}) -- src/marching_squares/surface.jl:92-110: the contour
+ * of each cut leaf's stored corner values, in leaf order, welded by exact coordinate equality; the
+ * same two-call protocol and buffers as cc_generate_mesh_ms. */
+int cc_generate_mesh_leaves(cc_cache *cache, const cc_leaves *tree, double *nodes, int64_t *nnodes,
+                            int64_t *lines, int64_t *nlines);
 
 /* generate_mesh(cache, grid) for a marching-cubes cache -- src/marching_cubes/surface.jl:82-112:
  * the cache's phi marched on the device (Lewiner cases, shared vertices, wound so each triangle's

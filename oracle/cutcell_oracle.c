@@ -369,6 +369,33 @@ static void ms_cut_cell_moments(const double nodes[4][2], const double phi[4],
     *vf = area / vol_cell;
 }
 
+/* _leaf_cells_kernel! (src/marching_squares/cache.jl:129-137): for each leaf of a quadtree, its four
+ * corners off the integer lattice (cell_nodes(mesh, c), marching_squares.jl:113-121:
+ * x0 + (coord + bit) * h), the body sampled there (cell_values(field, mesh, c), :152-155), and the
+ * moments from exactly those values with the leaf's own size h.  h = d / 2^level: CartesianMeshes'
+ * get_elem_size is un-vendored, but a power-of-two scaling is exact, so every way of forming it
+ * gives these bits.  coord is the leaf's lower corner on its level's lattice, 2 per leaf. */
+void orc_ms_leaves(const orc_geometry *g, const double *x0, const double *d, int64_t n,
+                   const int32_t *level, const int32_t *coord, double *phi4, const orc_cols *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n; ++c) {
+        static const int B[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}}; /* MS_NODE_BITS :68 */
+        double h[2] = {ldexp(d[0], -level[c]), ldexp(d[1], -level[c])};
+        double nodes[4][2], ph[4];
+        for (int k = 0; k < 4; ++k) {
+            for (int a = 0; a < 2; ++a)
+                nodes[k][a] = x0[a] + (double)(coord[2 * c + a] + B[k][a]) * h[a];
+            ph[k] = orc_sdf_value(g, 2, nodes[k]);
+            phi4[4 * c + k] = ph[k];
+        }
+        ms_cut_cell_moments(nodes, ph, h, &out->kind[c], &out->ambiguous[c],
+                            &out->volume_fraction[c], out->centroid + 2 * c,
+                            out->face_fraction + 4 * c, out->face_centroid_local + 4 * c,
+                            out->interface_centroid + 2 * c);
+    }
+}
+
 /* _nodal_cells_kernel! (src/marching_squares/cache.jl:75-78) over a 2D grid. */
 void orc_ms_cells(const double *phi, const double *x0, const double *d, const int64_t *n,
                   const orc_cols *out)

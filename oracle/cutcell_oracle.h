@@ -71,6 +71,12 @@ void orc_sample_nodes(const orc_geometry *g, int dim, const double *x0, const do
 void orc_ms_cells(const double *phi, const double *x0, const double *d, const int64_t *n,
                   const orc_cols *out);
 
+/* ---- marching squares on a quadtree's leaves: src/marching_squares/cache.jl:129-137 ----------
+ * Leaf c: level[c], lower corner coord[2c..2c+1] on its level's lattice (spacing d / 2^level from
+ * the base origin x0).  phi4: the four corner values per leaf; out: one entry per leaf. */
+void orc_ms_leaves(const orc_geometry *g, const double *x0, const double *d, int64_t n,
+                   const int32_t *level, const int32_t *coord, double *phi4, const orc_cols *out);
+
 /* ---- marching cubes: src/marching_cubes/moments.jl:275-346, cells with k in [k0, k1) -------
  * `out` columns are indexed relative to cell (0,0,k0), so a slab can be checked on its own. */
 void orc_mc_cells(const double *phi, const double *x0, const double *d, const int64_t *n,

@@ -61,6 +61,7 @@ def lib():
         L.orc_mc_cell_triangles.restype = ctypes.c_int
         L.orc_mc_edge_crossings.argtypes = [vp, vp, vp]
         L.orc_mc_cell.argtypes = [vp] * 10
+        L.orc_ms_leaves.argtypes = [vp, vp, vp, ctypes.c_int64, vp, vp, vp, vp]
         L.orc_mc_mesh.argtypes = [vp] * 9
         L.orc_mc_mesh.restype = ctypes.c_int
         L.orc_volume_fraction.argtypes = [ctypes.c_int, vp, ctypes.c_double]
@@ -163,6 +164,21 @@ def ms_cells(phi, x0, d, n):
     phi = _f64(phi)
     lib().orc_ms_cells(_p(phi), _p(_f64(x0, 3)), _p(_f64(d, 3)), _p(_i64(n)), ctypes.byref(s))
     return cols
+
+
+def ms_leaves(geo, x0, d, level, coord):
+    """A marching-squares cache over quadtree leaves (src/marching_squares/cache.jl:129-137):
+    (phi (n, 4) corner values, columns with one entry per leaf)."""
+    level = np.ascontiguousarray(np.asarray(level, dtype=np.int32))
+    coord = np.ascontiguousarray(np.asarray(coord, dtype=np.int32).reshape(-1, 2))
+    n = level.size
+    cols = alloc_cols(n, 2)
+    s = _cols_struct(cols)
+    phi4 = np.zeros((n, 4))
+    g = make_geometry(geo)
+    lib().orc_ms_leaves(ctypes.byref(g), _p(_f64(x0, 2)), _p(_f64(d, 2)), n, _p(level), _p(coord),
+                        _p(phi4), ctypes.byref(s))
+    return phi4, cols
 
 
 def mc_cells(phi, x0, d, n, k0=0, k1=None):

@@ -114,3 +114,55 @@ def surface_moments(nodes, tris, face_fraction, origin, h, face_area_of):
     V = vol3 / 3
     return dict(vector_area=S, interface_centroid=o + Ac / A if A > 0 else None,
                 volume_fraction=V / np.prod(h), centroid=o + cm2 / (2 * V) if V != 0 else None)
+
+
+# ---- quadtrees ---------------------------------------------------------------------------------
+GEO_CENTRE, GEO_RADIUS = (0.13, -0.07), 0.9        # test/test_moments.jl:30-32
+CACHE_CENTRE, CACHE_RADIUS = (0.013, -0.021), 0.83  # test/test_cache.jl:22
+
+
+def circle_sdf(x, c, r):
+    dx, dy = x[..., 0] - c[0], x[..., 1] - c[1]
+    return np.sqrt(dx * dx + dy * dy) - r
+
+
+def banded_tree(level=4, width=2):
+    """test/test_moments.jl:72-95: base [-2, 2]^2 of 8 x 8, initial level 1, a uniformly refined
+    band round the circle so every cut leaf sits at the finest level, with genuine level jumps."""
+    from cutcell import AdaptiveMesh, CartesianGrid
+    base = CartesianGrid((-2.0, -2.0), (8, 8), (0.5, 0.5))
+    mesh = AdaptiveMesh(base, max_level=5, initial_level=1)
+    for _ in range(level):
+        h = mesh.elem_size()
+        diag = np.sqrt((h * h).sum(axis=1))
+        d = np.abs(circle_sdf(mesh.elem_centroid(), GEO_CENTRE, GEO_RADIUS))
+        flags = (mesh.level < level) & (d <= (width + 0.5) * diag)
+        if not flags.any():
+            break
+        mesh.refine(flags)
+    return mesh
+
+
+def cache_tree():
+    """test/test_cache.jl:116-124: base [-1.5, 1.5]^2 of 8 x 8, initial level 1, refined twice near
+    the circle, so cut leaves come in several sizes."""
+    from cutcell import AdaptiveMesh, CartesianGrid
+    base = CartesianGrid((-1.5, -1.5), (8, 8), (3.0 / 8, 3.0 / 8))
+    tree = AdaptiveMesh(base, max_level=3, initial_level=1)
+    for _ in range(2):
+        near = np.abs(circle_sdf(tree.elem_centroid(), CACHE_CENTRE, CACHE_RADIUS)) < \
+            1.5 * tree.elem_size().max(axis=1)
+        tree.refine(near)
+    return tree
+
+
+def same_level_neighbours(mesh):
+    """Pairs (a, b, axis) of leaves at one level sharing a face, b on a's + side."""
+    at = {(int(l), int(x), int(y)): i for i, (l, (x, y)) in enumerate(zip(mesh.level, mesh.coord))}
+    out = []
+    for i, (l, (x, y)) in enumerate(zip(mesh.level, mesh.coord)):
+        for ax, key in ((0, (int(l), int(x) + 1, int(y))), (1, (int(l), int(x), int(y) + 1))):
+            j = at.get(key)
+            if j is not None:
+                out.append((i, j, ax))
+    return out
