@@ -42,7 +42,7 @@ def parse():
                          "autotuner picks among the fused z-march shapes on this GPU before warm-up)")
     ap.add_argument("--quick", action="store_true",
                     help="only the headline workload (skip the secondary workloads)")
-    ap.add_argument("--only", default=None, choices=["csg", "batched", "secondary", "surface"],
+    ap.add_argument("--only", default=None, choices=["csg", "batched", "secondary", "surface", "quadtree"],
                     help="run one secondary workload alone and print its JSON")
     ap.add_argument("--train-epochs", type=int, default=24,
                     help="surrogate training epochs of the composite-body workload")
@@ -181,6 +181,58 @@ def surface(seed, n=512, reps=8, cpu=True):
                                          "then filling; half the time is reported)",
                                "same_mesh": bool(len(nodes) == nv and len(tris) == nt)}
     del c
+    return out
+
+
+def quadtree(seed, level=12, reps=20, cpu=True):
+    """Marching squares on a quadtree's leaves (the reference's AdaptiveMesh{2} cache): the circle of
+    its tree tests (centre (0.13, -0.07), r 0.9) in a band refined to `level` round the circle
+    (base [-2, 2]^2 of 8 x 8, band half-width 2.5 leaf diagonals), about 10^6 leaves at level 12."""
+    from cutcell import AdaptiveMesh, CartesianGrid, MarchingSquaresCutCell, SDFCircle, allocate_cache
+    from cutcell import generate_mesh, update_cache
+    centre, radius = (0.13, -0.07), 0.9
+    base = CartesianGrid((-2.0, -2.0), (8, 8), (0.5, 0.5))
+    mesh = AdaptiveMesh(base, max_level=level, initial_level=1)
+    for _ in range(level):
+        h = mesh.elem_size()
+        c = mesh.elem_centroid() - np.asarray(centre)
+        dist = np.abs(np.sqrt((c * c).sum(axis=1)) - radius)
+        flags = (mesh.level < level) & (dist <= 2.5 * np.sqrt((h * h).sum(axis=1)))
+        if not flags.any():
+            break
+        mesh.refine(flags)
+    geo = SDFCircle(centre, radius)
+    cache = allocate_cache(mesh, MarchingSquaresCutCell)
+    cache.set_profiling(True)
+    for _ in range(3):
+        update_cache(cache, geo, mesh)
+    ts, ks = [], []
+    for _ in range(reps):
+        t = time.perf_counter()
+        update_cache(cache, geo, mesh)
+        ts.append(time.perf_counter() - t)
+        ks.append(dict(cache.kernel_times())["leaf_cells"])
+    t_up, k_ms = float(np.median(ts)), float(np.median(ks))
+    n = mesh.nleaves
+    alg = n * (106 + 32)  # each leaf's record (2D, FP64) and its four corner values
+    t = time.perf_counter()
+    nodes, lines = generate_mesh(cache, mesh)
+    t_mesh = time.perf_counter() - t
+    out = {"workload": f"MarchingSquaresCutCell on {n} quadtree leaves (levels 1-{level}), circle r={radius}",
+           "leaves": n, "cut_leaves": cache.cut_count(), "update_ms": round(t_up * 1e3, 4),
+           "leaves_per_s": n / t_up, "kernel_ms": round(k_ms, 4),
+           "kernel_hbm_frac": round(alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "generate_mesh_ms": round(t_mesh * 1e3, 2), "contour_segments": int(lines.shape[0])}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        orc.ms_leaves(geo, base.x0, base.d, mesh.level[:1000], mesh.coord[:1000])
+        t = time.perf_counter()
+        orc.ms_leaves(geo, base.x0, base.d, mesh.level, mesh.coord)
+        t_cpu = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(t_cpu * 1e3, 2), "unit": "ms per update", "cores": orc.num_threads(),
+                               "kind": "port", "sample": "the oracle's leaf pass over the same tree (OpenMP)"}
+    del cache
     return out
 
 
@@ -392,7 +444,8 @@ def main():
         fn = {"csg": lambda: csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples),
               "batched": lambda: batched(a.seed),
               "secondary": lambda: secondary(a.seed),
-              "surface": lambda: surface(a.seed, cpu=not a.no_cpu)}[a.only]
+              "surface": lambda: surface(a.seed, cpu=not a.no_cpu),
+              "quadtree": lambda: quadtree(a.seed, cpu=not a.no_cpu)}[a.only]
         print(json.dumps({a.only: fn()}))
         return
     grid, geo, centre = workload(a.n, a.seed)
@@ -489,6 +542,7 @@ def main():
         del cache
         out.update(secondary(a.seed))
         out["mc_surface_512"] = surface(a.seed, cpu=not a.no_cpu)
+        out["ms_quadtree"] = quadtree(a.seed, cpu=not a.no_cpu)
         out["batched_64_bodies"] = batched(a.seed)
         out["csg_512_spheres"] = csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples)
     print(json.dumps(out))

@@ -133,6 +133,8 @@ struct cc_cache {
     // marching squares laid out on a quadtree's leaves (ncell = leaves, phi = 4 values per leaf)
     bool leaves = false;
     int32_t *leaf_level = nullptr, *leaf_coord = nullptr;  // device copies of a host leaf list
+    const void *leaf_src[2] = {};  // the host arrays and version tag they were copied from
+    int32_t leaf_version = 0;
 };
 
 namespace {
@@ -569,12 +571,22 @@ int run_leaves(cc_cache *c, const cc::Geo *geo, const void *phi4, int on_dev, co
     if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
     const int32_t *level = t->level, *coord = t->coord;
     if (!t->on_device) {
-        // the host leaf list, copied in stream order into the cache's own buffers
-        HIPC(hipMemcpyAsync(c->leaf_level, t->level, sizeof(int32_t) * t->n, hipMemcpyHostToDevice, c->stream));
-        HIPC(hipMemcpyAsync(c->leaf_coord, t->coord, 2 * sizeof(int32_t) * t->n, hipMemcpyHostToDevice, c->stream));
+        // the host leaf list, copied in stream order into the cache's own buffers -- unless the
+        // caller's version tag says the copy from the last update is still these leaves
+        const bool same = t->version != 0 && t->version == c->leaf_version && c->leaf_src[0] == t->level &&
+                          c->leaf_src[1] == t->coord;
+        if (!same) {
+            HIPC(hipMemcpyAsync(c->leaf_level, t->level, sizeof(int32_t) * t->n, hipMemcpyHostToDevice,
+                                c->stream));
+            HIPC(hipMemcpyAsync(c->leaf_coord, t->coord, 2 * sizeof(int32_t) * t->n, hipMemcpyHostToDevice,
+                                c->stream));
+            c->leaf_src[0] = t->level;
+            c->leaf_src[1] = t->coord;
+            c->leaf_version = t->version;
+            mark(c, "copy_leaves");
+        }
         level = c->leaf_level;
         coord = c->leaf_coord;
-        mark(c, "copy_leaves");
     }
     if (phi4) {
         HIPC(hipMemcpyAsync(c->col[CC_FIELD_PHI], phi4, c->colbytes[CC_FIELD_PHI],

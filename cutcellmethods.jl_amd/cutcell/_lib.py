@@ -52,7 +52,7 @@ class CCLeaves(ctypes.Structure):
     """A quadtree's leaf list (include/cutcell_hip.h, cc_leaves)."""
     _fields_ = [("n", ctypes.c_int64), ("level", ctypes.c_void_p), ("coord", ctypes.c_void_p),
                 ("x0", ctypes.c_double * 2), ("d", ctypes.c_double * 2), ("on_device", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("version", ctypes.c_int32)]
 
 
 class DimensionMismatch(ValueError):

@@ -13,10 +13,13 @@ its four children in Z order ((0,0), (1,0), (0,1), (1,1)).  Every entry point of
 leaf, in whatever order the list has, so a tree with CartesianMeshes' own order is taken as it is.
 """
 import ctypes
+import itertools
 
 import numpy as np
 
 from ._lib import ArgumentError, CCLeaves
+
+_versions = itertools.count(1)  # one tag per state of any tree in this process
 
 
 class AdaptiveMesh:
@@ -36,6 +39,7 @@ class AdaptiveMesh:
         jj, ii = np.meshgrid(np.arange(n1), np.arange(n0), indexing="ij")
         self.level = np.zeros(n0 * n1, dtype=np.int32)
         self.coord = np.ascontiguousarray(np.stack([ii.ravel(), jj.ravel()], axis=1).astype(np.int32))
+        self.version = next(_versions)
         for _ in range(initial_level):
             self.refine(np.ones(self.nleaves, dtype=bool))
 
@@ -59,7 +63,6 @@ class AdaptiveMesh:
 
     def cell_nodes(self, i=None):
         """Corners (MS_NODE_BITS order, counter-clockwise from the lower corner): (4, 2), or (n, 4, 2)."""
-        lv = self.level if i is None else self.level[i]
         co = self.coord if i is None else self.coord[i]
         h = self.elem_size(i)
         x0 = np.asarray(self.base.x0, dtype=np.float64)
@@ -89,6 +92,12 @@ class AdaptiveMesh:
             lv[at] = self.level[flags] + 1
         self.level = np.ascontiguousarray(lv.astype(np.int32))
         self.coord = np.ascontiguousarray(co.astype(np.int32))
+        self.version = next(_versions)
+        return self
+
+    def touch(self):
+        """Declare ``level`` / ``coord`` edited by hand: the next update copies the leaves again."""
+        self.version = next(_versions)
         return self
 
     def refine_where(self, pred):
@@ -106,5 +115,6 @@ class AdaptiveMesh:
             t.x0[a] = float(self.base.x0[a])
             t.d[a] = float(self.base.d[a])
         t.on_device = 0
+        t.version = self.version & 0x7FFFFFFF  # changes whenever the leaf arrays do
         t._keep = (self.level, self.coord)
         return t

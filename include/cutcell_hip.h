@@ -161,7 +161,9 @@ typedef struct cc_leaves {
     double x0[2];          /* the base grid's origin */
     double d[2];           /* the base grid's cell size */
     int32_t on_device;     /* 1: level and coord are device pointers (the tree adapted to the device) */
-    int32_t reserved;
+    int32_t version;       /* host arrays: 0, or a tag the caller changes whenever the leaf arrays
+                              change; with the same nonzero tag, arrays and n as the cache's last update
+                              the library keeps its device copy instead of copying the leaves again */
 } cc_leaves;
 
 /* allocate_cache(mesh::AdaptiveMesh{2}, MarchingSquaresCutCell()) -- src/marching_squares/cache.jl:100-106:
