@@ -130,6 +130,9 @@ struct cc_cache {
     int64_t mesh_nv = -1, mesh_nt = -1, mesh_serial = 0;
     uint64_t mesh_gen = 0;
     double mesh_x0[3] = {}, mesh_d[3] = {};
+    double mesh_min_frac = -1.0;  // the PLIC surface's element floor
+    cc::PlicSurfBuffers ps = {};
+    std::vector<void *> ps_allocs;
     // marching squares laid out on a quadtree's leaves (ncell = leaves, phi = 4 values per leaf)
     bool leaves = false;
     int32_t *leaf_level = nullptr, *leaf_coord = nullptr;  // device copies of a host leaf list
@@ -619,7 +622,66 @@ int update_leaves_impl(cc_cache *c, const cc_geometry *geo, const void *phi4, in
     return CC_OK;
 }
 
+// The PLIC surface (src/plic/surface.jl:28-267): count every cell, prefix sums, one round trip for
+// the sizes, then every node and element at its offset.
+template <typename T> int plic_surface(cc_cache *c, const cc_grid *g, double min_frac) {
+    const int64_t ncell = c->ncell;
+    if (!c->ps.vcnt) {
+        auto get = [&](void **p, size_t bytes) -> bool {
+            if (hipMalloc(p, bytes < 8 ? 8 : bytes) != hipSuccess) return false;
+            c->ps_allocs.push_back(*p);
+            return true;
+        };
+        const size_t blocks = cc::mesh_scan_blocks((size_t)ncell);
+        cc::PlicSurfBuffers &b = c->ps;
+        b.part_cap = blocks > 0 ? blocks : 1;
+        if (!(get((void **)&b.vcnt, 4 * ncell) && get((void **)&b.ecnt, 4 * ncell) &&
+              get((void **)&b.vpre, 8 * ncell) && get((void **)&b.epre, 8 * ncell) &&
+              get((void **)&b.part, 8 * 2 * b.part_cap) && get((void **)&b.totals, 8 * 2))) {
+            for (void *p : c->ps_allocs) (void)hipFree(p);
+            c->ps_allocs.clear();
+            c->ps = {};
+            return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the PLIC surface's work arrays");
+        }
+    }
+    const cc::GridT<T> gt = grid_t<T>(g, c->dim);
+    const T *nrm = (const T *)c->col[CC_FIELD_NORMALS], *icp = (const T *)c->col[CC_FIELD_INTERCEPTS];
+    c->nev = 0;
+    c->ev_ready = false;
+    if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
+    cc::launch_plic_surface<T>(gt, c->dim, nrm, icp, nullptr, (size_t)ncell, min_frac, c->ps, false, nullptr, nullptr,
+                               c->stream);
+    mark(c, "plic_surface_count");
+    HIPC(hipGetLastError());
+    int64_t tot[2] = {0, 0};
+    HIPC(hipMemcpyAsync(tot, c->ps.totals, sizeof tot, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    int rc = ensure((char **)&c->mverts, &c->mverts_cap, c->dim * sizeof(T) * (tot[0] > 0 ? tot[0] : 1));
+    if (rc) return rc;
+    rc = ensure(&c->mtris, &c->mtris_cap, c->dim * (tot[1] > 0 ? tot[1] : 1));
+    if (rc) return rc;
+    mark(c, "plic_surface_host_sync");
+    cc::launch_plic_surface<T>(gt, c->dim, nrm, icp, nullptr, (size_t)ncell, min_frac, c->ps, true, (T *)c->mverts,
+                               c->mtris, c->stream);
+    mark(c, "plic_surface_emit");
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->stream));
+    c->ev_ready = c->profiling;
+    c->mesh_nv = tot[0];
+    c->mesh_nt = tot[1];
+    c->mesh_gen = c->gen;
+    c->mesh_min_frac = min_frac;
+    for (int a = 0; a < 3; ++a) {
+        c->mesh_x0[a] = g->x0[a];
+        c->mesh_d[a] = g->d[a];
+    }
+    c->mesh_serial += 1;
+    return CC_OK;
+}
+
 void free_all(cc_cache *c) {
+    for (void *p : c->ps_allocs) (void)hipFree(p);
+    c->ps_allocs.clear();
     for (void *p : c->mb_allocs) (void)hipFree(p);
     c->mb_allocs.clear();
     for (void *p : {(void *)c->mb.order, (void *)c->mb.tn, (void *)c->mb.cn, (void *)c->mb.tpre,
@@ -1202,6 +1264,188 @@ int cc_generate_mesh_leaves(cc_cache *c, const cc_leaves *tree, double *nodes, i
     return CC_OK;
 }
 
+int cc_calc_volume(int dtype, const cc_geometry *geo, const cc_grid *grid, double *volume) {
+    if (!geo || !grid || !volume) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    if (dtype != CC_FLOAT64 && dtype != CC_FLOAT32)
+        return fail(CC_ERR_UNSUPPORTED, "element type must be CC_FLOAT64 or CC_FLOAT32");
+    const int dim = grid->dim;
+    if (dim != 2 && dim != 3) return fail(CC_ERR_ARGUMENT, "grid.dim must be 2 or 3");
+    int64_t ncell = 1;
+    for (int a = 0; a < dim; ++a) {
+        if (grid->n[a] < 1) return fail(CC_ERR_ARGUMENT, "every axis needs at least one cell");
+        if (!(grid->d[a] > 0.0) || !std::isfinite(grid->d[a]) || !std::isfinite(grid->x0[a]))
+            return fail(CC_ERR_ARGUMENT, "grid.d must be positive and finite, x0 finite");
+        ncell *= grid->n[a];
+    }
+    if (ncell >= (int64_t(1) << 32)) return fail(CC_ERR_UNSUPPORTED, "grids of 2^32 cells or more are not supported");
+    for (int a = 1; a < dim; ++a)  // src/volume.jl:30-32
+        if (grid->d[a] != grid->d[0])
+            return fail(CC_ERR_ARGUMENT,
+                        "calc_volume needs an isotropic grid (equal cell size on every axis), got grid.d = "
+                        "(%g, %g%s)", grid->d[0], grid->d[1], dim == 3 ? ", ..." : "");
+    if (geo->kind < CC_GEO_EMPTY || geo->kind > CC_GEO_SPHERE_UNION)
+        return fail(CC_ERR_UNSUPPORTED, "unknown geometry kind %d", geo->kind);
+    if (geo->kind == CC_GEO_TORUS && dim != 3) return fail(CC_ERR_UNSUPPORTED, "a torus is a 3D body");
+    if (geo->kind == CC_GEO_SPHERE_UNION && (geo->n_prims < 1 || !geo->prims || !(geo->p[0] > 0.0)))
+        return fail(CC_ERR_ARGUMENT, "a sphere union needs primitives and a positive smoothing k");
+    cc::Geo gd;
+    gd.kind = geo->kind;
+    gd.nprims = geo->kind == CC_GEO_SPHERE_UNION ? geo->n_prims : 0;
+    std::memcpy(gd.p, geo->p, sizeof gd.p);
+    gd.prims = nullptr;
+    hipStream_t st = nullptr;
+    void *part = nullptr, *out = nullptr;
+    double *prims = nullptr;
+    int rc = CC_OK;
+    const size_t es = elem_size(dtype);
+    const cc::GridT<double> g64 = grid_t<double>(grid, dim);
+    const cc::GridT<float> g32 = grid_t<float>(grid, dim);
+    const int nb = dtype == CC_FLOAT32 ? cc::plic_volume_blocks(g32, dim) : cc::plic_volume_blocks(g64, dim);
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipMalloc(&part, es * nb) != hipSuccess ||
+        hipMalloc(&out, es) != hipSuccess ||
+        (gd.nprims > 0 && (hipMalloc((void **)&prims, 32 * (size_t)gd.nprims) != hipSuccess ||
+                           hipMemcpy(prims, geo->prims, 32 * (size_t)gd.nprims, hipMemcpyHostToDevice) != hipSuccess))) {
+        rc = fail(CC_ERR_OUT_OF_MEMORY, "calc_volume: could not set up its work arrays");
+    } else {
+        gd.prims = prims;
+        double sum = 0.0;
+        if (dtype == CC_FLOAT32) {
+            cc::launch_plic_volume<float>(g32, gd, dim, (float *)part, (float *)out, st);
+            float v = 0.0f;
+            if (hipMemcpyAsync(&v, out, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                rc = fail(CC_ERR_HIP, "calc_volume: %s", hipGetErrorString(hipGetLastError()));
+            float c32 = g32.d[0] * g32.d[1];
+            if (dim == 3) c32 = c32 * g32.d[2];
+            sum = (double)(v * c32);  // vol * prod(grid.d), in the grid's type
+        } else {
+            cc::launch_plic_volume<double>(g64, gd, dim, (double *)part, (double *)out, st);
+            if (hipMemcpyAsync(&sum, out, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                rc = fail(CC_ERR_HIP, "calc_volume: %s", hipGetErrorString(hipGetLastError()));
+            double cv = g64.d[0] * g64.d[1];
+            if (dim == 3) cv = cv * g64.d[2];
+            sum = sum * cv;
+        }
+        if (rc == CC_OK) *volume = sum;
+    }
+    if (prims) (void)hipFree(prims);
+    if (out) (void)hipFree(out);
+    if (part) (void)hipFree(part);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int cc_generate_mesh_plic(cc_cache *c, const cc_grid *grid, double min_frac, void *nodes, int64_t *nnodes,
+                          int64_t *elems, int64_t *nelems) {
+    if (!c || !nnodes || !nelems) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    if (c->method != CC_PLIC) return fail(CC_ERR_UNSUPPORTED, "this is the PLIC surface");
+    if (!(min_frac >= 0.0) || !std::isfinite(min_frac))
+        return fail(CC_ERR_ARGUMENT, "min_tri_area_fraction must be finite and non-negative");
+    int rc = check_grid(c, grid);
+    if (rc) return rc;
+    HIPC(hipSetDevice(c->device));
+    if (!mesh_current(c, grid) || c->mesh_min_frac != min_frac) {
+        rc = c->dtype == CC_FLOAT32 ? plic_surface<float>(c, grid, min_frac) : plic_surface<double>(c, grid, min_frac);
+        if (rc) return rc;
+    }
+    if (nodes && elems) {
+        if (*nnodes < c->mesh_nv || *nelems < c->mesh_nt)
+            return fail(CC_ERR_ARGUMENT, "buffers hold %lld nodes / %lld elements, need %lld / %lld",
+                        (long long)*nnodes, (long long)*nelems, (long long)c->mesh_nv, (long long)c->mesh_nt);
+        if (c->mesh_nv > 0)
+            HIPC(hipMemcpyAsync(nodes, c->mverts, c->dim * elem_size(c->dtype) * c->mesh_nv, hipMemcpyDeviceToHost,
+                                c->stream));
+        if (c->mesh_nt > 0)
+            HIPC(hipMemcpyAsync(elems, c->mtris, c->dim * sizeof(int64_t) * c->mesh_nt, hipMemcpyDeviceToHost,
+                                c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+    }
+    *nnodes = c->mesh_nv;
+    *nelems = c->mesh_nt;
+    return CC_OK;
+}
+
+int cc_extract_surface_plic(int dtype, const cc_grid *grid, int64_t count, const int64_t *cells, const void *normals,
+                            const void *intercepts, double min_frac, void *nodes, int64_t *nnodes, int64_t *elems,
+                            int64_t *nelems) {
+    if (!grid || !nnodes || !nelems || count < 0 || (count > 0 && (!cells || !normals || !intercepts)))
+        return fail(CC_ERR_ARGUMENT, "NULL argument or negative count");
+    if (dtype != CC_FLOAT64 && dtype != CC_FLOAT32)
+        return fail(CC_ERR_UNSUPPORTED, "element type must be CC_FLOAT64 or CC_FLOAT32");
+    if (!(min_frac >= 0.0) || !std::isfinite(min_frac))
+        return fail(CC_ERR_ARGUMENT, "min_tri_area_fraction must be finite and non-negative");
+    const int dim = grid->dim;
+    if (dim != 2 && dim != 3) return fail(CC_ERR_ARGUMENT, "grid.dim must be 2 or 3");
+    int64_t ncell = 1;
+    for (int a = 0; a < dim; ++a) {
+        if (grid->n[a] < 1 || !(grid->d[a] > 0.0) || !std::isfinite(grid->d[a]) || !std::isfinite(grid->x0[a]))
+            return fail(CC_ERR_ARGUMENT, "grid: n >= 1, d positive and finite, x0 finite");
+        ncell *= grid->n[a];
+    }
+    for (int64_t q = 0; q < count; ++q)
+        if (cells[q] < 0 || cells[q] >= ncell)
+            return fail(CC_ERR_ARGUMENT, "cell index %lld is outside the grid", (long long)cells[q]);
+    const size_t es = elem_size(dtype);
+    std::vector<void *> allocs;
+    hipStream_t st = nullptr;
+    auto get = [&](void **p, size_t bytes) -> bool {
+        if (hipMalloc(p, bytes < 8 ? 8 : bytes) != hipSuccess) return false;
+        allocs.push_back(*p);
+        return true;
+    };
+    auto cleanup = [&]() {
+        for (void *p : allocs) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+    };
+    cc::PlicSurfBuffers b = {};
+    void *dn = nullptr, *di = nullptr, *dc = nullptr, *vout = nullptr, *eout = nullptr;
+    const size_t n = (size_t)count;
+    b.part_cap = cc::mesh_scan_blocks(n) > 0 ? cc::mesh_scan_blocks(n) : 1;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        !(get((void **)&b.vcnt, 4 * n) && get((void **)&b.ecnt, 4 * n) && get((void **)&b.vpre, 8 * n) &&
+          get((void **)&b.epre, 8 * n) && get((void **)&b.part, 16 * b.part_cap) && get((void **)&b.totals, 16) &&
+          get(&dn, dim * es * n) && get(&di, es * n) && get(&dc, 8 * n))) {
+        cleanup();
+        return fail(CC_ERR_OUT_OF_MEMORY, "extract_surface_plic: could not allocate its work arrays");
+    }
+    int rc = CC_OK;
+    int64_t tot[2] = {0, 0};
+    auto run = [&](auto tag) -> int {
+        using T = decltype(tag);
+        const cc::GridT<T> gt = grid_t<T>(grid, dim);
+        if (n > 0) {
+            HIPC(hipMemcpyAsync(dn, normals, dim * es * n, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(di, intercepts, es * n, hipMemcpyHostToDevice, st));
+            HIPC(hipMemcpyAsync(dc, cells, 8 * n, hipMemcpyHostToDevice, st));
+        }
+        cc::launch_plic_surface<T>(gt, dim, (const T *)dn, (const T *)di, (const int64_t *)dc, n, min_frac, b, false,
+                                   nullptr, nullptr, st);
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(tot, b.totals, sizeof tot, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        if (!(nodes && elems)) return CC_OK;
+        if (*nnodes < tot[0] || *nelems < tot[1])
+            return fail(CC_ERR_ARGUMENT, "buffers hold %lld nodes / %lld elements, need %lld / %lld",
+                        (long long)*nnodes, (long long)*nelems, (long long)tot[0], (long long)tot[1]);
+        if (!get(&vout, dim * es * (tot[0] > 0 ? tot[0] : 1)) || !get(&eout, dim * 8 * (tot[1] > 0 ? tot[1] : 1)))
+            return fail(CC_ERR_OUT_OF_MEMORY, "extract_surface_plic: could not allocate its output");
+        cc::launch_plic_surface<T>(gt, dim, (const T *)dn, (const T *)di, (const int64_t *)dc, n, min_frac, b, true,
+                                   (T *)vout, (int64_t *)eout, st);
+        HIPC(hipGetLastError());
+        if (tot[0] > 0) HIPC(hipMemcpyAsync(nodes, vout, dim * es * tot[0], hipMemcpyDeviceToHost, st));
+        if (tot[1] > 0) HIPC(hipMemcpyAsync(elems, eout, dim * 8 * tot[1], hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        return CC_OK;
+    };
+    rc = dtype == CC_FLOAT32 ? run(0.0f) : run(0.0);
+    cleanup();
+    if (rc) return rc;
+    *nnodes = tot[0];
+    *nelems = tot[1];
+    return CC_OK;
+}
+
 int cc_generate_mesh_mc(cc_cache *c, const cc_grid *grid, void *nodes, int64_t *nnodes, int64_t *tris,
                         int64_t *ntris) {
     if (!c || !nnodes || !ntris) return fail(CC_ERR_ARGUMENT, "NULL argument");
@@ -1242,6 +1486,8 @@ int cc_mesh_device(cc_cache *c, const void **nodes, const int64_t **tris, int64_
 
 int cc_mesh_cell_indices(cc_cache *c, const cc_grid *grid, int64_t *cells, int on_device) {
     if (!c || !grid || !cells) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    if (c->method != CC_MARCHING_CUBES)
+        return fail(CC_ERR_UNSUPPORTED, "cell_indices is for a marching-cubes surface");
     if (c->mesh_nv < 0) return fail(CC_ERR_ARGUMENT, "no surface has been generated on this cache");
     if (grid->dim != 3) return fail(CC_ERR_DIMENSION_MISMATCH, "cell_indices takes a 3D grid");
     cc::MeshGrid mg;

@@ -63,6 +63,12 @@ template <typename T>
 void launch_plic_faces(const GridT<T> &g, int dim, const Cols<T> &cols, const T *normals,
                        const T *intercepts, const uint8_t *valid, hipStream_t s);
 
+// calc_volume: the sum of every cell's solid fraction (out[0]) through plic_volume_blocks(g, dim)
+// block sums in `part`; the caller multiplies by the cell volume.
+template <typename T> int plic_volume_blocks(const GridT<T> &g, int dim);
+template <typename T>
+void launch_plic_volume(const GridT<T> &g, const Geo &geo, int dim, T *part, T *out, hipStream_t s);
+
 // The fused 3D PLIC pass (fit + resolved faces in one sweep, no read-back).
 template <typename T>
 void launch_plic_fused(const GridT<T> &g, const Geo &geo, const Cols<T> &cols, T *normals,
@@ -161,6 +167,27 @@ struct MeshScan {
     size_t maxblocks;
     int64_t *totals;
 };
+
+// Two exclusive prefix sums in one set of launches (block sums in part, part_cap per sum; the two
+// totals into totals[0..1]).
+void launch_scan_pair(MeshScan::Seg s0, MeshScan::Seg s1, unsigned long long *part, size_t part_cap,
+                      int64_t *totals, hipStream_t s);
+
+// The PLIC surface's per-cell work arrays: vertex and element counts, their prefix sums.
+struct PlicSurfBuffers {
+    uint32_t *vcnt, *ecnt;
+    int64_t *vpre, *epre;
+    unsigned long long *part;
+    size_t part_cap;
+    int64_t *totals;  // [2] vertices, elements
+};
+// Over ncell entries, entry c being plane c in cell cells[c] (0-based linear index; cells == nullptr:
+// cell c).  emit == false: counts every entry's vertices and elements and their prefix sums (totals
+// on the device); emit == true: writes nodes (D T per vertex) and elements (D int64, 1-based).
+template <typename T>
+void launch_plic_surface(const GridT<T> &g, int dim, const T *normals, const T *intercepts, const int64_t *cells,
+                         size_t ncell, double min_frac, const PlicSurfBuffers &b, bool emit, T *nodes, int64_t *elems,
+                         hipStream_t s);
 
 // The grid cell_indices floors into (Float64, as the reference converts it).
 struct MeshGrid {

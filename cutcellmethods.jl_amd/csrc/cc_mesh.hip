@@ -379,14 +379,14 @@ MeshSizes mesh_sizes(const int *n) {
 
 size_t mesh_scan_blocks(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
-namespace {
-void scan2(const MeshBuffers &b, MeshScan::Seg s0, MeshScan::Seg s1, int64_t *totals, hipStream_t s) {
+void launch_scan_pair(MeshScan::Seg s0, MeshScan::Seg s1, unsigned long long *part, size_t part_cap,
+                      int64_t *totals, hipStream_t s) {
     MeshScan sc;
     sc.seg[0] = s0;
     sc.seg[1] = s1;
     sc.seg[2] = {nullptr, nullptr, nullptr, nullptr, 0};
-    sc.part = b.part;
-    sc.maxblocks = b.part_cap;
+    sc.part = part;
+    sc.maxblocks = part_cap;
     sc.totals = totals;
     const size_t nb = mesh_scan_blocks(s0.n > s1.n ? s0.n : s1.n);
     if (nb == 0) {
@@ -396,6 +396,11 @@ void scan2(const MeshBuffers &b, MeshScan::Seg s0, MeshScan::Seg s1, int64_t *to
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb, 2), dim3(SCAN_THREADS), 0, s, sc);
     hipLaunchKernelGGL(k_scan_parts, dim3(1, 2), dim3(SCAN_THREADS), 0, s, sc);
     hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb, 2), dim3(SCAN_THREADS), 0, s, sc);
+}
+
+namespace {
+void scan2(const MeshBuffers &b, MeshScan::Seg s0, MeshScan::Seg s1, int64_t *totals, hipStream_t s) {
+    launch_scan_pair(s0, s1, b.part, b.part_cap, totals, s);
 }
 
 unsigned grid_for(uint32_t n) {

@@ -38,7 +38,71 @@ __global__ __launch_bounds__(256) void k_plic_fused(GridT<T> g, Geo geo, Cols<T>
     plic_fused_item<T>(g, geo, cols, normals, intercepts, valid, w, nx, ny, kz, L);
 }
 
+// calc_volume (src/volume.jl:29-40): every cell's solid fraction from its plane (cell_plane, the fit
+// the PLIC cache stores), summed.  Block b sums a fixed contiguous range of cells, lane t the cells
+// t, t + 256, ... of it, then the block folds its 256 lane sums in a fixed tree; k_sum_parts folds
+// the block sums the same way.  The order is fixed by the launch, so the result is the same on every
+// run; it is a pairwise rather than the reference's left fold, so it agrees with that to rounding.
+constexpr int VOL_THREADS = 256;
+
+template <typename T> __device__ __forceinline__ T block_sum(T v) {
+    __shared__ T s[VOL_THREADS];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = VOL_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + w];
+        __syncthreads();
+    }
+    return s[0];
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(VOL_THREADS) void k_plic_volume(GridT<T> g, Geo geo, size_t ncell, size_t per,
+                                                             T *__restrict__ part) {
+    const size_t c0 = blockIdx.x * per, c1 = c0 + per < ncell ? c0 + per : ncell;
+    const uint32_t n0 = g.n[0], n1 = g.n[1];
+    T acc = T(0);
+    for (size_t c = c0 + threadIdx.x; c < c1; c += VOL_THREADS) {
+        const uint32_t i = (uint32_t)(c % n0), j = (uint32_t)((c / n0) % n1), k = (uint32_t)(c / ((size_t)n0 * n1));
+        acc = acc + plic_plane<T, D>(g, geo, i, j, k).solid;
+    }
+    const T sum = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = sum;
+}
+
+template <typename T>
+__global__ __launch_bounds__(VOL_THREADS) void k_sum_parts(const T *__restrict__ part, int nb, T *__restrict__ out) {
+    T acc = T(0);
+    for (int b = threadIdx.x; b < nb; b += VOL_THREADS) acc = acc + part[b];
+    const T sum = block_sum(acc);
+    if (threadIdx.x == 0) out[0] = sum;
+}
+
 }  // namespace
+
+template <typename T>
+int plic_volume_blocks(const GridT<T> &g, int dim) {
+    size_t ncell = (size_t)g.n[0] * g.n[1] * (dim == 3 ? g.n[2] : 1);
+    size_t nb = (ncell + VOL_THREADS - 1) / VOL_THREADS;
+    return (int)(nb < 2048 ? nb : 2048);
+}
+
+template <typename T>
+void launch_plic_volume(const GridT<T> &g, const Geo &geo, int dim, T *part, T *out, hipStream_t s) {
+    const size_t ncell = (size_t)g.n[0] * g.n[1] * (dim == 3 ? g.n[2] : 1);
+    const int nb = plic_volume_blocks(g, dim);
+    const size_t per = (ncell + nb - 1) / nb;
+    if (dim == 3)
+        hipLaunchKernelGGL((k_plic_volume<T, 3>), dim3(nb), dim3(VOL_THREADS), 0, s, g, geo, ncell, per, part);
+    else
+        hipLaunchKernelGGL((k_plic_volume<T, 2>), dim3(nb), dim3(VOL_THREADS), 0, s, g, geo, ncell, per, part);
+    hipLaunchKernelGGL(k_sum_parts<T>, dim3(1), dim3(VOL_THREADS), 0, s, (const T *)part, nb, out);
+}
+
+template int plic_volume_blocks<double>(const GridT<double> &, int);
+template int plic_volume_blocks<float>(const GridT<float> &, int);
+template void launch_plic_volume<double>(const GridT<double> &, const Geo &, int, double *, double *, hipStream_t);
+template void launch_plic_volume<float>(const GridT<float> &, const Geo &, int, float *, float *, hipStream_t);
 
 template <typename T>
 void launch_plic_fused(const GridT<T> &g, const Geo &geo, const Cols<T> &cols, T *normals,

@@ -95,7 +95,8 @@ __device__ __forceinline__ T face_area(const T *nv, T a, T frac, bool valid, con
 template <typename T> struct Plane {
     T n[3];
     T a;
-    T vf;  // the fluid fraction, 1 - solid
+    T vf;     // the fluid fraction, 1 - solid
+    T solid;  // cell_plane's fraction itself (calc_volume sums it)
     bool ok;
 };
 
@@ -126,6 +127,7 @@ __device__ __forceinline__ Plane<T> plic_plane(const GridT<T> &g, const Geo &geo
     }
     P.a = a;
     P.vf = T(1) - fraction;
+    P.solid = fraction;
     return P;
 }
 

@@ -6,8 +6,9 @@ The names mirror the reference's API (``src/CutCellMethods.jl:3-8``): method tag
 """
 from ._lib import ArgumentError, DimensionMismatch, HIPError, Unsupported
 from .cache import (Cache, MarchingCubesCutCell, MarchingSquaresCutCell, PLICCutCell, Surface,
-                    allocate_cache, cell_indices, face_fractions, generate_mesh, is_cell_open, kinds,
-                    update_cache, update_cache_from_device_phi, volume_fractions)
+                    allocate_cache, calc_volume, cell_indices, extract_surface_plic, face_fractions,
+                    generate_mesh, is_cell_open, kinds, update_cache, update_cache_from_device_phi,
+                    volume_fractions)
 from .geometry import (EmptyGeo, SDFBox, SDFCircle, SDFPlane, SDFSphere, SDFSphereUnion,
                        SDFTorus, random_rotation)
 from .grid import CartesianGrid, CartesianIsoGrid

@@ -31,7 +31,7 @@ EXPORTS = ("cc_cache_create", "cc_cache_destroy", "cc_update", "cc_update_from_p
            "cc_update_async", "cc_synchronize", "cc_column", "cc_copy_to_host", "cc_cut_count",
            "cc_stream", "cc_set_tuning", "cc_get_tuning", "cc_set_profiling", "cc_kernel_times", "cc_generate_mesh_ms",
            "cc_generate_mesh_mc", "cc_mesh_device", "cc_mesh_cell_indices",
-           "cc_leaf_cache_create", "cc_update_leaves", "cc_update_leaves_from_phi", "cc_generate_mesh_leaves",
+           "cc_calc_volume", "cc_generate_mesh_plic", "cc_extract_surface_plic", "cc_leaf_cache_create", "cc_update_leaves", "cc_update_leaves_from_phi", "cc_generate_mesh_leaves",
            "cc_probe_bandwidth", "cc_probe_write", "cc_last_error", "cc_version",
            "cc_mlp_create", "cc_mlp_destroy", "cc_mlp_num_params", "cc_mlp_get_params",
            "cc_mlp_set_params", "cc_mlp_eval", "cc_mlp_loss_grad", "cc_mlp_train",
@@ -101,6 +101,11 @@ def lib():
                                       ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_int)]
         L.cc_generate_mesh_ms.argtypes = [vp, ctypes.POINTER(CCGrid), vp, i64p, vp, i64p]
         L.cc_generate_mesh_mc.argtypes = [vp, ctypes.POINTER(CCGrid), vp, i64p, vp, i64p]
+        L.cc_calc_volume.argtypes = [ctypes.c_int, ctypes.POINTER(CCGeometry), ctypes.POINTER(CCGrid),
+                                     ctypes.POINTER(ctypes.c_double)]
+        L.cc_generate_mesh_plic.argtypes = [vp, ctypes.POINTER(CCGrid), ctypes.c_double, vp, i64p, vp, i64p]
+        L.cc_extract_surface_plic.argtypes = [ctypes.c_int, ctypes.POINTER(CCGrid), ctypes.c_int64, vp, vp, vp,
+                                              ctypes.c_double, vp, i64p, vp, i64p]
         lp = ctypes.POINTER(CCLeaves)
         L.cc_leaf_cache_create.argtypes = [ctypes.c_int, lp, ctypes.POINTER(vp)]
         L.cc_update_leaves.argtypes = [vp, ctypes.c_void_p, lp]

@@ -331,13 +331,16 @@ def _mesh_serial(cache):
     return serial.value
 
 
-def generate_mesh(cache, grid, warn=True):
+def generate_mesh(cache, grid, warn=True, min_tri_area_fraction=1e-4):
     """generate_mesh(cache, grid; warn=true).
 
     Marching squares (``src/marching_squares/surface.jl:59-86``): the contour of the cache's phi as
     ``(nodes (m, 2), lines (k, 2) 1-based)``.  Marching cubes (``src/marching_cubes/surface.jl:82-112``):
     the cache's phi marched on the device into a ``Surface`` of shared vertices and outward-wound
-    triangles, in the cache's element type.  An empty result warns unless ``warn`` is false."""
+    triangles, in the cache's element type.  PLIC (``src/plic/surface.jl:28-40``): every cell's stored
+    plane clipped to its cell on the device, ``(nodes (m, D), elements (k, D) 1-based)``, segments or
+    triangles smaller than ``min_tri_area_fraction`` of a cell edge / face dropped.  An empty result
+    warns unless ``warn`` is false."""
     nn = ctypes.c_int64()
     nl = ctypes.c_int64()
     if isinstance(grid, AdaptiveMesh):  # src/marching_squares/surface.jl:92-110
@@ -355,6 +358,19 @@ def generate_mesh(cache, grid, warn=True):
         return nodes, lines
     _check_same_n(cache, grid)
     g = make_grid(grid)
+    if _method_code(cache.method) == _lib.CC_PLIC:
+        D = grid.dim
+        frac = float(min_tri_area_fraction)
+        check(lib().cc_generate_mesh_plic(cache.handle, ctypes.byref(g), frac, None, ctypes.byref(nn), None,
+                                          ctypes.byref(nl)))
+        nodes = np.empty((nn.value, D), dtype=cache.dtype)
+        elems = np.empty((nl.value, D), dtype=np.int64)
+        check(lib().cc_generate_mesh_plic(cache.handle, ctypes.byref(g), frac, nodes.ctypes.data,
+                                          ctypes.byref(nn), elems.ctypes.data, ctypes.byref(nl)))
+        if warn and nl.value == 0:
+            warnings.warn("generate_mesh: no cut cell anywhere in `grid`, so the surface is empty (the grid "
+                          "misses the body, or lies wholly inside it)")
+        return nodes, elems
     if _method_code(cache.method) == _lib.CC_MARCHING_CUBES:
         check(lib().cc_generate_mesh_mc(cache.handle, ctypes.byref(g), None, ctypes.byref(nn), None,
                                         ctypes.byref(nl)))
@@ -378,6 +394,40 @@ def generate_mesh(cache, grid, warn=True):
     return nodes, lines
 
 
+def extract_surface_plic(normals, intercepts, cell_indices, grid, min_tri_area_fraction=1e-4):
+    """extract_surface_plic(normals, intercepts, cell_indices, grid) (``src/plic/surface.jl:44-182``):
+    the PLIC surface of planes the caller holds -- entry q is the plane (normals[q], intercepts[q]) in
+    the unit-cell frame of cell ``cell_indices[q]`` (0-based linear indices, or (i, j[, k]) rows) --
+    clipped on the device, in the given order.  The element type follows ``normals``."""
+    normals = np.asarray(normals)
+    dt = np.dtype(np.float32) if normals.dtype == np.float32 else np.dtype(np.float64)
+    D = grid.dim
+    nv = np.ascontiguousarray(normals, dtype=dt).reshape(-1, D)
+    ic = np.ascontiguousarray(intercepts, dtype=dt).reshape(-1)
+    cells = np.asarray(cell_indices, dtype=np.int64)
+    if cells.ndim == 2:  # (i, j[, k]) rows
+        lin = cells[:, 0].copy()
+        stride = 1
+        for a in range(1, D):
+            stride *= int(grid.n[a - 1])
+            lin += stride * cells[:, a]
+        cells = lin
+    cells = np.ascontiguousarray(cells.reshape(-1))
+    if not (len(cells) == len(ic) == len(nv)):
+        raise DimensionMismatch("one normal, one intercept and one cell per entry")
+    g = make_grid(grid)
+    code = _lib.CC_FLOAT32 if dt == np.float32 else _lib.CC_FLOAT64
+    nn, ne = ctypes.c_int64(), ctypes.c_int64()
+    args = (code, ctypes.byref(g), len(cells), cells.ctypes.data, nv.ctypes.data, ic.ctypes.data,
+            float(min_tri_area_fraction))
+    check(lib().cc_extract_surface_plic(*args, None, ctypes.byref(nn), None, ctypes.byref(ne)))
+    nodes = np.empty((nn.value, D), dtype=dt)
+    elems = np.empty((ne.value, D), dtype=np.int64)
+    check(lib().cc_extract_surface_plic(*args, nodes.ctypes.data, ctypes.byref(nn), elems.ctypes.data,
+                                        ctypes.byref(ne)))
+    return nodes, elems
+
+
 def cell_indices(mesh, grid):
     """cell_indices(mesh, grid) (``src/marching_cubes/surface.jl:128-141``): for each triangle of a
     marching-cubes surface, the 1-based linear index of the cell of ``grid`` its centroid floors
@@ -392,6 +442,19 @@ def cell_indices(mesh, grid):
     out = np.empty(len(mesh[1]), dtype=np.int64)
     check(lib().cc_mesh_cell_indices(cache.handle, ctypes.byref(make_grid(grid)), out.ctypes.data, 0))
     return out
+
+
+def calc_volume(geo, grid, dtype=np.float64):
+    """calc_volume(geo, grid) (src/volume.jl:29-40): the measure where the body's per-cell planes are
+    solid, over the grid's bounds, computed on the device in `dtype` (the grid's element type)."""
+    dt = np.dtype(dtype)
+    if dt not in (np.dtype(np.float64), np.dtype(np.float32)):
+        raise ArgumentError("dtype must be float64 or float32")
+    out = ctypes.c_double()
+    gg = make_geometry(geo)
+    check(lib().cc_calc_volume(_lib.CC_FLOAT32 if dt == np.float32 else _lib.CC_FLOAT64, ctypes.byref(gg),
+                               ctypes.byref(make_grid(grid)), ctypes.byref(out)))
+    return dt.type(out.value)
 
 
 def volume_fractions(cache):

@@ -188,6 +188,32 @@ int cc_update_leaves_from_phi(cc_cache *cache, const void *phi4, int on_device, 
 int cc_generate_mesh_leaves(cc_cache *cache, const cc_leaves *tree, double *nodes, int64_t *nnodes,
                             int64_t *lines, int64_t *nlines);
 
+/* generate_mesh(cache::PLICCutCellCache, grid; min_tri_area_fraction) -- src/plic/surface.jl:28-267:
+ * every cell's stored plane clipped to its cell on the device, in cell order: 2D segments (two nodes
+ * each, never shared) or 3D triangles (each crossed cell's polygon ordered about its centroid and
+ * fanned from its first vertex).  Elements shorter (2D) or smaller (3D) than min_frac of a cell
+ * edge / face are dropped.  nodes: D values per node in the cache's element type; elems: D int64 per
+ * element, 1-based.  Two-call protocol as cc_generate_mesh_mc; the surface stays in the cache
+ * (cc_mesh_device) until the next update. */
+int cc_generate_mesh_plic(cc_cache *cache, const cc_grid *grid, double min_frac, void *nodes, int64_t *nnodes,
+                          int64_t *elems, int64_t *nelems);
+
+/* extract_surface_plic(normals, intercepts, cell_indices, grid; min_tri_area_fraction) --
+ * src/plic/surface.jl:44-101, :126-182: the same surface for planes the caller holds: entry q is the
+ * plane (normals[D q .. D q + D - 1], intercepts[q]) in the unit-cell frame of cell cells[q] (0-based
+ * linear index).  Host arrays in; nodes / elems as cc_generate_mesh_plic, entries in the given order.
+ * Two-call protocol (counts with nodes/elems NULL; every call recomputes). */
+int cc_extract_surface_plic(int dtype, const cc_grid *grid, int64_t count, const int64_t *cells, const void *normals,
+                            const void *intercepts, double min_frac, void *nodes, int64_t *nnodes, int64_t *elems,
+                            int64_t *nelems);
+
+/* calc_volume(geo, grid) -- src/volume.jl:29-40: the measure of the region where geo's per-cell
+ * planes (cell_plane, the PLIC fit) are solid, over the grid's own bounds, in the element type dtype:
+ * the sum of every cell's solid fraction times the cell volume.  CC_ERR_ARGUMENT for an anisotropic
+ * grid.  The cells' fractions are summed on the device in a fixed order (the same result on every
+ * run); the reference's serial left fold can differ from it in the last bits. */
+int cc_calc_volume(int dtype, const cc_geometry *geo, const cc_grid *grid, double *volume);
+
 /* generate_mesh(cache, grid) for a marching-cubes cache -- src/marching_cubes/surface.jl:82-112:
  * the cache's phi marched on the device (Lewiner cases, shared vertices, wound so each triangle's
  * normal points out of the body) into a mesh the cache keeps until its next update.  Call with

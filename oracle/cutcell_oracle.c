@@ -899,7 +899,32 @@ double orc_volume_fraction(int dim, const double *normal, double a)
     return dim == 2 ? vf2(normal[0], normal[1], a) : vf3(normal, a);
 }
 
-/* cell_plane (src/plic/plic.jl:72-91) + _plic_fit_kernel! (src/plic/cache.jl:96-113). */
+/* cell_plane (src/plic/plic.jl:72-91) for cell (i, j, k): the fitted normal and intercept (zeroed
+ * for a degenerate fit), the solid fraction, and whether the fit is valid. */
+static void cell_plane(const orc_geometry *g, int dim, const double *x0, const double *d, int64_t i,
+                       int64_t j, int64_t k, double nrm[3], double *a_out, double *fraction, int *valid)
+{
+    double x[3] = {x0[0] + ((double)i + 0.5) * d[0], x0[1] + ((double)j + 0.5) * d[1],
+                   dim == 3 ? x0[2] + ((double)k + 0.5) * d[2] : 0.0};
+    nrm[0] = nrm[1] = nrm[2] = 0.0;
+    double dist = orc_get_sdf(g, dim, x, nrm);
+    double sum = nrm[0] + nrm[1];
+    if (dim == 3) sum = sum + nrm[2];
+    double a = 0.5 * sum - dist / d[0];
+    int zero = nrm[0] == 0.0 && nrm[1] == 0.0 && (dim == 2 || nrm[2] == 0.0);
+    if (isnan(a) || zero) {
+        for (int q = 0; q < dim; ++q) nrm[q] = 0.0;
+        a = 0.0;
+        *fraction = dist <= 0.0 ? 1.0 : 0.0;
+        *valid = 0;
+    } else {
+        *fraction = orc_volume_fraction(dim, nrm, a);
+        *valid = 1;
+    }
+    *a_out = a;
+}
+
+/* _plic_fit_kernel! (src/plic/cache.jl:96-113). */
 void orc_plic_fit(const orc_geometry *g, int dim, const double *x0, const double *d,
                   const int64_t *n, double *vf, int8_t *kind, double *normals,
                   double *intercepts, uint8_t *is_valid)
@@ -910,25 +935,9 @@ void orc_plic_fit(const orc_geometry *g, int dim, const double *x0, const double
         for (int64_t j = 0; j < n[1]; ++j)
             for (int64_t i = 0; i < n[0]; ++i) {
                 int64_t c = i + n[0] * (j + n[1] * k);
-                double x[3] = {x0[0] + ((double)i + 0.5) * d[0], x0[1] + ((double)j + 0.5) * d[1],
-                               dim == 3 ? x0[2] + ((double)k + 0.5) * d[2] : 0.0};
-                double nrm[3] = {0.0, 0.0, 0.0};
-                double dist = orc_get_sdf(g, dim, x, nrm);
-                double sum = nrm[0] + nrm[1];
-                if (dim == 3) sum = sum + nrm[2];
-                double a = 0.5 * sum - dist / d[0];
-                int zero = nrm[0] == 0.0 && nrm[1] == 0.0 && (dim == 2 || nrm[2] == 0.0);
-                double fraction;
+                double nrm[3], a, fraction;
                 int valid;
-                if (isnan(a) || zero) {
-                    for (int q = 0; q < dim; ++q) nrm[q] = 0.0;
-                    a = 0.0;
-                    fraction = dist <= 0.0 ? 1.0 : 0.0;
-                    valid = 0;
-                } else {
-                    fraction = orc_volume_fraction(dim, nrm, a);
-                    valid = 1;
-                }
+                cell_plane(g, dim, x0, d, i, j, k, nrm, &a, &fraction, &valid);
                 double frac = 1.0 - fraction;
                 for (int q = 0; q < dim; ++q) normals[dim * c + q] = nrm[q];
                 intercepts[c] = a;
@@ -936,6 +945,25 @@ void orc_plic_fit(const orc_geometry *g, int dim, const double *x0, const double
                 vf[c] = frac;
                 kind[c] = frac == 0.0 ? CELL_INSIDE : (frac == 1.0 ? CELL_OUTSIDE : CELL_CUT);
             }
+}
+
+/* calc_volume (src/volume.jl:29-40): the solid fractions of every cell's plane summed in
+ * CartesianIndices order (a left fold, one thread), times prod(grid.d). */
+double orc_calc_volume(const orc_geometry *g, int dim, const double *x0, const double *d, const int64_t *n)
+{
+    int64_t n2 = dim == 3 ? n[2] : 1;
+    double vol = 0.0;
+    for (int64_t k = 0; k < n2; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t i = 0; i < n[0]; ++i) {
+                double nrm[3], a, fraction;
+                int valid;
+                cell_plane(g, dim, x0, d, i, j, k, nrm, &a, &fraction, &valid);
+                vol += fraction;
+            }
+    double cv = d[0] * d[1];
+    if (dim == 3) cv = cv * d[2];
+    return vol * cv;
 }
 
 /* _cell_wet and _face_area (src/plic/apertures.jl:73-125).  k is the 0-based direction. */
@@ -1123,5 +1151,169 @@ int orc_mc_mesh(const double *phi, const double *x0, const double *d, const int6
     free(vid);
     *nverts = nv;
     *ntris = nt;
+    return 0;
+}
+
+/* =========================================================================================== */
+/* PLIC surface shards: generate_mesh(cache::PLICCutCellCache, grid), src/plic/surface.jl:28-267 */
+/* =========================================================================================== */
+/* Every cell's stored plane n.xi = a clipped to its unit cell xi in [0,1]^D (CartesianMeshes'
+ * cell_plane_clip, un-vendored, restated from the geometry): the corners lying on the plane, then
+ * the crossings of edges whose ends are strictly on opposite sides, x = x_lower + xi * d with
+ * x_lower = x0 + i d (local_to_global).  2D: the two ends ordered so the segment's right-hand normal
+ * points along n (out of the body), a plane that misses the cell or touches one corner giving a
+ * zero-length segment; segments shorter than min_frac * min(d) are dropped (surface.jl:82-88).
+ * 3D: polygons of 3-6 vertices ordered by angle about their centroid in the basis
+ * u = normalize(ref x n), v = n x u (ref = e_z unless |n_z| >= 0.9, then e_x) (surface.jl:219-266),
+ * fan-triangulated from their first vertex, triangles with |cross|^2 < (2 min_frac min(d)^2)^2
+ * dropped, every vertex of a polygon kept (surface.jl:150-182).  nodes == NULL counts. */
+static const int SQ_CORNER[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+
+static int plic_clip2(const double *nv, double a, double xi[2][2])
+{
+    double f[4];
+    int cnt = 0;
+    if (nv[0] == 0.0 && nv[1] == 0.0) return 0;
+    for (int k = 0; k < 4; ++k) f[k] = nv[0] * SQ_CORNER[k][0] + nv[1] * SQ_CORNER[k][1] - a;
+    for (int k = 0; k < 4 && cnt < 2; ++k) {
+        if (f[k] == 0.0) {
+            xi[cnt][0] = SQ_CORNER[k][0];
+            xi[cnt][1] = SQ_CORNER[k][1];
+            cnt++;
+        }
+    }
+    for (int k = 0; k < 4 && cnt < 2; ++k) {
+        int k2 = (k + 1) & 3;
+        if ((f[k] < 0.0 && f[k2] > 0.0) || (f[k] > 0.0 && f[k2] < 0.0)) {
+            double t = f[k] / (f[k] - f[k2]);
+            for (int x = 0; x < 2; ++x)
+                xi[cnt][x] = SQ_CORNER[k][x] + t * (double)(SQ_CORNER[k2][x] - SQ_CORNER[k][x]);
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
+static int plic_clip3(const double *nv, double a, double xi[6][3])
+{
+    if (nv[0] == 0.0 && nv[1] == 0.0 && nv[2] == 0.0) return 0;
+    double f[8];
+    int cnt = 0;
+    for (int c = 0; c < 8; ++c)
+        f[c] = (nv[0] * MC_BITS[c][0] + nv[1] * MC_BITS[c][1]) + nv[2] * MC_BITS[c][2] - a;
+    for (int c = 0; c < 8 && cnt < 6; ++c)
+        if (f[c] == 0.0) {
+            for (int x = 0; x < 3; ++x) xi[cnt][x] = MC_BITS[c][x];
+            cnt++;
+        }
+    for (int e = 0; e < 12 && cnt < 6; ++e) {
+        int p = MC_EDGE[e][0], q = MC_EDGE[e][1];
+        if ((f[p] < 0.0 && f[q] > 0.0) || (f[p] > 0.0 && f[q] < 0.0)) {
+            double t = f[p] / (f[p] - f[q]);
+            for (int x = 0; x < 3; ++x)
+                xi[cnt][x] = MC_BITS[p][x] + t * (double)(MC_BITS[q][x] - MC_BITS[p][x]);
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
+int orc_plic_surface(int dim, const double *x0, const double *d, const int64_t *n, const double *normals,
+                     const double *intercepts, double min_frac, int64_t *nverts, int64_t *nelems,
+                     double *nodes, int64_t *elems)
+{
+    int64_t n2 = dim == 3 ? n[2] : 1, nv = 0, ne = 0;
+    double dmin = d[0] < d[1] ? d[0] : d[1];
+    if (dim == 3 && d[2] < dmin) dmin = d[2];
+    for (int64_t k = 0; k < n2; ++k)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t i = 0; i < n[0]; ++i) {
+                int64_t c = i + n[0] * (j + n[1] * k);
+                double lo[3] = {x0[0] + (double)i * d[0], x0[1] + (double)j * d[1],
+                                dim == 3 ? x0[2] + (double)k * d[2] : 0.0};
+                if (dim == 2) {
+                    const double *nvv = normals + 2 * c;
+                    double xi[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+                    int cnt = plic_clip2(nvv, intercepts[c], xi);
+                    if (cnt == 1) { xi[1][0] = xi[0][0]; xi[1][1] = xi[0][1]; }
+                    if (cnt == 2) {
+                        double v0 = xi[1][0] - xi[0][0], v1 = xi[1][1] - xi[0][1];
+                        if (v1 * nvv[0] - v0 * nvv[1] < 0.0) {
+                            double t0 = xi[0][0], t1 = xi[0][1];
+                            xi[0][0] = xi[1][0]; xi[0][1] = xi[1][1];
+                            xi[1][0] = t0; xi[1][1] = t1;
+                        }
+                    }
+                    double p[2][2];
+                    for (int s = 0; s < 2; ++s)
+                        for (int x = 0; x < 2; ++x) p[s][x] = lo[x] + xi[s][x] * d[x];
+                    double l0 = p[1][0] - p[0][0], l1 = p[1][1] - p[0][1];
+                    if (sqrt(l0 * l0 + l1 * l1) < min_frac * dmin) continue;
+                    if (nodes)
+                        for (int s = 0; s < 2; ++s)
+                            for (int x = 0; x < 2; ++x) nodes[2 * (nv + s) + x] = p[s][x];
+                    if (elems) { elems[2 * ne] = nv + 1; elems[2 * ne + 1] = nv + 2; }
+                    nv += 2;
+                    ne += 1;
+                } else {
+                    const double *nvv = normals + 3 * c;
+                    double xi[6][3], p[6][3], ang[6];
+                    int cnt = plic_clip3(nvv, intercepts[c], xi);
+                    if (cnt < 3) continue;
+                    double cen[3] = {0.0, 0.0, 0.0};
+                    for (int m = 0; m < cnt; ++m)
+                        for (int x = 0; x < 3; ++x) {
+                            p[m][x] = lo[x] + xi[m][x] * d[x];
+                            cen[x] += p[m][x];
+                        }
+                    for (int x = 0; x < 3; ++x) cen[x] = cen[x] / (double)cnt;
+                    double ref[3] = {0.0, 0.0, 0.0};
+                    if (fabs(nvv[2]) < 0.9) ref[2] = 1.0; else ref[0] = 1.0;
+                    double u[3] = {ref[1] * nvv[2] - ref[2] * nvv[1], ref[2] * nvv[0] - ref[0] * nvv[2],
+                                   ref[0] * nvv[1] - ref[1] * nvv[0]};
+                    double un = sqrt((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+                    for (int x = 0; x < 3; ++x) u[x] = u[x] / un;
+                    double v[3] = {nvv[1] * u[2] - nvv[2] * u[1], nvv[2] * u[0] - nvv[0] * u[2],
+                                   nvv[0] * u[1] - nvv[1] * u[0]};
+                    for (int m = 0; m < cnt; ++m) {
+                        double dd[3] = {p[m][0] - cen[0], p[m][1] - cen[1], p[m][2] - cen[2]};
+                        ang[m] = atan2((dd[0] * v[0] + dd[1] * v[1]) + dd[2] * v[2],
+                                       (dd[0] * u[0] + dd[1] * u[1]) + dd[2] * u[2]);
+                    }
+                    for (int m = 1; m < cnt; ++m) { /* insertion sort by angle */
+                        double kp[3] = {p[m][0], p[m][1], p[m][2]}, ka = ang[m];
+                        int q = m - 1;
+                        while (q >= 0 && ang[q] > ka) {
+                            for (int x = 0; x < 3; ++x) p[q + 1][x] = p[q][x];
+                            ang[q + 1] = ang[q];
+                            q--;
+                        }
+                        for (int x = 0; x < 3; ++x) p[q + 1][x] = kp[x];
+                        ang[q + 1] = ka;
+                    }
+                    double min_area = min_frac * dmin * dmin, min_cross_sq = (2.0 * min_area) * (2.0 * min_area);
+                    if (nodes)
+                        for (int m = 0; m < cnt; ++m)
+                            for (int x = 0; x < 3; ++x) nodes[3 * (nv + m) + x] = p[m][x];
+                    for (int m = 1; m < cnt - 1; ++m) {
+                        double e1[3] = {p[m][0] - p[0][0], p[m][1] - p[0][1], p[m][2] - p[0][2]};
+                        double e2[3] = {p[m + 1][0] - p[0][0], p[m + 1][1] - p[0][1], p[m + 1][2] - p[0][2]};
+                        double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                        e1[0] * e2[1] - e1[1] * e2[0]};
+                        double s2 = (cr[0] * cr[0] + cr[1] * cr[1]) + cr[2] * cr[2];
+                        if (s2 >= min_cross_sq) {
+                            if (elems) {
+                                elems[3 * ne] = nv + 1;
+                                elems[3 * ne + 1] = nv + m + 1;
+                                elems[3 * ne + 2] = nv + m + 2;
+                            }
+                            ne += 1;
+                        }
+                    }
+                    nv += cnt;
+                }
+            }
+    *nverts = nv;
+    *nelems = ne;
     return 0;
 }

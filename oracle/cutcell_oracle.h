@@ -106,6 +106,16 @@ void orc_plic_faces(int dim, const double *d, const int64_t *n, const double *no
                     const double *intercepts, const uint8_t *is_valid, const double *vf,
                     double *face_fraction);
 
+/* generate_mesh(cache::PLICCutCellCache, grid) (src/plic/surface.jl:28-267): every cell's plane
+ * clipped to its cell -- 2D segments (2 nodes each) or 3D fan triangles of the ordered polygon --
+ * in cell order; elements are 1-based node ids, D per element.  nodes == NULL counts. */
+int orc_plic_surface(int dim, const double *x0, const double *d, const int64_t *n, const double *normals,
+                     const double *intercepts, double min_frac, int64_t *nverts, int64_t *nelems,
+                     double *nodes, int64_t *elems);
+
+/* calc_volume(geo, grid) (src/volume.jl:29-40): the measure where the body's planes are solid. */
+double orc_calc_volume(const orc_geometry *g, int dim, const double *x0, const double *d, const int64_t *n);
+
 /* ---- whole-grid drivers (OpenMP over k) ----------------------------------------------------- */
 int  orc_num_threads(void);
 

@@ -73,6 +73,10 @@ def lib():
         L.orc_get_sdf.argtypes = [vp, ctypes.c_int, vp, vp]
         L.orc_get_sdf.restype = ctypes.c_double
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_calc_volume.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+        L.orc_plic_surface.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, vp, vp]
+        L.orc_plic_surface.restype = ctypes.c_int
+        L.orc_calc_volume.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -267,3 +271,22 @@ def plic(geo, dim, x0, d, n):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def calc_volume(geo, dim, x0, d, n):
+    """calc_volume (src/volume.jl:29-40), summed in cell order."""
+    g = make_geometry(geo)
+    return lib().orc_calc_volume(ctypes.byref(g), dim, _p(_f64(x0, 3)), _p(_f64(d, 3)), _p(_i64(n)))
+
+
+def plic_surface(dim, x0, d, n, normals, intercepts, min_frac=1e-4):
+    """The PLIC surface shards (src/plic/surface.jl): (nodes (m, dim), elements (k, dim) 1-based)."""
+    normals = _f64(normals)
+    intercepts = _f64(intercepts)
+    args = (dim, _p(_f64(x0, 3)), _p(_f64(d, 3)), _p(_i64(n)), _p(normals), _p(intercepts), float(min_frac))
+    nv, ne = ctypes.c_int64(0), ctypes.c_int64(0)
+    lib().orc_plic_surface(*args, ctypes.byref(nv), ctypes.byref(ne), None, None)
+    nodes = np.zeros((nv.value, dim))
+    elems = np.zeros((ne.value, dim), np.int64)
+    lib().orc_plic_surface(*args, ctypes.byref(nv), ctypes.byref(ne), _p(nodes), _p(elems))
+    return nodes, elems
