@@ -42,7 +42,8 @@ def parse():
                          "autotuner picks among the fused z-march shapes on this GPU before warm-up)")
     ap.add_argument("--quick", action="store_true",
                     help="only the headline workload (skip the secondary workloads)")
-    ap.add_argument("--only", default=None, choices=["csg", "batched", "secondary", "surface", "quadtree"],
+    ap.add_argument("--only", default=None,
+                    choices=["csg", "batched", "secondary", "surface", "quadtree", "plic_surface"],
                     help="run one secondary workload alone and print its JSON")
     ap.add_argument("--train-epochs", type=int, default=24,
                     help="surrogate training epochs of the composite-body workload")
@@ -233,6 +234,56 @@ def quadtree(seed, level=12, reps=20, cpu=True):
         out["cpu_baseline"] = {"value": round(t_cpu * 1e3, 2), "unit": "ms per update", "cores": orc.num_threads(),
                                "kind": "port", "sample": "the oracle's leaf pass over the same tree (OpenMP)"}
     del cache
+    return out
+
+
+def plic_surface(seed, n=384, reps=5, cpu=True):
+    """generate_mesh on the PLIC cache (the torus R=0.5 r=0.15 with a seeded tilt at 384^3, every cell's
+    plane clipped on the device) and calc_volume of the same body on the same grid, each beside the
+    oracle's serial restatement."""
+    from cutcell import (CartesianIsoGrid, PLICCutCell, SDFTorus, allocate_cache, calc_volume,
+                         generate_mesh, random_rotation, update_cache)
+    rng = np.random.default_rng(seed)
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=n)
+    tor = SDFTorus((0.0, 0.0, 0.0), 0.5, 0.15, random_rotation(rng))
+    c = allocate_cache(g, PLICCutCell)
+    c.set_profiling(True)
+    dev, wall = [], []
+    for r in range(reps + 1):
+        update_cache(c, tor, g)  # a new update: the next call clips again
+        t = time.perf_counter()
+        nodes, el = generate_mesh(c, g)
+        t1 = time.perf_counter()
+        kt = dict(c.kernel_times())
+        if r:
+            dev.append(kt["plic_surface_count"] + kt["plic_surface_emit"])
+            wall.append((t1 - t) * 1e3)
+    vt = []
+    for r in range(reps + 1):
+        t = time.perf_counter()
+        vol = calc_volume(tor, g)
+        if r:
+            vt.append((time.perf_counter() - t) * 1e3)
+    out = {"workload": f"PLICCutCell surface and calc_volume, torus at {n}^3", "cells": g.ncells,
+           "triangles": int(len(el)), "vertices": int(len(nodes)),
+           "surface_device_ms": round(float(np.median(dev)), 4),
+           "surface_call_ms_with_host_copy": round(float(np.median(wall)), 3),
+           "calc_volume_ms": round(float(np.median(vt)), 3), "volume": float(vol)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        t = time.perf_counter()
+        v_ref = orc.calc_volume(tor, 3, g.x0, g.d, g.n)
+        t_v = time.perf_counter() - t
+        nrm, ic = c.normals, c.intercepts
+        t = time.perf_counter()
+        orc.plic_surface(3, g.x0, g.d, g.n, nrm, ic)
+        t_s = (time.perf_counter() - t) / 2  # it clips twice, counting then filling
+        out["cpu_baseline"] = {"calc_volume_ms": round(t_v * 1e3, 1), "surface_ms": round(t_s * 1e3, 1),
+                               "cores": 1, "kind": "port",
+                               "sample": "the oracle's serial calc_volume and surface over the same grid",
+                               "volume_rel_diff": abs(float(vol) - v_ref) / v_ref}
+    del c
     return out
 
 
@@ -445,7 +496,8 @@ def main():
               "batched": lambda: batched(a.seed),
               "secondary": lambda: secondary(a.seed),
               "surface": lambda: surface(a.seed, cpu=not a.no_cpu),
-              "quadtree": lambda: quadtree(a.seed, cpu=not a.no_cpu)}[a.only]
+              "quadtree": lambda: quadtree(a.seed, cpu=not a.no_cpu),
+              "plic_surface": lambda: plic_surface(a.seed, cpu=not a.no_cpu)}[a.only]
         print(json.dumps({a.only: fn()}))
         return
     grid, geo, centre = workload(a.n, a.seed)
@@ -543,6 +595,7 @@ def main():
         out.update(secondary(a.seed))
         out["mc_surface_512"] = surface(a.seed, cpu=not a.no_cpu)
         out["ms_quadtree"] = quadtree(a.seed, cpu=not a.no_cpu)
+        out["plic_surface_384"] = plic_surface(a.seed, cpu=not a.no_cpu)
         out["batched_64_bodies"] = batched(a.seed)
         out["csg_512_spheres"] = csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples)
     print(json.dumps(out))

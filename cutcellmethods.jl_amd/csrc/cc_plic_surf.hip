@@ -183,6 +183,7 @@ __global__ __launch_bounds__(PS_THREADS) void k_plic_surf(GridT<T> g, const T *_
                                                           int64_t *__restrict__ elems) {
     const size_t c = (size_t)blockIdx.x * PS_THREADS + threadIdx.x;
     if (c >= ncell) return;
+    if (emit && vcnt[c] == 0) return;  // nothing of this entry is drawn: skip its plane
     const uint32_t n0 = g.n[0], n1 = g.n[1];
     const size_t lin = cells ? (size_t)cells[c] : c;
     const uint32_t i = (uint32_t)(lin % n0), j = (uint32_t)((lin / n0) % n1), k = (uint32_t)(lin / ((size_t)n0 * n1));
