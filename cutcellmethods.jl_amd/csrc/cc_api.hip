@@ -133,6 +133,9 @@ struct cc_cache {
     double mesh_min_frac = -1.0;  // the PLIC surface's element floor
     cc::PlicSurfBuffers ps = {};
     std::vector<void *> ps_allocs;
+    // diagnostics: counters, block sums, totals (device), and a pinned copy of the results
+    void *dg = nullptr;
+    void *dg_host = nullptr;
     // marching squares laid out on a quadtree's leaves (ncell = leaves, phi = 4 values per leaf)
     bool leaves = false;
     int32_t *leaf_level = nullptr, *leaf_coord = nullptr;  // device copies of a host leaf list
@@ -679,7 +682,73 @@ template <typename T> int plic_surface(cc_cache *c, const cc_grid *g, double min
     return CC_OK;
 }
 
+// The diagnostics pass over cache c's columns, viewed with grid g (a pseudo-grid of n x 1 x 1 for a
+// leaf cache, whose report is all that is read back).
+template <typename T> int run_diagnostics(cc_cache *c, const cc::GridT<T> &g, int dim, unsigned long long *counts,
+                                          T *sums) {
+    const size_t most = (size_t)cc::diag_max_blocks();  // the pass's largest block count
+    const size_t bytes = 9 * 8 + 4 * 8 + 4 * 8 * most;
+    if (!c->dg) {
+        if (hipMalloc(&c->dg, bytes) != hipSuccess) { c->dg = nullptr; return fail(CC_ERR_OUT_OF_MEMORY, "diagnostics"); }
+        if (hipHostMalloc(&c->dg_host, 9 * 8 + 4 * 8, hipHostMallocDefault) != hipSuccess) {
+            c->dg_host = nullptr;
+            return fail(CC_ERR_OUT_OF_MEMORY, "diagnostics");
+        }
+    }
+    unsigned long long *dc = (unsigned long long *)c->dg;
+    T *dsum = (T *)((char *)c->dg + 9 * 8), *dpart = dsum + 4;
+    if (!cc::launch_diagnostics<T>(cols_t<T>(c), g, dim, dc, dpart, most, dsum, c->stream))
+        return fail(CC_ERR_ARGUMENT, "diagnostics: the grid needs more blocks than the partial sums hold");
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(c->dg_host, c->dg, 9 * 8 + 4 * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    std::memcpy(counts, c->dg_host, 9 * 8);
+    std::memcpy(sums, (char *)c->dg_host + 9 * 8, 4 * sizeof(T));
+    return CC_OK;
+}
+
+int diagnostics_impl(cc_cache *c, const cc_grid *grid, cc_diagnostics *out) {
+    unsigned long long cnt[9];
+    double sums[4];
+    int rc;
+    const int dim = c->leaves ? 2 : c->dim;
+    cc_grid pg{};
+    if (c->leaves) {  // the report reads kind, ambiguous and volume fraction only
+        pg.dim = 2;
+        pg.n[0] = c->ncell;
+        pg.n[1] = pg.n[2] = 1;
+        pg.d[0] = pg.d[1] = pg.d[2] = 1.0;
+        grid = &pg;
+    }
+    if (c->dtype == CC_FLOAT32) {
+        float fs[4];
+        rc = run_diagnostics<float>(c, grid_t<float>(grid, dim), dim, cnt, fs);
+        for (int q = 0; q < 4; ++q) sums[q] = fs[q];
+    } else {
+        rc = run_diagnostics<double>(c, grid_t<double>(grid, dim), dim, cnt, sums);
+    }
+    if (rc) return rc;
+    double minvf, resmax;
+    std::memcpy(&minvf, &cnt[7], 8);
+    std::memcpy(&resmax, &cnt[8], 8);
+    out->report.n = c->ncell;
+    out->report.cut = (int64_t)cnt[0];
+    out->report.inside = (int64_t)cnt[1];
+    out->report.outside = (int64_t)cnt[2];
+    out->report.ambiguous = (int64_t)cnt[3];
+    out->report.min_volume_fraction = minvf;
+    out->closure_nonzero = (int64_t)cnt[4];
+    out->closure_max = resmax;
+    out->face_mismatches = (int64_t)cnt[5];
+    out->centroid_mismatches = (int64_t)cnt[6];
+    out->interface_area = sums[0];
+    for (int q = 0; q < 3; ++q) out->interface_normal_area[q] = q < dim ? sums[q + 1] : 0.0;
+    return CC_OK;
+}
+
 void free_all(cc_cache *c) {
+    if (c->dg) (void)hipFree(c->dg);
+    if (c->dg_host) (void)hipHostFree(c->dg_host);
     for (void *p : c->ps_allocs) (void)hipFree(p);
     c->ps_allocs.clear();
     for (void *p : c->mb_allocs) (void)hipFree(p);
@@ -835,6 +904,9 @@ int cc_internal_body(cc_cache *c, int *method, int *dtype, int64_t *n, void **co
     *kz = c->tune.kz;
     return CC_OK;
 }
+
+// Whether the cache is laid out on a quadtree's leaves.
+int cc_internal_leaves(const cc_cache *c) { return c && c->leaves ? 1 : 0; }
 
 // A body of a batched step was updated: a surface marched before describes an older field.
 void cc_internal_touch(cc_cache *c) {
@@ -1444,6 +1516,30 @@ int cc_extract_surface_plic(int dtype, const cc_grid *grid, int64_t count, const
     *nnodes = tot[0];
     *nelems = tot[1];
     return CC_OK;
+}
+
+int cc_cut_cell_report(cc_cache *c, cc_report *out) {
+    if (!c || !out) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    HIPC(hipSetDevice(c->device));
+    cc_diagnostics d{};
+    cc_grid g{};
+    g.dim = c->dim;
+    for (int a = 0; a < 3; ++a) {
+        g.n[a] = a < c->dim ? c->n[a] : 1;
+        g.d[a] = 1.0;
+    }
+    const int rc = diagnostics_impl(c, &g, &d);
+    if (rc) return rc;
+    *out = d.report;
+    return CC_OK;
+}
+
+int cc_cache_diagnostics(cc_cache *c, const cc_grid *grid, cc_diagnostics *out) {
+    if (!c || !out) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    const int rc = check_grid(c, grid);  // also refuses a leaf cache: its faces need the tree
+    if (rc) return rc;
+    HIPC(hipSetDevice(c->device));
+    return diagnostics_impl(c, grid, out);
 }
 
 int cc_generate_mesh_mc(cc_cache *c, const cc_grid *grid, void *nodes, int64_t *nnodes, int64_t *tris,

@@ -189,6 +189,18 @@ void launch_plic_surface(const GridT<T> &g, int dim, const T *normals, const T *
                          size_t ncell, double min_frac, const PlicSurfBuffers &b, bool emit, T *nodes, int64_t *elems,
                          hipStream_t s);
 
+// Diagnostics over a cache's cells: counts[0..6] = cut, inside, outside, ambiguous, cells with a
+// nonzero closure residual, shared faces whose fractions differ, shared faces whose centroids differ;
+// counts[7] = the bits of the smallest positive cut volume fraction (1.0 if none); counts[8] = the bits
+// of the largest |closure residual| component.  sums[0..3] = total interface area and vector area.
+// part holds part_blocks x 4 values; a grid needing more than part_blocks blocks (diag_blocks) is
+// refused, with nothing launched (false).  No grid needs more than diag_max_blocks().
+int diag_max_blocks();
+int diag_blocks(size_t ncell);
+template <typename T>
+bool launch_diagnostics(const Cols<T> &cols, const GridT<T> &g, int dim, unsigned long long *counts, T *part,
+                        size_t part_blocks, T *sums, hipStream_t s);
+
 // The grid cell_indices floors into (Float64, as the reference converts it).
 struct MeshGrid {
     double x0[3], d[3];

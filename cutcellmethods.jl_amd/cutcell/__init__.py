@@ -6,16 +6,17 @@ The names mirror the reference's API (``src/CutCellMethods.jl:3-8``): method tag
 """
 from ._lib import ArgumentError, DimensionMismatch, HIPError, Unsupported
 from .cache import (Cache, MarchingCubesCutCell, MarchingSquaresCutCell, PLICCutCell, Surface,
-                    allocate_cache, calc_volume, cell_indices, extract_surface_plic, face_fractions,
-                    generate_mesh, is_cell_open, kinds, update_cache, update_cache_from_device_phi,
-                    volume_fractions)
+                    allocate_cache, calc_volume, cell_indices, cut_cell_report, diagnostics,
+                    extract_surface_plic, face_fractions, generate_mesh, is_cell_open, kinds,
+                    update_cache, update_cache_from_device_phi, volume_fractions, write_cache_vtk,
+                    write_npy)
 from .geometry import (EmptyGeo, SDFBox, SDFCircle, SDFPlane, SDFSphere, SDFSphereUnion,
                        SDFTorus, random_rotation)
 from .grid import CartesianGrid, CartesianIsoGrid
+from .readers import (CELL_CUT, CELL_INSIDE, CELL_OUTSIDE, closure_residual, face_area_of,
+                      face_centroid, full_face_area, interface_area, interface_normal,
+                      interface_normal_area)
 from .surrogate import Surrogate
 from .tree import AdaptiveMesh
-from .readers import (CELL_CUT, CELL_INSIDE, CELL_OUTSIDE, closure_residual, cut_cell_report,
-                      face_area_of, face_centroid, full_face_area, interface_area,
-                      interface_normal, interface_normal_area)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -31,7 +31,7 @@ EXPORTS = ("cc_cache_create", "cc_cache_destroy", "cc_update", "cc_update_from_p
            "cc_update_async", "cc_synchronize", "cc_column", "cc_copy_to_host", "cc_cut_count",
            "cc_stream", "cc_set_tuning", "cc_get_tuning", "cc_set_profiling", "cc_kernel_times", "cc_generate_mesh_ms",
            "cc_generate_mesh_mc", "cc_mesh_device", "cc_mesh_cell_indices",
-           "cc_calc_volume", "cc_generate_mesh_plic", "cc_extract_surface_plic", "cc_leaf_cache_create", "cc_update_leaves", "cc_update_leaves_from_phi", "cc_generate_mesh_leaves",
+           "cc_calc_volume", "cc_cut_cell_report", "cc_cache_diagnostics", "cc_write_cache_vtk", "cc_write_npy", "cc_generate_mesh_plic", "cc_extract_surface_plic", "cc_leaf_cache_create", "cc_update_leaves", "cc_update_leaves_from_phi", "cc_generate_mesh_leaves",
            "cc_probe_bandwidth", "cc_probe_write", "cc_last_error", "cc_version",
            "cc_mlp_create", "cc_mlp_destroy", "cc_mlp_num_params", "cc_mlp_get_params",
            "cc_mlp_set_params", "cc_mlp_eval", "cc_mlp_loss_grad", "cc_mlp_train",
@@ -53,6 +53,18 @@ class CCLeaves(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("level", ctypes.c_void_p), ("coord", ctypes.c_void_p),
                 ("x0", ctypes.c_double * 2), ("d", ctypes.c_double * 2), ("on_device", ctypes.c_int32),
                 ("version", ctypes.c_int32)]
+
+
+class CCReport(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("cut", ctypes.c_int64), ("inside", ctypes.c_int64),
+                ("outside", ctypes.c_int64), ("ambiguous", ctypes.c_int64),
+                ("min_volume_fraction", ctypes.c_double)]
+
+
+class CCDiagnostics(ctypes.Structure):
+    _fields_ = [("report", CCReport), ("closure_nonzero", ctypes.c_int64), ("closure_max", ctypes.c_double),
+                ("face_mismatches", ctypes.c_int64), ("centroid_mismatches", ctypes.c_int64),
+                ("interface_area", ctypes.c_double), ("interface_normal_area", ctypes.c_double * 3)]
 
 
 class DimensionMismatch(ValueError):
@@ -106,6 +118,10 @@ def lib():
         L.cc_generate_mesh_plic.argtypes = [vp, ctypes.POINTER(CCGrid), ctypes.c_double, vp, i64p, vp, i64p]
         L.cc_extract_surface_plic.argtypes = [ctypes.c_int, ctypes.POINTER(CCGrid), ctypes.c_int64, vp, vp, vp,
                                               ctypes.c_double, vp, i64p, vp, i64p]
+        L.cc_cut_cell_report.argtypes = [vp, ctypes.POINTER(CCReport)]
+        L.cc_write_cache_vtk.argtypes = [vp, ctypes.POINTER(CCGrid), ctypes.c_char_p, ctypes.c_int]
+        L.cc_write_npy.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
+        L.cc_cache_diagnostics.argtypes = [vp, ctypes.POINTER(CCGrid), ctypes.POINTER(CCDiagnostics)]
         lp = ctypes.POINTER(CCLeaves)
         L.cc_leaf_cache_create.argtypes = [ctypes.c_int, lp, ctypes.POINTER(vp)]
         L.cc_update_leaves.argtypes = [vp, ctypes.c_void_p, lp]

@@ -457,6 +457,55 @@ def calc_volume(geo, grid, dtype=np.float64):
     return dt.type(out.value)
 
 
+def _report_dict(r, dtype):
+    return {"n": r.n, "cut": r.cut, "inside": r.inside, "outside": r.outside, "ambiguous": r.ambiguous,
+            "min_volume_fraction": np.dtype(dtype).type(r.min_volume_fraction)}
+
+
+def cut_cell_report(cache, *rest):
+    """cut_cell_report(moments) (src/cut_cell.jl:294-316): counts by kind, ambiguous cells and the
+    smallest nonzero cut volume fraction -- reduced on the device for a cache; given host columns
+    (kind, ambiguous, volume_fraction) it is the host reader."""
+    if not isinstance(cache, Cache):
+        from . import readers
+        return readers.cut_cell_report(cache, *rest)
+    r = _lib.CCReport()
+    check(lib().cc_cut_cell_report(cache.handle, ctypes.byref(r)))
+    return _report_dict(r, cache.dtype)
+
+
+def diagnostics(cache, grid):
+    """The report plus, over every cell (on the device): cells with a nonzero closure_residual and its
+    largest component; shared faces whose fractions or centroids differ bit for bit between their two
+    cells; total interface_area and interface_normal_area (src/cut_cell.jl:176-347)."""
+    _check_same_n(cache, grid)
+    dg = _lib.CCDiagnostics()
+    check(lib().cc_cache_diagnostics(cache.handle, ctypes.byref(make_grid(grid)), ctypes.byref(dg)))
+    out = _report_dict(dg.report, cache.dtype)
+    out.update(closure_nonzero=dg.closure_nonzero, closure_max=dg.closure_max,
+               face_mismatches=dg.face_mismatches, centroid_mismatches=dg.centroid_mismatches,
+               interface_area=dg.interface_area,
+               interface_normal_area=np.array(dg.interface_normal_area[:grid.dim]))
+    return out
+
+
+def write_cache_vtk(prefix, cache, grid, surface=True):
+    """write_cache_vtk(prefix, cache, grid; surface=true) -> files written: ``prefix_cells.vti`` (the
+    cells, with phi or the planes) and, with ``surface``, ``prefix_surface.vtu`` (generate_mesh), the
+    layout of the reference's write_cache_vtk (src/tri_clipping/surface.jl:214-252)."""
+    _check_same_n(cache, grid)
+    check(lib().cc_write_cache_vtk(cache.handle, ctypes.byref(make_grid(grid)), str(prefix).encode(),
+                                   1 if surface else 0))
+    return [str(prefix) + "_cells.vti"] + ([str(prefix) + "_surface.vtu"] if surface else [])
+
+
+def write_npy(cache, name, path):
+    """One column as a .npy file in the reference's layout (components first, then the cells, column
+    major): ``numpy.load(path)`` gives e.g. face_fraction as (2D, n0, n1[, n2])."""
+    check(lib().cc_write_npy(cache.handle, _lib.FIELDS[name], str(path).encode()))
+    return str(path)
+
+
 def volume_fractions(cache):
     return cache.read("volume_fraction")
 

@@ -142,6 +142,43 @@ int cc_get_tuning(cc_cache *cache, int *variant, int *kz, int *nontemporal);
 int cc_set_profiling(cc_cache *cache, int on);
 int cc_kernel_times(cc_cache *cache, int max, float *ms, const char **names, int *count);
 
+/* ---- diagnostics over a cache's cells, reduced on the device ------------------------------------ */
+/* cut_cell_report(moments) -- src/cut_cell.jl:294-316: counts by kind, ambiguous cells, and the
+ * smallest nonzero volume fraction of a cut cell (1 when there is none). */
+typedef struct cc_report {
+    int64_t n, cut, inside, outside, ambiguous;
+    double min_volume_fraction;
+} cc_report;
+int cc_cut_cell_report(cc_cache *cache, cc_report *out);
+
+/* The report plus, over every cell of a grid cache: closure_residual (src/cut_cell.jl:318-347) --
+ * how many cells have a nonzero one and its largest component magnitude (both zero by construction);
+ * the shared faces (interior faces, each once) whose two cells store different bits for the face
+ * fraction or the face centroid (zero for a watertight cache; test/test_moments.jl:318-340); and the
+ * totals of interface_area and interface_normal_area (src/cut_cell.jl:176-220).  Counts and extremes
+ * are exact; the totals are summed in a fixed order (the same on every run). */
+typedef struct cc_diagnostics {
+    cc_report report;
+    int64_t closure_nonzero;
+    double closure_max;
+    int64_t face_mismatches, centroid_mismatches;
+    double interface_area;
+    double interface_normal_area[3];
+} cc_diagnostics;
+int cc_cache_diagnostics(cc_cache *cache, const cc_grid *grid, cc_diagnostics *out);
+
+/* ---- files ------------------------------------------------------------------------------------- */
+/* For ParaView: `prefix_cells.vti` (VTK XML ImageData over the grid: volume_fraction, kind, ambiguous,
+ * face_fraction_-x ... +z, and centroid / interface_centroid with phi as point data, or for PLIC
+ * normal / intercept / is_valid) and, with surface != 0, `prefix_surface.vtu` (the generate_mesh
+ * result), as write_cache_vtk writes them for the reference's triangle-clipping cache
+ * (src/tri_clipping/surface.jl:214-252).  Raw little-endian appended data.  Not for leaf caches. */
+int cc_write_cache_vtk(cc_cache *cache, const cc_grid *grid, const char *prefix, int surface);
+
+/* One column as a .npy file in the reference's layout: fortran_order, components first, then the
+ * cells (or nodes, for phi): e.g. face_fraction of an n0 x n1 x n2 cache is (6, n0, n1, n2) float64. */
+int cc_write_npy(cc_cache *cache, int field, const char *path);
+
 /* generate_mesh(cache, grid) for a marching-squares cache -- src/marching_squares/surface.jl:59-86:
  * the contour of the cache's phi, welded by exact coordinate equality.  Two passes: call with
  * nodes/lines NULL to get the counts, then again with buffers of that size.  nodes: 2*nnodes
