@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--quick", action="store_true",
                     help="only the headline workload (skip the secondary workloads)")
     ap.add_argument("--only", default=None,
-                    choices=["csg", "batched", "secondary", "surface", "quadtree", "plic_surface"],
+                    choices=["csg", "batched", "secondary", "surface", "quadtree", "plic_surface", "diagnostics"],
                     help="run one secondary workload alone and print its JSON")
     ap.add_argument("--train-epochs", type=int, default=24,
                     help="surrogate training epochs of the composite-body workload")
@@ -287,6 +287,62 @@ def plic_surface(seed, n=384, reps=5, cpu=True):
     return out
 
 
+DIAG_BYTES_PER_CELL_3D_F64 = 1 + 1 + 8 + 6 * 8 + 12 * 8  # kind, ambiguous, volume, faces, face centroids
+
+
+def diagnostics_pass(seed, n=512, reps=8, cpu=True, layers=16):
+    """The diagnostics pass (report, closure residual, interface areas, shared faces) over the
+    marching-cubes workload's cache at n^3: the fused device reduction (HIP events), the whole call,
+    and the host's numpy readers over `layers` k-layers of the same columns, scaled to the grid."""
+    from cutcell import MarchingCubesCutCell, allocate_cache, diagnostics, readers, update_cache
+    grid, geo, _ = workload(n, seed)
+    c = allocate_cache(grid, MarchingCubesCutCell)
+    update_cache(c, geo, grid)
+    c.set_profiling(True)
+    dev, wall = [], []
+    for r in range(reps + 1):
+        t = time.perf_counter()
+        d = diagnostics(c, grid)
+        t1 = time.perf_counter()
+        if r:
+            dev.append(dict(c.kernel_times())["diagnostics"])
+            wall.append((t1 - t) * 1e3)
+    ms = float(np.median(dev))
+    nbytes = grid.ncells * DIAG_BYTES_PER_CELL_3D_F64
+    out = {"workload": f"diagnostics of the {n}^3 marching-cubes sphere cache", "cells": grid.ncells,
+           "cut": d["cut"], "face_mismatches": d["face_mismatches"], "closure_nonzero": d["closure_nonzero"],
+           "interface_area": d["interface_area"], "device_ms": round(ms, 4),
+           "call_ms": round(float(np.median(wall)), 4),
+           "roofline": {"bound": "hbm", "achieved": round(nbytes / (ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                        "unit": "GB/s", "frac": round(nbytes / (ms * 1e-3) / 8e12, 4),
+                        "bytes_per_cell": DIAG_BYTES_PER_CELL_3D_F64}}
+    if cpu:
+        mid = grid.n[2] // 2 - layers // 2
+        lo, hi = grid.n[0] * grid.n[1] * mid, grid.n[0] * grid.n[1] * (mid + layers)
+        cols = {k: c.read(k, lo, hi) for k in ("kind", "ambiguous", "volume_fraction", "face_fraction",
+                                               "face_centroid_local")}
+        t = time.perf_counter()
+        readers.cut_cell_report(cols["kind"], cols["ambiguous"], cols["volume_fraction"])
+        res = readers.closure_residual(cols["face_fraction"], grid.d)
+        int(np.any(res != 0, axis=1).sum())
+        float(readers.interface_area(cols["face_fraction"], grid.d).sum())
+        readers.interface_normal_area(cols["face_fraction"], grid.d).sum(axis=0)
+        ff = cols["face_fraction"].reshape(layers, grid.n[1], grid.n[0], 6)
+        fcl = cols["face_centroid_local"].reshape(layers, grid.n[1], grid.n[0], 6, 2)
+        int((ff[:, :, :-1, 1] != ff[:, :, 1:, 0]).sum() + (ff[:, :-1, :, 3] != ff[:, 1:, :, 2]).sum() +
+            (ff[:-1, :, :, 5] != ff[1:, :, :, 4]).sum())
+        int(np.any(fcl[:, :, :-1, 1] != fcl[:, :, 1:, 0], axis=-1).sum() +
+            np.any(fcl[:, :-1, :, 3] != fcl[:, 1:, :, 2], axis=-1).sum() +
+            np.any(fcl[:-1, :, :, 5] != fcl[1:, :, :, 4], axis=-1).sum())
+        t_cpu = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(t_cpu * grid.n[2] / layers * 1e3, 1), "unit": "ms per pass",
+                               "cores": 1, "kind": "host numpy readers",
+                               "sample": f"{layers} middle k-layers ({hi - lo} cells) already on the host, "
+                                         f"scaled to the {n}^3 grid"}
+    del c
+    return out
+
+
 def fp32_error(grid, geo, cache64):
     """The Float32 variant: its update time and its error against the Float64 cache (bitwise the
     oracle) on a slab through the body."""
@@ -497,7 +553,8 @@ def main():
               "secondary": lambda: secondary(a.seed),
               "surface": lambda: surface(a.seed, cpu=not a.no_cpu),
               "quadtree": lambda: quadtree(a.seed, cpu=not a.no_cpu),
-              "plic_surface": lambda: plic_surface(a.seed, cpu=not a.no_cpu)}[a.only]
+              "plic_surface": lambda: plic_surface(a.seed, cpu=not a.no_cpu),
+              "diagnostics": lambda: diagnostics_pass(a.seed, cpu=not a.no_cpu)}[a.only]
         print(json.dumps({a.only: fn()}))
         return
     grid, geo, centre = workload(a.n, a.seed)
@@ -596,6 +653,7 @@ def main():
         out["mc_surface_512"] = surface(a.seed, cpu=not a.no_cpu)
         out["ms_quadtree"] = quadtree(a.seed, cpu=not a.no_cpu)
         out["plic_surface_384"] = plic_surface(a.seed, cpu=not a.no_cpu)
+        out["diagnostics_512"] = diagnostics_pass(a.seed, cpu=not a.no_cpu)
         out["batched_64_bodies"] = batched(a.seed)
         out["csg_512_spheres"] = csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples)
     print(json.dumps(out))

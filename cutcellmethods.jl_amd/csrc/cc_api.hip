@@ -697,11 +697,16 @@ template <typename T> int run_diagnostics(cc_cache *c, const cc::GridT<T> &g, in
     }
     unsigned long long *dc = (unsigned long long *)c->dg;
     T *dsum = (T *)((char *)c->dg + 9 * 8), *dpart = dsum + 4;
+    c->nev = 0;
+    c->ev_ready = false;
+    if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
     if (!cc::launch_diagnostics<T>(cols_t<T>(c), g, dim, dc, dpart, most, dsum, c->stream))
         return fail(CC_ERR_ARGUMENT, "diagnostics: the grid needs more blocks than the partial sums hold");
+    mark(c, "diagnostics");
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(c->dg_host, c->dg, 9 * 8 + 4 * sizeof(T), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
+    c->ev_ready = c->profiling;
     std::memcpy(counts, c->dg_host, 9 * 8);
     std::memcpy(sums, (char *)c->dg_host + 9 * 8, 4 * sizeof(T));
     return CC_OK;

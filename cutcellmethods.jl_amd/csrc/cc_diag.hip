@@ -4,17 +4,22 @@
 // bit for bit: test/test_mc_moments.jl:445-462, test/test_moments.jl:318-340).
 //
 // One fused pass reads each cell's kind, ambiguous flag, volume fraction, face fractions and face
-// centroids (and its +x/+y/+z neighbours' for the shared faces).  Integer counts, the minimum volume
-// fraction and the largest residual are combined with integer atomics (order-free); the floating
-// sums (interface area and vector area) are folded per block over a fixed range of cells and then
-// over the blocks in a fixed tree, so every result is the same on every run.
+// centroids once.  A wave owns a strip of 64 cells along x by DG_TJ rows by DG_KZ layers and marches
+// it row by row, layer by layer: the +x neighbour's low face comes from the next lane, the +y and +z
+// neighbours' from the row and layer before, held in registers, so only the strip's far edges read a
+// neighbour again.  Integer counts, the minimum volume fraction and the largest residual are combined
+// with integer atomics (order-free); the floating sums (interface area and vector area) are folded per
+// lane in a fixed order, per block in a fixed tree and then over the blocks in a fixed tree, so every
+// result is the same on every run.
 #include "cc_internal.h"
 
 namespace cc {
 namespace {
 
-constexpr int DG_THREADS = 256;
+constexpr int DG_THREADS = 256;  // four waves, one strip each
 constexpr int DG_BLOCKS = 2048;
+constexpr int DG_TJ = 8;         // rows per strip
+constexpr int DG_KZ = 32;        // layers per strip (3D)
 
 typedef unsigned long long u64;
 
@@ -34,6 +39,21 @@ __device__ __forceinline__ u64 bits_of(double v) { return (u64)__double_as_longl
 __device__ __forceinline__ bool differ(double a, double b) { return __double_as_longlong(a) != __double_as_longlong(b); }
 __device__ __forceinline__ bool differ(float a, float b) { return __float_as_uint(a) != __float_as_uint(b); }
 
+template <typename T> struct PairOf;
+template <> struct PairOf<double> { typedef double2 type; };
+template <> struct PairOf<float> { typedef float2 type; };
+
+// N consecutive values from p (N even, p aligned to two values) as N/2 paired loads.
+template <typename T, int N> __device__ __forceinline__ void load_pairs(const T *__restrict__ p, T *out) {
+    typedef typename PairOf<T>::type P;
+#pragma unroll
+    for (int q = 0; q < N / 2; ++q) {
+        const P v = reinterpret_cast<const P *>(p)[q];
+        out[2 * q] = v.x;
+        out[2 * q + 1] = v.y;
+    }
+}
+
 // full_face_area (src/cut_cell.jl:246-253): the product of the sizes along every other axis.
 template <typename T, int D> __device__ __forceinline__ T full_face(const T *cs, int dir) {
     const int ax = dir >> 1;
@@ -45,84 +65,159 @@ template <typename T, int D> __device__ __forceinline__ T full_face(const T *cs,
     return p;
 }
 
+template <typename T> struct Acc {
+    u64 cnt[7];  // cut, inside, outside, ambiguous, closure nonzero, face mismatches, centroid mismatches
+    u64 minvf, maxres;
+    T area, vec[3];
+};
+
+// One cell's report terms, interface_normal_area and closure_residual (the reference's expressions
+// in its order), folded into the lane's accumulators.
 template <typename T, int D>
-__global__ __launch_bounds__(DG_THREADS) void k_diag(Cols<T> cols, GridT<T> g, size_t ncell, size_t per,
+__device__ __forceinline__ void cell_terms(Acc<T> &a, int8_t kd, uint8_t amb, T vf, const T *ff, const T *cs) {
+    a.cnt[0] += kd == CELL_CUT;
+    a.cnt[1] += kd == CELL_INSIDE;
+    a.cnt[2] += kd == CELL_OUTSIDE;
+    a.cnt[3] += amb != 0;
+    if (kd == CELL_CUT && vf > T(0)) {
+        const u64 b = bits_of((double)vf);
+        a.minvf = b < a.minvf ? b : a.minvf;  // positive doubles order as their bit patterns
+    }
+    T ina[D], acc[D];
+#pragma unroll
+    for (int x = 0; x < D; ++x) {
+        ina[x] = ff[2 * x + 1] * full_face<T, D>(cs, 2 * x + 1) - ff[2 * x] * full_face<T, D>(cs, 2 * x);
+        acc[x] = T(0);
+    }
+#pragma unroll
+    for (int dir = 0; dir < 2 * D; ++dir) {
+        const T s = (dir & 1 ? T(1) : T(-1)) * (ff[dir] * full_face<T, D>(cs, dir));
+#pragma unroll
+        for (int x = 0; x < D; ++x) acc[x] = acc[x] + (x == (dir >> 1) ? s : T(0));
+    }
+    bool nonzero = false;
+    T sq = T(0);
+#pragma unroll
+    for (int x = 0; x < D; ++x) {
+        const T r = acc[x] - ina[x];
+        nonzero |= r != T(0);
+        const u64 b = bits_of(fabs((double)r));
+        a.maxres = b > a.maxres ? b : a.maxres;
+        sq = sq + ina[x] * ina[x];
+        a.vec[x] = a.vec[x] + ina[x];
+    }
+    a.cnt[4] += nonzero;
+    a.area = a.area + sqrt(sq);
+}
+
+// A face two cells share: this cell's fraction and centroid on it against the neighbour's.
+template <typename T, int D>
+__device__ __forceinline__ void shared_face(Acc<T> &a, T f, const T *fc, T g, const T *gc) {
+    a.cnt[5] += differ(f, g);
+    bool m = false;
+#pragma unroll
+    for (int q = 0; q < D - 1; ++q) m |= differ(fc[q], gc[q]);
+    a.cnt[6] += m;
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(DG_THREADS) void k_diag(Cols<T> cols, GridT<T> g, int sx, int sy, int nitems,
                                                      unsigned long long *__restrict__ counts,
                                                      T *__restrict__ part) {
-    __shared__ T red[DG_THREADS];
-    const size_t c0 = blockIdx.x * per, c1 = c0 + per < ncell ? c0 + per : ncell;
-    const uint32_t n0 = g.n[0], n1 = g.n[1], n2 = D == 3 ? g.n[2] : 1;
-    const size_t st[3] = {1, n0, (size_t)n0 * n1};
+    constexpr int NF = 2 * D, NC = 2 * D * (D - 1);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = D == 3 ? g.n[2] : 1;
+    const size_t row = (size_t)n0, plane = (size_t)n0 * n1;
     const T cs[3] = {g.d[0], g.d[1], D == 3 ? g.d[2] : T(1)};
-    u64 ncut = 0, nin = 0, nout = 0, namb = 0, nclo = 0, nff = 0, nfc = 0;
-    u64 minvf = bits_of(1.0), maxres = 0;
-    T area = T(0), vec[3] = {T(0), T(0), T(0)};
-    for (size_t c = c0 + threadIdx.x; c < c1; c += DG_THREADS) {
-        const int8_t kd = cols.kind[c];
-        ncut += kd == CELL_CUT;
-        nin += kd == CELL_INSIDE;
-        nout += kd == CELL_OUTSIDE;
-        namb += cols.amb[c] != 0;
-        const T vf = cols.vf[c];
-        if (kd == CELL_CUT && vf > T(0)) {
-            const u64 b = bits_of((double)vf);
-            minvf = b < minvf ? b : minvf;  // positive doubles order as their bit patterns
+    // blocks are dealt to the 8 XCDs in turn: renumber so that consecutive strips (neighbouring rows
+    // of one layer range) run on one XCD and share its L2 for the rows a strip reads again
+    const int nb = gridDim.x;
+    const int lb = nb % 8 == 0 ? (int)(blockIdx.x % 8) * (nb / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+    Acc<T> a;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) a.cnt[q] = 0;
+    a.minvf = bits_of(1.0);
+    a.maxres = 0;
+    a.area = a.vec[0] = a.vec[1] = a.vec[2] = T(0);
+    for (int item = lb * 4 + wave; item < nitems; item += nb * 4) {
+        const int tx = item % sx, r = item / sx, ty = r % sy, tz = r / sy;
+        const int i = tx * 64 + lane, j0 = ty * DG_TJ, k0 = D == 3 ? tz * DG_KZ : 0;
+        const int j1 = j0 + DG_TJ < n1 ? j0 + DG_TJ : n1;
+        const int k1 = D == 3 ? (k0 + DG_KZ < n2 ? k0 + DG_KZ : n2) : 1;
+        const bool vi = i < n0, xn = i + 1 < n0;
+        T zf[DG_TJ], zc[DG_TJ][D - 1];  // the layer before: each row's +z face
+        for (int k = k0; k < k1; ++k) {
+            T yf = T(0), yc[D - 1];  // the row before: its +y face
+#pragma unroll
+            for (int q = 0; q < D - 1; ++q) yc[q] = T(0);
+#pragma unroll
+            for (int jj = 0; jj < DG_TJ; ++jj) {
+                const int j = j0 + jj;
+                if (j >= j1) break;
+                const size_t c = (size_t)i + row * j + plane * k;
+                T ff[NF], fc[NC];
+                if (vi) {
+                    load_pairs<T, NF>(cols.ff + NF * c, ff);
+                    load_pairs<T, NC>(cols.fcl + NC * c, fc);
+                    cell_terms<T, D>(a, cols.kind[c], cols.amb[c], cols.vf[c], ff, cs);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NF; ++q) ff[q] = T(0);
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) fc[q] = T(0);
+                }
+                // +x: the next lane's -x face (the next strip's first cell for the last lane)
+                T nf = __shfl_down(ff[0], 1), nc[D - 1];
+#pragma unroll
+                for (int q = 0; q < D - 1; ++q) nc[q] = __shfl_down(fc[q], 1);
+                if (lane == 63 && xn) {
+                    nf = cols.ff[NF * (c + 1)];
+#pragma unroll
+                    for (int q = 0; q < D - 1; ++q) nc[q] = cols.fcl[NC * (c + 1) + q];
+                }
+                if (vi && xn) shared_face<T, D>(a, ff[1], fc + (D - 1), nf, nc);
+                // +y: the row before's +y face against this row's -y face
+                if (vi && jj > 0) shared_face<T, D>(a, yf, yc, ff[2], fc + 2 * (D - 1));
+                yf = ff[3];
+#pragma unroll
+                for (int q = 0; q < D - 1; ++q) yc[q] = fc[3 * (D - 1) + q];
+                if (D == 3) {  // +z: the layer before's +z face against this layer's -z face
+                    if (vi && k > k0) shared_face<T, D>(a, zf[jj], zc[jj], ff[4], fc + 4 * (D - 1));
+                    zf[jj] = ff[5];
+#pragma unroll
+                    for (int q = 0; q < D - 1; ++q) zc[jj][q] = fc[5 * (D - 1) + q];
+                }
+            }
+            if (vi && j1 < n1) {  // the strip's last row against the first row past it
+                const size_t c = (size_t)i + row * j1 + plane * k;
+                T gc[D - 1];
+#pragma unroll
+                for (int q = 0; q < D - 1; ++q) gc[q] = cols.fcl[NC * c + 2 * (D - 1) + q];
+                shared_face<T, D>(a, yf, yc, cols.ff[NF * c + 2], gc);
+            }
         }
-        T ff[2 * D];
+        if (D == 3 && vi && k1 < n2) {  // the strip's last layer against the first layer past it
 #pragma unroll
-        for (int k = 0; k < 2 * D; ++k) ff[k] = cols.ff[2 * D * c + k];
-        // interface_normal_area and closure_residual, the reference's expressions in its order
-        T ina[D], acc[D];
+            for (int jj = 0; jj < DG_TJ; ++jj) {
+                const int j = j0 + jj;
+                if (j >= j1) break;
+                const size_t c = (size_t)i + row * j + plane * k1;
+                T gc[D - 1];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            ina[a] = ff[2 * a + 1] * full_face<T, D>(cs, 2 * a + 1) - ff[2 * a] * full_face<T, D>(cs, 2 * a);
-            acc[a] = T(0);
-        }
-#pragma unroll
-        for (int dir = 0; dir < 2 * D; ++dir) {
-            const T s = (dir & 1 ? T(1) : T(-1)) * (ff[dir] * full_face<T, D>(cs, dir));
-#pragma unroll
-            for (int a = 0; a < D; ++a) acc[a] = acc[a] + (a == (dir >> 1) ? s : T(0));
-        }
-        bool nonzero = false;
-        T sq = T(0);
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const T r = acc[a] - ina[a];
-            nonzero |= r != T(0);
-            const u64 b = bits_of(fabs((double)r));
-            maxres = b > maxres ? b : maxres;
-            sq = sq + ina[a] * ina[a];
-            vec[a] = vec[a] + ina[a];
-        }
-        nclo += nonzero;
-        area = area + sqrt(sq);
-        // the shared faces towards +x, +y, +z: the neighbour's low slot against this cell's high slot
-        const uint32_t ijk[3] = {(uint32_t)(c % n0), (uint32_t)((c / n0) % n1), (uint32_t)(c / st[2])};
-        const uint32_t nn[3] = {n0, n1, n2};
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            if (ijk[a] + 1 >= nn[a]) continue;
-            const size_t o = c + st[a];
-            nff += differ(ff[2 * a + 1], cols.ff[2 * D * o + 2 * a]);
-            bool fc = false;
-#pragma unroll
-            for (int q = 0; q < D - 1; ++q)
-                fc |= differ(cols.fcl[2 * D * (D - 1) * c + (2 * a + 1) * (D - 1) + q],
-                             cols.fcl[2 * D * (D - 1) * o + (2 * a) * (D - 1) + q]);
-            nfc += fc;
+                for (int q = 0; q < D - 1; ++q) gc[q] = cols.fcl[NC * c + 4 * (D - 1) + q];
+                shared_face<T, D>(a, zf[jj], zc[jj], cols.ff[NF * c + 4], gc);
+            }
         }
     }
     // the counts: folded per block, then one atomic per block
     __shared__ u64 ired[DG_THREADS];
-    const u64 vals[7] = {ncut, nin, nout, namb, nclo, nff, nfc};
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
-        const u64 s = block_fold(vals[q], ired);
+        const u64 s = block_fold(a.cnt[q], ired);
         if (threadIdx.x == 0 && s) atomicAdd(&counts[q], s);
     }
     // the minimum volume fraction and the largest residual (bit patterns of non-negative doubles)
-    ired[threadIdx.x] = minvf;
+    ired[threadIdx.x] = a.minvf;
     __syncthreads();
     for (int w = DG_THREADS / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w && ired[threadIdx.x + w] < ired[threadIdx.x]) ired[threadIdx.x] = ired[threadIdx.x + w];
@@ -130,7 +225,7 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag(Cols<T> cols, GridT<T> g, s
     }
     if (threadIdx.x == 0) atomicMin(&counts[7], ired[0]);
     __syncthreads();
-    ired[threadIdx.x] = maxres;
+    ired[threadIdx.x] = a.maxres;
     __syncthreads();
     for (int w = DG_THREADS / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w && ired[threadIdx.x + w] > ired[threadIdx.x]) ired[threadIdx.x] = ired[threadIdx.x + w];
@@ -138,10 +233,11 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag(Cols<T> cols, GridT<T> g, s
     }
     if (threadIdx.x == 0) atomicMax(&counts[8], ired[0]);
     // the floating sums, per block in a fixed tree
-    const T a0 = block_fold(area, red);
-    const T v0 = block_fold(vec[0], red);
-    const T v1 = block_fold(vec[1], red);
-    const T v2 = D == 3 ? block_fold(vec[2], red) : T(0);
+    __shared__ T red[DG_THREADS];
+    const T a0 = block_fold(a.area, red);
+    const T v0 = block_fold(a.vec[0], red);
+    const T v1 = block_fold(a.vec[1], red);
+    const T v2 = D == 3 ? block_fold(a.vec[2], red) : T(0);
     if (threadIdx.x == 0) {
         part[4 * blockIdx.x] = a0;
         part[4 * blockIdx.x + 1] = v0;
@@ -172,23 +268,32 @@ __global__ void k_diag_init(unsigned long long *counts) {
 
 int diag_max_blocks() { return DG_BLOCKS; }
 
-int diag_blocks(size_t ncell) {
-    const size_t nb = ncell / DG_THREADS + (ncell % DG_THREADS != 0);  // no overflow near SIZE_MAX
-    return (int)(nb == 0 ? 1 : (nb < (size_t)DG_BLOCKS ? nb : DG_BLOCKS));
+// Strips of 64 x DG_TJ x DG_KZ cells, four per block, at most DG_BLOCKS blocks (each loops over
+// strips), at least one.
+void diag_shape(const int *n, int dim, int &sx, int &sy, int64_t &items, int &blocks) {
+    const int n2 = dim == 3 ? n[2] : 1;
+    sx = (n[0] + 63) / 64;
+    sy = (n[1] + DG_TJ - 1) / DG_TJ;
+    const int64_t sz = dim == 3 ? (n2 + DG_KZ - 1) / DG_KZ : 1;
+    items = (int64_t)sx * sy * sz;
+    const int64_t nb = (items + 3) / 4;
+    blocks = (int)(nb < 1 ? 1 : (nb < DG_BLOCKS ? nb : DG_BLOCKS));
 }
 
 template <typename T>
 bool launch_diagnostics(const Cols<T> &cols, const GridT<T> &g, int dim, unsigned long long *counts, T *part,
                         size_t part_blocks, T *sums, hipStream_t s) {
-    const size_t ncell = (size_t)g.n[0] * g.n[1] * (dim == 3 ? g.n[2] : 1);
-    const int nb = diag_blocks(ncell);
-    if ((size_t)nb > part_blocks) return false;  // part holds 4 values per block
-    const size_t per = (ncell + nb - 1) / nb;
+    int sx, sy, nb;
+    int64_t items;
+    diag_shape(g.n, dim, sx, sy, items, nb);
+    if ((size_t)nb > part_blocks || items > INT32_MAX) return false;  // part holds 4 values per block
     hipLaunchKernelGGL(k_diag_init, dim3(1), dim3(64), 0, s, counts);
     if (dim == 3)
-        hipLaunchKernelGGL((k_diag<T, 3>), dim3(nb), dim3(DG_THREADS), 0, s, cols, g, ncell, per, counts, part);
+        hipLaunchKernelGGL((k_diag<T, 3>), dim3(nb), dim3(DG_THREADS), 0, s, cols, g, sx, sy, (int)items, counts,
+                           part);
     else
-        hipLaunchKernelGGL((k_diag<T, 2>), dim3(nb), dim3(DG_THREADS), 0, s, cols, g, ncell, per, counts, part);
+        hipLaunchKernelGGL((k_diag<T, 2>), dim3(nb), dim3(DG_THREADS), 0, s, cols, g, sx, sy, (int)items, counts,
+                           part);
     hipLaunchKernelGGL(k_diag_sums<T>, dim3(1), dim3(DG_THREADS), 0, s, (const T *)part, nb, sums);
     return true;
 }

@@ -193,10 +193,9 @@ void launch_plic_surface(const GridT<T> &g, int dim, const T *normals, const T *
 // nonzero closure residual, shared faces whose fractions differ, shared faces whose centroids differ;
 // counts[7] = the bits of the smallest positive cut volume fraction (1.0 if none); counts[8] = the bits
 // of the largest |closure residual| component.  sums[0..3] = total interface area and vector area.
-// part holds part_blocks x 4 values; a grid needing more than part_blocks blocks (diag_blocks) is
-// refused, with nothing launched (false).  No grid needs more than diag_max_blocks().
+// part holds part_blocks x 4 values; a grid needing more than part_blocks blocks is refused, with
+// nothing launched (false).  No grid needs more than diag_max_blocks().
 int diag_max_blocks();
-int diag_blocks(size_t ncell);
 template <typename T>
 bool launch_diagnostics(const Cols<T> &cols, const GridT<T> &g, int dim, unsigned long long *counts, T *part,
                         size_t part_blocks, T *sums, hipStream_t s);
