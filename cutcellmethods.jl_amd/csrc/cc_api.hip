@@ -686,17 +686,18 @@ template <typename T> int plic_surface(cc_cache *c, const cc_grid *g, double min
 // leaf cache, whose report is all that is read back).
 template <typename T> int run_diagnostics(cc_cache *c, const cc::GridT<T> &g, int dim, unsigned long long *counts,
                                           T *sums) {
+    constexpr size_t NCOUNT = 10;  // counters the pass writes (cc_internal.h)
     const size_t most = (size_t)cc::diag_max_blocks();  // the pass's largest block count
-    const size_t bytes = 9 * 8 + 4 * 8 + 4 * 8 * most;
+    const size_t bytes = NCOUNT * 8 + 4 * 8 + 4 * 8 * most;
     if (!c->dg) {
         if (hipMalloc(&c->dg, bytes) != hipSuccess) { c->dg = nullptr; return fail(CC_ERR_OUT_OF_MEMORY, "diagnostics"); }
-        if (hipHostMalloc(&c->dg_host, 9 * 8 + 4 * 8, hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc(&c->dg_host, NCOUNT * 8 + 4 * 8, hipHostMallocDefault) != hipSuccess) {
             c->dg_host = nullptr;
             return fail(CC_ERR_OUT_OF_MEMORY, "diagnostics");
         }
     }
     unsigned long long *dc = (unsigned long long *)c->dg;
-    T *dsum = (T *)((char *)c->dg + 9 * 8), *dpart = dsum + 4;
+    T *dsum = (T *)((char *)c->dg + NCOUNT * 8), *dpart = dsum + 4;
     c->nev = 0;
     c->ev_ready = false;
     if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
@@ -704,16 +705,16 @@ template <typename T> int run_diagnostics(cc_cache *c, const cc::GridT<T> &g, in
         return fail(CC_ERR_ARGUMENT, "diagnostics: the grid needs more blocks than the partial sums hold");
     mark(c, "diagnostics");
     HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(c->dg_host, c->dg, 9 * 8 + 4 * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipMemcpyAsync(c->dg_host, c->dg, NCOUNT * 8 + 4 * sizeof(T), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
     c->ev_ready = c->profiling;
-    std::memcpy(counts, c->dg_host, 9 * 8);
-    std::memcpy(sums, (char *)c->dg_host + 9 * 8, 4 * sizeof(T));
+    std::memcpy(counts, c->dg_host, NCOUNT * 8);
+    std::memcpy(sums, (char *)c->dg_host + NCOUNT * 8, 4 * sizeof(T));
     return CC_OK;
 }
 
 int diagnostics_impl(cc_cache *c, const cc_grid *grid, cc_diagnostics *out) {
-    unsigned long long cnt[9];
+    unsigned long long cnt[10];  // the pass's counters, cc_internal.h
     double sums[4];
     int rc;
     const int dim = c->leaves ? 2 : c->dim;
@@ -746,6 +747,7 @@ int diagnostics_impl(cc_cache *c, const cc_grid *grid, cc_diagnostics *out) {
     out->closure_max = resmax;
     out->face_mismatches = (int64_t)cnt[5];
     out->centroid_mismatches = (int64_t)cnt[6];
+    out->nonfinite = (int64_t)cnt[9];
     out->interface_area = sums[0];
     for (int q = 0; q < 3; ++q) out->interface_normal_area[q] = q < dim ? sums[q + 1] : 0.0;
     return CC_OK;

@@ -1,5 +1,5 @@
-// Diagnostics over a cache's cells, reduced on the device: cut_cell_report (src/cut_cell.jl:294-316),
-// closure_residual (:318-347) and interface_normal_area / interface_area (:176-220) over every cell,
+// Diagnostics over a cache's cells, reduced on the device: cut_cell_report (src/cut_cell.jl:302-317),
+// closure_residual (:340-347), interface_normal_area (:224-226) and interface_area (:190-191) over every cell,
 // and the shared-face check the reference's tests run (a face two cells share must be single-valued,
 // bit for bit: test/test_mc_moments.jl:445-462, test/test_moments.jl:318-340).
 //
@@ -68,6 +68,7 @@ template <typename T, int D> __device__ __forceinline__ T full_face(const T *cs,
 template <typename T> struct Acc {
     u64 cnt[7];  // cut, inside, outside, ambiguous, closure nonzero, face mismatches, centroid mismatches
     u64 minvf, maxres;
+    u64 nonfinite;  // cells whose volume fraction or a face fraction is NaN or infinite
     T area, vec[3];
 };
 
@@ -83,6 +84,10 @@ __device__ __forceinline__ void cell_terms(Acc<T> &a, int8_t kd, uint8_t amb, T 
         const u64 b = bits_of((double)vf);
         a.minvf = b < a.minvf ? b : a.minvf;  // positive doubles order as their bit patterns
     }
+    bool finite = isfinite(vf);
+#pragma unroll
+    for (int k = 0; k < 2 * D; ++k) finite &= isfinite(ff[k]);
+    a.nonfinite += !finite;
     T ina[D], acc[D];
 #pragma unroll
     for (int x = 0; x < D; ++x) {
@@ -138,6 +143,7 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag(Cols<T> cols, GridT<T> g, i
     for (int q = 0; q < 7; ++q) a.cnt[q] = 0;
     a.minvf = bits_of(1.0);
     a.maxres = 0;
+    a.nonfinite = 0;
     a.area = a.vec[0] = a.vec[1] = a.vec[2] = T(0);
     for (int item = lb * 4 + wave; item < nitems; item += nb * 4) {
         const int tx = item % sx, r = item / sx, ty = r % sy, tz = r / sy;
@@ -216,6 +222,10 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag(Cols<T> cols, GridT<T> g, i
         const u64 s = block_fold(a.cnt[q], ired);
         if (threadIdx.x == 0 && s) atomicAdd(&counts[q], s);
     }
+    {
+        const u64 s = block_fold(a.nonfinite, ired);
+        if (threadIdx.x == 0 && s) atomicAdd(&counts[9], s);
+    }
     // the minimum volume fraction and the largest residual (bit patterns of non-negative doubles)
     ired[threadIdx.x] = a.minvf;
     __syncthreads();
@@ -262,6 +272,7 @@ __global__ void k_diag_init(unsigned long long *counts) {
     if (t < 7) counts[t] = 0;
     if (t == 7) counts[7] = bits_of(1.0);
     if (t == 8) counts[8] = 0;
+    if (t == 9) counts[9] = 0;
 }
 
 }  // namespace

@@ -192,7 +192,8 @@ void launch_plic_surface(const GridT<T> &g, int dim, const T *normals, const T *
 // Diagnostics over a cache's cells: counts[0..6] = cut, inside, outside, ambiguous, cells with a
 // nonzero closure residual, shared faces whose fractions differ, shared faces whose centroids differ;
 // counts[7] = the bits of the smallest positive cut volume fraction (1.0 if none); counts[8] = the bits
-// of the largest |closure residual| component.  sums[0..3] = total interface area and vector area.
+// of the largest |closure residual| component; counts[9] = cells with a NaN or infinite volume
+// fraction or face fraction.  sums[0..3] = total interface area and vector area.
 // part holds part_blocks x 4 values; a grid needing more than part_blocks blocks is refused, with
 // nothing launched (false).  No grid needs more than diag_max_blocks().
 int diag_max_blocks();

@@ -64,7 +64,8 @@ class CCReport(ctypes.Structure):
 class CCDiagnostics(ctypes.Structure):
     _fields_ = [("report", CCReport), ("closure_nonzero", ctypes.c_int64), ("closure_max", ctypes.c_double),
                 ("face_mismatches", ctypes.c_int64), ("centroid_mismatches", ctypes.c_int64),
-                ("interface_area", ctypes.c_double), ("interface_normal_area", ctypes.c_double * 3)]
+                ("interface_area", ctypes.c_double), ("interface_normal_area", ctypes.c_double * 3),
+                ("nonfinite", ctypes.c_int64)]
 
 
 class DimensionMismatch(ValueError):

@@ -463,7 +463,7 @@ def _report_dict(r, dtype):
 
 
 def cut_cell_report(cache, *rest):
-    """cut_cell_report(moments) (src/cut_cell.jl:294-316): counts by kind, ambiguous cells and the
+    """cut_cell_report(moments) (src/cut_cell.jl:302-317): counts by kind, ambiguous cells and the
     smallest nonzero cut volume fraction -- reduced on the device for a cache; given host columns
     (kind, ambiguous, volume_fraction) it is the host reader."""
     if not isinstance(cache, Cache):
@@ -477,7 +477,8 @@ def cut_cell_report(cache, *rest):
 def diagnostics(cache, grid):
     """The report plus, over every cell (on the device): cells with a nonzero closure_residual and its
     largest component; shared faces whose fractions or centroids differ bit for bit between their two
-    cells; total interface_area and interface_normal_area (src/cut_cell.jl:176-347)."""
+    cells; total interface_area and interface_normal_area (src/cut_cell.jl:190-226, :340-347); cells whose
+    volume fraction or a face fraction is NaN or infinite (an audit with no reference counterpart)."""
     _check_same_n(cache, grid)
     dg = _lib.CCDiagnostics()
     check(lib().cc_cache_diagnostics(cache.handle, ctypes.byref(make_grid(grid)), ctypes.byref(dg)))
@@ -485,7 +486,7 @@ def diagnostics(cache, grid):
     out.update(closure_nonzero=dg.closure_nonzero, closure_max=dg.closure_max,
                face_mismatches=dg.face_mismatches, centroid_mismatches=dg.centroid_mismatches,
                interface_area=dg.interface_area,
-               interface_normal_area=np.array(dg.interface_normal_area[:grid.dim]))
+               interface_normal_area=np.array(dg.interface_normal_area[:grid.dim]), nonfinite=dg.nonfinite)
     return out
 
 

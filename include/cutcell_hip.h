@@ -143,7 +143,7 @@ int cc_set_profiling(cc_cache *cache, int on);
 int cc_kernel_times(cc_cache *cache, int max, float *ms, const char **names, int *count);
 
 /* ---- diagnostics over a cache's cells, reduced on the device ------------------------------------ */
-/* cut_cell_report(moments) -- src/cut_cell.jl:294-316: counts by kind, ambiguous cells, and the
+/* cut_cell_report(moments) -- src/cut_cell.jl:302-317: counts by kind, ambiguous cells, and the
  * smallest nonzero volume fraction of a cut cell (1 when there is none). */
 typedef struct cc_report {
     int64_t n, cut, inside, outside, ambiguous;
@@ -151,11 +151,11 @@ typedef struct cc_report {
 } cc_report;
 int cc_cut_cell_report(cc_cache *cache, cc_report *out);
 
-/* The report plus, over every cell of a grid cache: closure_residual (src/cut_cell.jl:318-347) --
+/* The report plus, over every cell of a grid cache: closure_residual (src/cut_cell.jl:340-347) --
  * how many cells have a nonzero one and its largest component magnitude (both zero by construction);
  * the shared faces (interior faces, each once) whose two cells store different bits for the face
  * fraction or the face centroid (zero for a watertight cache; test/test_moments.jl:318-340); and the
- * totals of interface_area and interface_normal_area (src/cut_cell.jl:176-220).  Counts and extremes
+ * totals of interface_area and interface_normal_area (src/cut_cell.jl:190-191, :224-226).  Counts and extremes
  * are exact; the totals are summed in a fixed order (the same on every run). */
 typedef struct cc_diagnostics {
     cc_report report;
@@ -164,6 +164,7 @@ typedef struct cc_diagnostics {
     int64_t face_mismatches, centroid_mismatches;
     double interface_area;
     double interface_normal_area[3];
+    int64_t nonfinite; /* cells whose volume fraction or a face fraction is NaN or infinite */
 } cc_diagnostics;
 int cc_cache_diagnostics(cc_cache *cache, const cc_grid *grid, cc_diagnostics *out);
 

@@ -1,5 +1,5 @@
 """GPU checks of the device-side diagnostics (cut_cell_report, closure_residual, interface areas,
-shared faces; src/cut_cell.jl:176-347) and of the VTK / .npy writers, through the C ABI.
+shared faces, a NaN audit; src/cut_cell.jl:190-347) and of the VTK / .npy writers, through the C ABI.
 
 Bar: the report and every count, minimum and maximum equal to a host computation on the copied
 columns; the interface totals to 1e-12 of the host's; a changed face fraction found as exactly one
@@ -34,7 +34,7 @@ def check(c, g):
     for k, v in r.items():
         assert d[k] == v, (k, d[k], v)
     assert d["closure_nonzero"] == 0 and d["closure_max"] == 0.0
-    assert d["face_mismatches"] == 0 and d["centroid_mismatches"] == 0
+    assert d["face_mismatches"] == 0 and d["centroid_mismatches"] == 0 and d["nonfinite"] == 0
     ff = c.read("face_fraction")
     ina = readers.interface_normal_area(ff, g.d)
     area = readers.interface_area(ff, g.d).sum()
@@ -64,6 +64,25 @@ def test_reports_and_diagnostics_of_every_method():
         diagnostics(tc, g2)
 
 
+def test_launch_shapes_and_float32():
+    # 2 x 16 x 2 strips in 16 blocks (renumbered across the XCDs); 64 x 263 strips, more than the
+    # 2048 blocks x 4 waves, so waves loop over several; the Float32 variant
+    g = CartesianGrid((-1.0, -1.0, -0.5), (128, 128, 64), (1 / 64, 1 / 64, 1 / 64))
+    c = update_cache(allocate_cache(g, MarchingCubesCutCell), SDFSphere((0.01, 0.02, -0.03), 0.45), g)
+    check(c, g)
+    g2 = CartesianGrid((-1.0, -0.52), (4096, 2100), (1 / 2048, 1 / 2048))
+    check(update_cache(allocate_cache(g2, MarchingSquaresCutCell), SDFSphere((0.001, 0.002), 0.4), g2), g2)
+    c32 = update_cache(allocate_cache(g, MarchingCubesCutCell, np.float32), SDFSphere((0.01, 0.02, -0.03), 0.45), g)
+    d = diagnostics(c32, g)
+    r = host_report(c32)
+    for k, v in r.items():
+        assert d[k] == v, (k, d[k], v)
+    assert d["closure_nonzero"] == 0 and d["face_mismatches"] == 0 and d["centroid_mismatches"] == 0
+    area = readers.interface_area(c32.read("face_fraction").astype(np.float64), g.d).sum()
+    assert abs(d["interface_area"] - area) <= 1e-5 * area
+    assert cut_cell_report(c32) == r
+
+
 def test_a_changed_face_is_found():
     g = CartesianGrid((-1.5,) * 3, (24,) * 3, (0.125,) * 3)
     c = update_cache(allocate_cache(g, MarchingCubesCutCell), SDFSphere((0.01, 0.02, -0.03), 1.0), g)
@@ -76,7 +95,17 @@ def test_a_changed_face_is_found():
     assert hip.hipMemcpy(ctypes.c_void_p(ptr + (6 * cell + 1) * 8), v.ctypes.data, 8, 1) == 0  # host to device
     assert hip.hipDeviceSynchronize() == 0  # a pageable copy may return before it lands
     d = diagnostics(c, g)
-    assert d["face_mismatches"] == 1 and d["centroid_mismatches"] == 0
+    assert d["face_mismatches"] == 1 and d["centroid_mismatches"] == 0 and d["nonfinite"] == 0
+    # a NaN volume fraction in an uncut cell: the audit counts it, nothing else moves
+    other = int(np.flatnonzero(c.read("kind") == 1)[0])
+    vptr, _ = c.column_ptr("volume_fraction")
+    nan = np.array([np.nan])
+    assert hip.hipMemcpy(ctypes.c_void_p(vptr + other * 8), nan.ctypes.data, 8, 1) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    d2 = diagnostics(c, g)
+    assert d2["nonfinite"] == 1 and d2["face_mismatches"] == 1
+    assert {k: d2[k] for k in ("cut", "inside", "outside", "min_volume_fraction")} == \
+        {k: d[k] for k in ("cut", "inside", "outside", "min_volume_fraction")}
 
 
 def read_vtk(path):
