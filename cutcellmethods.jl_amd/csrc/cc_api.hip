@@ -133,6 +133,13 @@ struct cc_cache {
     double mesh_min_frac = -1.0;  // the PLIC surface's element floor
     cc::PlicSurfBuffers ps = {};
     std::vector<void *> ps_allocs;
+    int64_t ps_cap = 0;  // entries the PLIC surface's work arrays hold
+    unsigned long long *pt_bits = nullptr;  // generate_mesh's touch bitmap, its word ranks, the marked cells
+    int64_t pt_bits_cap = 0;
+    int64_t *pt_rank = nullptr;
+    int64_t pt_rank_cap = 0;
+    int64_t *pt_cells = nullptr;
+    int64_t pt_cells_cap = 0;
     // diagnostics: counters, block sums, totals (device), and a pinned copy of the results
     void *dg = nullptr;
     void *dg_host = nullptr;
@@ -627,44 +634,89 @@ int update_leaves_impl(cc_cache *c, const cc_geometry *geo, const void *phi4, in
 
 // The PLIC surface (src/plic/surface.jl:28-267): count every cell, prefix sums, one round trip for
 // the sizes, then every node and element at its offset.
+// The PLIC surface's per-entry work arrays for n entries, grown on demand (contents not kept).
+int plic_work(cc_cache *c, int64_t n) {
+    if (n < 1) n = 1;
+    if (c->ps.vcnt && n <= c->ps_cap) return CC_OK;
+    for (void *p : c->ps_allocs) (void)hipFree(p);
+    c->ps_allocs.clear();
+    c->ps = {};
+    c->ps_cap = 0;
+    auto get = [&](void **p, size_t bytes) -> bool {
+        if (hipMalloc(p, bytes < 8 ? 8 : bytes) != hipSuccess) return false;
+        c->ps_allocs.push_back(*p);
+        return true;
+    };
+    const size_t blocks = cc::mesh_scan_blocks((size_t)n);
+    cc::PlicSurfBuffers &b = c->ps;
+    b.part_cap = blocks > 0 ? blocks : 1;
+    if (!(get((void **)&b.vcnt, 4 * n) && get((void **)&b.ecnt, 4 * n) && get((void **)&b.vpre, 8 * n) &&
+          get((void **)&b.epre, 8 * n) && get((void **)&b.part, 8 * 2 * b.part_cap) && get((void **)&b.totals, 8 * 2))) {
+        for (void *p : c->ps_allocs) (void)hipFree(p);
+        c->ps_allocs.clear();
+        c->ps = {};
+        return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the PLIC surface's work arrays");
+    }
+    c->ps_cap = n;
+    return CC_OK;
+}
+
+// generate_mesh for a PLIC cache: the cells whose plane can touch them are marked and listed (one
+// host round trip for their number), then counted, ranked (a second round trip for the output
+// sizes) and emitted in cell order.  In 2D with a zero floor every cell draws, so all are entries.
 template <typename T> int plic_surface(cc_cache *c, const cc_grid *g, double min_frac) {
     const int64_t ncell = c->ncell;
-    if (!c->ps.vcnt) {
-        auto get = [&](void **p, size_t bytes) -> bool {
-            if (hipMalloc(p, bytes < 8 ? 8 : bytes) != hipSuccess) return false;
-            c->ps_allocs.push_back(*p);
-            return true;
-        };
-        const size_t blocks = cc::mesh_scan_blocks((size_t)ncell);
-        cc::PlicSurfBuffers &b = c->ps;
-        b.part_cap = blocks > 0 ? blocks : 1;
-        if (!(get((void **)&b.vcnt, 4 * ncell) && get((void **)&b.ecnt, 4 * ncell) &&
-              get((void **)&b.vpre, 8 * ncell) && get((void **)&b.epre, 8 * ncell) &&
-              get((void **)&b.part, 8 * 2 * b.part_cap) && get((void **)&b.totals, 8 * 2))) {
-            for (void *p : c->ps_allocs) (void)hipFree(p);
-            c->ps_allocs.clear();
-            c->ps = {};
-            return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the PLIC surface's work arrays");
-        }
-    }
     const cc::GridT<T> gt = grid_t<T>(g, c->dim);
     const T *nrm = (const T *)c->col[CC_FIELD_NORMALS], *icp = (const T *)c->col[CC_FIELD_INTERCEPTS];
+    double dmin = (double)gt.d[0] < (double)gt.d[1] ? (double)gt.d[0] : (double)gt.d[1];
+    if (c->dim == 3 && (double)gt.d[2] < dmin) dmin = (double)gt.d[2];
+    const bool filter = c->dim == 3 || min_frac * dmin > 0.0;  // the 2D floor launch_plic_surface applies
     c->nev = 0;
     c->ev_ready = false;
     if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
-    cc::launch_plic_surface<T>(gt, c->dim, nrm, icp, nullptr, (size_t)ncell, min_frac, c->ps, false, nullptr, nullptr,
+    int64_t n = ncell;
+    const int64_t *cells = nullptr;
+    int rc;
+    if (filter) {
+        const int64_t nwords = (ncell + 63) / 64;
+        rc = ensure(&c->pt_bits, &c->pt_bits_cap, nwords);
+        if (rc == CC_OK) rc = ensure(&c->pt_rank, &c->pt_rank_cap, nwords);
+        if (rc == CC_OK) rc = plic_work(c, nwords);
+        if (rc) return rc;
+        cc::launch_plic_touch<T>(nrm, icp, (size_t)ncell, c->dim, c->pt_bits, c->pt_rank, c->ps.part, c->ps.part_cap,
+                                 c->ps.totals, c->stream);
+        mark(c, "plic_surface_touch");
+        HIPC(hipGetLastError());
+        int64_t marked[2] = {0, 0};
+        HIPC(hipMemcpyAsync(marked, c->ps.totals, sizeof marked, hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        n = marked[0];
+        if (n < 0 || n > ncell) return fail(CC_ERR_HIP, "PLIC surface: %lld cells marked of %lld", (long long)n,
+                                            (long long)ncell);
+        rc = ensure(&c->pt_cells, &c->pt_cells_cap, n > 0 ? n : 1);
+        if (rc == CC_OK) rc = plic_work(c, n);
+        if (rc) return rc;
+        mark(c, "plic_surface_host_sync0");  // the round trip above: host time, not a kernel
+        if (n > 0) cc::launch_plic_marked(c->pt_bits, c->pt_rank, (size_t)ncell, c->pt_cells, c->stream);
+        mark(c, "plic_surface_marked");
+        cells = c->pt_cells;
+    } else {
+        rc = plic_work(c, ncell);
+        if (rc) return rc;
+    }
+    cc::launch_plic_surface<T>(gt, c->dim, nrm, icp, cells, true, (size_t)n, min_frac, c->ps, false, nullptr, nullptr,
                                c->stream);
     mark(c, "plic_surface_count");
     HIPC(hipGetLastError());
     int64_t tot[2] = {0, 0};
     HIPC(hipMemcpyAsync(tot, c->ps.totals, sizeof tot, hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
-    int rc = ensure((char **)&c->mverts, &c->mverts_cap, c->dim * sizeof(T) * (tot[0] > 0 ? tot[0] : 1));
+    rc = ensure((char **)&c->mverts, &c->mverts_cap, c->dim * sizeof(T) * (tot[0] > 0 ? tot[0] : 1));
     if (rc) return rc;
     rc = ensure(&c->mtris, &c->mtris_cap, c->dim * (tot[1] > 0 ? tot[1] : 1));
     if (rc) return rc;
     mark(c, "plic_surface_host_sync");
-    cc::launch_plic_surface<T>(gt, c->dim, nrm, icp, nullptr, (size_t)ncell, min_frac, c->ps, true, (T *)c->mverts,
+    cc::launch_plic_surface<T>(gt, c->dim, nrm, icp, cells, true, (size_t)n, min_frac, c->ps, true, (T *)c->mverts,
                                c->mtris, c->stream);
     mark(c, "plic_surface_emit");
     HIPC(hipGetLastError());
@@ -767,6 +819,9 @@ void free_all(cc_cache *c) {
     if (c->mverts) (void)hipFree(c->mverts);
     if (c->mtris) (void)hipFree(c->mtris);
     if (c->mcells) (void)hipFree(c->mcells);
+    if (c->pt_bits) (void)hipFree(c->pt_bits);
+    if (c->pt_rank) (void)hipFree(c->pt_rank);
+    if (c->pt_cells) (void)hipFree(c->pt_cells);
     if (c->leaf_level) (void)hipFree(c->leaf_level);
     if (c->leaf_coord) (void)hipFree(c->leaf_coord);
     for (auto &p : c->col)
@@ -1498,8 +1553,8 @@ int cc_extract_surface_plic(int dtype, const cc_grid *grid, int64_t count, const
             HIPC(hipMemcpyAsync(di, intercepts, es * n, hipMemcpyHostToDevice, st));
             HIPC(hipMemcpyAsync(dc, cells, 8 * n, hipMemcpyHostToDevice, st));
         }
-        cc::launch_plic_surface<T>(gt, dim, (const T *)dn, (const T *)di, (const int64_t *)dc, n, min_frac, b, false,
-                                   nullptr, nullptr, st);
+        cc::launch_plic_surface<T>(gt, dim, (const T *)dn, (const T *)di, (const int64_t *)dc, false, n, min_frac, b,
+                                   false, nullptr, nullptr, st);
         HIPC(hipGetLastError());
         HIPC(hipMemcpyAsync(tot, b.totals, sizeof tot, hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
@@ -1509,8 +1564,8 @@ int cc_extract_surface_plic(int dtype, const cc_grid *grid, int64_t count, const
                         (long long)*nnodes, (long long)*nelems, (long long)tot[0], (long long)tot[1]);
         if (!get(&vout, dim * es * (tot[0] > 0 ? tot[0] : 1)) || !get(&eout, dim * 8 * (tot[1] > 0 ? tot[1] : 1)))
             return fail(CC_ERR_OUT_OF_MEMORY, "extract_surface_plic: could not allocate its output");
-        cc::launch_plic_surface<T>(gt, dim, (const T *)dn, (const T *)di, (const int64_t *)dc, n, min_frac, b, true,
-                                   (T *)vout, (int64_t *)eout, st);
+        cc::launch_plic_surface<T>(gt, dim, (const T *)dn, (const T *)di, (const int64_t *)dc, false, n, min_frac, b,
+                                   true, (T *)vout, (int64_t *)eout, st);
         HIPC(hipGetLastError());
         if (tot[0] > 0) HIPC(hipMemcpyAsync(nodes, vout, dim * es * tot[0], hipMemcpyDeviceToHost, st));
         if (tot[1] > 0) HIPC(hipMemcpyAsync(elems, eout, dim * 8 * tot[1], hipMemcpyDeviceToHost, st));

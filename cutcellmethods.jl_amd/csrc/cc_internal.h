@@ -186,8 +186,18 @@ struct PlicSurfBuffers {
 // on the device); emit == true: writes nodes (D T per vertex) and elements (D int64, 1-based).
 template <typename T>
 void launch_plic_surface(const GridT<T> &g, int dim, const T *normals, const T *intercepts, const int64_t *cells,
-                         size_t ncell, double min_frac, const PlicSurfBuffers &b, bool emit, T *nodes, int64_t *elems,
-                         hipStream_t s);
+                         bool by_cell, size_t ncell, double min_frac, const PlicSurfBuffers &b, bool emit, T *nodes,
+                         int64_t *elems, hipStream_t s);
+
+// generate_mesh's filter: bits (one per cell, (ncell + 63) / 64 words) marks each cell whose plane can
+// touch it; wrank receives each word's exclusive rank and totals[0] the number of marked cells (part
+// holds the scan's block sums, part_cap per sum).  launch_plic_marked then lists the marked cells in
+// cell order.
+template <typename T>
+void launch_plic_touch(const T *normals, const T *intercepts, size_t ncell, int dim, unsigned long long *bits,
+                       int64_t *wrank, unsigned long long *part, size_t part_cap, int64_t *totals, hipStream_t s);
+void launch_plic_marked(const unsigned long long *bits, const int64_t *wrank, size_t ncell, int64_t *cells,
+                        hipStream_t s);
 
 // Diagnostics over a cache's cells: counts[0..6] = cut, inside, outside, ambiguous, cells with a
 // nonzero closure residual, shared faces whose fractions differ, shared faces whose centroids differ;

@@ -14,6 +14,13 @@
 //       |cross|^2 < (2 min_frac min(d)^2)^2 is dropped, every vertex is kept (surface.jl:150-182).
 // Two launches over the cells -- counts, then (after prefix sums) the elements at their offsets --
 // keep the reference's cell order with no atomics: the surface is the same on every run.
+//
+// generate_mesh first marks, one bit per cell, the cells whose plane can touch the closed cell at
+// all: some corner value <= 0 and some >= 0, the corner values formed by the very expression the
+// clip uses, so every cell the clip would draw from is marked (a cell drawing nothing may be marked
+// too).  The marked cells, ranked by a popcount scan, are the entries of the count and emit
+// launches, in cell order, so the result is the all-cells result bit for bit.  Not used in 2D with a
+// zero floor, where every cell draws a (possibly zero-length) segment.
 #include "cc_internal.h"
 
 namespace cc {
@@ -26,14 +33,54 @@ template <typename T> struct Poly {
     int n;
 };
 
+constexpr int SQ[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+constexpr int CB[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+// n.xi - a at the unit cell's corners (MS_NODE_BITS / MC_NODE_BITS order): the clip and the touch
+// test both take their corner values from here.
+template <typename T> __device__ __forceinline__ void corners2(const T *nv, T a, T (&f)[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) f[k] = nv[0] * T(SQ[k][0]) + nv[1] * T(SQ[k][1]) - a;
+}
+template <typename T> __device__ __forceinline__ void corners3(const T *nv, T a, T (&f)[8]) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) f[c] = (nv[0] * T(CB[c][0]) + nv[1] * T(CB[c][1])) + nv[2] * T(CB[c][2]) - a;
+}
+
+// Whether the clip of this plane can give any point: a nonzero normal, some corner value <= 0 and
+// some >= 0 (a zero corner, or corners of both strict signs and so an edge between them).
+template <typename T, int D> __device__ __forceinline__ bool touches(const T *nv, T a) {
+    if (D == 2) {
+        if (nv[0] == T(0) && nv[1] == T(0)) return false;
+        T f[4];
+        corners2(nv, a, f);
+        bool le = false, ge = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            le |= f[k] <= T(0);
+            ge |= f[k] >= T(0);
+        }
+        return le && ge;
+    }
+    if (nv[0] == T(0) && nv[1] == T(0) && nv[2] == T(0)) return false;
+    T f[8];
+    corners3(nv, a, f);
+    bool le = false, ge = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        le |= f[k] <= T(0);
+        ge |= f[k] >= T(0);
+    }
+    return le && ge;
+}
+
 // The 2D clip: 0, 1 or 2 points in the unit square (MS_NODE_BITS corner order).
 template <typename T> __device__ __forceinline__ int clip2(const T *nv, T a, T (&xi)[2][2]) {
     xi[0][0] = xi[0][1] = xi[1][0] = xi[1][1] = T(0);
     if (nv[0] == T(0) && nv[1] == T(0)) return 0;
-    constexpr int C[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+    const int (&C)[4][2] = SQ;
     T f[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) f[k] = nv[0] * T(C[k][0]) + nv[1] * T(C[k][1]) - a;
+    corners2(nv, a, f);
     int cnt = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -89,12 +136,11 @@ __device__ __forceinline__ void polygon(const GridT<T> &g, const T *nv, T a, uin
                                         Poly<T> &P) {
     P.n = 0;
     if (nv[0] == T(0) && nv[1] == T(0) && nv[2] == T(0)) return;
-    constexpr int B[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    const int (&B)[8][3] = CB;
     constexpr int E[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
                               {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
     T f[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) f[c] = (nv[0] * T(B[c][0]) + nv[1] * T(B[c][1])) + nv[2] * T(B[c][2]) - a;
+    corners3(nv, a, f);
     T xi[6][3];
     int cnt = 0;
 #pragma unroll
@@ -170,12 +216,14 @@ template <typename T> __device__ __forceinline__ bool keep_tri(const Poly<T> &P,
     return (double)s2 >= min_cross_sq;
 }
 
-// Entry c draws plane c (normals[D c], intercepts[c]) in cell cells[c] (a 0-based linear index), or
-// in cell c when cells is null (generate_mesh: every cell, its own plane).
+// Entry c draws, in cell cells[c] (a 0-based linear index; cell c when cells is null), the plane
+// stored at index c (extract_surface_plic: the caller's planes, one per entry) or, with by_cell, the
+// plane stored at the cell's own index (generate_mesh over the marked cells).
 template <typename T, int D>
 __global__ __launch_bounds__(PS_THREADS) void k_plic_surf(GridT<T> g, const T *__restrict__ normals,
                                                           const T *__restrict__ intercepts,
-                                                          const int64_t *__restrict__ cells, size_t ncell,
+                                                          const int64_t *__restrict__ cells, bool by_cell,
+                                                          size_t ncell,
                                                           double min_len, double min_cross_sq, bool emit,
                                                           uint32_t *__restrict__ vcnt, uint32_t *__restrict__ ecnt,
                                                           const int64_t *__restrict__ vpre,
@@ -187,10 +235,12 @@ __global__ __launch_bounds__(PS_THREADS) void k_plic_surf(GridT<T> g, const T *_
     const uint32_t n0 = g.n[0], n1 = g.n[1];
     const size_t lin = cells ? (size_t)cells[c] : c;
     const uint32_t i = (uint32_t)(lin % n0), j = (uint32_t)((lin / n0) % n1), k = (uint32_t)(lin / ((size_t)n0 * n1));
-    const T *nv = normals + D * c;
+    const size_t pl = by_cell ? lin : c;
+    const T *nv = normals + D * pl;
+    const T a = intercepts[pl];
     if (D == 2) {
         T p[2][2];
-        const T len = segment(g, nv, intercepts[c], i, j, p);
+        const T len = segment(g, nv, a, i, j, p);
         const bool kept = (double)len >= min_len;
         if (!emit) {
             vcnt[c] = kept ? 2u : 0u;
@@ -206,7 +256,7 @@ __global__ __launch_bounds__(PS_THREADS) void k_plic_surf(GridT<T> g, const T *_
         }
     } else {
         Poly<T> P;
-        polygon(g, nv, intercepts[c], i, j, k, P);
+        polygon(g, nv, a, i, j, k, P);
         if (!emit) {
             uint32_t nt = 0;
             if (P.n >= 3)
@@ -232,12 +282,64 @@ __global__ __launch_bounds__(PS_THREADS) void k_plic_surf(GridT<T> g, const T *_
     }
 }
 
+// One bit per cell (bit c % 64 of word c / 64): whether cell c's plane can touch it.
+template <typename T, int D>
+__global__ __launch_bounds__(PS_THREADS) void k_plic_touch(const T *__restrict__ normals,
+                                                           const T *__restrict__ intercepts, size_t ncell,
+                                                           size_t nwords, unsigned long long *__restrict__ bits) {
+    const size_t c = (size_t)blockIdx.x * PS_THREADS + threadIdx.x;
+    bool t = false;
+    if (c < ncell) t = touches<T, D>(normals + D * c, intercepts[c]);
+    const unsigned long long m = __ballot(t);
+    const size_t w = c / 64;  // lane 0 holds a multiple of 64: the wave's word
+    if ((threadIdx.x & 63) == 0 && w < nwords) bits[w] = m;
+}
+
+// The marked cells in cell order: word w's set bits at wrank[w] onwards.
+__global__ __launch_bounds__(PS_THREADS) void k_plic_marked(const unsigned long long *__restrict__ bits,
+                                                            const int64_t *__restrict__ wrank, size_t nwords,
+                                                            int64_t *__restrict__ cells) {
+    const size_t w = (size_t)blockIdx.x * PS_THREADS + threadIdx.x;
+    if (w >= nwords) return;
+    unsigned long long m = bits[w];
+    int64_t r = wrank[w];
+    while (m) {
+        cells[r++] = (int64_t)(w * 64 + (size_t)(__ffsll((long long)m) - 1));
+        m &= m - 1;
+    }
+}
+
 }  // namespace
 
 template <typename T>
+void launch_plic_touch(const T *normals, const T *intercepts, size_t ncell, int dim, unsigned long long *bits,
+                       int64_t *wrank, unsigned long long *part, size_t part_cap, int64_t *totals, hipStream_t s) {
+    const size_t nwords = (ncell + 63) / 64;
+    const dim3 grid((unsigned)((ncell + PS_THREADS - 1) / PS_THREADS));
+    if (dim == 3)
+        hipLaunchKernelGGL((k_plic_touch<T, 3>), grid, dim3(PS_THREADS), 0, s, normals, intercepts, ncell, nwords, bits);
+    else
+        hipLaunchKernelGGL((k_plic_touch<T, 2>), grid, dim3(PS_THREADS), 0, s, normals, intercepts, ncell, nwords, bits);
+    launch_scan_pair({nullptr, bits, wrank, nullptr, nwords}, {nullptr, nullptr, nullptr, nullptr, 0}, part, part_cap,
+                     totals, s);
+}
+
+void launch_plic_marked(const unsigned long long *bits, const int64_t *wrank, size_t ncell, int64_t *cells,
+                        hipStream_t s) {
+    const size_t nwords = (ncell + 63) / 64;
+    hipLaunchKernelGGL(k_plic_marked, dim3((unsigned)((nwords + PS_THREADS - 1) / PS_THREADS)), dim3(PS_THREADS), 0, s,
+                       bits, wrank, nwords, cells);
+}
+
+template void launch_plic_touch<double>(const double *, const double *, size_t, int, unsigned long long *, int64_t *,
+                                        unsigned long long *, size_t, int64_t *, hipStream_t);
+template void launch_plic_touch<float>(const float *, const float *, size_t, int, unsigned long long *, int64_t *,
+                                       unsigned long long *, size_t, int64_t *, hipStream_t);
+
+template <typename T>
 void launch_plic_surface(const GridT<T> &g, int dim, const T *normals, const T *intercepts, const int64_t *cells,
-                         size_t ncell, double min_frac, const PlicSurfBuffers &b, bool emit, T *nodes, int64_t *elems,
-                         hipStream_t s) {
+                         bool by_cell, size_t ncell, double min_frac, const PlicSurfBuffers &b, bool emit, T *nodes,
+                         int64_t *elems, hipStream_t s) {
     if (ncell == 0) {
         if (!emit) launch_scan_pair({b.vcnt, nullptr, b.vpre, nullptr, 0}, {b.ecnt, nullptr, b.epre, nullptr, 0},
                                     b.part, b.part_cap, b.totals, s);
@@ -250,11 +352,11 @@ void launch_plic_surface(const GridT<T> &g, int dim, const T *normals, const T *
     const double min_cross_sq = (2.0 * min_area) * (2.0 * min_area);
     const dim3 grid((unsigned)((ncell + PS_THREADS - 1) / PS_THREADS));
     if (dim == 3)
-        hipLaunchKernelGGL((k_plic_surf<T, 3>), grid, dim3(PS_THREADS), 0, s, g, normals, intercepts, cells, ncell, min_len,
+        hipLaunchKernelGGL((k_plic_surf<T, 3>), grid, dim3(PS_THREADS), 0, s, g, normals, intercepts, cells, by_cell, ncell, min_len,
                            min_cross_sq, emit, b.vcnt, b.ecnt, (const int64_t *)b.vpre, (const int64_t *)b.epre,
                            nodes, elems);
     else
-        hipLaunchKernelGGL((k_plic_surf<T, 2>), grid, dim3(PS_THREADS), 0, s, g, normals, intercepts, cells, ncell, min_len,
+        hipLaunchKernelGGL((k_plic_surf<T, 2>), grid, dim3(PS_THREADS), 0, s, g, normals, intercepts, cells, by_cell, ncell, min_len,
                            min_cross_sq, emit, b.vcnt, b.ecnt, (const int64_t *)b.vpre, (const int64_t *)b.epre,
                            nodes, elems);
     if (!emit)
@@ -263,10 +365,10 @@ void launch_plic_surface(const GridT<T> &g, int dim, const T *normals, const T *
 }
 
 template void launch_plic_surface<double>(const GridT<double> &, int, const double *, const double *,
-                                          const int64_t *, size_t, double, const PlicSurfBuffers &, bool, double *,
-                                          int64_t *, hipStream_t);
+                                          const int64_t *, bool, size_t, double, const PlicSurfBuffers &, bool,
+                                          double *, int64_t *, hipStream_t);
 template void launch_plic_surface<float>(const GridT<float> &, int, const float *, const float *, const int64_t *,
-                                         size_t, double, const PlicSurfBuffers &, bool, float *, int64_t *,
+                                         bool, size_t, double, const PlicSurfBuffers &, bool, float *, int64_t *,
                                          hipStream_t);
 
 }  // namespace cc

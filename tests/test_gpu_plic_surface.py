@@ -81,6 +81,23 @@ def test_3d_sphere_golden_torus_box_union_bitwise():
         check_surface(g2, geo, fracs=(1e-4, 0.0))
 
 
+def test_3d_planes_on_lattice_faces_and_nodes():
+    # a plane lying on a lattice face: the cells beside it are not cut (fraction 0 or 1) yet their
+    # planes run along a face of theirs and are drawn; a plane through lattice nodes touches cells at
+    # corners and edges only.  generate_mesh's cell filter must keep every such cell.
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=16)
+    for nrm, pt, on_faces in (((0.0, 0.0, 1.0), (0.0, 0.0, 0.0), True), ((0.0, 1.0, 0.0), (0.0, 0.25, 0.0), True),
+                              ((1.0, 1.0, 1.0), (0.0, 0.0, 0.0), False),
+                              ((1.0, -1.0, 0.0), (0.125, 0.0, 0.0), False)):
+        c, cols, out = check_surface(g, SDFPlane(nrm, pt), fracs=(1e-4, 0.0))
+        nodes, el = out[0.0]
+        if on_faces:  # cells that are not cut draw too
+            assert len(el) > int((cols["kind"] == 2).sum())
+    c, cols, out = check_surface(g, SDFPlane((0.0, 0.0, 1.0), (0.0, 0.0, 0.0)), fracs=(1e-4,))
+    nodes, el = out[1e-4]
+    assert int((cols["kind"] == 2).sum()) == 0 and len(el) == 2 * 2 * 16 * 16 and np.all(nodes[:, 2] == 0.0)
+
+
 def test_calc_volume_against_the_oracle_and_the_cache():
     rng = np.random.default_rng(11)
     cases = [(CartesianIsoGrid((-1.5,) * 3, (1.5,) * 3, sz=0.15), SDFSphere((0.0, 0.0, 0.0), 1.0)),
