@@ -1006,6 +1006,10 @@ int cc_cache_create(int method, int dtype, const cc_grid *grid, cc_cache **out) 
     for (int a = 0; a < dim; ++a) c->n[a] = grid->n[a];
     c->ncell = ncell;
     c->nnode = nnode;
+    // The fused PLIC sweep fits the layer below and above each work item's z-range again, so it
+    // marches deeper than the nodal stream pass (measured on the 384^3 torus: kz 4 1.67 ms, 16
+    // 1.48 ms, 32 1.46 ms, 64 1.48 ms).
+    if (method == CC_PLIC) c->tune.kz = 32;
     (void)hipGetDevice(&c->device);
     int rc = check_grid(c, grid);
     if (rc) { delete c; return rc; }
