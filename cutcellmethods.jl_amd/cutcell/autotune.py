@@ -82,16 +82,20 @@ def tune_stream(cache, geo, grid, candidates=None, repeats=3, use_cache=True):
 
 
 def tune_batch(batch, geos, grids, kz_candidates=(2, 4, 8, 16, 32), classes=3, repeats=2):
-    """Per size class of the batch's marching-cubes bodies, the slab depth that minimises the
-    graph's device time (coordinate descent over the classes).  Returns the chosen kz per body."""
-    from .cache import MarchingCubesCutCell
+    """Per size class of the batch's marching-cubes bodies, and for its PLIC bodies as one more class,
+    the slab depth that minimises the graph's device time (coordinate descent over the classes).
+    Returns the chosen kz per body."""
+    from .cache import MarchingCubesCutCell, PLICCutCell
     sizes = np.array([c.grid.ncells for c in batch.caches])
     mc = [i for i, c in enumerate(batch.caches) if c.method is MarchingCubesCutCell]
+    plic = [i for i, c in enumerate(batch.caches) if c.method is PLICCutCell]
     kz = {i: 8 for i in mc}
-    if not mc:
+    kz.update({i: 4 for i in plic})
+    if not kz:
         return kz
-    edges = np.quantile(sizes[mc], np.linspace(0, 1, classes + 1)[1:-1]) if classes > 1 else []
+    edges = np.quantile(sizes[mc], np.linspace(0, 1, classes + 1)[1:-1]) if classes > 1 and mc else []
     cls = {i: int(np.searchsorted(edges, sizes[i], side="right")) for i in mc}
+    cls.update({i: classes for i in plic})
 
     def timed():
         ts = []
@@ -99,11 +103,11 @@ def tune_batch(batch, geos, grids, kz_candidates=(2, 4, 8, 16, 32), classes=3, r
             ts.append(batch.step(geos, grids))
         return min(ts[1:])
 
-    for i in mc:
+    for i in kz:
         batch.set_kz(i, kz[i])
     best_t = timed()
-    for c in range(classes):
-        members = [i for i in mc if cls[i] == c]
+    for c in range(classes + 1):
+        members = [i for i in kz if cls[i] == c]
         if not members:
             continue
         best_k = kz[members[0]]

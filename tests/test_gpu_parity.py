@@ -329,6 +329,18 @@ def test_batched_bodies_match_single_updates_and_oracle():
     assert_cells_equal(caches[b], orc.mc_cells(orc.sample_nodes(wl.geometry(b, 2), 3, wl.grid(b, 2).x0,
                                                                 wl.grid(b, 2).d, wl.grid(b, 2).n),
                                                wl.grid(b, 2).x0, wl.grid(b, 2).d, wl.grid(b, 2).n))
+    # the batch autotuner gives every body a depth (the PLIC bodies as a class of their own); the
+    # tuned step is still what each body gets alone
+    from cutcell.autotune import tune_batch
+    geos = [wl.geometry(i, 3) for i in range(6)]
+    grids = [wl.grid(i, 3) for i in range(6)]
+    kz = tune_batch(batch, geos, grids, kz_candidates=(2, 32), classes=2, repeats=1)
+    assert sorted(kz) == list(range(6)) and all(k in (2, 4, 8, 32) for k in kz.values())
+    batch.step(geos, grids)
+    for i in range(6):
+        update_cache(solo[i], geos[i], grids[i])
+        for name in COLS:
+            assert same_bits(caches[i].read(name), solo[i].read(name)), (i, name)
 
 
 def test_autotuner_picks_a_shape_and_results_stay_identical(tmp_path, monkeypatch):
