@@ -26,12 +26,12 @@ __global__ __launch_bounds__(256) void k_ms_cells(const T *__restrict__ phi, Gri
         const T nx[4] = {xl, xh, xh, xl}, ny[4] = {yl, yl, yh, yh};
         cut = ms_cell(nx, ny, ph, g.d[0], g.d[1], cols, c);
     }
-    const uint64_t m = __ballot(cut);
-    if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(counter, (uint32_t)__popcll(m));
+    add_block_cuts(cut ? 1u : 0u, counter);
 }
 
-// _leaf_cells_kernel! (src/marching_squares/cache.jl:129-137): one thread per quadtree leaf.  Its
-// corners come off the integer lattice, x0 + T(coord + bit) h with h = d / 2^level
+// _leaf_cells_kernel! (src/marching_squares/cache.jl:129-137): a thread per quadtree leaf (a
+// grid-stride loop past 2048 blocks, so the cut count takes at most 2048 atomics).  Its corners
+// come off the integer lattice, x0 + T(coord + bit) h with h = d / 2^level
 // (cell_nodes(mesh, c), marching_squares.jl:113-121): a corner two leaves share across a level jump
 // is then one product, (2a)(h/2) = a h exactly, so both sample the body at the same point.  The
 // values are sampled there (cell_values(field, mesh, c), :152-155) and kept, four per leaf -- or,
@@ -41,9 +41,8 @@ __global__ __launch_bounds__(256) void k_leaf_cells(Geo geo, const int32_t *__re
                                                     const int32_t *__restrict__ coord, uint32_t n, T x0x, T x0y,
                                                     T dx, T dy, T *__restrict__ phi4, Cols<T> cols,
                                                     uint32_t *__restrict__ counter) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    bool cut = false;
-    if (c < n) {
+    uint32_t cuts = 0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         const int lv = level[c];
         const T hx = ldexp(dx, -lv), hy = ldexp(dy, -lv);  // exact: a power-of-two scaling
         const int cx = coord[2 * (size_t)c], cy = coord[2 * (size_t)c + 1];
@@ -61,10 +60,9 @@ __global__ __launch_bounds__(256) void k_leaf_cells(Geo geo, const int32_t *__re
                 ph[k] = phi4[4 * (size_t)c + k];
             }
         }
-        cut = ms_cell(nx, ny, ph, hx, hy, cols, c);
+        cuts += ms_cell(nx, ny, ph, hx, hy, cols, c) ? 1u : 0u;
     }
-    const uint64_t m = __ballot(cut);
-    if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(counter, (uint32_t)__popcll(m));
+    add_block_cuts(cuts, counter);
 }
 
 }  // namespace
@@ -81,7 +79,8 @@ template <typename T>
 void launch_leaf_cells(const Geo *geo, const int32_t *level, const int32_t *coord, uint32_t n, const double *x0,
                        const double *d, T *phi4, const Cols<T> &cols, uint32_t *counter, hipStream_t s) {
     if (n == 0) return;
-    const dim3 grid((n + 255) / 256);
+    const uint32_t blocks = (n + 255) / 256;
+    const dim3 grid(blocks < 2048 ? blocks : 2048);  // leaves beyond one per thread: a grid-stride loop
     if (geo)
         hipLaunchKernelGGL((k_leaf_cells<T, true>), grid, dim3(256), 0, s, *geo, level, coord, n, T(x0[0]),
                            T(x0[1]), T(d[0]), T(d[1]), phi4, cols, counter);

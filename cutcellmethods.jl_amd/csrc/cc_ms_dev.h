@@ -143,4 +143,18 @@ __device__ __forceinline__ bool ms_cell(const T (&nx)[4], const T (&ny)[4], cons
     return cut;
 }
 
+// Adds the block's cut cells to the cache's counter with one atomic per block (256 threads, every
+// thread arriving): a counter taking one atomic per wave serialises them at one L2 address.
+__device__ __forceinline__ void add_block_cuts(uint32_t v, uint32_t *counter) {
+    __shared__ uint32_t per_wave[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) per_wave[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = (per_wave[0] + per_wave[1]) + (per_wave[2] + per_wave[3]);
+        if (t) atomicAdd(counter, t);
+    }
+}
+
 }  // namespace cc
