@@ -134,12 +134,14 @@ struct cc_cache {
     cc::PlicSurfBuffers ps = {};
     std::vector<void *> ps_allocs;
     int64_t ps_cap = 0;  // entries the PLIC surface's work arrays hold
-    unsigned long long *pt_bits = nullptr;  // generate_mesh's touch bitmap, its word ranks, the marked cells
-    int64_t pt_bits_cap = 0;
-    int64_t *pt_rank = nullptr;
-    int64_t pt_rank_cap = 0;
-    int64_t *pt_cells = nullptr;
-    int64_t pt_cells_cap = 0;
+    // generate_mesh's cell marks (the PLIC cells a plane can touch, the cut leaves of a tree): one
+    // bit per cell, each word's rank, the marked cells in order
+    unsigned long long *mark_bits = nullptr;
+    int64_t mark_bits_cap = 0;
+    int64_t *mark_rank = nullptr;
+    int64_t mark_rank_cap = 0;
+    int64_t *mark_cells = nullptr;
+    int64_t mark_cells_cap = 0;
     // diagnostics: counters, block sums, totals (device), and a pinned copy of the results
     void *dg = nullptr;
     void *dg_host = nullptr;
@@ -148,6 +150,22 @@ struct cc_cache {
     int32_t *leaf_level = nullptr, *leaf_coord = nullptr;  // device copies of a host leaf list
     const void *leaf_src[2] = {};  // the host arrays and version tag they were copied from
     int32_t leaf_version = 0;
+    // the last tree contour, for the fill call of generate_mesh's two calls: valid for this update
+    // (gen) and this tagged tree (arrays, count, version, base grid)
+    std::vector<double> tree_nodes;
+    std::vector<int64_t> tree_lines;
+    bool tree_mesh_ok = false;
+    uint64_t tree_mesh_gen = 0;
+    const void *tree_mesh_src[2] = {};
+    int64_t tree_mesh_n = -1;
+    int32_t tree_mesh_version = 0;
+    double tree_mesh_frame[4] = {};
+    int64_t *mark_idx = nullptr;  // the cut leaves gathered for the tree contour
+    int64_t mark_idx_cap = 0;
+    char *mark_phi = nullptr;
+    int64_t mark_phi_cap = 0;
+    int32_t *mark_lc = nullptr;
+    int64_t mark_lc_cap = 0;
 };
 
 namespace {
@@ -634,8 +652,9 @@ int update_leaves_impl(cc_cache *c, const cc_geometry *geo, const void *phi4, in
 
 // The PLIC surface (src/plic/surface.jl:28-267): count every cell, prefix sums, one round trip for
 // the sizes, then every node and element at its offset.
-// The PLIC surface's per-entry work arrays for n entries, grown on demand (contents not kept).
-int plic_work(cc_cache *c, int64_t n) {
+// The surface passes' per-entry work arrays (counts, their prefix sums, the scans' block sums and
+// totals) for n entries, grown on demand (contents not kept).
+int surface_work(cc_cache *c, int64_t n) {
     if (n < 1) n = 1;
     if (c->ps.vcnt && n <= c->ps_cap) return CC_OK;
     for (void *p : c->ps_allocs) (void)hipFree(p);
@@ -655,7 +674,7 @@ int plic_work(cc_cache *c, int64_t n) {
         for (void *p : c->ps_allocs) (void)hipFree(p);
         c->ps_allocs.clear();
         c->ps = {};
-        return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the PLIC surface's work arrays");
+        return fail(CC_ERR_OUT_OF_MEMORY, "could not allocate the surface work arrays");
     }
     c->ps_cap = n;
     return CC_OK;
@@ -679,11 +698,11 @@ template <typename T> int plic_surface(cc_cache *c, const cc_grid *g, double min
     int rc;
     if (filter) {
         const int64_t nwords = (ncell + 63) / 64;
-        rc = ensure(&c->pt_bits, &c->pt_bits_cap, nwords);
-        if (rc == CC_OK) rc = ensure(&c->pt_rank, &c->pt_rank_cap, nwords);
-        if (rc == CC_OK) rc = plic_work(c, nwords);
+        rc = ensure(&c->mark_bits, &c->mark_bits_cap, nwords);
+        if (rc == CC_OK) rc = ensure(&c->mark_rank, &c->mark_rank_cap, nwords);
+        if (rc == CC_OK) rc = surface_work(c, nwords);
         if (rc) return rc;
-        cc::launch_plic_touch<T>(nrm, icp, (size_t)ncell, c->dim, c->pt_bits, c->pt_rank, c->ps.part, c->ps.part_cap,
+        cc::launch_plic_touch<T>(nrm, icp, (size_t)ncell, c->dim, c->mark_bits, c->mark_rank, c->ps.part, c->ps.part_cap,
                                  c->ps.totals, c->stream);
         mark(c, "plic_surface_touch");
         HIPC(hipGetLastError());
@@ -693,15 +712,15 @@ template <typename T> int plic_surface(cc_cache *c, const cc_grid *g, double min
         n = marked[0];
         if (n < 0 || n > ncell) return fail(CC_ERR_HIP, "PLIC surface: %lld cells marked of %lld", (long long)n,
                                             (long long)ncell);
-        rc = ensure(&c->pt_cells, &c->pt_cells_cap, n > 0 ? n : 1);
-        if (rc == CC_OK) rc = plic_work(c, n);
+        rc = ensure(&c->mark_cells, &c->mark_cells_cap, n > 0 ? n : 1);
+        if (rc == CC_OK) rc = surface_work(c, n);
         if (rc) return rc;
         mark(c, "plic_surface_host_sync0");  // the round trip above: host time, not a kernel
-        if (n > 0) cc::launch_plic_marked(c->pt_bits, c->pt_rank, (size_t)ncell, c->pt_cells, c->stream);
+        if (n > 0) cc::launch_plic_marked(c->mark_bits, c->mark_rank, (size_t)ncell, c->mark_cells, c->stream);
         mark(c, "plic_surface_marked");
-        cells = c->pt_cells;
+        cells = c->mark_cells;
     } else {
-        rc = plic_work(c, ncell);
+        rc = surface_work(c, ncell);
         if (rc) return rc;
     }
     cc::launch_plic_surface<T>(gt, c->dim, nrm, icp, cells, true, (size_t)n, min_frac, c->ps, false, nullptr, nullptr,
@@ -819,9 +838,12 @@ void free_all(cc_cache *c) {
     if (c->mverts) (void)hipFree(c->mverts);
     if (c->mtris) (void)hipFree(c->mtris);
     if (c->mcells) (void)hipFree(c->mcells);
-    if (c->pt_bits) (void)hipFree(c->pt_bits);
-    if (c->pt_rank) (void)hipFree(c->pt_rank);
-    if (c->pt_cells) (void)hipFree(c->pt_cells);
+    if (c->mark_bits) (void)hipFree(c->mark_bits);
+    if (c->mark_rank) (void)hipFree(c->mark_rank);
+    if (c->mark_cells) (void)hipFree(c->mark_cells);
+    if (c->mark_idx) (void)hipFree(c->mark_idx);
+    if (c->mark_phi) (void)hipFree(c->mark_phi);
+    if (c->mark_lc) (void)hipFree(c->mark_lc);
     if (c->leaf_level) (void)hipFree(c->leaf_level);
     if (c->leaf_coord) (void)hipFree(c->leaf_coord);
     for (auto &p : c->col)
@@ -927,23 +949,71 @@ void contour(const std::vector<T> &phi, const std::vector<int8_t> &kind, const c
 }
 
 // The tree contour: cut leaves in leaf order, each from its own four stored corner values
-// (surface.jl:92-110), corners off the lattice as the leaf pass places them.
+// (surface.jl:92-110), corners off the lattice as the leaf pass places them.  Entry r is leaf idx[r],
+// its corner values phi[4r..4r+3], its level and lattice corner lc[3r..3r+2] (or, lc null, from the
+// host leaf list).
 template <typename T>
-void contour_leaves(const std::vector<T> &phi4, const std::vector<int8_t> &kind, const std::vector<int32_t> &level,
-                    const std::vector<int32_t> &coord, const cc_leaves *t, std::vector<double> &nodes,
-                    std::vector<int64_t> &lines) {
+void contour_leaves(int64_t ncut, const int64_t *idx, const T *phi, const int32_t *lc, const cc_leaves *t,
+                    std::vector<double> &nodes, std::vector<int64_t> &lines) {
     Welder vid(nodes);
+    vid.ids.reserve((size_t)(2 * ncut));
+    nodes.reserve((size_t)(4 * ncut));
+    lines.reserve((size_t)(2 * ncut));
     const T x0 = T(t->x0[0]), y0 = T(t->x0[1]), dx = T(t->d[0]), dy = T(t->d[1]);
-    for (int64_t c = 0; c < t->n; ++c) {
-        if (kind[c] != CC_CELL_CUT) continue;
-        const T hx = std::ldexp(dx, -level[c]), hy = std::ldexp(dy, -level[c]);
-        const int cx = coord[2 * c], cy = coord[2 * c + 1];
+    for (int64_t r = 0; r < ncut; ++r) {
+        const int64_t c = idx[r];
+        const int lv = lc ? lc[3 * r] : t->level[c];
+        const int cx = lc ? lc[3 * r + 1] : t->coord[2 * c], cy = lc ? lc[3 * r + 2] : t->coord[2 * c + 1];
+        const T hx = std::ldexp(dx, -lv), hy = std::ldexp(dy, -lv);
         const T xl = x0 + T(cx) * hx, xh = x0 + T(cx + 1) * hx;
         const T yl = y0 + T(cy) * hy, yh = y0 + T(cy + 1) * hy;
         const T nx[4] = {xl, xh, xh, xl}, ny[4] = {yl, yl, yh, yh};
-        const T ph[4] = {phi4[4 * c], phi4[4 * c + 1], phi4[4 * c + 2], phi4[4 * c + 3]};
+        const T ph[4] = {phi[4 * r], phi[4 * r + 1], phi[4 * r + 2], phi[4 * r + 3]};
         cell_segments(nx, ny, ph, vid, lines);
     }
+}
+
+// Marks, ranks and gathers the cut leaves on the device, copies them to the host and contours them
+// into the cache's kept tree contour.
+template <typename T> int tree_contour(cc_cache *c, const cc_leaves *tree) {
+    const int64_t n = tree->n, nwords = (n + 63) / 64;
+    int rc = ensure(&c->mark_bits, &c->mark_bits_cap, nwords);
+    if (rc == CC_OK) rc = ensure(&c->mark_rank, &c->mark_rank_cap, nwords);
+    if (rc == CC_OK) rc = surface_work(c, nwords);
+    if (rc) return rc;
+    cc::launch_leaf_marks((const int8_t *)c->col[CC_FIELD_KIND], (size_t)n, c->mark_bits, c->mark_rank, c->ps.part,
+                          c->ps.part_cap, c->ps.totals, c->stream);
+    HIPC(hipGetLastError());
+    int64_t tot[2] = {0, 0};
+    HIPC(hipMemcpyAsync(tot, c->ps.totals, sizeof tot, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    const int64_t ncut = tot[0];
+    if (ncut < 0 || ncut > n) return fail(CC_ERR_HIP, "tree contour: %lld cut leaves of %lld", (long long)ncut,
+                                          (long long)n);
+    const int64_t m = ncut > 0 ? ncut : 1;
+    rc = ensure(&c->mark_idx, &c->mark_idx_cap, m);
+    if (rc == CC_OK) rc = ensure(&c->mark_phi, &c->mark_phi_cap, 4 * (int64_t)sizeof(T) * m);
+    if (rc == CC_OK && tree->on_device) rc = ensure(&c->mark_lc, &c->mark_lc_cap, 3 * m);
+    if (rc) return rc;
+    std::vector<int64_t> idx(m);
+    std::vector<T> phi(4 * m);
+    std::vector<int32_t> lc(tree->on_device ? 3 * m : 0);
+    if (ncut > 0) {
+        cc::launch_leaf_gather<T>(c->mark_bits, c->mark_rank, (size_t)n, (const T *)c->col[CC_FIELD_PHI],
+                                  tree->on_device ? tree->level : nullptr, tree->on_device ? tree->coord : nullptr,
+                                  c->mark_idx, (T *)c->mark_phi, c->mark_lc, c->stream);
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(idx.data(), c->mark_idx, sizeof(int64_t) * ncut, hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipMemcpyAsync(phi.data(), c->mark_phi, 4 * sizeof(T) * ncut, hipMemcpyDeviceToHost, c->stream));
+        if (tree->on_device)
+            HIPC(hipMemcpyAsync(lc.data(), c->mark_lc, 3 * sizeof(int32_t) * ncut, hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+    }
+    c->tree_nodes.clear();
+    c->tree_lines.clear();
+    contour_leaves<T>(ncut, idx.data(), phi.data(), tree->on_device ? lc.data() : nullptr, tree, c->tree_nodes,
+                      c->tree_lines);
+    return CC_OK;
 }
 
 }  // namespace
@@ -1368,27 +1438,25 @@ int cc_generate_mesh_leaves(cc_cache *c, const cc_leaves *tree, double *nodes, i
     HIPC(hipSetDevice(c->device));
     HIPC(hipStreamSynchronize(c->stream));
     const int64_t n = tree->n;
-    std::vector<int8_t> kind(n);
-    std::vector<int32_t> level(n), coord(2 * n);
-    HIPC(hipMemcpy(kind.data(), c->col[CC_FIELD_KIND], n, hipMemcpyDeviceToHost));
-    if (tree->on_device) {
-        HIPC(hipMemcpy(level.data(), tree->level, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-        HIPC(hipMemcpy(coord.data(), tree->coord, 2 * sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-    } else {
-        std::memcpy(level.data(), tree->level, sizeof(int32_t) * n);
-        std::memcpy(coord.data(), tree->coord, 2 * sizeof(int32_t) * n);
+    const double frame[4] = {tree->x0[0], tree->x0[1], tree->d[0], tree->d[1]};
+    const bool reuse = c->tree_mesh_ok && c->tree_mesh_gen == c->gen && tree->version != 0 &&
+                       tree->version == c->tree_mesh_version && n == c->tree_mesh_n &&
+                       c->tree_mesh_src[0] == tree->level && c->tree_mesh_src[1] == tree->coord &&
+                       std::memcmp(frame, c->tree_mesh_frame, sizeof frame) == 0;
+    if (!reuse) {
+        c->tree_mesh_ok = false;
+        rc = c->dtype == CC_FLOAT32 ? tree_contour<float>(c, tree) : tree_contour<double>(c, tree);
+        if (rc) return rc;
+        c->tree_mesh_ok = true;
+        c->tree_mesh_gen = c->gen;
+        c->tree_mesh_src[0] = tree->level;
+        c->tree_mesh_src[1] = tree->coord;
+        c->tree_mesh_n = n;
+        c->tree_mesh_version = tree->version;
+        std::memcpy(c->tree_mesh_frame, frame, sizeof frame);
     }
-    std::vector<double> vn;
-    std::vector<int64_t> vl;
-    if (c->dtype == CC_FLOAT32) {
-        std::vector<float> phi4(4 * n);
-        HIPC(hipMemcpy(phi4.data(), c->col[CC_FIELD_PHI], c->colbytes[CC_FIELD_PHI], hipMemcpyDeviceToHost));
-        contour_leaves<float>(phi4, kind, level, coord, tree, vn, vl);
-    } else {
-        std::vector<double> phi4(4 * n);
-        HIPC(hipMemcpy(phi4.data(), c->col[CC_FIELD_PHI], c->colbytes[CC_FIELD_PHI], hipMemcpyDeviceToHost));
-        contour_leaves<double>(phi4, kind, level, coord, tree, vn, vl);
-    }
+    const std::vector<double> &vn = c->tree_nodes;
+    const std::vector<int64_t> &vl = c->tree_lines;
     const int64_t nn = (int64_t)vn.size() / 2, nl = (int64_t)vl.size() / 2;
     if (nodes && lines) {
         if (*nnodes < nn || *nlines < nl)

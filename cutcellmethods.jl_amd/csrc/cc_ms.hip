@@ -65,7 +65,68 @@ __global__ __launch_bounds__(256) void k_leaf_cells(Geo geo, const int32_t *__re
     add_block_cuts(cuts, counter);
 }
 
+// generate_mesh over a tree (src/marching_squares/surface.jl:92-110) draws only the cut leaves: one
+// bit per leaf, ranked by a popcount scan, then each cut leaf's index and four corner values (and,
+// for a tree held on the device, its level and lattice corner) gathered in leaf order, so only they
+// cross to the host contour.
+__global__ __launch_bounds__(256) void k_leaf_cut_bits(const int8_t *__restrict__ kind, size_t n, size_t nwords,
+                                                       unsigned long long *__restrict__ bits) {
+    const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const bool cut = c < n && kind[c] == CELL_CUT;
+    const unsigned long long m = __ballot(cut);
+    const size_t w = c / 64;  // lane 0 holds a multiple of 64: the wave's word
+    if ((threadIdx.x & 63) == 0 && w < nwords) bits[w] = m;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_leaf_gather(const unsigned long long *__restrict__ bits,
+                                                     const int64_t *__restrict__ wrank, size_t nwords,
+                                                     const T *__restrict__ phi4, const int32_t *__restrict__ level,
+                                                     const int32_t *__restrict__ coord, int64_t *__restrict__ idx,
+                                                     T *__restrict__ phi_out, int32_t *__restrict__ lc_out) {
+    const size_t w = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (w >= nwords) return;
+    unsigned long long m = bits[w];
+    int64_t r = wrank[w];
+    while (m) {
+        const size_t c = w * 64 + (size_t)(__ffsll((long long)m) - 1);
+        idx[r] = (int64_t)c;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) phi_out[4 * r + q] = phi4[4 * c + q];
+        if (level) {
+            lc_out[3 * r] = level[c];
+            lc_out[3 * r + 1] = coord[2 * c];
+            lc_out[3 * r + 2] = coord[2 * c + 1];
+        }
+        ++r;
+        m &= m - 1;
+    }
+}
+
 }  // namespace
+
+void launch_leaf_marks(const int8_t *kind, size_t n, unsigned long long *bits, int64_t *wrank,
+                       unsigned long long *part, size_t part_cap, int64_t *totals, hipStream_t s) {
+    const size_t nwords = (n + 63) / 64;
+    hipLaunchKernelGGL(k_leaf_cut_bits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, kind, n, nwords, bits);
+    launch_scan_pair({nullptr, bits, wrank, nullptr, nwords}, {nullptr, nullptr, nullptr, nullptr, 0}, part, part_cap,
+                     totals, s);
+}
+
+template <typename T>
+void launch_leaf_gather(const unsigned long long *bits, const int64_t *wrank, size_t n, const T *phi4,
+                        const int32_t *level, const int32_t *coord, int64_t *idx, T *phi_out, int32_t *lc_out,
+                        hipStream_t s) {
+    const size_t nwords = (n + 63) / 64;
+    hipLaunchKernelGGL((k_leaf_gather<T>), dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, bits, wrank,
+                       nwords, phi4, level, coord, idx, phi_out, lc_out);
+}
+
+template void launch_leaf_gather<double>(const unsigned long long *, const int64_t *, size_t, const double *,
+                                         const int32_t *, const int32_t *, int64_t *, double *, int32_t *,
+                                         hipStream_t);
+template void launch_leaf_gather<float>(const unsigned long long *, const int64_t *, size_t, const float *,
+                                        const int32_t *, const int32_t *, int64_t *, float *, int32_t *, hipStream_t);
 
 template <typename T>
 void launch_ms_cells(const T *phi, const GridT<T> &g, const Cols<T> &cols, uint32_t *counter,

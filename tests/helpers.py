@@ -166,3 +166,31 @@ def same_level_neighbours(mesh):
             if j is not None:
                 out.append((i, j, ax))
     return out
+
+
+class DeviceArray:
+    """A device copy of a host array, made through the HIP runtime the library itself links (so its
+    pointer is valid in the library's kernels).  For tests of the entry points that take device
+    pointers."""
+
+    def __init__(self, arr):
+        import ctypes
+        import cutcell._lib as L
+        L.lib()  # the library, and with it its HIP runtime, loaded first
+        self._hip = hip = ctypes.CDLL("libamdhip64.so.7")
+        hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        hip.hipFree.argtypes = [ctypes.c_void_p]
+        a = np.ascontiguousarray(arr)
+        self.ptr = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(self.ptr), max(a.nbytes, 8)) == 0
+        assert hip.hipMemcpy(self.ptr, ctypes.c_void_p(a.ctypes.data), a.nbytes, 1) == 0  # host to device
+        assert hip.hipDeviceSynchronize() == 0
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            self._hip.hipFree(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        self.free()

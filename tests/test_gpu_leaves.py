@@ -15,7 +15,7 @@ from cutcell import (CartesianGrid, DimensionMismatch, MarchingCubesCutCell, Mar
                      SDFCircle, SDFPlane, SDFSphereUnion, Unsupported, allocate_cache, generate_mesh,
                      readers, update_cache)
 from cutcell.tree import AdaptiveMesh
-from helpers import (CACHE_CENTRE, CACHE_RADIUS, GEO_CENTRE, GEO_RADIUS, banded_tree, cache_tree,
+from helpers import (CACHE_CENTRE, CACHE_RADIUS, GEO_CENTRE, GEO_RADIUS, DeviceArray, banded_tree, cache_tree,
                      circle_sdf, same_level_neighbours)
 
 pytestmark = pytest.mark.gpu
@@ -124,6 +124,45 @@ def test_tree_invariants_and_contour():
     nrm = np.stack([v[:, 1], -v[:, 0]], axis=1)
     out = 0.5 * (p + q) - np.asarray(GEO_CENTRE)
     assert np.all(np.einsum("ij,ij->i", nrm, out) > 0)
+    # the library keeps a contour between the size and fill calls: a later update, or a refined
+    # tree, must draw afresh
+    update_cache(cache, SDFCircle((0.2, 0.1), 0.7), mesh)
+    n2, l2 = generate_mesh(cache, mesh)
+    rn, rl = contour_reference(cache, mesh)
+    assert np.array_equal(n2, rn) and np.array_equal(l2, rl) and not np.array_equal(n2[:10], nodes[:10])
+    mesh.refine(np.abs(circle_sdf(mesh.elem_centroid(), (0.2, 0.1), 0.7)) < 0.05)
+    cache3 = update_cache(allocate_cache(mesh, MarchingSquaresCutCell), SDFCircle((0.2, 0.1), 0.7), mesh)
+    n3, l3 = generate_mesh(cache3, mesh)
+    rn, rl = contour_reference(cache3, mesh)
+    assert np.array_equal(n3, rn) and np.array_equal(l3, rl) and len(l3) > len(l2)
+
+
+def test_tree_held_on_the_device():
+    # the leaf list as device arrays (on_device = 1, the reference's tree adapted to the device): the
+    # update and the contour are the host tree's bit for bit
+    import ctypes
+    import cutcell._lib as L
+    mesh = banded_tree()
+    geo = SDFCircle(GEO_CENTRE, GEO_RADIUS)
+    host = update_cache(allocate_cache(mesh, MarchingSquaresCutCell), geo, mesh)
+    hn, hl = generate_mesh(host, mesh)
+    lv, co = DeviceArray(mesh.level.astype(np.int32)), DeviceArray(mesh.coord.astype(np.int32))
+    t = mesh.leaves()
+    t.level, t.coord, t.on_device, t.version = lv.ptr, co.ptr, 1, 0
+    dev = allocate_cache(mesh, MarchingSquaresCutCell)
+    L.check(L.lib().cc_update_leaves(dev.handle, ctypes.byref(L.make_geometry(geo)), ctypes.byref(t)))
+    for name in ("kind", "ambiguous", "volume_fraction", "centroid", "face_fraction", "face_centroid_local",
+                 "interface_centroid", "phi"):
+        assert np.array_equal(dev.read(name), host.read(name)), name
+    nn, nl = ctypes.c_int64(), ctypes.c_int64()
+    L.check(L.lib().cc_generate_mesh_leaves(dev.handle, ctypes.byref(t), None, ctypes.byref(nn), None, ctypes.byref(nl)))
+    nodes, lines = np.empty((nn.value, 2)), np.empty((nl.value, 2), dtype=np.int64)
+    L.check(L.lib().cc_generate_mesh_leaves(dev.handle, ctypes.byref(t), nodes.ctypes.data, ctypes.byref(nn),
+                                            lines.ctypes.data, ctypes.byref(nl)))
+    assert np.array_equal(nodes, hn) and np.array_equal(lines, hl)
+    del dev
+    lv.free()
+    co.free()
 
 
 def test_uniform_tree_is_the_grid_cache():

@@ -13,7 +13,7 @@ from cutcell import (ArgumentError, CartesianGrid, CartesianIsoGrid, DimensionMi
                      MarchingCubesCutCell, MarchingSquaresCutCell, PLICCutCell, SDFBox, SDFPlane,
                      SDFSphere, SDFSphereUnion, SDFTorus, allocate_cache, closure_residual,
                      generate_mesh, random_rotation, update_cache)
-from helpers import MC_BITS, fortran, sample_vec, shared_face_mismatches
+from helpers import MC_BITS, DeviceArray, fortran, sample_vec, shared_face_mismatches
 from refrng import XorShift, randn_field
 
 pytestmark = pytest.mark.gpu
@@ -138,6 +138,22 @@ def test_mc_random_field_bitwise():
     assert shared_face_mismatches(c.cells.face_fraction, g.n, 3) == 0
     assert shared_face_mismatches(c.cells.face_centroid_local, g.n, 3) == 0
     assert np.count_nonzero(closure_residual(c.cells.face_fraction, g.d)) == 0
+
+
+def test_update_from_a_device_field():
+    # cc_update_from_phi with a device pointer (a caller's field already in HBM): the same cells as
+    # the host-array update, and the field kept as phi
+    from cutcell import update_cache_from_device_phi
+    vals = randn_field(XorShift(4242), (21, 18, 25))
+    g = CartesianGrid((-0.5, 0.25, -1.0), (20, 17, 24), (0.1, 0.15, 0.08))
+    host = update_cache(allocate_cache(g, MarchingCubesCutCell), vals, g)
+    buf = DeviceArray(vals)
+    dev = update_cache_from_device_phi(allocate_cache(g, MarchingCubesCutCell), buf.ptr.value, g)
+    assert same_bits(dev.phi, vals)
+    for name in COLS:
+        assert same_bits(dev.read(name), host.read(name)), name
+    del dev
+    buf.free()
 
 
 def test_mc_every_sign_pattern_single_cubes():
