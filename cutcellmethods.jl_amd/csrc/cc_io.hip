@@ -65,12 +65,13 @@ int body_of(cc_cache *c, Body &b) {
 }
 
 // Streams component `comp` of an interleaved column (`ncomp` values of `es` bytes per entry, `count`
-// entries) to f, converted by `convert` (bytes in, bytes out) when given.
+// entries) to f, `chunk` entries per round trip (host and stage hold chunk * es bytes), converted by
+// `convert` (bytes in, bytes out) when given.
 int stream_column(FILE *f, const void *dev, int64_t count, int ncomp, int comp, size_t es, hipStream_t st,
-                  char *stage, std::vector<char> &host, size_t out_es = 0,
+                  char *stage, std::vector<char> &host, int64_t chunk, size_t out_es = 0,
                   void (*convert)(const char *, char *, int64_t) = nullptr) {
-    for (int64_t q0 = 0; q0 < count; q0 += CHUNK) {
-        const int64_t m = count - q0 < CHUNK ? count - q0 : CHUNK;
+    for (int64_t q0 = 0; q0 < count; q0 += chunk) {
+        const int64_t m = count - q0 < chunk ? count - q0 : chunk;
         const char *src = (const char *)dev + (size_t)(q0 * ncomp + comp) * es;
         if (ncomp == 1) {
             if (hipMemcpyAsync(host.data(), src, m * es, hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -150,20 +151,30 @@ int write_vti(const char *path, const Body &b, const cc_grid *g, const std::vect
     }
     xml += "    </Piece>\n  </ImageData>\n  <AppendedData encoding=\"raw\">\n   _";
     if (fwrite(xml.data(), 1, xml.size(), f) != xml.size()) { fclose(f); return io_fail(CC_ERR_ARGUMENT, "write failed"); }
+    // the largest single round trip: every column's entries, at most CHUNK of them
+    int64_t chunk = 1;
+    for (const Array &a : arrays) {
+        const int64_t e = a.ncomp == a.stride ? a.count * a.stride : a.count;
+        chunk = e > chunk ? e : chunk;
+    }
+    chunk = chunk < CHUNK ? chunk : CHUNK;
     char *stage = nullptr;
     int rc = CC_OK;
-    std::vector<char> host(CHUNK * 8);
-    if (hipMalloc(&stage, CHUNK * 8) != hipSuccess) { fclose(f); return io_fail(CC_ERR_OUT_OF_MEMORY, "staging"); }
+    std::vector<char> host((size_t)chunk * 8);
+    if (hipMalloc(&stage, (size_t)chunk * 8) != hipSuccess) { fclose(f); return io_fail(CC_ERR_OUT_OF_MEMORY, "staging"); }
     for (int pass = 0; pass < 2 && rc == CC_OK; ++pass)
         for (const Array &a : arrays) {
             if (a.point != (pass == 0)) continue;
             const uint64_t nbytes = (uint64_t)a.count * a.ncomp * a.out_es;
-            fwrite(&nbytes, 8, 1, f);
+            if (fwrite(&nbytes, 8, 1, f) != 1) {
+                rc = io_fail(CC_ERR_ARGUMENT, "write failed");
+                break;
+            }
             if (a.ncomp == a.stride) {  // a whole column, entry by entry
-                rc = stream_column(f, b.cols[a.field], a.count * a.stride, 1, 0, a.in_es, st, stage, host, a.out_es,
-                                   a.convert);
+                rc = stream_column(f, b.cols[a.field], a.count * a.stride, 1, 0, a.in_es, st, stage, host, chunk,
+                                   a.out_es, a.convert);
             } else {  // one component of an interleaved column
-                rc = stream_column(f, b.cols[a.field], a.count, a.stride, a.comp, a.in_es, st, stage, host,
+                rc = stream_column(f, b.cols[a.field], a.count, a.stride, a.comp, a.in_es, st, stage, host, chunk,
                                    a.out_es, a.convert);
             }
             if (rc) break;
@@ -329,8 +340,9 @@ int cc_write_npy(cc_cache *c, int field, const char *path) {
     rc = ok ? cc_stream(c, &sp) : io_fail(CC_ERR_ARGUMENT, "write failed");
     if (rc == CC_OK && cc_synchronize(c) != CC_OK) rc = CC_ERR_HIP;
     if (rc == CC_OK) {
-        std::vector<char> host(CHUNK * 8);
-        rc = stream_column(f, dptr, nbytes / (int64_t)es, 1, 0, es, (hipStream_t)sp, nullptr, host);
+        const int64_t count = nbytes / (int64_t)es, chunk = count < CHUNK ? (count > 0 ? count : 1) : CHUNK;
+        std::vector<char> host((size_t)chunk * es);
+        rc = stream_column(f, dptr, count, 1, 0, es, (hipStream_t)sp, nullptr, host, chunk);
     }
     fclose(f);
     return rc;
