@@ -224,20 +224,38 @@ template <typename T> cc::Cols<T> cols_t(const cc_cache *c) {
     return r;
 }
 
-int stage_geometry(cc_cache *c, const cc_geometry *geo, cc::Geo *out) {
+// A geometry descriptor's kind and parameters, checked on the host before any launch.
+int check_geometry(const cc_geometry *geo, int dim) {
     if (!geo) return fail(CC_ERR_ARGUMENT, "geometry is NULL");
-    if (geo->kind < CC_GEO_EMPTY || geo->kind > CC_GEO_SPHERE_UNION)
+    const int base = geo->kind & ~CC_GEO_BOUNDED;
+    if ((geo->kind & ~(CC_GEO_BOUNDED | 0xff)) != 0 || base < CC_GEO_EMPTY || base > CC_GEO_SPHERE_UNION)
         return fail(CC_ERR_UNSUPPORTED, "unknown geometry kind %d", geo->kind);
-    if (geo->kind == CC_GEO_TORUS && c->dim != 3)
-        return fail(CC_ERR_UNSUPPORTED, "a torus is a 3D body");
+    if (base == CC_GEO_TORUS && dim != 3) return fail(CC_ERR_UNSUPPORTED, "a torus is a 3D body");
+    if (base == CC_GEO_SPHERE_UNION) {
+        if (geo->n_prims < 1 || !geo->prims)
+            return fail(CC_ERR_ARGUMENT, "a sphere union needs at least one primitive");
+        if (!(geo->p[0] > 0.0)) return fail(CC_ERR_ARGUMENT, "the smoothing k must be positive");
+    }
+    if (geo->kind & CC_GEO_BOUNDED) {
+        if (base == CC_GEO_EMPTY || base == CC_GEO_PLANE)
+            return fail(CC_ERR_ARGUMENT, "a bounded geometry needs a bounded child (no empty body or plane)");
+        for (int a = 0; a < dim; ++a)
+            if (!(geo->p[16 + a] <= geo->p[19 + a]))
+                return fail(CC_ERR_ARGUMENT, "a bounded geometry's box must have low <= high on every axis");
+        if (!(geo->p[22] >= 0.0)) return fail(CC_ERR_ARGUMENT, "a bounded geometry's pad_distance must be >= 0");
+        if (!(geo->p[23] > 0.0)) return fail(CC_ERR_ARGUMENT, "a bounded geometry's fill_distance must be positive");
+    }
+    return CC_OK;
+}
+
+int stage_geometry(cc_cache *c, const cc_geometry *geo, cc::Geo *out) {
+    const int rc = check_geometry(geo, c->dim);
+    if (rc) return rc;
     out->kind = geo->kind;
     out->nprims = 0;
     std::memcpy(out->p, geo->p, sizeof out->p);
     out->prims = nullptr;
-    if (geo->kind == CC_GEO_SPHERE_UNION) {
-        if (geo->n_prims < 1 || !geo->prims)
-            return fail(CC_ERR_ARGUMENT, "a sphere union needs at least one primitive");
-        if (!(geo->p[0] > 0.0)) return fail(CC_ERR_ARGUMENT, "the smoothing k must be positive");
+    if ((geo->kind & ~CC_GEO_BOUNDED) == CC_GEO_SPHERE_UNION) {
         const int64_t nd = 4 * (int64_t)geo->n_prims;
         bool same = (int64_t)c->prims_host.size() == nd &&
                     std::memcmp(c->prims_host.data(), geo->prims, nd * sizeof(double)) == 0;
@@ -1489,14 +1507,11 @@ int cc_calc_volume(int dtype, const cc_geometry *geo, const cc_grid *grid, doubl
             return fail(CC_ERR_ARGUMENT,
                         "calc_volume needs an isotropic grid (equal cell size on every axis), got grid.d = "
                         "(%g, %g%s)", grid->d[0], grid->d[1], dim == 3 ? ", ..." : "");
-    if (geo->kind < CC_GEO_EMPTY || geo->kind > CC_GEO_SPHERE_UNION)
-        return fail(CC_ERR_UNSUPPORTED, "unknown geometry kind %d", geo->kind);
-    if (geo->kind == CC_GEO_TORUS && dim != 3) return fail(CC_ERR_UNSUPPORTED, "a torus is a 3D body");
-    if (geo->kind == CC_GEO_SPHERE_UNION && (geo->n_prims < 1 || !geo->prims || !(geo->p[0] > 0.0)))
-        return fail(CC_ERR_ARGUMENT, "a sphere union needs primitives and a positive smoothing k");
+    const int grc = check_geometry(geo, dim);
+    if (grc) return grc;
     cc::Geo gd;
     gd.kind = geo->kind;
-    gd.nprims = geo->kind == CC_GEO_SPHERE_UNION ? geo->n_prims : 0;
+    gd.nprims = (geo->kind & ~CC_GEO_BOUNDED) == CC_GEO_SPHERE_UNION ? geo->n_prims : 0;
     std::memcpy(gd.p, geo->p, sizeof gd.p);
     gd.prims = nullptr;
     hipStream_t st = nullptr;

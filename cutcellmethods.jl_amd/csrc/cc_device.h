@@ -17,7 +17,7 @@ template <> struct Eps<float> { static constexpr float v = 1.1920928955078125e-0
 enum : int8_t { CELL_INSIDE = 0, CELL_OUTSIDE = 1, CELL_CUT = 2 };  // src/CutCellMethods.jl:56-58
 
 enum { GEO_EMPTY = 0, GEO_SPHERE = 1, GEO_TORUS = 2, GEO_BOX = 3, GEO_PLANE = 4,
-       GEO_SPHERE_UNION = 5 };
+       GEO_SPHERE_UNION = 5, GEO_BOUNDED = 0x100, GEO_KIND = 0xff };
 
 // The body as the kernels see it: parameters by value, the primitive table in device memory.
 struct Geo {
@@ -128,13 +128,35 @@ __device__ __forceinline__ T union_dist(const Geo &g, const T *x) {
     return acc;
 }
 
+// SDFBounded (SDFLibrary, un-vendored; its use documented at src/marching_cubes/surface.jl:70-71
+// and src/volume.jl:26-27): beyond pad (p[22]) of the child's bounding box (p[16..18] to p[19..21])
+// the value is the positive fill (p[23]), with no normal, so those nodes and centroids read as
+// outside.  The band is the Euclidean distance to the box, summed over the axes in order and
+// compared squared.
+template <typename T, int D> __device__ __forceinline__ bool beyond_band(const Geo &g, const T *x) {
+    T s = T(0);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const T lo = T(g.p[16 + a]), hi = T(g.p[19 + a]);
+        const T e = x[a] < lo ? lo - x[a] : (x[a] > hi ? x[a] - hi : T(0));
+        s = s + e * e;
+    }
+    const T pad = T(g.p[22]);
+    return s > pad * pad;
+}
+
 // get_sdf(geo, x) -> (distance, normal) (SDFLibrary, un-vendored: the build's own bodies).  The
 // normal is the analytic gradient; it is NaN or zero where the distance is not differentiable,
 // which the PLIC fit reports as a degenerate plane (src/plic/plic.jl:82-85).
 template <typename T, int D>
 __device__ __forceinline__ T get_sdf(const Geo &g, const T *x, T *normal) {
     T v[3] = {T(0), T(0), T(0)}, p[3] = {T(0), T(0), T(0)}, gl[3] = {T(0), T(0), T(0)};
-    switch (g.kind) {
+    if ((g.kind & GEO_BOUNDED) && beyond_band<T, D>(g, x)) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) normal[a] = T(0);
+        return T(g.p[23]);
+    }
+    switch (g.kind & GEO_KIND) {
     case GEO_SPHERE: {
         T nr;
         T dist = sphere_dist<T, D>(T(g.p[0]), T(g.p[1]), T(g.p[2]), T(g.p[3]), x[0], x[1],
@@ -197,7 +219,7 @@ __device__ __forceinline__ T get_sdf(const Geo &g, const T *x, T *normal) {
     }
 #pragma unroll
     for (int a = 0; a < D; ++a) normal[a] = T(0);
-    if (g.kind == GEO_EMPTY) return T(g.p[0]);
+    if ((g.kind & GEO_KIND) == GEO_EMPTY) return T(g.p[0]);
     return union_dist<T, D>(g, x);  // the sphere union offers no analytic normal
 }
 
@@ -205,7 +227,8 @@ __device__ __forceinline__ T get_sdf(const Geo &g, const T *x, T *normal) {
 template <typename T, int D>
 __device__ __forceinline__ T sdf_value(const Geo &g, const T *x) {
     const T z = D == 3 ? x[2] : T(0);
-    switch (g.kind) {
+    if ((g.kind & GEO_BOUNDED) && beyond_band<T, D>(g, x)) return T(g.p[23]);
+    switch (g.kind & GEO_KIND) {
     case GEO_EMPTY: return T(g.p[0]);
     case GEO_SPHERE:
         return sphere_dist<T, D>(T(g.p[0]), T(g.p[1]), T(g.p[2]), T(g.p[3]), x[0], x[1], z);

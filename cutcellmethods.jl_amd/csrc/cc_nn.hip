@@ -816,6 +816,9 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
         nsamples > (int64_t(1) << 30) || !(band > 0.0f))
         return nfail(CC_ERR_ARGUMENT, "bad training arguments");
     (void)hipSetDevice(m->device);
+    if (geo->kind < CC_GEO_EMPTY || geo->kind > CC_GEO_SPHERE_UNION)
+        return nfail(CC_ERR_UNSUPPORTED, "the surrogate trains on an unbounded body (sphere, torus, box, plane, "
+                                         "sphere union)");
     if (geo->kind == CC_GEO_SPHERE_UNION && (!geo->prims || geo->n_prims < 1))
         return nfail(CC_ERR_ARGUMENT, "sphere union without primitives");
     if (!(hi > lo)) return nfail(CC_ERR_ARGUMENT, "the training box needs hi > lo");

@@ -19,12 +19,17 @@ SDFLibrary is not vendored with the reference; these are the build's own signed-
   ``h = max(k - |a-b|, 0)/k; min(a, b) - h*h*k/4``.
 * ``EmptyGeo``: the constant field ``value`` (the reference seeds every cache with ``EmptyGeo(1)``,
   ``src/marching_squares/cache.jl:46``).
+* ``SDFBounded(child, pad_distance, fill_distance)``: the child within ``pad_distance`` (Euclidean)
+  of a box holding it, ``fill_distance`` (positive) with no normal beyond -- SDFLibrary's
+  ``SDFBounded`` as the reference uses it (``src/marching_cubes/surface.jl:70-71``,
+  ``src/volume.jl:26-27``, ``test/test_bounded.jl``).
 """
 from dataclasses import dataclass, field
 
 import numpy as np
 
 GEO_EMPTY, GEO_SPHERE, GEO_TORUS, GEO_BOX, GEO_PLANE, GEO_SPHERE_UNION = range(6)
+GEO_BOUNDED = 0x100  # a flag on a bounded body's kind
 
 _EYE = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
 
@@ -129,6 +134,74 @@ class SDFSphereUnion:
     @property
     def params(self):
         return [float(self.smoothing)]
+
+
+@dataclass
+class SDFBounded:
+    """SDFLibrary's ``SDFBounded(child; pad_distance, fill_distance)``: within ``pad_distance`` of a
+    box holding the child, the child's own value; beyond it ``fill_distance``, which must be positive so
+    those nodes and centroids read as outside (``src/marching_cubes/surface.jl:70-71``,
+    ``src/volume.jl:26-27``), with no normal.  A pad wider than a cell diagonal leaves every cut cell,
+    and so the measured volume and the contour, bit for bit alone (``test/test_bounded.jl``).
+
+    SDFLibrary is not vendored: the box is the one ``bounds()`` gives (a sphere's ``c +- r``, a
+    rotated box's world-axis extents, a torus's ``c +- (R + r)``, a sphere union's spheres widened by
+    its smoothing) unless ``box = (lo, hi)`` is given, the band is the Euclidean distance to the box,
+    and ``fill_distance`` defaults to ``pad_distance`` (a lower bound of the true distance beyond the
+    band)."""
+    child: object
+    pad_distance: float
+    fill_distance: float = None
+    box: tuple = None
+
+    def __post_init__(self):
+        if int(self.child.kind) & GEO_BOUNDED or int(self.child.kind) in (GEO_EMPTY, GEO_PLANE):
+            raise ValueError("SDFBounded needs a bounded child: a sphere, torus, box or sphere union")
+        if len(list(self.child.params)) > 16:
+            raise ValueError("the child's parameters do not fit a bounded descriptor")
+        if not float(self.pad_distance) >= 0.0:
+            raise ValueError("pad_distance must be >= 0")
+        if not self.fill() > 0.0:
+            raise ValueError("fill_distance must be positive")
+
+    @property
+    def kind(self):
+        return int(self.child.kind) | GEO_BOUNDED
+
+    @property
+    def prims(self):
+        return self.child.prims
+
+    def fill(self):
+        if self.fill_distance is not None:
+            return float(self.fill_distance)
+        return float(self.pad_distance) if float(self.pad_distance) > 0.0 else 1.0
+
+    def bounds(self):
+        """(lo, hi), each three floats, of the box the band is measured from."""
+        if self.box is not None:
+            return _vec3(self.box[0]), _vec3(self.box[1])
+        g = self.child
+        if int(g.kind) == GEO_SPHERE:
+            c, r = np.array(_vec3(g.center)), float(g.radius)
+            return list(c - r), list(c + r)
+        if int(g.kind) == GEO_TORUS:
+            c, e = np.array(_vec3(g.center)), float(g.major_radius) + float(g.minor_radius)
+            return list(c - e), list(c + e)
+        if int(g.kind) == GEO_BOX:
+            c = np.array(_vec3(g.center))
+            R = np.array(_rot(g.rotation)).reshape(3, 3)
+            e = np.abs(R) @ np.array(_vec3(g.half_extents))  # world = c + R p
+            return list(c - e), list(c + e)
+        p = np.asarray(g.prims)
+        k = float(g.smoothing)
+        return list((p[:, :3] - p[:, 3:]).min(axis=0) - k), list((p[:, :3] + p[:, 3:]).max(axis=0) + k)
+
+    @property
+    def params(self):
+        p = [float(v) for v in self.child.params]
+        lo, hi = self.bounds()
+        return p + [0.0] * (16 - len(p)) + lo + hi + [float(self.pad_distance), self.fill()]
 
 
 def random_rotation(rng):

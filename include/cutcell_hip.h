@@ -64,6 +64,12 @@ extern "C" {
 #define CC_GEO_BOX 3          /* p[0..2] centre, p[3..5] half extents, p[6..14] rotation */
 #define CC_GEO_PLANE 4        /* p[0..2] unit normal, p[3..5] point on the plane */
 #define CC_GEO_SPHERE_UNION 5 /* p[0] smoothing k; prims = n_prims x (cx, cy, cz, r), host memory */
+/* A flag on any of the bounded kinds above: SDFBounded(child; pad_distance, fill_distance) (SDFLibrary,
+ * whose use the reference documents at src/marching_cubes/surface.jl:70-71 and src/volume.jl:26-27).
+ * The child's own parameters stay in p[0..15]; p[16..18] / p[19..21] are a box holding the child,
+ * p[22] the pad and p[23] the fill (> 0).  Points farther than the pad from the box evaluate to the
+ * fill, with no normal; every other point to the child. */
+#define CC_GEO_BOUNDED 0x100
 
 typedef struct cc_geometry {
     int32_t kind;

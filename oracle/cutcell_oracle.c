@@ -84,9 +84,33 @@ static double smin_poly(double a, double b, double k)
     return m - ((h * h) * k) / 4.0;
 }
 
+/* SDFBounded (SDFLibrary 0.10.5, un-vendored; its use documented at src/marching_cubes/surface.jl:70-71
+ * and src/volume.jl:26-27): beyond pad of the child's bounding box the value is fill (> 0, so such
+ * nodes and centroids read as outside), with no normal.  The band is the Euclidean distance to the
+ * box, summed over the axes in order and compared squared. */
+static int beyond_band(const orc_geometry *g, int dim, const double *x)
+{
+    double s = 0.0;
+    for (int a = 0; a < dim; ++a) {
+        double lo = g->p[16 + a], hi = g->p[19 + a];
+        double e = x[a] < lo ? lo - x[a] : (x[a] > hi ? x[a] - hi : 0.0);
+        s = s + e * e;
+    }
+    return s > g->p[22] * g->p[22];
+}
+
 double orc_get_sdf(const orc_geometry *g, int dim, const double *x, double *normal)
 {
     double v[3] = {0.0, 0.0, 0.0}, p[3] = {0.0, 0.0, 0.0}, gl[3] = {0.0, 0.0, 0.0};
+    if (g->kind & ORC_GEO_BOUNDED) {
+        if (beyond_band(g, dim, x)) {
+            for (int a = 0; a < dim; ++a) normal[a] = 0.0;
+            return g->p[23];
+        }
+        orc_geometry child = *g;
+        child.kind &= ~ORC_GEO_BOUNDED;
+        return orc_get_sdf(&child, dim, x, normal);
+    }
     switch (g->kind) {
     case ORC_GEO_EMPTY:
         for (int a = 0; a < dim; ++a) normal[a] = 0.0;
@@ -153,6 +177,12 @@ double orc_get_sdf(const orc_geometry *g, int dim, const double *x, double *norm
 double orc_sdf_value(const orc_geometry *g, int dim, const double *x)
 {
     double v[3] = {0.0, 0.0, 0.0};
+    if (g->kind & ORC_GEO_BOUNDED) {
+        if (beyond_band(g, dim, x)) return g->p[23];
+        orc_geometry child = *g;
+        child.kind &= ~ORC_GEO_BOUNDED;
+        return orc_sdf_value(&child, dim, x);
+    }
     if (g->kind == ORC_GEO_SPHERE_UNION) {
         const double *s = g->prims;
         double k = g->p[0];

@@ -3,6 +3,7 @@ mathematics that fixes them: brute-force distances to densely sampled surfaces, 
 closed forms, and the reduction of the polynomial smooth union to the plain minimum.  A wrong sign,
 a dropped rotation or a transposed frame fails one of these."""
 import numpy as np
+import pytest
 
 import oracle as orc
 from cutcell import SDFBox, SDFPlane, SDFSphere, SDFSphereUnion, SDFTorus, random_rotation
@@ -84,3 +85,44 @@ def test_smooth_union_fold_and_its_reduction_to_min():
                    np.linalg.norm(far - [0.5, 0, 0], axis=1) - 0.1], 1)
     assert np.all(np.abs(dd[:, 0] - dd[:, 1]) >= k)
     assert np.array_equal(sdf(geo2, far), dd.min(axis=1))
+
+
+def test_bounded_body_band_and_the_reference_test():
+    # SDFBounded (test/test_bounded.jl): the child within the pad of its box, the fill beyond, and a
+    # pad wider than a cell diagonal leaves every cut cell -- and so the volume and the contour -- alone
+    from cutcell import CartesianIsoGrid, SDFBounded, SDFBox, SDFCircle, SDFPlane, SDFSphere, SDFTorus
+    circle = SDFCircle((0.0, 0.0), 1.0)
+    b = SDFBounded(circle, pad_distance=0.5)
+    rng = np.random.default_rng(9)
+    pts = rng.uniform(-3.0, 3.0, (4000, 2))
+    for x in pts:
+        lo = np.clip(x, -1.0, 1.0)
+        far = np.sum((x - lo) ** 2) > 0.25
+        v = orc.sdf_value(b, 2, x)
+        assert v == (0.5 if far else orc.sdf_value(circle, 2, x))
+        val, nrm = orc.get_sdf(b, 2, x)
+        assert val == v and (np.all(nrm == 0.0) if far else np.array_equal(nrm, orc.get_sdf(circle, 2, x)[1]))
+    # test/test_bounded.jl:15-19: cell 0.1, pad 0.5
+    g = CartesianIsoGrid((-2.0, -2.0), (2.0, 2.0), n=40)
+    vb = orc.calc_volume(b, 2, g.x0, g.d, g.n)
+    assert vb == orc.calc_volume(circle, 2, g.x0, g.d, g.n)
+    assert abs(vb - np.pi) < 1e-3 * np.pi
+    # :21-25, the nodal reconstruction: every cell the same (only far phi differs)
+    pb, pc = orc.sample_nodes(b, 2, g.x0, g.d, g.n), orc.sample_nodes(circle, 2, g.x0, g.d, g.n)
+    cb, cc = orc.ms_cells(pb, g.x0, g.d, g.n), orc.ms_cells(pc, g.x0, g.d, g.n)
+    for k in cc:
+        assert np.array_equal(cb[k], cc[k]), k
+    assert np.any(pb != pc) and np.all((pb == pc) | (pb == 0.5))
+    # bounding boxes contain their bodies: on the box's boundary every child is >= 0 (outside)
+    for child in (SDFSphere((0.1, -0.2, 0.3), 0.7), SDFTorus((0.0, 0.1, 0.0), 0.5, 0.15, np.eye(3)),
+                  SDFBox((0.0, 0.0, 0.0), (0.3, 0.2, 0.5), random_rotation(rng))):
+        lo, hi = SDFBounded(child, 0.1).bounds()
+        face = rng.uniform(lo, hi, (2000, 3))
+        ax = rng.integers(0, 3, 2000)
+        side = rng.integers(0, 2, 2000)
+        face[np.arange(2000), ax] = np.where(side == 1, np.array(hi)[ax], np.array(lo)[ax])
+        assert all(orc.sdf_value(child, 3, p) >= -1e-12 for p in face), type(child).__name__
+    with pytest.raises(ValueError):
+        SDFBounded(SDFPlane((0.0, 0.0, 1.0), (0.0, 0.0, 0.0)), 0.5)
+    with pytest.raises(ValueError):
+        SDFBounded(circle, 0.5, fill_distance=0.0)
