@@ -150,16 +150,16 @@ struct cc_cache {
     int32_t *leaf_level = nullptr, *leaf_coord = nullptr;  // device copies of a host leaf list
     const void *leaf_src[2] = {};  // the host arrays and version tag they were copied from
     int32_t leaf_version = 0;
-    // the last tree contour, for the fill call of generate_mesh's two calls: valid for this update
-    // (gen) and this tagged tree (arrays, count, version, base grid)
-    std::vector<double> tree_nodes;
-    std::vector<int64_t> tree_lines;
-    bool tree_mesh_ok = false;
-    uint64_t tree_mesh_gen = 0;
-    const void *tree_mesh_src[2] = {};
-    int64_t tree_mesh_n = -1;
-    int32_t tree_mesh_version = 0;
-    double tree_mesh_frame[4] = {};
+    // the last contour (grid or tree), for the fill call of generate_mesh's two calls: valid for
+    // this update (gen) and this grid frame, or this tagged tree (arrays, count, version, base grid)
+    std::vector<double> contour_nodes;
+    std::vector<int64_t> contour_lines;
+    bool contour_ok = false;
+    uint64_t contour_gen = 0;
+    const void *contour_src[2] = {};
+    int64_t contour_n = -1;
+    int32_t contour_version = 0;
+    double contour_frame[4] = {};
     int64_t *mark_idx = nullptr;  // the cut leaves gathered for the tree contour
     int64_t mark_idx_cap = 0;
     char *mark_phi = nullptr;
@@ -948,17 +948,20 @@ void cell_segments(const T (&nx)[4], const T (&ny)[4], const T (&ph)[4], Welder 
 }
 
 // The grid contour: cut cells in linear order (surface.jl:59-86).
+// The grid contour: cut cells in cell order (the screen: corners not all of one sign), entry r being
+// cell idx[r] with its corner values phi[4r..4r+3] (surface.jl:59-86).
 template <typename T>
-void contour(const std::vector<T> &phi, const std::vector<int8_t> &kind, const cc_grid *g,
-             std::vector<double> &nodes, std::vector<int64_t> &lines) {
-    const int64_t n0 = g->n[0], n1 = g->n[1], sy = n0 + 1;
+void contour(int64_t ncut, const int64_t *idx, const T *phi, const cc_grid *g, std::vector<double> &nodes,
+             std::vector<int64_t> &lines) {
+    const int64_t n0 = g->n[0];
     Welder vid(nodes);
+    vid.ids.reserve((size_t)(2 * ncut));
+    nodes.reserve((size_t)(4 * ncut));
+    lines.reserve((size_t)(2 * ncut));
     const T x0 = T(g->x0[0]), y0 = T(g->x0[1]), dx = T(g->d[0]), dy = T(g->d[1]);
-    for (int64_t c = 0; c < n0 * n1; ++c) {
-        if (kind[c] != CC_CELL_CUT) continue;  // the screen: corners not all of one sign
-        const int64_t i = c % n0, j = c / n0;
-        const T ph[4] = {phi[i + sy * j], phi[i + 1 + sy * j], phi[i + 1 + sy * (j + 1)],
-                         phi[i + sy * (j + 1)]};
+    for (int64_t r = 0; r < ncut; ++r) {
+        const int64_t c = idx[r], i = c % n0, j = c / n0;
+        const T ph[4] = {phi[4 * r], phi[4 * r + 1], phi[4 * r + 2], phi[4 * r + 3]};
         const T xl = x0 + T(i) * dx, xh = x0 + T(i + 1) * dx;
         const T yl = y0 + T(j) * dy, yh = y0 + T(j + 1) * dy;
         const T nx[4] = {xl, xh, xh, xl}, ny[4] = {yl, yl, yh, yh};
@@ -991,23 +994,58 @@ void contour_leaves(int64_t ncut, const int64_t *idx, const T *phi, const int32_
     }
 }
 
-// Marks, ranks and gathers the cut leaves on the device, copies them to the host and contours them
-// into the cache's kept tree contour.
-template <typename T> int tree_contour(cc_cache *c, const cc_leaves *tree) {
-    const int64_t n = tree->n, nwords = (n + 63) / 64;
+// Marks and ranks the cut cells (or leaves) of the cache on the device; *ncut receives their number.
+int mark_cut_cells(cc_cache *c, int64_t n, int64_t *ncut) {
+    const int64_t nwords = (n + 63) / 64;
     int rc = ensure(&c->mark_bits, &c->mark_bits_cap, nwords);
     if (rc == CC_OK) rc = ensure(&c->mark_rank, &c->mark_rank_cap, nwords);
     if (rc == CC_OK) rc = surface_work(c, nwords);
     if (rc) return rc;
-    cc::launch_leaf_marks((const int8_t *)c->col[CC_FIELD_KIND], (size_t)n, c->mark_bits, c->mark_rank, c->ps.part,
-                          c->ps.part_cap, c->ps.totals, c->stream);
+    cc::launch_cut_marks((const int8_t *)c->col[CC_FIELD_KIND], (size_t)n, c->mark_bits, c->mark_rank, c->ps.part,
+                         c->ps.part_cap, c->ps.totals, c->stream);
     HIPC(hipGetLastError());
     int64_t tot[2] = {0, 0};
     HIPC(hipMemcpyAsync(tot, c->ps.totals, sizeof tot, hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
-    const int64_t ncut = tot[0];
-    if (ncut < 0 || ncut > n) return fail(CC_ERR_HIP, "tree contour: %lld cut leaves of %lld", (long long)ncut,
-                                          (long long)n);
+    if (tot[0] < 0 || tot[0] > n) return fail(CC_ERR_HIP, "contour: %lld cut cells of %lld", (long long)tot[0],
+                                             (long long)n);
+    *ncut = tot[0];
+    return CC_OK;
+}
+
+// The grid contour of a marching-squares cache into its kept contour: the cut cells' corners gathered
+// on the device, welded on the host.
+template <typename T> int grid_contour(cc_cache *c, const cc_grid *g) {
+    int64_t ncut = 0;
+    int rc = mark_cut_cells(c, c->ncell, &ncut);
+    if (rc) return rc;
+    const int64_t m = ncut > 0 ? ncut : 1;
+    rc = ensure(&c->mark_idx, &c->mark_idx_cap, m);
+    if (rc == CC_OK) rc = ensure(&c->mark_phi, &c->mark_phi_cap, 4 * (int64_t)sizeof(T) * m);
+    if (rc) return rc;
+    std::vector<int64_t> idx(m);
+    std::vector<T> phi(4 * m);
+    if (ncut > 0) {
+        cc::launch_grid_gather<T>(c->mark_bits, c->mark_rank, (size_t)c->ncell, (const T *)c->col[CC_FIELD_PHI],
+                                  (size_t)c->n[0], c->mark_idx, (T *)c->mark_phi, c->stream);
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(idx.data(), c->mark_idx, sizeof(int64_t) * ncut, hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipMemcpyAsync(phi.data(), c->mark_phi, 4 * sizeof(T) * ncut, hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+    }
+    c->contour_nodes.clear();
+    c->contour_lines.clear();
+    contour<T>(ncut, idx.data(), phi.data(), g, c->contour_nodes, c->contour_lines);
+    return CC_OK;
+}
+
+// Marks, ranks and gathers the cut leaves on the device, copies them to the host and contours them
+// into the cache's kept contour.
+template <typename T> int tree_contour(cc_cache *c, const cc_leaves *tree) {
+    const int64_t n = tree->n;
+    int64_t ncut = 0;
+    int rc = mark_cut_cells(c, n, &ncut);
+    if (rc) return rc;
     const int64_t m = ncut > 0 ? ncut : 1;
     rc = ensure(&c->mark_idx, &c->mark_idx_cap, m);
     if (rc == CC_OK) rc = ensure(&c->mark_phi, &c->mark_phi_cap, 4 * (int64_t)sizeof(T) * m);
@@ -1027,10 +1065,10 @@ template <typename T> int tree_contour(cc_cache *c, const cc_leaves *tree) {
             HIPC(hipMemcpyAsync(lc.data(), c->mark_lc, 3 * sizeof(int32_t) * ncut, hipMemcpyDeviceToHost, c->stream));
         HIPC(hipStreamSynchronize(c->stream));
     }
-    c->tree_nodes.clear();
-    c->tree_lines.clear();
-    contour_leaves<T>(ncut, idx.data(), phi.data(), tree->on_device ? lc.data() : nullptr, tree, c->tree_nodes,
-                      c->tree_lines);
+    c->contour_nodes.clear();
+    c->contour_lines.clear();
+    contour_leaves<T>(ncut, idx.data(), phi.data(), tree->on_device ? lc.data() : nullptr, tree, c->contour_nodes,
+                      c->contour_lines);
     return CC_OK;
 }
 
@@ -1370,19 +1408,21 @@ int cc_generate_mesh_ms(cc_cache *c, const cc_grid *grid, double *nodes, int64_t
     if (rc) return rc;
     HIPC(hipSetDevice(c->device));
     HIPC(hipStreamSynchronize(c->stream));
-    std::vector<int8_t> kind(c->ncell);
-    HIPC(hipMemcpy(kind.data(), c->col[CC_FIELD_KIND], c->ncell, hipMemcpyDeviceToHost));
-    std::vector<double> vn;
-    std::vector<int64_t> vl;
-    if (c->dtype == CC_FLOAT32) {
-        std::vector<float> phi(c->nnode);
-        HIPC(hipMemcpy(phi.data(), c->col[CC_FIELD_PHI], c->colbytes[CC_FIELD_PHI], hipMemcpyDeviceToHost));
-        contour<float>(phi, kind, grid, vn, vl);
-    } else {
-        std::vector<double> phi(c->nnode);
-        HIPC(hipMemcpy(phi.data(), c->col[CC_FIELD_PHI], c->colbytes[CC_FIELD_PHI], hipMemcpyDeviceToHost));
-        contour<double>(phi, kind, grid, vn, vl);
+    // the fill call of the two-call protocol takes the contour its size call made, when nothing was
+    // updated and the grid's frame is the same; every size call contours afresh
+    const double frame[4] = {grid->x0[0], grid->x0[1], grid->d[0], grid->d[1]};
+    const bool reuse = nodes && lines && c->contour_ok && c->contour_gen == c->gen &&
+                       std::memcmp(frame, c->contour_frame, sizeof frame) == 0;
+    if (!reuse) {
+        c->contour_ok = false;
+        rc = c->dtype == CC_FLOAT32 ? grid_contour<float>(c, grid) : grid_contour<double>(c, grid);
+        if (rc) return rc;
+        c->contour_ok = true;
+        c->contour_gen = c->gen;
+        std::memcpy(c->contour_frame, frame, sizeof frame);
     }
+    const std::vector<double> &vn = c->contour_nodes;
+    const std::vector<int64_t> &vl = c->contour_lines;
     const int64_t nn = (int64_t)vn.size() / 2, nl = (int64_t)vl.size() / 2;
     if (nodes && lines) {
         if (*nnodes < nn || *nlines < nl)
@@ -1457,24 +1497,24 @@ int cc_generate_mesh_leaves(cc_cache *c, const cc_leaves *tree, double *nodes, i
     HIPC(hipStreamSynchronize(c->stream));
     const int64_t n = tree->n;
     const double frame[4] = {tree->x0[0], tree->x0[1], tree->d[0], tree->d[1]};
-    const bool reuse = c->tree_mesh_ok && c->tree_mesh_gen == c->gen && tree->version != 0 &&
-                       tree->version == c->tree_mesh_version && n == c->tree_mesh_n &&
-                       c->tree_mesh_src[0] == tree->level && c->tree_mesh_src[1] == tree->coord &&
-                       std::memcmp(frame, c->tree_mesh_frame, sizeof frame) == 0;
+    const bool reuse = nodes && lines && c->contour_ok && c->contour_gen == c->gen && tree->version != 0 &&
+                       tree->version == c->contour_version && n == c->contour_n &&
+                       c->contour_src[0] == tree->level && c->contour_src[1] == tree->coord &&
+                       std::memcmp(frame, c->contour_frame, sizeof frame) == 0;
     if (!reuse) {
-        c->tree_mesh_ok = false;
+        c->contour_ok = false;
         rc = c->dtype == CC_FLOAT32 ? tree_contour<float>(c, tree) : tree_contour<double>(c, tree);
         if (rc) return rc;
-        c->tree_mesh_ok = true;
-        c->tree_mesh_gen = c->gen;
-        c->tree_mesh_src[0] = tree->level;
-        c->tree_mesh_src[1] = tree->coord;
-        c->tree_mesh_n = n;
-        c->tree_mesh_version = tree->version;
-        std::memcpy(c->tree_mesh_frame, frame, sizeof frame);
+        c->contour_ok = true;
+        c->contour_gen = c->gen;
+        c->contour_src[0] = tree->level;
+        c->contour_src[1] = tree->coord;
+        c->contour_n = n;
+        c->contour_version = tree->version;
+        std::memcpy(c->contour_frame, frame, sizeof frame);
     }
-    const std::vector<double> &vn = c->tree_nodes;
-    const std::vector<int64_t> &vl = c->tree_lines;
+    const std::vector<double> &vn = c->contour_nodes;
+    const std::vector<int64_t> &vl = c->contour_lines;
     const int64_t nn = (int64_t)vn.size() / 2, nl = (int64_t)vl.size() / 2;
     if (nodes && lines) {
         if (*nnodes < nn || *nlines < nl)

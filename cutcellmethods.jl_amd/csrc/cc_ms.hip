@@ -65,10 +65,10 @@ __global__ __launch_bounds__(256) void k_leaf_cells(Geo geo, const int32_t *__re
     add_block_cuts(cuts, counter);
 }
 
-// generate_mesh over a tree (src/marching_squares/surface.jl:92-110) draws only the cut leaves: one
-// bit per leaf, ranked by a popcount scan, then each cut leaf's index and four corner values (and,
-// for a tree held on the device, its level and lattice corner) gathered in leaf order, so only they
-// cross to the host contour.
+// generate_mesh over a grid or a tree (src/marching_squares/surface.jl:59-110) draws only the cut
+// cells: one bit per cell or leaf, ranked by a popcount scan, then each cut one's index and four
+// corner values (and, for a tree held on the device, its level and lattice corner) gathered in
+// order, so only they cross to the host contour.
 __global__ __launch_bounds__(256) void k_leaf_cut_bits(const int8_t *__restrict__ kind, size_t n, size_t nwords,
                                                        unsigned long long *__restrict__ bits) {
     const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -103,9 +103,47 @@ __global__ __launch_bounds__(256) void k_leaf_gather(const unsigned long long *_
     }
 }
 
+// The grid contour's gather: cut cell c's index and its four corner values from the nodal field
+// (MS_NODE_BITS order), in cell order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_grid_gather(const unsigned long long *__restrict__ bits,
+                                                     const int64_t *__restrict__ wrank, size_t nwords,
+                                                     const T *__restrict__ phi, size_t n0, int64_t *__restrict__ idx,
+                                                     T *__restrict__ phi_out) {
+    const size_t w = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (w >= nwords) return;
+    unsigned long long m = bits[w];
+    int64_t r = wrank[w];
+    const size_t sy = n0 + 1;
+    while (m) {
+        const size_t c = w * 64 + (size_t)(__ffsll((long long)m) - 1);
+        const size_t i = c % n0, j = c / n0, base = i + sy * j;
+        idx[r] = (int64_t)c;
+        phi_out[4 * r] = phi[base];
+        phi_out[4 * r + 1] = phi[base + 1];
+        phi_out[4 * r + 2] = phi[base + sy + 1];
+        phi_out[4 * r + 3] = phi[base + sy];
+        ++r;
+        m &= m - 1;
+    }
+}
+
 }  // namespace
 
-void launch_leaf_marks(const int8_t *kind, size_t n, unsigned long long *bits, int64_t *wrank,
+template <typename T>
+void launch_grid_gather(const unsigned long long *bits, const int64_t *wrank, size_t n, const T *phi, size_t n0,
+                        int64_t *idx, T *phi_out, hipStream_t s) {
+    const size_t nwords = (n + 63) / 64;
+    hipLaunchKernelGGL((k_grid_gather<T>), dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, bits, wrank,
+                       nwords, phi, n0, idx, phi_out);
+}
+
+template void launch_grid_gather<double>(const unsigned long long *, const int64_t *, size_t, const double *, size_t,
+                                         int64_t *, double *, hipStream_t);
+template void launch_grid_gather<float>(const unsigned long long *, const int64_t *, size_t, const float *, size_t,
+                                        int64_t *, float *, hipStream_t);
+
+void launch_cut_marks(const int8_t *kind, size_t n, unsigned long long *bits, int64_t *wrank,
                        unsigned long long *part, size_t part_cap, int64_t *totals, hipStream_t s) {
     const size_t nwords = (n + 63) / 64;
     hipLaunchKernelGGL(k_leaf_cut_bits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, kind, n, nwords, bits);

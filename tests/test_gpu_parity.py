@@ -231,6 +231,25 @@ def test_ms_circle_config_bitwise_and_contour():
     v = b - a
     nrm = np.stack([v[:, 1], -v[:, 0]], axis=1)
     assert np.all(np.sum(nrm * (0.5 * (a + b)), axis=1) > 0)
+    # the library keeps a contour only from a size call to its fill call: a moved grid, or a field
+    # changed through the column's own pointer with no update, is drawn afresh by the next call
+    moved = g.moved((0.003, -0.002))
+    n2, l2 = generate_mesh(c, moved)
+    assert np.array_equal(n2, contour_reference(phi_ref, moved)[0]) and np.array_equal(l2, lines)
+    import ctypes
+    ptr, _ = c.column_ptr("phi")
+    P = fortran(phi_ref, g.nodegrid_size())
+    i, j = np.argwhere((P[:-1] > 0) & (P[1:] < 0))[0]  # a node whose +x edge the surface crosses
+    node = int(i) + 201 * int(j)
+    new = phi_ref.copy()
+    new[node] = 0.5 * phi_ref[node]  # the same sign, so the same cut cells
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.hipMemcpy(ctypes.c_void_p(ptr + 8 * node), new[node:node + 1].ctypes.data, 8, 1) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    n3, l3 = generate_mesh(c, g)
+    w3, wl3 = contour_reference(new, g)
+    assert np.array_equal(n3, w3) and np.array_equal(l3, wl3) and not np.array_equal(n3, nodes)
 
 
 def test_ms_callable_field_and_errors():
