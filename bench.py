@@ -344,11 +344,19 @@ def diagnostics_pass(seed, n=512, reps=8, cpu=True, layers=16):
     return out
 
 
-def fp32_error(grid, geo, cache64):
+def fp32_error(grid, geo, cache64, tune=None):
     """The Float32 variant: its update time and its error against the Float64 cache (bitwise the
-    oracle) on a slab through the body."""
+    oracle) on a slab through the body.  With ``tune`` "auto" its stream shape is tuned on the GPU
+    the same way as the Float64 cache's; otherwise ``tune`` is the (variant, kz, nontemporal) used."""
     from cutcell import MarchingCubesCutCell, allocate_cache, update_cache
     c32 = allocate_cache(grid, MarchingCubesCutCell, np.float32)
+    if tune == "auto":
+        from cutcell.autotune import tune_stream
+        cands = [(2, kz, nt) for kz in (1, 2, 4, 8) for nt in (False, True)]
+        rec = tune_stream(c32, geo, grid, candidates=cands, repeats=3, use_cache=False)
+        c32.set_tuning(rec["variant"], rec["kz"], bool(rec["nontemporal"]))
+    elif tune is not None:
+        c32.set_tuning(*tune)
     for _ in range(3):
         update_cache(c32, geo, grid)
     t32, _ = _timed(lambda: update_cache(c32, geo, grid), 8)
@@ -372,8 +380,9 @@ def fp32_error(grid, geo, cache64):
         sel = (k64 == 2) & (vf >= lim) & (vf <= 1.0 - lim)
         err[f"centroid_max_abs_cut_vf_{lim:g}_to_1-{lim:g}"] = float(dc[sel].max()) if sel.any() else 0.0
     err["cell_size"] = float(grid.d[0])
+    tuning = list(c32.get_tuning())
     del c32
-    return {"update_ms": round(t32 * 1e3, 3), "cells_per_s": grid.ncells / t32,
+    return {"update_ms": round(t32 * 1e3, 3), "cells_per_s": grid.ncells / t32, "stream_tuning": tuning,
             "error_vs_fp64_on_16_layers": err}
 
 
@@ -648,7 +657,7 @@ def main():
     if sweep is not None:
         out["tuning_sweep"] = sweep
     if not a.quick:
-        out["fp32"] = fp32_error(grid, geo, cache)
+        out["fp32"] = fp32_error(grid, geo, cache, "auto" if a.tune == "auto" else (tv, tk, bool(tn)))
         del cache
         out.update(secondary(a.seed))
         out["mc_surface_512"] = surface(a.seed, cpu=not a.no_cpu)
