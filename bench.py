@@ -216,14 +216,18 @@ def quadtree(seed, level=12, reps=20, cpu=True):
     t_up, k_ms = float(np.median(ts)), float(np.median(ks))
     n = mesh.nleaves
     alg = n * (106 + 32)  # each leaf's record (2D, FP64) and its four corner values
-    t = time.perf_counter()
-    nodes, lines = generate_mesh(cache, mesh)
-    t_mesh = time.perf_counter() - t
+    tm = []
+    for _ in range(4):  # the first call also sizes the contour's work arrays
+        t = time.perf_counter()
+        nodes, lines = generate_mesh(cache, mesh)
+        tm.append(time.perf_counter() - t)
+    t_mesh = float(np.median(tm[1:]))
     out = {"workload": f"MarchingSquaresCutCell on {n} quadtree leaves (levels 1-{level}), circle r={radius}",
            "leaves": n, "cut_leaves": cache.cut_count(), "update_ms": round(t_up * 1e3, 4),
            "leaves_per_s": n / t_up, "kernel_ms": round(k_ms, 4),
            "kernel_hbm_frac": round(alg / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-           "generate_mesh_ms": round(t_mesh * 1e3, 2), "contour_segments": int(lines.shape[0])}
+           "generate_mesh_ms": round(t_mesh * 1e3, 2), "generate_mesh_first_call_ms": round(tm[0] * 1e3, 2),
+           "contour_segments": int(lines.shape[0])}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc

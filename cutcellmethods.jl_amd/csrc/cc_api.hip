@@ -879,30 +879,61 @@ void free_all(cc_cache *c) {
 }
 
 // ---- the marching-squares contour (src/marching_squares/surface.jl:59-177), on the host --------
-struct KeyHash {
-    size_t operator()(const std::pair<uint64_t, uint64_t> &k) const {
-        return std::hash<uint64_t>()(k.first * 0x9E3779B97F4A7C15ull ^ k.second);
-    }
-};
-
 // Welds contour vertices by exact coordinate equality (surface.jl:170-177): `+ 0` folds -0.0 onto
-// 0.0; ids are 1-based, as MeshLibrary numbers nodes.
+// 0.0; ids are 1-based, in order of first appearance, as MeshLibrary numbers nodes.  The keys are the
+// coordinates' bit patterns in a flat open-addressing table (linear probing, at most half full).
 struct Welder {
-    std::unordered_map<std::pair<uint64_t, uint64_t>, int64_t, KeyHash> ids;
+    std::vector<uint64_t> kx, ky;  // slot keys; an empty slot has id 0
+    std::vector<int64_t> id;
+    uint64_t mask = 0;
+    size_t used = 0;
     std::vector<double> &nodes;
-    explicit Welder(std::vector<double> &n) : nodes(n) {}
+    explicit Welder(std::vector<double> &n) : nodes(n) { grow(64); }
+    void reserve(size_t keys) {
+        size_t cap = 64;
+        while (cap < 2 * keys) cap <<= 1;
+        if (cap > mask + 1) grow(cap);
+    }
+    static uint64_t mix(uint64_t a, uint64_t b) {
+        uint64_t h = a * 0x9E3779B97F4A7C15ull ^ (b + 0x632BE59BD9B4E019ull + (a << 6) + (a >> 2));
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ull;
+        return h ^ (h >> 29);
+    }
+    void grow(size_t cap) {
+        std::vector<uint64_t> ox(std::move(kx)), oy(std::move(ky));
+        std::vector<int64_t> oi(std::move(id));
+        kx.assign(cap, 0);
+        ky.assign(cap, 0);
+        id.assign(cap, 0);
+        mask = cap - 1;
+        for (size_t q = 0; q < oi.size(); ++q)
+            if (oi[q]) {
+                uint64_t h = mix(ox[q], oy[q]) & mask;
+                while (id[h]) h = (h + 1) & mask;
+                kx[h] = ox[q];
+                ky[h] = oy[q];
+                id[h] = oi[q];
+            }
+    }
     template <typename T> int64_t operator()(T x, T y) {
-        const double kx = (double)(x + T(0)), ky = (double)(y + T(0));
+        const double px = (double)(x + T(0)), py = (double)(y + T(0));
         uint64_t bx, by;
-        std::memcpy(&bx, &kx, 8);
-        std::memcpy(&by, &ky, 8);
-        auto it = ids.find({bx, by});
-        if (it != ids.end()) return it->second;
-        nodes.push_back(kx);
-        nodes.push_back(ky);
-        const int64_t id = (int64_t)nodes.size() / 2;
-        ids.emplace(std::make_pair(bx, by), id);
-        return id;
+        std::memcpy(&bx, &px, 8);
+        std::memcpy(&by, &py, 8);
+        uint64_t h = mix(bx, by) & mask;
+        while (id[h]) {
+            if (kx[h] == bx && ky[h] == by) return id[h];
+            h = (h + 1) & mask;
+        }
+        nodes.push_back(px);
+        nodes.push_back(py);
+        const int64_t v = (int64_t)nodes.size() / 2;
+        kx[h] = bx;
+        ky[h] = by;
+        id[h] = v;
+        if (2 * ++used > mask + 1) grow(2 * (mask + 1));
+        return v;
     }
 };
 
@@ -955,7 +986,7 @@ void contour(int64_t ncut, const int64_t *idx, const T *phi, const cc_grid *g, s
              std::vector<int64_t> &lines) {
     const int64_t n0 = g->n[0];
     Welder vid(nodes);
-    vid.ids.reserve((size_t)(2 * ncut));
+    vid.reserve((size_t)(2 * ncut));
     nodes.reserve((size_t)(4 * ncut));
     lines.reserve((size_t)(2 * ncut));
     const T x0 = T(g->x0[0]), y0 = T(g->x0[1]), dx = T(g->d[0]), dy = T(g->d[1]);
@@ -977,7 +1008,7 @@ template <typename T>
 void contour_leaves(int64_t ncut, const int64_t *idx, const T *phi, const int32_t *lc, const cc_leaves *t,
                     std::vector<double> &nodes, std::vector<int64_t> &lines) {
     Welder vid(nodes);
-    vid.ids.reserve((size_t)(2 * ncut));
+    vid.reserve((size_t)(2 * ncut));
     nodes.reserve((size_t)(4 * ncut));
     lines.reserve((size_t)(2 * ncut));
     const T x0 = T(t->x0[0]), y0 = T(t->x0[1]), dx = T(t->d[0]), dy = T(t->d[1]);
