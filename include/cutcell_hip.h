@@ -188,8 +188,11 @@ int cc_write_npy(cc_cache *cache, int field, const char *path);
 
 /* generate_mesh(cache, grid) for a marching-squares cache -- src/marching_squares/surface.jl:59-86:
  * the contour of the cache's phi, welded by exact coordinate equality.  Two passes: call with
- * nodes/lines NULL to get the counts, then again with buffers of that size.  nodes: 2*nnodes
- * doubles; lines: 2*nlines int64 (1-based node ids, as MeshLibrary stores them). */
+ * nodes/lines NULL to get the counts, then again with buffers of that size; the size call draws the
+ * contour (the cut cells gathered on the device, contoured on the host) and the fill call right after
+ * it copies that contour out unless an update or another grid frame came between.  nodes: 2*nnodes
+ * doubles; lines: 2*nlines int64 (1-based node ids, in order of first appearance, as MeshLibrary
+ * stores them). */
 int cc_generate_mesh_ms(cc_cache *cache, const cc_grid *grid, double *nodes, int64_t *nnodes,
                         int64_t *lines, int64_t *nlines);
 
@@ -228,7 +231,8 @@ int cc_update_leaves_from_phi(cc_cache *cache, const void *phi4, int on_device, 
 
 /* generate_mesh(cache, mesh::AdaptiveMesh{2}) -- src/marching_squares/surface.jl:92-110: the contour
  * of each cut leaf's stored corner values, in leaf order, welded by exact coordinate equality; the
- * same two-call protocol and buffers as cc_generate_mesh_ms. */
+ * same two-call protocol and buffers as cc_generate_mesh_ms (the fill call reuses the size call's
+ * contour only for a tree with a nonzero version tag, the same arrays and count). */
 int cc_generate_mesh_leaves(cc_cache *cache, const cc_leaves *tree, double *nodes, int64_t *nnodes,
                             int64_t *lines, int64_t *nlines);
 
