@@ -977,7 +977,6 @@ void cell_segments(const T (&nx)[4], const T (&ny)[4], const T (&ph)[4], Welder 
     }
 }
 
-// The grid contour: cut cells in linear order (surface.jl:59-86).
 // The grid contour: cut cells in cell order (the screen: corners not all of one sign), entry r being
 // cell idx[r] with its corner values phi[4r..4r+3] (surface.jl:59-86).
 template <typename T>

@@ -44,15 +44,18 @@ def assert_cells_equal(cache, ref, start=0, stop=None):
                                  f"got {got[bad[:2]]} want {want[bad[:2]]}")
 
 
-def contour_reference(phi, g):
+def contour_reference(phi, g, dtype=np.float64):
     """An independent host restatement of generate_mesh on a grid (surface.jl:59-177):
-    cell_interface per cut cell in linear order, vertices welded by exact (+0-normalised) keys."""
-    P = fortran(phi, g.nodegrid_size())
+    cell_interface per cut cell in linear order, vertices welded by exact (+0-normalised) keys;
+    the arithmetic in ``dtype``, the cache's element type."""
+    T = np.dtype(dtype).type
+    P = fortran(np.asarray(phi, dtype=dtype), g.nodegrid_size())
     n0, n1 = g.n
     ids, nodes, lines = {}, [], []
+    x0, y0, dx, dy = T(g.x0[0]), T(g.x0[1]), T(g.d[0]), T(g.d[1])
 
     def vid(p):
-        key = (p[0] + 0.0, p[1] + 0.0)
+        key = (float(p[0] + T(0)), float(p[1] + T(0)))
         if key not in ids:
             nodes.append(key)
             ids[key] = len(nodes)
@@ -64,7 +67,7 @@ def contour_reference(phi, g):
         ph = [P[a, b] for a, b in corners]
         if len({v >= 0 for v in ph}) == 1:
             continue
-        X = [(g.x0[0] + a * g.d[0], g.x0[1] + b * g.d[1]) for a, b in corners]
+        X = [(x0 + T(a) * dx, y0 + T(b) * dy) for a, b in corners]
         pres, sa, sb = [], [], []
         for k in range(4):
             k2 = (k + 1) % 4
@@ -250,6 +253,15 @@ def test_ms_circle_config_bitwise_and_contour():
     n3, l3 = generate_mesh(c, g)
     w3, wl3 = contour_reference(new, g)
     assert np.array_equal(n3, w3) and np.array_equal(l3, wl3) and not np.array_equal(n3, nodes)
+
+
+def test_ms_float32_contour():
+    # a Float32 cache contours in Float32, as the reference converts into the grid's float type
+    g = CartesianIsoGrid((-1.0, -1.0), (1.0, 1.0), sz=0.01)
+    c = update_cache(allocate_cache(g, MarchingSquaresCutCell, np.float32), SDFSphere((0.0, 0.0), 0.5), g)
+    nodes, lines = generate_mesh(c, g)
+    want_nodes, want_lines = contour_reference(c.phi, g, np.float32)
+    assert np.array_equal(nodes, want_nodes) and np.array_equal(lines, want_lines) and len(lines) > 300
 
 
 def test_ms_callable_field_and_errors():
