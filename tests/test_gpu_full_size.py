@@ -1,0 +1,83 @@
+"""GPU parity at the benchmarked sizes, through the C ABI.
+
+Bar: bit-exact against the oracle at full size, as at the small sizes of the other parity tests,
+on the benchmark's own inputs (its default seed, 20261203):
+- the 512^3 Float64 marching-cubes workload (sphere r = 0.4, centre U[-0.1, 0.1]^3 from the seed) at
+  the position of its last timed step (step 35, the body 17.5 cells along +x) and under the stream
+  shape the autotuner picks for it (fused z-march, one layer per work item, ordinary stores): phi
+  at all 513^3 nodes, then every column of all 512^3 cells, slab by slab;
+- the 384^3 Float64 PLIC workload (torus R = 0.5, r = 0.15, tilted by the rotation drawn from the
+  seed): every column, plane and valid flag.
+The reference's invariants on the device output at the same sizes: shared faces bitwise
+single-valued, closure exactly zero (src/cut_cell.jl:302-317; test/test_cache.jl:187-190).
+"""
+import gc
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from cutcell import (CartesianIsoGrid, MarchingCubesCutCell, PLICCutCell, SDFSphere, SDFTorus,
+                     allocate_cache, closure_residual, random_rotation, update_cache)
+from helpers import shared_face_mismatches
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261203  # the benchmark's default seed
+
+COLS = ("kind", "ambiguous", "volume_fraction", "centroid", "face_fraction",
+        "face_centroid_local", "interface_centroid")
+
+
+def same_bits(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and a.dtype == b.dtype and bool(np.array_equal(a.view(np.uint8), b.view(np.uint8)))
+
+
+def test_marching_cubes_512_sphere_full_size_bitwise():
+    rng = np.random.default_rng(SEED)
+    centre = tuple(rng.uniform(-0.1, 0.1, 3))
+    geo = SDFSphere(centre, 0.4)
+    base = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=512)
+    g = base.moved((-0.5 * base.d[0] * 35, 0.0, 0.0))
+    c = allocate_cache(g, MarchingCubesCutCell)
+    c.set_tuning(2, 1, False)
+    update_cache(c, geo, g)
+    print(f"512^3 sphere at {centre}, x0 {g.x0}", flush=True)
+    phi = orc.sample_nodes(geo, 3, g.x0, g.d, g.n)
+    assert same_bits(c.phi, phi)
+    plane, slab, cut = 512 * 512, 32, 0
+    for k0 in range(0, 512, slab):
+        ref = orc.mc_cells(phi, g.x0, g.d, g.n, k0, k0 + slab)
+        for name in COLS:
+            assert same_bits(c.read(name, k0 * plane, (k0 + slab) * plane), ref[name]), (k0, name)
+        cut += int((ref["kind"] == 2).sum())
+        del ref
+    assert cut == c.cut_count() and cut > 150_000
+    del phi
+    ff = c.cells.face_fraction
+    assert shared_face_mismatches(ff, g.n, 3) == 0
+    assert np.count_nonzero(closure_residual(ff, g.d)) == 0
+    del ff, c
+    gc.collect()
+
+
+def test_plic_384_torus_full_size_bitwise():
+    rng = np.random.default_rng(SEED)
+    geo = SDFTorus((0.0, 0.0, 0.0), 0.5, 0.15, random_rotation(rng))
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=384)
+    c = update_cache(allocate_cache(g, PLICCutCell), geo, g)
+    print("384^3 torus", flush=True)
+    cols, normals, intercepts, valid = orc.plic(geo, 3, g.x0, g.d, g.n)
+    assert same_bits(c.normals, normals.reshape(c.normals.shape))
+    assert same_bits(c.intercepts, intercepts) and np.array_equal(c.is_valid, valid)
+    del normals, intercepts, valid
+    for name in ("kind", "volume_fraction", "face_fraction"):
+        assert same_bits(c.read(name), cols[name]), name
+    assert int((cols["kind"] == 2).sum()) > 100_000
+    del cols
+    ff = c.cells.face_fraction
+    assert shared_face_mismatches(ff, g.n, 3) == 0
+    assert np.count_nonzero(closure_residual(ff, g.d)) == 0
+    del ff, c
+    gc.collect()
