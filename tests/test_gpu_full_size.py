@@ -7,7 +7,13 @@ on the benchmark's own inputs (its default seed, 20261203):
   shape the autotuner picks for it (fused z-march, one layer per work item, ordinary stores): phi
   at all 513^3 nodes, then every column of all 512^3 cells, slab by slab;
 - the 384^3 Float64 PLIC workload (torus R = 0.5, r = 0.15, tilted by the rotation drawn from the
-  seed): every column, plane and valid flag.
+  seed): every column, plane and valid flag;
+- the 512^3 composite body (the smooth union of 512 spheres drawn from the seed) at the grid
+  position the benchmark compares: the culled fold's phi and every column bitwise the plain
+  fold's at every node and cell, three slabs of four layers through the body bitwise the oracle's
+  (the lattice is dyadic, so a slab's own origin reproduces the nodes exactly), and the surrogate
+  (a network trained briefly here) with every column bitwise the plain fold's, phi bitwise at
+  every corner of every cut cell and of the same sign everywhere.
 The reference's invariants on the device output at the same sizes: shared faces bitwise
 single-valued, closure exactly zero (src/cut_cell.jl:302-317; test/test_cache.jl:187-190).
 """
@@ -18,7 +24,8 @@ import pytest
 
 import oracle as orc
 from cutcell import (CartesianIsoGrid, MarchingCubesCutCell, PLICCutCell, SDFSphere, SDFTorus,
-                     allocate_cache, closure_residual, random_rotation, update_cache)
+                     Surrogate, allocate_cache, closure_residual, random_rotation, update_cache)
+from cutcell.workloads import csg_spheres
 from helpers import shared_face_mismatches
 
 pytestmark = pytest.mark.gpu
@@ -80,4 +87,59 @@ def test_plic_384_torus_full_size_bitwise():
     assert shared_face_mismatches(ff, g.n, 3) == 0
     assert np.count_nonzero(closure_residual(ff, g.d)) == 0
     del ff, c
+    gc.collect()
+
+
+def test_composite_body_512_full_size_bitwise():
+    body = csg_spheres(SEED, 512)
+    base = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=512)
+    d = base.d[0]
+    g = base.moved((-0.5 * d * 20, 0.0, 0.0))
+    plain = update_cache(allocate_cache(g, MarchingCubesCutCell), body, g)
+    culled = allocate_cache(g, MarchingCubesCutCell)
+    culled.set_phi_source("culled")
+    update_cache(culled, body, g)
+    net = Surrogate(nfreq=64, width=256, depth=4, sigma=8.0, seed=SEED & 0xFFFF)
+    net.train(body, nsamples=1 << 21, lo=-1.0, hi=1.0, band=0.05, epochs=8, batch=16384, lr=5e-3,
+              seed=SEED)
+    sur = allocate_cache(g, MarchingCubesCutCell)
+    sur.set_phi_source("surrogate", net, coarse=8, band_cells=3.0)
+    update_cache(sur, body, g)
+    print(f"512^3 composite at x0 {g.x0}; surrogate {sur.phi_stats()}", flush=True)
+    n0 = n1 = n2 = 512
+    m0, m1 = n0 + 1, n1 + 1
+    plane, layers, cut, differing = n0 * n1, 32, 0, 0
+    for k0 in range(0, n2, layers):
+        k1 = k0 + layers
+        for name in COLS:
+            a = plain.read(name, k0 * plane, k1 * plane)
+            assert same_bits(culled.read(name, k0 * plane, k1 * plane), a), (k0, name)
+            assert same_bits(sur.read(name, k0 * plane, k1 * plane), a), (k0, name)
+        lo, hi = m0 * m1 * k0, m0 * m1 * (k1 + 1)
+        pe = plain.read("phi", lo, hi)
+        assert same_bits(culled.read("phi", lo, hi), pe), k0
+        ps = sur.read("phi", lo, hi)
+        assert np.array_equal(pe > 0, ps > 0), k0
+        body_nodes = pe.size if k1 == n2 else m0 * m1 * layers  # each node layer once
+        differing += int((pe[:body_nodes].view(np.uint64) != ps[:body_nodes].view(np.uint64)).sum())
+        cells = np.flatnonzero(plain.read("kind", k0 * plane, k1 * plane) == 2)
+        cut += cells.size
+        i, j, k = cells % n0, (cells // n0) % n1, cells // plane
+        for dk in (0, 1):
+            for dj in (0, 1):
+                for di in (0, 1):
+                    idx = (i + di) + m0 * ((j + dj) + m1 * (k + dk))
+                    assert same_bits(ps[idx], pe[idx]), k0
+    assert cut == plain.cut_count() and cut > 1_000_000
+    assert differing > 0  # the network's values are kept away from the surface
+    for k0 in (160, 256, 352):
+        x0 = (g.x0[0], g.x0[1], g.x0[2] + k0 * d)
+        phi = orc.sample_nodes(body, 3, x0, g.d, (n0, n1, 4))
+        assert same_bits(plain.read("phi", m0 * m1 * k0, m0 * m1 * (k0 + 5)), phi), k0
+        ref = orc.mc_cells(phi, x0, g.d, (n0, n1, 4))
+        assert int((ref["kind"] == 2).sum()) > 10_000
+        for name in COLS:
+            assert same_bits(plain.read(name, k0 * plane, (k0 + 4) * plane), ref[name]), (k0, name)
+    del plain, culled, sur
+    net.close()
     gc.collect()
