@@ -13,7 +13,11 @@ on the benchmark's own inputs (its default seed, 20261203):
   fold's at every node and cell, three slabs of four layers through the body bitwise the oracle's
   (the lattice is dyadic, so a slab's own origin reproduces the nodes exactly), and the surrogate
   (a network trained briefly here) with every column bitwise the plain fold's, phi bitwise at
-  every corner of every cut cell and of the same sign everywhere.
+  every corner of every cut cell and of the same sign everywhere;
+- the batched workload (64 moving bodies from the seed, 64^3-256^3 cells each, marching cubes and
+  PLIC mixed) at its last step after the batch autotuner: every body's columns from the one graph
+  launch bitwise what cc_update gives that body alone, and the largest marching-cubes and PLIC
+  bodies bitwise the oracle's.
 The reference's invariants on the device output at the same sizes: shared faces bitwise
 single-valued, closure exactly zero (src/cut_cell.jl:302-317; test/test_cache.jl:187-190).
 """
@@ -142,4 +146,41 @@ def test_composite_body_512_full_size_bitwise():
             assert same_bits(plain.read(name, k0 * plane, (k0 + 4) * plane), ref[name]), (k0, name)
     del plain, culled, sur
     net.close()
+    gc.collect()
+
+
+def test_batched_64_bodies_full_size_bitwise():
+    from cutcell.autotune import tune_batch
+    from cutcell.batch import Batch
+    from cutcell.workloads import MovingBodies
+    wl = MovingBodies(seed=SEED, nbodies=64)
+    meth = {"mc": MarchingCubesCutCell, "plic": PLICCutCell}
+    nb = len(wl.bodies)
+    caches = [allocate_cache(wl.grid(b, 0), meth[B["method"]]) for b, B in enumerate(wl.bodies)]
+    batch = Batch(caches)
+    step = 31
+    geos = [wl.geometry(b, step) for b in range(nb)]
+    grids = [wl.grid(b, step) for b in range(nb)]
+    kz = tune_batch(batch, [wl.geometry(b, 0) for b in range(nb)], [wl.grid(b, 0) for b in range(nb)])
+    batch.step(geos, grids)
+    print(f"64 bodies, {wl.ncells()} cells, depths {sorted(set(kz.values()))}", flush=True)
+    for b in range(nb):
+        solo = update_cache(allocate_cache(grids[b], meth[wl.bodies[b]["method"]]), geos[b], grids[b])
+        for name in COLS:
+            assert same_bits(caches[b].read(name), solo.read(name)), (b, name)
+        solo.close()
+    for m in ("mc", "plic"):
+        b = max((i for i in range(nb) if wl.bodies[i]["method"] == m), key=lambda i: grids[i].ncells)
+        g = grids[b]
+        if m == "mc":
+            ref = orc.mc_cells(orc.sample_nodes(geos[b], 3, g.x0, g.d, g.n), g.x0, g.d, g.n)
+            names = COLS
+        else:
+            ref, *_ = orc.plic(geos[b], 3, g.x0, g.d, g.n)
+            names = ("kind", "volume_fraction", "face_fraction")
+        assert int((ref["kind"] == 2).sum()) > 1000, (m, b)
+        for name in names:
+            assert same_bits(caches[b].read(name), ref[name]), (m, b, name)
+    batch.close()
+    del caches
     gc.collect()
