@@ -108,6 +108,44 @@ def test_a_changed_face_is_found():
         {k: d[k] for k in ("cut", "inside", "outside", "min_volume_fraction")}
 
 
+def test_planted_faces_at_every_seam_are_found():
+    # one changed face fraction or face centroid per seam of the pass's march, each a single mismatch:
+    # +x inside a strip and from lane 63 to the next strip; +y inside a strip, between the strips of
+    # a band (wave to wave) and past a band (16 rows, read again); +z inside a band of layers and past
+    # it (64 layers); rows and lanes past the grid's edges on a 70 x 40 x 140 grid
+    g = CartesianGrid((-1.05, -0.6, -2.1), (70, 40, 140), (0.03, 0.03, 0.03))
+    c = update_cache(allocate_cache(g, MarchingCubesCutCell), SDFSphere((0.0, 0.0, 0.0), 0.5), g)
+    assert check(c, g)["face_mismatches"] == 0
+    n0, n1 = g.n[0], g.n[1]
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime the library links
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    ffp, _ = c.column_ptr("face_fraction")
+    fcp, _ = c.column_ptr("face_centroid_local")
+    ff = c.read("face_fraction")
+    fc = c.read("face_centroid_local").reshape(-1, 12)
+
+    def cell(i, j, k):
+        return i + n0 * (j + n1 * k)
+
+    def plant(ptr, offset, old):
+        v = np.array([old + 0.123456], dtype=np.float64)
+        assert hip.hipMemcpy(ctypes.c_void_p(ptr + offset * 8), v.ctypes.data, 8, 1) == 0
+
+    faces = [(10, 20, 70, 1), (63, 20, 70, 1), (30, 5, 70, 3), (30, 3, 71, 3), (30, 7, 72, 2),
+             (30, 15, 73, 3), (31, 35, 74, 3), (30, 20, 10, 5), (30, 21, 63, 5), (30, 22, 64, 4),
+             (69, 39, 139, 0)]
+    for i, j, k, slot in faces:
+        c0 = cell(i, j, k)
+        plant(ffp, 6 * c0 + slot, ff[c0, slot])
+    cents = [(62, 11, 50, 1), (40, 11, 50, 3), (41, 31, 127, 5)]  # +x, wave seam +y, band-of-layers +z
+    for i, j, k, slot in cents:
+        c0 = cell(i, j, k)
+        plant(fcp, 12 * c0 + 2 * slot + 1, fc[c0, 2 * slot + 1])
+    assert hip.hipDeviceSynchronize() == 0  # a pageable copy may return before it lands
+    d = diagnostics(c, g)
+    assert d["face_mismatches"] == len(faces) and d["centroid_mismatches"] == len(cents), d
+
+
 def read_vtk(path):
     """The arrays of a VTK XML file with raw appended data and UInt64 block headers, by name (the
     points of an unstructured grid as "points"); the header must parse as XML."""
