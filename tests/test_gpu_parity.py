@@ -295,6 +295,56 @@ def test_plic_2d_golden_and_bitwise():
         update_cache(ci, geo, CartesianGrid((-1.5, -1.5), (30, 15), (0.1, 0.2)))
 
 
+def test_plic_float32_2d():
+    # test/test_plic.jl:238-250: a Float32 grid and body; every column, plane and intercept in
+    # Float32, every face fraction finite, every volume fraction in [0, 1]
+    f = lambda v: float(np.float32(v))
+    g = CartesianGrid((f(-0.6), f(-0.6)), (24, 24), (f(0.05), f(0.05)))
+    c = update_cache(allocate_cache(g, PLICCutCell, np.float32), SDFSphere((f(0.02), f(0.03)), f(0.31)), g)
+    for name in ("volume_fraction", "face_fraction", "centroid", "face_centroid_local", "interface_centroid"):
+        assert c.read(name).dtype == np.float32, name
+    assert c.normals.dtype == np.float32 and c.intercepts.dtype == np.float32
+    assert np.all(np.isfinite(c.read("face_fraction")))
+    vf = c.read("volume_fraction")
+    assert np.all((vf >= 0) & (vf <= 1)) and int((c.read("kind") == 2).sum()) > 20
+
+
+def test_the_uniform_readers_read_the_caches_own_storage():
+    # test/test_cache.jl:250-287: every cache answers face_fractions / volume_fractions / kinds from its
+    # own storage, is_cell_open agrees with the per-cell reading, and a write into that storage (the
+    # device columns) is what the readers then return
+    import ctypes
+    from cutcell import face_fractions, is_cell_open, kinds, volume_fractions
+    g2 = CartesianGrid((-1.3, -1.25), (48, 44), (0.055, 0.055))
+    g3 = CartesianGrid((-1.5,) * 3, (24,) * 3, (0.125,) * 3)
+    circle, sphere = SDFSphere((0.013, -0.021), 0.83), SDFSphere((0.01, 0.02, -0.03), 1.0)
+    ms = update_cache(allocate_cache(g2, MarchingSquaresCutCell), circle, g2)
+    mc = update_cache(allocate_cache(g3, MarchingCubesCutCell), sphere, g3)
+    pl = update_cache(allocate_cache(g2, PLICCutCell), circle, g2)
+    for c, g in ((ms, g2), (mc, g3), (pl, g2)):
+        n = int(np.prod(g.n))
+        for reader, name, per in ((face_fractions, "face_fraction", 2 * g.dim * 8),
+                                  (volume_fractions, "volume_fraction", 8), (kinds, "kind", 1)):
+            assert same_bits(reader(c), c.read(name)), name
+            assert c.column_ptr(name)[1] == n * per, name  # the column is the cache's storage
+        ff = face_fractions(c)
+        opened = is_cell_open(c)
+        assert np.array_equal(opened, np.array([bool(np.any(row != 0)) for row in ff]))
+        assert int((~opened).sum()) > 0
+    cell = int(np.flatnonzero(kinds(ms) == 2)[0])
+    assert is_cell_open(ms)[cell]
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime the library links
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    zeros, inside = np.zeros(4), np.array([0], dtype=np.int8)
+    ffp, _ = ms.column_ptr("face_fraction")
+    kp, _ = ms.column_ptr("kind")
+    assert hip.hipMemcpy(ctypes.c_void_p(ffp + 4 * 8 * cell), zeros.ctypes.data, 32, 1) == 0
+    assert hip.hipMemcpy(ctypes.c_void_p(kp + cell), inside.ctypes.data, 1, 1) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert np.all(face_fractions(ms)[cell] == 0) and kinds(ms)[cell] == 0
+    assert not is_cell_open(ms)[cell]  # it reads the edited fractions
+
+
 def test_plic_3d_torus_box_degenerate_bitwise():
     rng = np.random.default_rng(11)
     g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=48)
