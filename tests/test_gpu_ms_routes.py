@@ -1,8 +1,9 @@
 """The reference's marching-squares moment tests that go through a cache
 (test/test_moments.jl:286-659), on the device through the C ABI: convergence of the apertures and of
 the volume against closed forms, the interface facet recomputed independently from the cache's own
-corner values, exact closure, the report of a banded tree, a plain field, the contour against the
-moments, and face centroids single-valued on an off-lattice grid.
+corner values, exact closure, the report of a banded tree, children of a refined leaf cut afresh, a
+plain field, the contour against the moments and on its own domain only, and face centroids
+single-valued on an off-lattice grid.
 
 Bar: the reference's own thresholds (rates, tolerances, counts), and bitwise equality where the
 reference asks `===` or `==`.
@@ -12,8 +13,9 @@ import math
 import numpy as np
 import pytest
 
-from cutcell import (CartesianGrid, CartesianIsoGrid, MarchingSquaresCutCell, SDFSphere, allocate_cache,
-                     cut_cell_report, generate_mesh, readers, update_cache)
+from cutcell import (AdaptiveMesh, CartesianGrid, CartesianIsoGrid, DimensionMismatch, MarchingSquaresCutCell,
+                     SDFSphere, Unsupported, allocate_cache, cut_cell_report, generate_mesh, readers,
+                     update_cache)
 from helpers import GEO_CENTRE, GEO_RADIUS, banded_tree, circle_sdf
 
 pytestmark = pytest.mark.gpu
@@ -176,3 +178,44 @@ def test_face_centroids_are_single_valued_on_an_off_lattice_grid():
         outside_by_cut += int((((kind[ci] == 1) & (kind[cj] == 2)) | ((kind[ci] == 2) & (kind[cj] == 1))).sum())
     assert outside_by_cut > 50  # the case the test exists for is exercised
     assert mismatches == 0
+
+
+def test_children_are_recomputed_never_inherited():
+    # test/test_moments.jl:556-585: one cut leaf of a uniform 32 x 32 tree refined; its four children
+    # are cut afresh from the body, not copied from the parent
+    mesh = AdaptiveMesh(uniform(32), max_level=1, initial_level=0)
+    c = update_cache(allocate_cache(mesh, MarchingSquaresCutCell), GEO, mesh)
+    kind, vf = c.read("kind"), c.read("volume_fraction")
+    i = int(np.flatnonzero((kind == 2) & (vf < 0.35))[0])
+    parent_frac = float(vf[i])
+    h = mesh.elem_size(i)
+    parent_vol = parent_frac * h[0] * h[1]
+    pcoord = mesh.coord[i].copy()
+    flags = np.zeros(mesh.nleaves, dtype=bool)
+    flags[i] = True
+    mesh.refine(flags)
+    c = update_cache(allocate_cache(mesh, MarchingSquaresCutCell), GEO, mesh)
+    kids = np.flatnonzero((mesh.level == 1) & np.all(mesh.coord // 2 == pcoord, axis=1))
+    assert kids.size == 4
+    vf = c.read("volume_fraction")
+    child = vf[kids]
+    assert not np.allclose(child, parent_frac)  # not a copied fraction
+    assert np.any(child == 0)                   # at least one wholly inside
+    hk = mesh.elem_size(kids[0])
+    assert abs(float(child.sum()) * hk[0] * hk[1] - parent_vol) <= 0.2 * parent_vol
+    ff = c.read("face_fraction")[kids]
+    assert np.count_nonzero(readers.closure_residual(ff, tuple(hk))) == 0
+
+
+def test_a_caches_contour_is_asked_for_on_its_own_domain():
+    # test/test_moments.jl:623-633: another grid, another tree, or a tree for a grid cache is refused
+    g = CartesianIsoGrid((-1.5, -1.5), (1.5, 1.5), sz=0.1)
+    c = circle_cache(g)
+    with pytest.raises(DimensionMismatch):
+        generate_mesh(c, CartesianGrid(g.x0, tuple(n + 1 for n in g.n), g.d))
+    tree = banded_tree()
+    tc = update_cache(allocate_cache(tree, MarchingSquaresCutCell), GEO, tree)
+    with pytest.raises(DimensionMismatch):
+        generate_mesh(tc, banded_tree(level=2))
+    with pytest.raises(Unsupported):
+        generate_mesh(c, tree)
