@@ -144,9 +144,9 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag(Cols<T> cols, GridT<T> g, F
     // the layer before: each row's +z face fraction and centroid, per wave, row and lane (lane
     // fastest, so a wave's access is one contiguous run); a lane reads back only what it wrote
     __shared__ T zq[NW][DG_TJ][ZQ][64];
-    // each wave's first row's -y face, for the wave below it in the band; two buffers by layer parity,
-    // so one barrier per layer separates a layer's writes from its reads and from the writes of the
-    // layer after next
+    // each wave's first row's -y face, for the previous wave (the rows just before it); two buffers
+    // used in turn layer by layer, so one barrier per layer separates a layer's writes from its reads
+    // and from the writes of the layer after next
     __shared__ T yq[2][NW][D][64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: strip bounds in scalars
@@ -174,8 +174,8 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag(Cols<T> cols, GridT<T> g, F
         const int k0 = D == 3 ? tz * DG_KZ : 0;
         const int k1 = D == 3 ? (k0 + DG_KZ < n2 ? k0 + DG_KZ : n2) : 1;
         const bool vi = i < n0, xn = i + 1 < n0;
-        // the wave below in the band starts at j1 and has rows: its first -y face comes through yq
-        const bool below = wave + 1 < NW && j1 < n1;
+        // the next wave in the band starts at j1 and has rows: its first -y face comes through yq
+        const bool from_next = wave + 1 < NW && j1 < n1;
         // lanes past the grid read its last cell and count nothing; a lane's offset within its row
         // of the strip is small, the row's start is the same for the whole wave
         const uint32_t lo = (uint32_t)(vi ? lane : n0 - 1 - tx * 64);
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag(Cols<T> cols, GridT<T> g, F
                 if (vi && xn) shared_face<T, D>(a, ff[1], fc + (D - 1), nf, nc);
                 // +y: the row before's +y face against this row's -y face
                 if (vi && j > j0) shared_face<T, D>(a, yf, yc, ff[2], fc + 2 * (D - 1));
-                if (j == j0 && wave > 0) {  // the wave above compares its last row against this row
+                if (j == j0 && wave > 0) {  // the previous wave compares its last row against this row
                     yq[par][wave][0][lane] = ff[2];
 #pragma unroll
                     for (int q = 0; q < D - 1; ++q) yq[par][wave][1 + q][lane] = fc[2 * (D - 1) + q];
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(DG_THREADS) void k_diag(Cols<T> cols, GridT<T> g, F
             __syncthreads();
             if (vi && j0 < j1 && j1 < n1) {  // the strip's last row against the first row past it
                 T g0, gc[D - 1];
-                if (below) {
+                if (from_next) {
                     g0 = yq[par][wave + 1][0][lane];
 #pragma unroll
                     for (int q = 0; q < D - 1; ++q) gc[q] = yq[par][wave + 1][1 + q][lane];
