@@ -227,13 +227,25 @@ template <typename T> cc::Cols<T> cols_t(const cc_cache *c) {
 int check_geometry(const cc_geometry *geo, int dim) {
     if (!geo) return fail(CC_ERR_ARGUMENT, "geometry is NULL");
     const int base = geo->kind & ~CC_GEO_BOUNDED;
-    if ((geo->kind & ~(CC_GEO_BOUNDED | 0xff)) != 0 || base < CC_GEO_EMPTY || base > CC_GEO_SPHERE_UNION)
+    if ((geo->kind & ~(CC_GEO_BOUNDED | 0xff)) != 0 || base < CC_GEO_EMPTY || base > CC_GEO_UNION)
         return fail(CC_ERR_UNSUPPORTED, "unknown geometry kind %d", geo->kind);
     if (base == CC_GEO_TORUS && dim != 3) return fail(CC_ERR_UNSUPPORTED, "a torus is a 3D body");
     if (base == CC_GEO_SPHERE_UNION) {
         if (geo->n_prims < 1 || !geo->prims)
             return fail(CC_ERR_ARGUMENT, "a sphere union needs at least one primitive");
         if (!(geo->p[0] > 0.0)) return fail(CC_ERR_ARGUMENT, "the smoothing k must be positive");
+    }
+    if (base == CC_GEO_UNION) {
+        if (geo->n_prims < 1 || !geo->prims)
+            return fail(CC_ERR_ARGUMENT, "a union needs at least one child");
+        if (!(geo->p[0] >= 0.0) || !std::isfinite(geo->p[0]))
+            return fail(CC_ERR_ARGUMENT, "the smoothing k must be finite and >= 0");
+        for (int i = 0; i < geo->n_prims; ++i) {
+            const double ck = geo->prims[(size_t)CC_UNION_RECORD * i];
+            if (!(ck == CC_GEO_SPHERE || ck == CC_GEO_TORUS || ck == CC_GEO_BOX || ck == CC_GEO_PLANE))
+                return fail(CC_ERR_UNSUPPORTED, "union child %d: kind %g is not a sphere, torus, box or plane", i, ck);
+            if (ck == CC_GEO_TORUS && dim != 3) return fail(CC_ERR_UNSUPPORTED, "a torus is a 3D body");
+        }
     }
     if (geo->kind & CC_GEO_BOUNDED) {
         if (base == CC_GEO_EMPTY || base == CC_GEO_PLANE)
@@ -247,6 +259,15 @@ int check_geometry(const cc_geometry *geo, int dim) {
     return CC_OK;
 }
 
+// Doubles in a checked geometry's primitive table (host memory): four per sphere of a sphere union,
+// CC_UNION_RECORD per child of a mixed union, none for any other body.
+int64_t prim_doubles(const cc_geometry *geo) {
+    const int base = geo->kind & ~CC_GEO_BOUNDED;
+    if (base == CC_GEO_SPHERE_UNION) return 4 * (int64_t)geo->n_prims;
+    if (base == CC_GEO_UNION) return CC_UNION_RECORD * (int64_t)geo->n_prims;
+    return 0;
+}
+
 int stage_geometry(cc_cache *c, const cc_geometry *geo, cc::Geo *out) {
     const int rc = check_geometry(geo, c->dim);
     if (rc) return rc;
@@ -254,8 +275,8 @@ int stage_geometry(cc_cache *c, const cc_geometry *geo, cc::Geo *out) {
     out->nprims = 0;
     std::memcpy(out->p, geo->p, sizeof out->p);
     out->prims = nullptr;
-    if ((geo->kind & ~CC_GEO_BOUNDED) == CC_GEO_SPHERE_UNION) {
-        const int64_t nd = 4 * (int64_t)geo->n_prims;
+    if (prim_doubles(geo) > 0) {
+        const int64_t nd = prim_doubles(geo);
         bool same = (int64_t)c->prims_host.size() == nd &&
                     std::memcmp(c->prims_host.data(), geo->prims, nd * sizeof(double)) == 0;
         if (!same) {
@@ -359,7 +380,7 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
         // re-read); a sphere union is sampled once by its own pass and then read.
         cc::McTuning tn = c->tune;
         const bool fuse = c->method == CC_MARCHING_CUBES && geo && tn.variant == cc::MC_MARCH_FUSED &&
-                          geo->kind != cc::GEO_SPHERE_UNION;
+                          geo->kind != cc::GEO_SPHERE_UNION && (geo->kind & cc::GEO_KIND) != cc::GEO_UNION;
         if (!fuse && tn.variant == cc::MC_MARCH_FUSED) tn.variant = cc::MC_MARCH;
         if (geo && !fuse && geo->kind == cc::GEO_SPHERE_UNION && c->phi_source != CC_PHI_NAIVE) {
             const int rc = phi_bricks<T>(c, *geo, gt, phi);
@@ -383,8 +404,9 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
             cc::launch_ms_cells<T>(phi, gt, cols, c->counter, c->stream);
             mark(c, "ms_cells");
         }
-    } else if (c->dim == 3 && c->tune.variant != 0) {
-        // the fused 3D pass: fit and resolved faces in one sweep
+    } else if (c->dim == 3 && c->tune.variant != 0 && (geo->kind & cc::GEO_KIND) != cc::GEO_UNION) {
+        // the fused 3D pass: fit and resolved faces in one sweep (a mixed union takes the two
+        // passes, whose fit kernel carries its evaluation)
         cc::launch_plic_fused<T>(gt, *geo, cols, (T *)c->col[CC_FIELD_NORMALS],
                                  (T *)c->col[CC_FIELD_INTERCEPTS], (uint8_t *)c->col[CC_FIELD_IS_VALID],
                                  (uint32_t)c->tune.kz, c->stream);
@@ -1023,15 +1045,16 @@ void contour_leaves(int64_t ncut, const int64_t *idx, const T *phi, const int32_
     }
 }
 
-// Marks and ranks the cut cells (or leaves) of the cache on the device; *ncut receives their number.
-int mark_cut_cells(cc_cache *c, int64_t n, int64_t *ncut) {
+// Marks and ranks the cut cells (or leaves) of the cache on the device from the corner signs of its
+// field; *ncut receives their number.  n0: the grid's cells along x, or 0 for a leaf cache.
+template <typename T> int mark_cut_cells(cc_cache *c, int64_t n, int64_t n0, int64_t *ncut) {
     const int64_t nwords = (n + 63) / 64;
     int rc = ensure(&c->mark_bits, &c->mark_bits_cap, nwords);
     if (rc == CC_OK) rc = ensure(&c->mark_rank, &c->mark_rank_cap, nwords);
     if (rc == CC_OK) rc = surface_work(c, nwords);
     if (rc) return rc;
-    cc::launch_cut_marks((const int8_t *)c->col[CC_FIELD_KIND], (size_t)n, c->mark_bits, c->mark_rank, c->ps.part,
-                         c->ps.part_cap, c->ps.totals, c->stream);
+    cc::launch_cut_marks<T>((const T *)c->col[CC_FIELD_PHI], (size_t)n, (size_t)n0, c->mark_bits, c->mark_rank,
+                            c->ps.part, c->ps.part_cap, c->ps.totals, c->stream);
     HIPC(hipGetLastError());
     int64_t tot[2] = {0, 0};
     HIPC(hipMemcpyAsync(tot, c->ps.totals, sizeof tot, hipMemcpyDeviceToHost, c->stream));
@@ -1046,7 +1069,7 @@ int mark_cut_cells(cc_cache *c, int64_t n, int64_t *ncut) {
 // on the device, welded on the host.
 template <typename T> int grid_contour(cc_cache *c, const cc_grid *g) {
     int64_t ncut = 0;
-    int rc = mark_cut_cells(c, c->ncell, &ncut);
+    int rc = mark_cut_cells<T>(c, c->ncell, c->n[0], &ncut);
     if (rc) return rc;
     const int64_t m = ncut > 0 ? ncut : 1;
     rc = ensure(&c->mark_idx, &c->mark_idx_cap, m);
@@ -1073,7 +1096,7 @@ template <typename T> int grid_contour(cc_cache *c, const cc_grid *g) {
 template <typename T> int tree_contour(cc_cache *c, const cc_leaves *tree) {
     const int64_t n = tree->n;
     int64_t ncut = 0;
-    int rc = mark_cut_cells(c, n, &ncut);
+    int rc = mark_cut_cells<T>(c, n, 0, &ncut);
     if (rc) return rc;
     const int64_t m = ncut > 0 ? ncut : 1;
     rc = ensure(&c->mark_idx, &c->mark_idx_cap, m);
@@ -1580,7 +1603,8 @@ int cc_calc_volume(int dtype, const cc_geometry *geo, const cc_grid *grid, doubl
     if (grc) return grc;
     cc::Geo gd;
     gd.kind = geo->kind;
-    gd.nprims = (geo->kind & ~CC_GEO_BOUNDED) == CC_GEO_SPHERE_UNION ? geo->n_prims : 0;
+    gd.nprims = prim_doubles(geo) > 0 ? geo->n_prims : 0;
+    const size_t prim_bytes = 8 * (size_t)prim_doubles(geo);
     std::memcpy(gd.p, geo->p, sizeof gd.p);
     gd.prims = nullptr;
     hipStream_t st = nullptr;
@@ -1593,8 +1617,8 @@ int cc_calc_volume(int dtype, const cc_geometry *geo, const cc_grid *grid, doubl
     const int nb = dtype == CC_FLOAT32 ? cc::plic_volume_blocks(g32, dim) : cc::plic_volume_blocks(g64, dim);
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipMalloc(&part, es * nb) != hipSuccess ||
         hipMalloc(&out, es) != hipSuccess ||
-        (gd.nprims > 0 && (hipMalloc((void **)&prims, 32 * (size_t)gd.nprims) != hipSuccess ||
-                           hipMemcpy(prims, geo->prims, 32 * (size_t)gd.nprims, hipMemcpyHostToDevice) != hipSuccess))) {
+        (gd.nprims > 0 && (hipMalloc((void **)&prims, prim_bytes) != hipSuccess ||
+                           hipMemcpy(prims, geo->prims, prim_bytes, hipMemcpyHostToDevice) != hipSuccess))) {
         rc = fail(CC_ERR_OUT_OF_MEMORY, "calc_volume: could not set up its work arrays");
     } else {
         gd.prims = prims;

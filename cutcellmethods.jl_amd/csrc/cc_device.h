@@ -17,14 +17,17 @@ template <> struct Eps<float> { static constexpr float v = 1.1920928955078125e-0
 enum : int8_t { CELL_INSIDE = 0, CELL_OUTSIDE = 1, CELL_CUT = 2 };  // src/CutCellMethods.jl:56-58
 
 enum { GEO_EMPTY = 0, GEO_SPHERE = 1, GEO_TORUS = 2, GEO_BOX = 3, GEO_PLANE = 4,
-       GEO_SPHERE_UNION = 5, GEO_BOUNDED = 0x100, GEO_KIND = 0xff };
+       GEO_SPHERE_UNION = 5, GEO_UNION = 6, GEO_BOUNDED = 0x100, GEO_KIND = 0xff };
+
+// Doubles per child of a mixed union: the child's kind, then its own parameters p[0..14].
+constexpr int UNION_REC = 16;
 
 // The body as the kernels see it: parameters by value, the primitive table in device memory.
 struct Geo {
     int kind;
     int nprims;
     double p[24];
-    const double *prims;  // device pointer, nprims x (cx, cy, cz, r)
+    const double *prims;  // device pointer: nprims x (cx, cy, cz, r), or nprims x UNION_REC
 };
 
 // The grid as the kernels see it, converted to the cache's element type once on the host
@@ -145,46 +148,42 @@ template <typename T, int D> __device__ __forceinline__ bool beyond_band(const G
     return s > pad * pad;
 }
 
-// get_sdf(geo, x) -> (distance, normal) (SDFLibrary, un-vendored: the build's own bodies).  The
-// normal is the analytic gradient; it is NaN or zero where the distance is not differentiable,
-// which the PLIC fit reports as a degenerate plane (src/plic/plic.jl:82-85).
+// One primitive body's get_sdf -> (distance, normal) from its kind and parameters p (a body's own
+// p, or a union child's record after its kind).  The normal is the analytic gradient; it is NaN or
+// zero where the distance is not differentiable, which the PLIC fit reports as a degenerate plane
+// (src/plic/plic.jl:82-85).  Returns NaN with a zero normal for a kind that is not a primitive.
 template <typename T, int D>
-__device__ __forceinline__ T get_sdf(const Geo &g, const T *x, T *normal) {
+__device__ __forceinline__ T primitive_sdf(int kind, const double *pp, const T *x, T *normal) {
     T v[3] = {T(0), T(0), T(0)}, p[3] = {T(0), T(0), T(0)}, gl[3] = {T(0), T(0), T(0)};
-    if ((g.kind & GEO_BOUNDED) && beyond_band<T, D>(g, x)) {
-#pragma unroll
-        for (int a = 0; a < D; ++a) normal[a] = T(0);
-        return T(g.p[23]);
-    }
-    switch (g.kind & GEO_KIND) {
+    switch (kind) {
     case GEO_SPHERE: {
         T nr;
-        T dist = sphere_dist<T, D>(T(g.p[0]), T(g.p[1]), T(g.p[2]), T(g.p[3]), x[0], x[1],
+        T dist = sphere_dist<T, D>(T(pp[0]), T(pp[1]), T(pp[2]), T(pp[3]), x[0], x[1],
                                    D == 3 ? x[2] : T(0), &nr, v);
 #pragma unroll
         for (int a = 0; a < D; ++a) normal[a] = v[a] / nr;
         return dist;
     }
     case GEO_TORUS: {  // 3D only
-        const double *r = g.p + 5;
-        to_local<T, 3>(r, g.p, x, p);
+        const double *r = pp + 5;
+        to_local<T, 3>(r, pp, x, p);
         T rho = sqrt(p[0] * p[0] + p[1] * p[1]);
-        T qx = rho - T(g.p[3]), qz = p[2];
+        T qx = rho - T(pp[3]), qz = p[2];
         T nq = sqrt(qx * qx + qz * qz);
         T u = qx / nq;
         gl[0] = u * (p[0] / rho);
         gl[1] = u * (p[1] / rho);
         gl[2] = qz / nq;
         from_local<T, 3>(r, gl, normal);
-        return nq - T(g.p[4]);
+        return nq - T(pp[4]);
     }
     case GEO_BOX: {
-        const double *r = g.p + 6;
+        const double *r = pp + 6;
         T q[3] = {T(0), T(0), T(0)}, mx[3] = {T(0), T(0), T(0)};
-        to_local<T, D>(r, g.p, x, p);
+        to_local<T, D>(r, pp, x, p);
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            q[a] = fabs(p[a]) - T(g.p[3 + a]);
+            q[a] = fabs(p[a]) - T(pp[3 + a]);
             mx[a] = q[a] > T(0) ? q[a] : T(0);
         }
         T o = mx[0] * mx[0];
@@ -208,23 +207,88 @@ __device__ __forceinline__ T get_sdf(const Geo &g, const T *x, T *normal) {
         return o + inner;
     }
     case GEO_PLANE: {
-        T s = T(g.p[0]) * (x[0] - T(g.p[3]));
+        T s = T(pp[0]) * (x[0] - T(pp[3]));
 #pragma unroll
-        for (int a = 1; a < D; ++a) s = s + T(g.p[a]) * (x[a] - T(g.p[3 + a]));
+        for (int a = 1; a < D; ++a) s = s + T(pp[a]) * (x[a] - T(pp[3 + a]));
 #pragma unroll
-        for (int a = 0; a < D; ++a) normal[a] = T(g.p[a]);
+        for (int a = 0; a < D; ++a) normal[a] = T(pp[a]);
         return s;
     }
     default: break;
     }
 #pragma unroll
     for (int a = 0; a < D; ++a) normal[a] = T(0);
-    if ((g.kind & GEO_KIND) == GEO_EMPTY) return T(g.p[0]);
-    return union_dist<T, D>(g, x);  // the sphere union offers no analytic normal
+    return T(NAN);
+}
+
+// The union of mixed primitives (SDFUnion(children...; smoothing), SDFLibrary, un-vendored): the
+// children folded left with the polynomial smooth minimum of smoothing k = p[0]; k = 0 is the plain
+// minimum (a tie keeps the later child, as min(a, b) = a < b ? a : b does).  The normal is the
+// gradient of the fold: where a step takes the plain minimum it is the winner's normal unchanged;
+// where it blends (h = (k - |a - b|)/k > 0) it is (1 - h/2) n_min + (h/2) n_max, the derivative of
+// min(a, b) - h^2 k/4, and a blended normal is scaled to unit length once at the end.
+template <typename T, int D>
+__device__ __forceinline__ T mixed_union_sdf(const Geo &g, const T *x, T *normal) {
+    const T k = T(g.p[0]);
+    T acc = T(0);
+    bool blended = false;
+    for (int i = 0; i < g.nprims; ++i) {
+        const double *ri = g.prims + UNION_REC * i;
+        T nb[3] = {T(0), T(0), T(0)};
+        const T b = primitive_sdf<T, D>((int)ri[0], ri + 1, x, nb);
+        T h = k - fabs(acc - b);
+        const bool a_min = acc < b;
+        if (i == 0 || !(h > T(0))) {
+            if (i == 0 || !a_min) {
+                acc = b;
+#pragma unroll
+                for (int a = 0; a < D; ++a) normal[a] = nb[a];
+            }
+            continue;
+        }
+        h = h / k;
+        const T wmax = h / T(2), wmin = T(1) - wmax;
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+            normal[a] = a_min ? wmin * normal[a] + wmax * nb[a] : wmax * normal[a] + wmin * nb[a];
+        acc = (a_min ? acc : b) - ((h * h) * k) / T(4);
+        blended = true;
+    }
+    if (blended) {
+        T s = normal[0] * normal[0];
+#pragma unroll
+        for (int a = 1; a < D; ++a) s = s + normal[a] * normal[a];
+        s = sqrt(s);
+#pragma unroll
+        for (int a = 0; a < D; ++a) normal[a] = normal[a] / s;
+    }
+    return acc;
+}
+
+// get_sdf(geo, x) -> (distance, normal) (SDFLibrary, un-vendored: the build's own bodies).  MIXED =
+// false leaves the mixed union out of the code, for the fused kernels whose register budget it
+// would raise; the host never hands them a mixed union (they read NaN for one).
+template <typename T, int D, bool MIXED = true>
+__device__ __forceinline__ T get_sdf(const Geo &g, const T *x, T *normal) {
+    if ((g.kind & GEO_BOUNDED) && beyond_band<T, D>(g, x)) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) normal[a] = T(0);
+        return T(g.p[23]);
+    }
+    const int kind = g.kind & GEO_KIND;
+    if (kind >= GEO_SPHERE && kind <= GEO_PLANE) return primitive_sdf<T, D>(kind, g.p, x, normal);
+    if constexpr (MIXED) {
+        if (kind == GEO_UNION) return mixed_union_sdf<T, D>(g, x, normal);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) normal[a] = T(0);
+    if (kind == GEO_EMPTY) return T(g.p[0]);
+    if (kind == GEO_SPHERE_UNION) return union_dist<T, D>(g, x);  // no analytic normal offered
+    return T(NAN);
 }
 
 // sdf_value(geo, x) (SDFLibrary, un-vendored: the build's own bodies).
-template <typename T, int D>
+template <typename T, int D, bool MIXED = true>
 __device__ __forceinline__ T sdf_value(const Geo &g, const T *x) {
     const T z = D == 3 ? x[2] : T(0);
     if ((g.kind & GEO_BOUNDED) && beyond_band<T, D>(g, x)) return T(g.p[23]);
@@ -236,7 +300,7 @@ __device__ __forceinline__ T sdf_value(const Geo &g, const T *x) {
     default: break;
     }
     T nrm[3];
-    return get_sdf<T, D>(g, x, nrm);
+    return get_sdf<T, D, MIXED>(g, x, nrm);
 }
 
 }  // namespace cc

@@ -667,7 +667,7 @@ __device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L) {
             if (FUSED) {
                 const T x[3] = {g.x0[0] + T(gi) * g.d[0], g.x0[1] + T(gj) * g.d[1],
                                 g.x0[2] + T(k) * g.d[2]};
-                val = sdf_value<T, 3>(B.geo, x);
+                val = sdf_value<T, 3, false>(B.geo, x);
                 const bool own = own_layer && (r < rc || gj == n1) && (ii < wc || gi == n0);
                 if (own) B.phi[idx] = val;
             } else {

@@ -100,7 +100,7 @@ template <typename T> struct Plane {
     bool ok;
 };
 
-template <typename T, int D>
+template <typename T, int D, bool MIXED = true>
 __device__ __forceinline__ Plane<T> plic_plane(const GridT<T> &g, const Geo &geo, uint32_t i,
                                                uint32_t j, uint32_t k) {
     const uint32_t ijk[3] = {i, j, k};
@@ -109,7 +109,7 @@ __device__ __forceinline__ Plane<T> plic_plane(const GridT<T> &g, const Geo &geo
     for (int a = 0; a < D; ++a) x[a] = g.x0[a] + (T(ijk[a]) + T(0.5)) * g.d[a];  // get_elem_centroid
     Plane<T> P;
     P.n[0] = P.n[1] = P.n[2] = T(0);
-    const T dist = get_sdf<T, D>(geo, x, P.n);
+    const T dist = get_sdf<T, D, MIXED>(geo, x, P.n);
     T sum = P.n[0] + P.n[1];
     if (D == 3) sum = sum + P.n[2];
     T a = T(0.5) * sum - dist / g.d[0];
@@ -258,14 +258,14 @@ __device__ void plic_fused_item(const GridT<T> &g, const Geo &geo, const Cols<T>
     T cz_prev = T(0);
     T cc[6], cn[6];
     if (inside) {
-        if (k0 >= 1) cz_prev = plane_face(plic_plane<T, 3>(g, geo, i, j, k0 - 1), g.d, 5) / fullf[2];
-        cur = plic_plane<T, 3>(g, geo, i, j, k0);
+        if (k0 >= 1) cz_prev = plane_face(plic_plane<T, 3, false>(g, geo, i, j, k0 - 1), g.d, 5) / fullf[2];
+        cur = plic_plane<T, 3, false>(g, geo, i, j, k0);
 #pragma unroll
         for (int q = 0; q < 6; ++q) cc[q] = plane_face(cur, g.d, q) / fullf[q >> 1];
     }
     for (uint32_t k = k0; k < k1; ++k) {
         if (inside && k + 1 < n2) {
-            nxt = plic_plane<T, 3>(g, geo, i, j, k + 1);
+            nxt = plic_plane<T, 3, false>(g, geo, i, j, k + 1);
 #pragma unroll
             for (int q = 0; q < 6; ++q) cn[q] = plane_face(nxt, g.d, q) / fullf[q >> 1];
         }
@@ -274,7 +274,7 @@ __device__ void plic_fused_item(const GridT<T> &g, const Geo &geo, const Cols<T>
             for (int q = 0; q < 6; ++q) L.cand[r + 1][ii + 1][q] = cc[q];
         }
         if (hr >= 0)
-            L.cand[hr][hc][hslot] = plane_face(plic_plane<T, 3>(g, geo, hi_, hj, k), g.d, hslot) / fullf[hslot >> 1];
+            L.cand[hr][hc][hslot] = plane_face(plic_plane<T, 3, false>(g, geo, hi_, hj, k), g.d, hslot) / fullf[hslot >> 1];
         __syncthreads();
         if (inside) {
             T ff[6];

@@ -10,8 +10,8 @@ from .cache import (Cache, MarchingCubesCutCell, MarchingSquaresCutCell, PLICCut
                     extract_surface_plic, face_fractions, generate_mesh, is_cell_open, kinds,
                     update_cache, update_cache_from_device_phi, volume_fractions, write_cache_vtk,
                     write_npy)
-from .geometry import (EmptyGeo, SDFBounded, SDFBox, SDFCircle, SDFPlane, SDFSphere, SDFSphereUnion,
-                       SDFTorus, random_rotation)
+from .geometry import (EmptyGeo, SDFBounded, SDFBox, SDFCircle, SDFPlane, SDFRectangle, SDFSphere,
+                       SDFSphereUnion, SDFTorus, SDFUnion, random_rotation)
 from .grid import CartesianGrid, CartesianIsoGrid
 from .readers import (CELL_CUT, CELL_INSIDE, CELL_OUTSIDE, closure_residual, face_area_of,
                       face_centroid, full_face_area, interface_area, interface_normal,

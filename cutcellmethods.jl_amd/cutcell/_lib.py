@@ -190,7 +190,8 @@ def make_geometry(geo):
         g.p[i] = float(v)
     prims = getattr(geo, "prims", None)
     if prims is not None:
-        arr = np.ascontiguousarray(np.asarray(prims, dtype=np.float64).reshape(-1, 4))
+        arr = np.asarray(prims, dtype=np.float64)
+        arr = np.ascontiguousarray(arr if arr.ndim == 2 else arr.reshape(-1, 4))
         g.n_prims = arr.shape[0]
         g.prims = arr.ctypes.data
         g._keep = arr

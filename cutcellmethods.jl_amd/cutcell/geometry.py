@@ -17,6 +17,12 @@ SDFLibrary is not vendored with the reference; these are the build's own signed-
 * ``SDFPlane``: ``n . (x - p)``.
 * ``SDFSphereUnion``: spheres folded left with the polynomial smooth minimum
   ``h = max(k - |a-b|, 0)/k; min(a, b) - h*h*k/4``.
+* ``SDFUnion(children...; smoothing)``: spheres, tori, boxes and planes folded left the same way,
+  ``smoothing = 0`` being the plain minimum (``test/test_plic.jl:96-112`` unions a circle and a
+  rectangle so).  Its normal is the fold's gradient: the winning child's own normal wherever the fold
+  took a plain minimum, ``(1 - h/2) n_min + (h/2) n_max`` where it blended, scaled to unit length.
+* ``SDFRectangle(center, dims)``: a 2D box of side lengths ``dims`` (SDFLibrary's own reading of
+  ``dims`` is not available; side lengths, not half-widths, is this build's).
 * ``EmptyGeo``: the constant field ``value`` (the reference seeds every cache with ``EmptyGeo(1)``,
   ``src/marching_squares/cache.jl:46``).
 * ``SDFBounded(child, pad_distance, fill_distance)``: the child within ``pad_distance`` (Euclidean)
@@ -28,7 +34,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-GEO_EMPTY, GEO_SPHERE, GEO_TORUS, GEO_BOX, GEO_PLANE, GEO_SPHERE_UNION = range(6)
+GEO_EMPTY, GEO_SPHERE, GEO_TORUS, GEO_BOX, GEO_PLANE, GEO_SPHERE_UNION, GEO_UNION = range(7)
+UNION_RECORD = 16  # doubles per union child: its kind, then its own parameters
 GEO_BOUNDED = 0x100  # a flag on a bounded body's kind
 
 _EYE = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
@@ -136,6 +143,45 @@ class SDFSphereUnion:
         return [float(self.smoothing)]
 
 
+def SDFRectangle(center, dims, rotation=None):
+    """A 2D box of side lengths ``dims`` centred at ``center`` (``test/test_plic.jl:101``)."""
+    return SDFBox(tuple(center), tuple(0.5 * float(v) for v in dims), rotation)
+
+
+class SDFUnion:
+    """``SDFUnion(children...; smoothing)``: the children (spheres, tori, boxes, planes) folded left
+    with the polynomial smooth minimum, ``smoothing = 0`` the plain minimum."""
+    kind = GEO_UNION
+
+    def __init__(self, *children, smoothing=0.0):
+        if not children:
+            raise ValueError("a union needs at least one child")
+        for c in children:
+            if int(c.kind) not in (GEO_SPHERE, GEO_TORUS, GEO_BOX, GEO_PLANE):
+                raise ValueError("a union's children are spheres, tori, boxes or planes")
+        if not float(smoothing) >= 0.0:
+            raise ValueError("smoothing must be >= 0")
+        self.children = tuple(children)
+        self.smoothing = float(smoothing)
+        rec = np.zeros((len(children), UNION_RECORD))
+        for i, c in enumerate(children):
+            p = [float(v) for v in c.params]
+            rec[i, 0] = int(c.kind)
+            rec[i, 1:1 + len(p)] = p
+        self._prims = rec
+
+    def __repr__(self):
+        return f"SDFUnion({', '.join(map(repr, self.children))}; smoothing={self.smoothing})"
+
+    @property
+    def prims(self):
+        return self._prims
+
+    @property
+    def params(self):
+        return [self.smoothing]
+
+
 @dataclass
 class SDFBounded:
     """SDFLibrary's ``SDFBounded(child; pad_distance, fill_distance)``: within ``pad_distance`` of a
@@ -156,7 +202,7 @@ class SDFBounded:
 
     def __post_init__(self):
         if int(self.child.kind) & GEO_BOUNDED or int(self.child.kind) in (GEO_EMPTY, GEO_PLANE):
-            raise ValueError("SDFBounded needs a bounded child: a sphere, torus, box or sphere union")
+            raise ValueError("SDFBounded needs a bounded child: a sphere, torus, box or union")
         if len(list(self.child.params)) > 16:
             raise ValueError("the child's parameters do not fit a bounded descriptor")
         if not float(self.pad_distance) >= 0.0:
@@ -193,6 +239,13 @@ class SDFBounded:
             R = np.array(_rot(g.rotation)).reshape(3, 3)
             e = np.abs(R) @ np.array(_vec3(g.half_extents))  # world = c + R p
             return list(c - e), list(c + e)
+        if int(g.kind) == GEO_UNION:
+            if any(int(c.kind) == GEO_PLANE for c in g.children):
+                raise ValueError("a union with a plane has no bounding box: give box = (lo, hi)")
+            b = [SDFBounded(c, 0.0).bounds() for c in g.children]
+            lo = np.min([v[0] for v in b], axis=0) - g.smoothing
+            hi = np.max([v[1] for v in b], axis=0) + g.smoothing
+            return list(lo), list(hi)
         p = np.asarray(g.prims)
         k = float(g.smoothing)
         return list((p[:, :3] - p[:, 3:]).min(axis=0) - k), list((p[:, :3] + p[:, 3:]).max(axis=0) + k)

@@ -64,6 +64,14 @@ extern "C" {
 #define CC_GEO_BOX 3          /* p[0..2] centre, p[3..5] half extents, p[6..14] rotation */
 #define CC_GEO_PLANE 4        /* p[0..2] unit normal, p[3..5] point on the plane */
 #define CC_GEO_SPHERE_UNION 5 /* p[0] smoothing k; prims = n_prims x (cx, cy, cz, r), host memory */
+/* SDFUnion(children...; smoothing) (SDFLibrary, whose use the reference tests at test/test_plic.jl:96-112):
+ * p[0] smoothing k >= 0 (0 = the plain minimum); prims = n_prims records of CC_UNION_RECORD doubles in
+ * host memory, each the child's kind (CC_GEO_SPHERE, _TORUS, _BOX or _PLANE, as a double) followed by
+ * that child's own p[0..14].  The children fold left with the polynomial smooth minimum
+ * min(a,b) - h*h*k/4, h = max(k - |a-b|, 0)/k; the normal is the fold's gradient (a blended one scaled
+ * to unit length), the winning child's own normal wherever no step blended. */
+#define CC_GEO_UNION 6
+#define CC_UNION_RECORD 16
 /* A flag on any of the bounded kinds above: SDFBounded(child; pad_distance, fill_distance) (SDFLibrary,
  * whose use the reference documents at src/marching_cubes/surface.jl:70-71 and src/volume.jl:26-27).
  * The child's own parameters stay in p[0..15]; p[16..18] / p[19..21] are a box holding the child,

@@ -77,10 +77,13 @@ static double sphere_dist(const double *c, double r, int dim, const double *x, d
 
 static double smin_poly(double a, double b, double k)
 {
-    /* h = max(k - |a-b|, 0)/k; smin = min(a,b) - h*h*k/4 (polynomial smooth minimum). */
+    /* h = max(k - |a-b|, 0)/k; smin = min(a,b) - h*h*k/4 (polynomial smooth minimum).  Where
+     * k - |a-b| is not positive the correction is exactly zero, min(a,b) - 0 is min(a,b), and k = 0
+     * (the plain minimum) never divides. */
     double h = k - fabs(a - b);
-    h = (h > 0.0 ? h : 0.0) / k;
     double m = a < b ? a : b;
+    if (!(h > 0.0)) return m;
+    h = h / k;
     return m - ((h * h) * k) / 4.0;
 }
 
@@ -99,47 +102,37 @@ static int beyond_band(const orc_geometry *g, int dim, const double *x)
     return s > g->p[22] * g->p[22];
 }
 
-double orc_get_sdf(const orc_geometry *g, int dim, const double *x, double *normal)
+/* One primitive's get_sdf from its kind and parameters pp (a body's own p, or a union child's record
+ * after its kind): the distance, and the analytic gradient as the normal. */
+static double primitive_sdf(int kind, const double *pp, int dim, const double *x, double *normal)
 {
     double v[3] = {0.0, 0.0, 0.0}, p[3] = {0.0, 0.0, 0.0}, gl[3] = {0.0, 0.0, 0.0};
-    if (g->kind & ORC_GEO_BOUNDED) {
-        if (beyond_band(g, dim, x)) {
-            for (int a = 0; a < dim; ++a) normal[a] = 0.0;
-            return g->p[23];
-        }
-        orc_geometry child = *g;
-        child.kind &= ~ORC_GEO_BOUNDED;
-        return orc_get_sdf(&child, dim, x, normal);
-    }
-    switch (g->kind) {
-    case ORC_GEO_EMPTY:
-        for (int a = 0; a < dim; ++a) normal[a] = 0.0;
-        return g->p[0];
+    switch (kind) {
     case ORC_GEO_SPHERE: {
-        double c[3] = {g->p[0], g->p[1], g->p[2]}, nr;
-        double dist = sphere_dist(c, g->p[3], dim, x, &nr, v);
+        double c[3] = {pp[0], pp[1], pp[2]}, nr;
+        double dist = sphere_dist(c, pp[3], dim, x, &nr, v);
         for (int a = 0; a < dim; ++a) normal[a] = v[a] / nr;
         return dist;
     }
     case ORC_GEO_TORUS: { /* 3D only */
-        const double *r = g->p + 5;
-        to_local(r, g->p, x, 3, p);
+        const double *r = pp + 5;
+        to_local(r, pp, x, 3, p);
         double rho = sqrt(p[0] * p[0] + p[1] * p[1]);
-        double qx = rho - g->p[3], qz = p[2];
+        double qx = rho - pp[3], qz = p[2];
         double nq = sqrt(qx * qx + qz * qz);
         double u = qx / nq;
         gl[0] = u * (p[0] / rho);
         gl[1] = u * (p[1] / rho);
         gl[2] = qz / nq;
         from_local(r, gl, 3, normal);
-        return nq - g->p[4];
+        return nq - pp[4];
     }
     case ORC_GEO_BOX: {
-        const double *r = g->p + 6;
+        const double *r = pp + 6;
         double q[3] = {0.0, 0.0, 0.0}, mx[3] = {0.0, 0.0, 0.0};
-        to_local(r, g->p, x, dim, p);
+        to_local(r, pp, x, dim, p);
         for (int a = 0; a < dim; ++a) {
-            q[a] = fabs(p[a]) - g->p[3 + a];
+            q[a] = fabs(p[a]) - pp[3 + a];
             mx[a] = q[a] > 0.0 ? q[a] : 0.0;
         }
         double o = mx[0] * mx[0];
@@ -160,11 +153,82 @@ double orc_get_sdf(const orc_geometry *g, int dim, const double *x, double *norm
         return o + inner;
     }
     case ORC_GEO_PLANE: {
-        double s = g->p[0] * (x[0] - g->p[3]);
-        for (int a = 1; a < dim; ++a) s = s + g->p[a] * (x[a] - g->p[3 + a]);
-        for (int a = 0; a < dim; ++a) normal[a] = g->p[a];
+        double s = pp[0] * (x[0] - pp[3]);
+        for (int a = 1; a < dim; ++a) s = s + pp[a] * (x[a] - pp[3 + a]);
+        for (int a = 0; a < dim; ++a) normal[a] = pp[a];
         return s;
     }
+    }
+    for (int a = 0; a < dim; ++a) normal[a] = 0.0;
+    return NAN;
+}
+
+/* SDFUnion(children...; smoothing) (SDFLibrary, un-vendored; used by test/test_plic.jl:96-112): the
+ * children folded left with smin_poly (k = 0 the plain minimum, a tie keeping the later child).  The
+ * normal is the fold's gradient: the winner's own normal where a step takes the plain minimum,
+ * (1 - h/2) n_min + (h/2) n_max where it blends (the derivative of min(a,b) - h*h*k/4 with
+ * h = (k - |a-b|)/k), and a blended normal is scaled to unit length at the end. */
+static double union_sdf(const orc_geometry *g, int dim, const double *x, double *normal)
+{
+    const double k = g->p[0];
+    double acc = 0.0;
+    int blended = 0;
+    for (int i = 0; i < g->n_prims; ++i) {
+        const double *rec = g->prims + ORC_UNION_RECORD * i;
+        double nb[3] = {0.0, 0.0, 0.0};
+        double b = primitive_sdf((int)rec[0], rec + 1, dim, x, nb);
+        if (i == 0) {
+            acc = b;
+            for (int a = 0; a < dim; ++a) normal[a] = nb[a];
+            continue;
+        }
+        double h = k - fabs(acc - b);
+        int a_min = acc < b;
+        if (!(h > 0.0)) {
+            if (!a_min) {
+                acc = b;
+                for (int a = 0; a < dim; ++a) normal[a] = nb[a];
+            }
+            continue;
+        }
+        h = h / k;
+        double wmax = h / 2.0, wmin = 1.0 - wmax;
+        for (int a = 0; a < dim; ++a)
+            normal[a] = a_min ? wmin * normal[a] + wmax * nb[a] : wmax * normal[a] + wmin * nb[a];
+        acc = (a_min ? acc : b) - ((h * h) * k) / 4.0;
+        blended = 1;
+    }
+    if (blended) {
+        double s = normal[0] * normal[0];
+        for (int a = 1; a < dim; ++a) s = s + normal[a] * normal[a];
+        s = sqrt(s);
+        for (int a = 0; a < dim; ++a) normal[a] = normal[a] / s;
+    }
+    return acc;
+}
+
+double orc_get_sdf(const orc_geometry *g, int dim, const double *x, double *normal)
+{
+    if (g->kind & ORC_GEO_BOUNDED) {
+        if (beyond_band(g, dim, x)) {
+            for (int a = 0; a < dim; ++a) normal[a] = 0.0;
+            return g->p[23];
+        }
+        orc_geometry child = *g;
+        child.kind &= ~ORC_GEO_BOUNDED;
+        return orc_get_sdf(&child, dim, x, normal);
+    }
+    switch (g->kind) {
+    case ORC_GEO_EMPTY:
+        for (int a = 0; a < dim; ++a) normal[a] = 0.0;
+        return g->p[0];
+    case ORC_GEO_SPHERE:
+    case ORC_GEO_TORUS:
+    case ORC_GEO_BOX:
+    case ORC_GEO_PLANE:
+        return primitive_sdf(g->kind, g->p, dim, x, normal);
+    case ORC_GEO_UNION:
+        return union_sdf(g, dim, x, normal);
     case ORC_GEO_SPHERE_UNION: { /* distance only; no analytic normal is offered */
         for (int a = 0; a < dim; ++a) normal[a] = 0.0;
         return orc_sdf_value(g, dim, x);

@@ -49,6 +49,9 @@ enum {
     ORC_GEO_BOX = 3,          /* p[0..2] centre, p[3..5] half extents, p[6..14] rotation        */
     ORC_GEO_PLANE = 4,        /* p[0..2] unit normal, p[3..5] point on the plane                */
     ORC_GEO_SPHERE_UNION = 5, /* p[0] smoothing k; prims = n_prims x (cx,cy,cz,r)               */
+    ORC_GEO_UNION = 6,        /* p[0] smoothing k >= 0; prims = n_prims x ORC_UNION_RECORD: the
+                                 child's kind (sphere, torus, box, plane), then its p[0..14]      */
+    ORC_UNION_RECORD = 16,
     ORC_GEO_BOUNDED = 0x100   /* flag on a kind: SDFBounded(child; pad, fill) -- the child's own
                                  parameters in p[0..15], its bounding box p[16..18] (low corner),
                                  p[19..21] (high), p[22] pad, p[23] fill                          */

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "liborc.so")
 FIELDS = ("kind", "ambiguous", "volume_fraction", "centroid", "face_fraction",
           "face_centroid_local", "interface_centroid")
 
-GEO_EMPTY, GEO_SPHERE, GEO_TORUS, GEO_BOX, GEO_PLANE, GEO_SPHERE_UNION = range(6)
+GEO_EMPTY, GEO_SPHERE, GEO_TORUS, GEO_BOX, GEO_PLANE, GEO_SPHERE_UNION, GEO_UNION = range(7)
 
 
 class OrcCols(ctypes.Structure):
@@ -99,7 +99,7 @@ def _i64(x, n=3):
 
 def make_geometry(geo):
     """An OrcGeometry from any object exposing ``kind``, ``params`` (<= 24 floats) and ``prims``
-    (None or an (m, 4) array of sphere centres and radii).  The prims array is kept alive on the
+    (None, an (m, 4) array of sphere centres and radii, or an (m, 16) array of union children).  The prims array is kept alive on the
     returned struct."""
     g = OrcGeometry()
     g.kind = int(geo.kind)
@@ -108,7 +108,8 @@ def make_geometry(geo):
         g.p[i] = v
     prims = getattr(geo, "prims", None)
     if prims is not None:
-        arr = np.ascontiguousarray(np.asarray(prims, dtype=np.float64).reshape(-1, 4))
+        arr = np.asarray(prims, dtype=np.float64)
+        arr = np.ascontiguousarray(arr if arr.ndim == 2 else arr.reshape(-1, 4))
         g.n_prims = arr.shape[0]
         g.prims = arr.ctypes.data
         g._keep = arr
