@@ -199,11 +199,13 @@ void launch_plic_touch(const T *normals, const T *intercepts, size_t ncell, int 
 void launch_plic_marked(const unsigned long long *bits, const int64_t *wrank, size_t ncell, int64_t *cells,
                         hipStream_t s);
 
-// generate_mesh over a grid or a tree: bits marks the cut cells (kind == CELL_CUT), wrank ranks each
-// word and totals[0] counts them (part: the scan's block sums, part_cap per sum).  launch_leaf_gather
-// then writes, in leaf order, each cut leaf's index, its four corner values and -- when level is not
-// null -- its level and lattice corner (three int32 each).
-void launch_cut_marks(const int8_t *kind, size_t n, unsigned long long *bits, int64_t *wrank,
+// generate_mesh over a grid or a tree: bits marks the cut cells (corner values not all of one sign;
+// n0 > 0 a grid n0 cells wide over the nodal field, n0 == 0 leaves with four values each), wrank
+// ranks each word and totals[0] counts them (part: the scan's block sums, part_cap per sum).
+// launch_leaf_gather then writes, in leaf order, each cut leaf's index, its four corner values and
+// -- when level is not null -- its level and lattice corner (three int32 each).
+template <typename T>
+void launch_cut_marks(const T *phi, size_t n, size_t n0, unsigned long long *bits, int64_t *wrank,
                       unsigned long long *part, size_t part_cap, int64_t *totals, hipStream_t s);
 // The grid's cut cells in cell order: each one's index and its four corner values from the nodal
 // field of an n0-wide grid (n cells).

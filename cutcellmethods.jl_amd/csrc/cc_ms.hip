@@ -68,11 +68,30 @@ __global__ __launch_bounds__(256) void k_leaf_cells(Geo geo, const int32_t *__re
 // generate_mesh over a grid or a tree (src/marching_squares/surface.jl:59-110) draws only the cut
 // cells: one bit per cell or leaf, ranked by a popcount scan, then each cut one's index and four
 // corner values (and, for a tree held on the device, its level and lattice corner) gathered in
-// order, so only they cross to the host contour.
-__global__ __launch_bounds__(256) void k_leaf_cut_bits(const int8_t *__restrict__ kind, size_t n, size_t nwords,
-                                                       unsigned long long *__restrict__ bits) {
+// order, so only they cross to the host contour.  A cell is cut when its four corner values do not
+// all share a sign (phi >= 0 is outside) -- _ms_screen_cell (src/marching_squares/surface.jl:131-135),
+// read from the field itself, so a field changed through the column's pointer is screened afresh.
+// n0 > 0: a grid n0 cells wide over the nodal field; n0 == 0: leaves, four values each.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sign_bits(const T *__restrict__ phi, size_t n, size_t n0, size_t nwords,
+                                                   unsigned long long *__restrict__ bits) {
     const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const bool cut = c < n && kind[c] == CELL_CUT;
+    bool cut = false;
+    if (c < n) {
+        T p[4];
+        if (n0) {
+            const size_t i = c % n0, j = c / n0, sy = n0 + 1, base = i + sy * j;
+            p[0] = phi[base];
+            p[1] = phi[base + 1];
+            p[2] = phi[base + sy + 1];
+            p[3] = phi[base + sy];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) p[q] = phi[4 * c + q];
+        }
+        const bool s0 = p[0] >= T(0), s1 = p[1] >= T(0), s2 = p[2] >= T(0), s3 = p[3] >= T(0);
+        cut = !(s0 == s1 && s1 == s2 && s2 == s3);
+    }
     const unsigned long long m = __ballot(cut);
     const size_t w = c / 64;  // lane 0 holds a multiple of 64: the wave's word
     if ((threadIdx.x & 63) == 0 && w < nwords) bits[w] = m;
@@ -143,13 +162,19 @@ template void launch_grid_gather<double>(const unsigned long long *, const int64
 template void launch_grid_gather<float>(const unsigned long long *, const int64_t *, size_t, const float *, size_t,
                                         int64_t *, float *, hipStream_t);
 
-void launch_cut_marks(const int8_t *kind, size_t n, unsigned long long *bits, int64_t *wrank,
-                       unsigned long long *part, size_t part_cap, int64_t *totals, hipStream_t s) {
+template <typename T>
+void launch_cut_marks(const T *phi, size_t n, size_t n0, unsigned long long *bits, int64_t *wrank,
+                      unsigned long long *part, size_t part_cap, int64_t *totals, hipStream_t s) {
     const size_t nwords = (n + 63) / 64;
-    hipLaunchKernelGGL(k_leaf_cut_bits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, kind, n, nwords, bits);
+    hipLaunchKernelGGL((k_sign_bits<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, phi, n, n0, nwords, bits);
     launch_scan_pair({nullptr, bits, wrank, nullptr, nwords}, {nullptr, nullptr, nullptr, nullptr, 0}, part, part_cap,
                      totals, s);
 }
+
+template void launch_cut_marks<double>(const double *, size_t, size_t, unsigned long long *, int64_t *,
+                                       unsigned long long *, size_t, int64_t *, hipStream_t);
+template void launch_cut_marks<float>(const float *, size_t, size_t, unsigned long long *, int64_t *,
+                                      unsigned long long *, size_t, int64_t *, hipStream_t);
 
 template <typename T>
 void launch_leaf_gather(const unsigned long long *bits, const int64_t *wrank, size_t n, const T *phi4,

@@ -34,11 +34,12 @@ def assert_leaves_equal(cache, mesh, geo):
 
 
 def contour_reference(cache, mesh):
-    """Host restatement of the tree contour: each cut leaf's cell_interface (boundary walk, gap
-    segments, outward orientation) from its stored corners, in leaf order, welded by exact keys."""
+    """Host restatement of the tree contour: cell_interface (boundary walk, gap segments, outward
+    orientation) of every leaf whose stored corners do not all share a sign, in leaf order, welded
+    by exact keys (src/marching_squares/surface.jl:92-110)."""
     phi4 = cache.phi
-    kind = cache.read("kind")
     X = mesh.cell_nodes()
+    signs = phi4 >= 0
     ids, nodes, lines = {}, [], []
 
     def vid(p):
@@ -48,7 +49,7 @@ def contour_reference(cache, mesh):
             ids[key] = len(nodes)
         return ids[key]
 
-    for c in np.flatnonzero(kind == 2):
+    for c in np.flatnonzero(signs.any(axis=1) & ~signs.all(axis=1)):
         ph, nd = phi4[c], X[c]
         sa, sb, pres = [None] * 4, [None] * 4, [False] * 4
         for k in range(4):
@@ -135,6 +136,31 @@ def test_tree_invariants_and_contour():
     n3, l3 = generate_mesh(cache3, mesh)
     rn, rl = contour_reference(cache3, mesh)
     assert np.array_equal(n3, rn) and np.array_equal(l3, rl) and len(l3) > len(l2)
+
+
+def test_tree_contour_reads_the_field_not_kind():
+    # the contour screens each leaf by its corner signs, so a corner value changed through the
+    # column's own pointer -- with no update, kind left stale -- is drawn by the next generate_mesh
+    import ctypes
+    mesh = banded_tree()
+    geo = SDFCircle(GEO_CENTRE, GEO_RADIUS)
+    cache = update_cache(allocate_cache(mesh, MarchingSquaresCutCell), geo, mesh)
+    n0, l0 = generate_mesh(cache, mesh)
+    phi4 = cache.phi
+    kind = cache.read("kind")
+    inside = np.flatnonzero((kind == 0) & np.all(phi4 > -0.05, axis=1))[0]  # an inside leaf by the band
+    new = phi4[inside].copy()
+    new[2] = -new[2]  # one corner now outside: the leaf is cut, its kind still says inside
+    ptr, _ = cache.column_ptr("phi")
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.hipMemcpy(ctypes.c_void_p(ptr + 32 * int(inside)), new.ctypes.data, 32, 1) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert cache.read("kind")[inside] == 0
+    n1, l1 = generate_mesh(cache, mesh)
+    rn, rl = contour_reference(cache, mesh)
+    assert np.array_equal(n1, rn) and np.array_equal(l1, rl)
+    assert len(l1) == len(l0) + 1
 
 
 def test_tree_held_on_the_device():

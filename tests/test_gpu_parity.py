@@ -255,6 +255,35 @@ def test_ms_circle_config_bitwise_and_contour():
     assert np.array_equal(n3, w3) and np.array_equal(l3, wl3) and not np.array_equal(n3, nodes)
 
 
+def test_ms_contour_reads_the_field_not_kind():
+    # generate_mesh screens cells by their corner signs (_ms_screen_cell, surface.jl:131-135), so a
+    # node whose sign is flipped through the column's own pointer -- no update, kind left stale --
+    # changes which cells the next contour draws
+    import ctypes
+    g = CartesianIsoGrid((-1.0, -1.0), (1.0, 1.0), sz=0.01)
+    geo = SDFSphere((0.0, 0.0), 0.5)
+    c = update_cache(allocate_cache(g, MarchingSquaresCutCell), geo, g)
+    phi = c.phi.copy()
+    P = fortran(phi, g.nodegrid_size())
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    ptr, _ = c.column_ptr("phi")
+    n0, l0 = generate_mesh(c, g)
+    # a node 3 cells deep turned positive: its 8 neighbours stay negative (phi is 1-Lipschitz), so
+    # the four cells around it are newly cut, one segment each
+    i, j = np.argwhere(P < -0.03)[0]
+    node = int(i) + 201 * int(j)
+    assert np.all(P[i - 1:i + 2, j - 1:j + 2] < 0)
+    new = phi.copy()
+    new[node] = 1e-3
+    assert hip.hipMemcpy(ctypes.c_void_p(ptr + 8 * node), new[node:node + 1].ctypes.data, 8, 1) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    n1, l1 = generate_mesh(c, g)
+    w1, wl1 = contour_reference(new, g)
+    assert np.array_equal(n1, w1) and np.array_equal(l1, wl1)
+    assert len(l1) == len(l0) + 4  # the stale kind would have drawn none of them
+
+
 def test_ms_float32_contour():
     # a Float32 cache contours in Float32, as the reference converts into the grid's float type
     g = CartesianIsoGrid((-1.0, -1.0), (1.0, 1.0), sz=0.01)
