@@ -1124,6 +1124,64 @@ template <typename T> int tree_contour(cc_cache *c, const cc_leaves *tree) {
     return CC_OK;
 }
 
+// calc_volume's host checks (src/volume.jl:29-40): element type, grid, isotropy, body; fills the
+// kernel's body descriptor without its primitive table.
+int check_volume_args(int dtype, const cc_geometry *geo, const cc_grid *grid, double *volume, cc::Geo *gd) {
+    if (!geo || !grid || !volume) return fail(CC_ERR_ARGUMENT, "NULL argument");
+    if (dtype != CC_FLOAT64 && dtype != CC_FLOAT32)
+        return fail(CC_ERR_UNSUPPORTED, "element type must be CC_FLOAT64 or CC_FLOAT32");
+    const int dim = grid->dim;
+    if (dim != 2 && dim != 3) return fail(CC_ERR_ARGUMENT, "grid.dim must be 2 or 3");
+    int64_t ncell = 1;
+    for (int a = 0; a < dim; ++a) {
+        if (grid->n[a] < 1) return fail(CC_ERR_ARGUMENT, "every axis needs at least one cell");
+        if (!(grid->d[a] > 0.0) || !std::isfinite(grid->d[a]) || !std::isfinite(grid->x0[a]))
+            return fail(CC_ERR_ARGUMENT, "grid.d must be positive and finite, x0 finite");
+        ncell *= grid->n[a];
+    }
+    if (ncell >= (int64_t(1) << 32)) return fail(CC_ERR_UNSUPPORTED, "grids of 2^32 cells or more are not supported");
+    for (int a = 1; a < dim; ++a)  // src/volume.jl:30-32
+        if (grid->d[a] != grid->d[0])
+            return fail(CC_ERR_ARGUMENT,
+                        "calc_volume needs an isotropic grid (equal cell size on every axis), got grid.d = "
+                        "(%g, %g%s)", grid->d[0], grid->d[1], dim == 3 ? ", ..." : "");
+    const int grc = check_geometry(geo, dim);
+    if (grc) return grc;
+    gd->kind = geo->kind;
+    gd->nprims = prim_doubles(geo) > 0 ? geo->n_prims : 0;
+    std::memcpy(gd->p, geo->p, sizeof gd->p);
+    gd->prims = nullptr;
+    return CC_OK;
+}
+
+// calc_volume on stream st with the given block sums and result arrays: the fold of every cell's solid
+// fraction, times prod(grid.d) in the grid's type; blocks until the result is on the host.
+int run_volume(int dtype, const cc_grid *grid, const cc::Geo &gd, void *part, void *out, hipStream_t st,
+               double *volume) {
+    const int dim = grid->dim;
+    double sum = 0.0;
+    if (dtype == CC_FLOAT32) {
+        const cc::GridT<float> g32 = grid_t<float>(grid, dim);
+        cc::launch_plic_volume<float>(g32, gd, dim, (float *)part, (float *)out, st);
+        float v = 0.0f;
+        if (hipMemcpyAsync(&v, out, 4, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+            return fail(CC_ERR_HIP, "calc_volume: %s", hipGetErrorString(hipGetLastError()));
+        float c32 = g32.d[0] * g32.d[1];
+        if (dim == 3) c32 = c32 * g32.d[2];
+        sum = (double)(v * c32);  // vol * prod(grid.d), in the grid's type
+    } else {
+        const cc::GridT<double> g64 = grid_t<double>(grid, dim);
+        cc::launch_plic_volume<double>(g64, gd, dim, (double *)part, (double *)out, st);
+        if (hipMemcpyAsync(&sum, out, 8, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+            return fail(CC_ERR_HIP, "calc_volume: %s", hipGetErrorString(hipGetLastError()));
+        double cv = g64.d[0] * g64.d[1];
+        if (dim == 3) cv = cv * g64.d[2];
+        sum = sum * cv;
+    }
+    *volume = sum;
+    return CC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1581,40 +1639,17 @@ int cc_generate_mesh_leaves(cc_cache *c, const cc_leaves *tree, double *nodes, i
 }
 
 int cc_calc_volume(int dtype, const cc_geometry *geo, const cc_grid *grid, double *volume) {
-    if (!geo || !grid || !volume) return fail(CC_ERR_ARGUMENT, "NULL argument");
-    if (dtype != CC_FLOAT64 && dtype != CC_FLOAT32)
-        return fail(CC_ERR_UNSUPPORTED, "element type must be CC_FLOAT64 or CC_FLOAT32");
-    const int dim = grid->dim;
-    if (dim != 2 && dim != 3) return fail(CC_ERR_ARGUMENT, "grid.dim must be 2 or 3");
-    int64_t ncell = 1;
-    for (int a = 0; a < dim; ++a) {
-        if (grid->n[a] < 1) return fail(CC_ERR_ARGUMENT, "every axis needs at least one cell");
-        if (!(grid->d[a] > 0.0) || !std::isfinite(grid->d[a]) || !std::isfinite(grid->x0[a]))
-            return fail(CC_ERR_ARGUMENT, "grid.d must be positive and finite, x0 finite");
-        ncell *= grid->n[a];
-    }
-    if (ncell >= (int64_t(1) << 32)) return fail(CC_ERR_UNSUPPORTED, "grids of 2^32 cells or more are not supported");
-    for (int a = 1; a < dim; ++a)  // src/volume.jl:30-32
-        if (grid->d[a] != grid->d[0])
-            return fail(CC_ERR_ARGUMENT,
-                        "calc_volume needs an isotropic grid (equal cell size on every axis), got grid.d = "
-                        "(%g, %g%s)", grid->d[0], grid->d[1], dim == 3 ? ", ..." : "");
-    const int grc = check_geometry(geo, dim);
-    if (grc) return grc;
     cc::Geo gd;
-    gd.kind = geo->kind;
-    gd.nprims = prim_doubles(geo) > 0 ? geo->n_prims : 0;
+    int rc = check_volume_args(dtype, geo, grid, volume, &gd);
+    if (rc) return rc;
+    const int dim = grid->dim;
     const size_t prim_bytes = 8 * (size_t)prim_doubles(geo);
-    std::memcpy(gd.p, geo->p, sizeof gd.p);
-    gd.prims = nullptr;
+    const size_t es = elem_size(dtype);
+    const int nb = dtype == CC_FLOAT32 ? cc::plic_volume_blocks(grid_t<float>(grid, dim), dim)
+                                       : cc::plic_volume_blocks(grid_t<double>(grid, dim), dim);
     hipStream_t st = nullptr;
     void *part = nullptr, *out = nullptr;
     double *prims = nullptr;
-    int rc = CC_OK;
-    const size_t es = elem_size(dtype);
-    const cc::GridT<double> g64 = grid_t<double>(grid, dim);
-    const cc::GridT<float> g32 = grid_t<float>(grid, dim);
-    const int nb = dtype == CC_FLOAT32 ? cc::plic_volume_blocks(g32, dim) : cc::plic_volume_blocks(g64, dim);
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipMalloc(&part, es * nb) != hipSuccess ||
         hipMalloc(&out, es) != hipSuccess ||
         (gd.nprims > 0 && (hipMalloc((void **)&prims, prim_bytes) != hipSuccess ||
@@ -1622,32 +1657,38 @@ int cc_calc_volume(int dtype, const cc_geometry *geo, const cc_grid *grid, doubl
         rc = fail(CC_ERR_OUT_OF_MEMORY, "calc_volume: could not set up its work arrays");
     } else {
         gd.prims = prims;
-        double sum = 0.0;
-        if (dtype == CC_FLOAT32) {
-            cc::launch_plic_volume<float>(g32, gd, dim, (float *)part, (float *)out, st);
-            float v = 0.0f;
-            if (hipMemcpyAsync(&v, out, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess)
-                rc = fail(CC_ERR_HIP, "calc_volume: %s", hipGetErrorString(hipGetLastError()));
-            float c32 = g32.d[0] * g32.d[1];
-            if (dim == 3) c32 = c32 * g32.d[2];
-            sum = (double)(v * c32);  // vol * prod(grid.d), in the grid's type
-        } else {
-            cc::launch_plic_volume<double>(g64, gd, dim, (double *)part, (double *)out, st);
-            if (hipMemcpyAsync(&sum, out, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess)
-                rc = fail(CC_ERR_HIP, "calc_volume: %s", hipGetErrorString(hipGetLastError()));
-            double cv = g64.d[0] * g64.d[1];
-            if (dim == 3) cv = cv * g64.d[2];
-            sum = sum * cv;
-        }
-        if (rc == CC_OK) *volume = sum;
+        rc = run_volume(dtype, grid, gd, part, out, st, volume);
     }
     if (prims) (void)hipFree(prims);
     if (out) (void)hipFree(out);
     if (part) (void)hipFree(part);
     if (st) (void)hipStreamDestroy(st);
     return rc;
+}
+
+int cc_calc_volume_noalloc(int dtype, const cc_geometry *geo, const cc_grid *grid, void *stream, double *volume) {
+    cc::Geo gd;
+    int rc = check_volume_args(dtype, geo, grid, volume, &gd);
+    if (rc) return rc;
+    const int64_t nd = prim_doubles(geo);
+    if (nd > (int64_t)cc::VOL_SCRATCH_MAX_PRIM_DOUBLES)
+        return fail(CC_ERR_ARGUMENT,
+                    "calc_volume without allocation: the body's primitive table holds %lld doubles, more than "
+                    "the %lld its scratch array keeps (%lld spheres of a sphere union, %lld children of a union)",
+                    (long long)nd, (long long)cc::VOL_SCRATCH_MAX_PRIM_DOUBLES,
+                    (long long)(cc::VOL_SCRATCH_MAX_PRIM_DOUBLES / 4),
+                    (long long)(cc::VOL_SCRATCH_MAX_PRIM_DOUBLES / CC_UNION_RECORD));
+    void *base = nullptr;
+    const int e = cc::volume_scratch(&base);
+    if (e != (int)hipSuccess)
+        return fail(CC_ERR_HIP, "calc_volume without allocation: scratch lookup: %s", hipGetErrorString((hipError_t)e));
+    unsigned char *ptr = (unsigned char *)base;
+    hipStream_t st = (hipStream_t)stream;
+    if (nd > 0) {
+        HIPC(hipMemcpyAsync(ptr + cc::VOL_SCRATCH_PRIMS, geo->prims, 8 * (size_t)nd, hipMemcpyHostToDevice, st));
+        gd.prims = (const double *)(ptr + cc::VOL_SCRATCH_PRIMS);
+    }
+    return run_volume(dtype, grid, gd, ptr, ptr + cc::VOL_SCRATCH_OUT, st, volume);
 }
 
 int cc_generate_mesh_plic(cc_cache *c, const cc_grid *grid, double min_frac, void *nodes, int64_t *nnodes,

@@ -69,6 +69,18 @@ template <typename T> int plic_volume_blocks(const GridT<T> &g, int dim);
 template <typename T>
 void launch_plic_volume(const GridT<T> &g, const Geo &geo, int dim, T *part, T *out, hipStream_t s);
 
+// calc_volume's work arrays without allocation: a file-scope device array of VOL_SCRATCH_BYTES per
+// GPU, laid out as the block sums (at most VOL_SCRATCH_PARTS of them, which every grid's
+// plic_volume_blocks fits), the result at VOL_SCRATCH_OUT, and a body's primitive table from
+// VOL_SCRATCH_PRIMS on (at most VOL_SCRATCH_MAX_PRIM_DOUBLES doubles).  volume_scratch gives the
+// current device's address of that array, looked up once per device; it returns a hipError_t.
+constexpr size_t VOL_SCRATCH_BYTES = 256 * 1024;
+constexpr size_t VOL_SCRATCH_PARTS = 2048;
+constexpr size_t VOL_SCRATCH_OUT = VOL_SCRATCH_PARTS * 8;
+constexpr size_t VOL_SCRATCH_PRIMS = VOL_SCRATCH_OUT + 256;
+constexpr size_t VOL_SCRATCH_MAX_PRIM_DOUBLES = (VOL_SCRATCH_BYTES - VOL_SCRATCH_PRIMS) / 8;
+int volume_scratch(void **ptr);
+
 // The fused 3D PLIC pass (fit + resolved faces in one sweep, no read-back).
 template <typename T>
 void launch_plic_fused(const GridT<T> &g, const Geo &geo, const Cols<T> &cols, T *normals,

@@ -6,6 +6,8 @@
 // cell's own one-sided wet measure and its neighbour's through the one compiled `face_area` below,
 // then the mean `(neg + pos)/2`, negative side first.  Both cells of an interior face form the same
 // sum of the same two numbers from one code path, so the resolved face is bitwise single-valued.
+#include <mutex>
+
 #include "cc_plic_dev.h"
 
 namespace cc {
@@ -84,7 +86,33 @@ template <typename T>
 int plic_volume_blocks(const GridT<T> &g, int dim) {
     size_t ncell = (size_t)g.n[0] * g.n[1] * (dim == 3 ? g.n[2] : 1);
     size_t nb = (ncell + VOL_THREADS - 1) / VOL_THREADS;
-    return (int)(nb < 2048 ? nb : 2048);
+    return (int)(nb < VOL_SCRATCH_PARTS ? nb : VOL_SCRATCH_PARTS);
+}
+
+namespace {
+// The allocation-free calc_volume's work arrays (block sums, result, primitive table), one copy per
+// GPU for the life of the process.  Host code never passes this name to a kernel: it stands for a
+// host-side shadow, not the device address, which volume_scratch looks up.
+static __device__ __align__(256) unsigned char scratch[VOL_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64] = {};
+}  // namespace
+
+int volume_scratch(void **ptr) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= 64) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return (int)hipSuccess;
 }
 
 template <typename T>

@@ -270,6 +270,16 @@ int cc_extract_surface_plic(int dtype, const cc_grid *grid, int64_t count, const
  * run); the reference's serial left fold can differ from it in the last bits. */
 int cc_calc_volume(int dtype, const cc_geometry *geo, const cc_grid *grid, double *volume);
 
+/* cc_calc_volume without allocating device memory.  No counterpart in the reference: this entry point
+ * is for callers that cannot call hipMalloc (code under graph capture, tight serving loops, harnesses
+ * that forbid allocation).  It makes no hipMalloc/hipFree and creates no stream: its block sums, its
+ * result and the body's primitive table live in a 256 KiB device array that each GPU holds for the
+ * life of the process (at most 30 688 doubles of primitive table: 7672 spheres of a sphere union or
+ * 1918 children of a union; a larger table is CC_ERR_ARGUMENT).  It runs on `stream` (a hipStream_t;
+ * NULL for the null stream) and blocks until the volume is on the host.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU race, so the caller serialises them. */
+int cc_calc_volume_noalloc(int dtype, const cc_geometry *geo, const cc_grid *grid, void *stream, double *volume);
+
 /* generate_mesh(cache, grid) for a marching-cubes cache -- src/marching_cubes/surface.jl:82-112:
  * the cache's phi marched on the device (Lewiner cases, shared vertices, wound so each triangle's
  * normal points out of the body) into a mesh the cache keeps until its next update.  Call with
