@@ -91,6 +91,10 @@ __device__ __forceinline__ T face_area(const T *nv, T a, T frac, bool valid, con
     return U * V * (T(1) - solid);
 }
 
+// Where every face of a cell is known without evaluating it (see plic_plane): not known, all six
+// faces wholly wet (one-sided fraction exactly 1), or all wholly dry (exactly 0).
+enum : int { FACES_EVAL = 0, FACES_WET = 1, FACES_DRY = 2 };
+
 // One cell's fitted plane: cell_plane (src/plic/plic.jl:72-91) as the fit pass stores it.
 template <typename T> struct Plane {
     T n[3];
@@ -98,6 +102,7 @@ template <typename T> struct Plane {
     T vf;     // the fluid fraction, 1 - solid
     T solid;  // cell_plane's fraction itself (calc_volume sums it)
     bool ok;
+    int faces;  // FACES_*: whether face_area needs evaluating on this cell's faces
 };
 
 template <typename T, int D, bool MIXED = true>
@@ -121,9 +126,26 @@ __device__ __forceinline__ Plane<T> plic_plane(const GridT<T> &g, const Geo &geo
         a = T(0);
         fraction = dist <= T(0) ? T(1) : T(0);
         P.ok = false;
+        // face_area's invalid branch: frac < 1/2 ? 0 : the full face, whose quotient is exactly 1
+        P.faces = T(1) - fraction < T(0.5) ? FACES_DRY : FACES_WET;
     } else {
         fraction = D == 2 ? vfrac2(P.n[0], P.n[1], a) : vfrac3(P.n, a);
         P.ok = true;
+        // A plane clear of the cell's closed cube leaves each face's own clip at an early exit of
+        // vfrac2.  With a3 = a minus the negative components and S = sum |n_i|, every face's
+        // reflected offset A (face_area: -(n_c s - a) minus the negative in-plane components) obeys
+        // A <= a3 and A - (|n_p| + |n_q|) >= a3 - S exactly; each side is a handful of roundings of
+        // magnitude at most |a| + S.  So a3 below -16 eps (|a| + S) makes every face's solid share
+        // exactly 0 (wet area U*V, quotient 1), and a3 - S above the same margin makes it exactly 1
+        // (area +0, quotient +0), the parallel-face guard included.
+        T a3 = a, S = T(0);
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            if (P.n[q] < T(0)) a3 -= P.n[q];
+            S = S + fabs(P.n[q]);
+        }
+        const T margin = T(16) * Eps<T>::v * (fabs(a) + S);
+        P.faces = a3 < -margin ? FACES_WET : (a3 - S > margin ? FACES_DRY : FACES_EVAL);
     }
     P.a = a;
     P.vf = T(1) - fraction;
@@ -199,6 +221,20 @@ template <typename T> __device__ __forceinline__ T plane_face(const Plane<T> &P,
     return face_area<T, 3>(P.n, P.a, P.vf, P.ok, cs, k);
 }
 
+// _stored_face_area(k) / full_face_area(k) for a fitted plane: the quotient both cells of a face
+// resolve with.  A face known wholly wet or dry is exactly 1 or +0 (U*V equals the full area, both
+// the product of the same two cell sizes in increasing axis order); an area of exactly 0 or of the
+// full face gives those quotients too, so the division runs only on partly wet faces.
+template <typename T>
+__device__ __forceinline__ T plane_candidate(const Plane<T> &P, const T *cs, int k, T full) {
+    if (P.faces == FACES_WET) return T(1);
+    if (P.faces == FACES_DRY) return T(0);
+    const T own = plane_face(P, cs, k);
+    if (own == T(0)) return T(0);
+    if (own == full) return T(1);
+    return own / full;
+}
+
 // Tile of the fused 3D PLIC pass: PF_W cells along x by PF_R rows, marched in z.
 constexpr int PF_W = 32, PF_R = 8;
 
@@ -258,23 +294,24 @@ __device__ void plic_fused_item(const GridT<T> &g, const Geo &geo, const Cols<T>
     T cz_prev = T(0);
     T cc[6], cn[6];
     if (inside) {
-        if (k0 >= 1) cz_prev = plane_face(plic_plane<T, 3, false>(g, geo, i, j, k0 - 1), g.d, 5) / fullf[2];
+        if (k0 >= 1) cz_prev = plane_candidate(plic_plane<T, 3, false>(g, geo, i, j, k0 - 1), g.d, 5, fullf[2]);
         cur = plic_plane<T, 3, false>(g, geo, i, j, k0);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) cc[q] = plane_face(cur, g.d, q) / fullf[q >> 1];
+        for (int q = 0; q < 6; ++q) cc[q] = plane_candidate(cur, g.d, q, fullf[q >> 1]);
     }
     for (uint32_t k = k0; k < k1; ++k) {
         if (inside && k + 1 < n2) {
             nxt = plic_plane<T, 3, false>(g, geo, i, j, k + 1);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) cn[q] = plane_face(nxt, g.d, q) / fullf[q >> 1];
+            for (int q = 0; q < 6; ++q) cn[q] = plane_candidate(nxt, g.d, q, fullf[q >> 1]);
         }
         if (inside) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) L.cand[r + 1][ii + 1][q] = cc[q];
         }
         if (hr >= 0)
-            L.cand[hr][hc][hslot] = plane_face(plic_plane<T, 3, false>(g, geo, hi_, hj, k), g.d, hslot) / fullf[hslot >> 1];
+            L.cand[hr][hc][hslot] =
+                plane_candidate(plic_plane<T, 3, false>(g, geo, hi_, hj, k), g.d, hslot, fullf[hslot >> 1]);
         __syncthreads();
         if (inside) {
             T ff[6];

@@ -499,3 +499,25 @@ def test_plic_fused_and_two_pass_bitwise(variant, kz):
         assert_cells_equal(c, ref)
         assert same_bits(c.normals, normals) and same_bits(c.intercepts, intercepts)
         assert np.array_equal(c.is_valid, valid)
+
+
+@pytest.mark.parametrize("nrm", [(1.0, 0.0, 0.0), (0.0, -1.0, 0.0), (1.0, 1.0, 0.0), (1.0, 1.0, 1.0),
+                                 (-1.0, 2.0, 2.0), (0.3, -0.4, 1e-17)])
+def test_plic_fused_planes_through_lattice_nodes(nrm):
+    # planes through a lattice node on a dyadic grid touch many cells exactly at a corner, an edge or
+    # a face: where a cell's faces are exactly wet, exactly dry or partly cut is decided at the
+    # boundary, and the fused sweep's shortcuts for wholly wet or dry cells must agree with the
+    # oracle's full evaluation bit for bit
+    nrm = np.asarray(nrm) / np.linalg.norm(nrm)
+    g = CartesianGrid((-1.0, -0.75, -0.5), (40, 36, 33), (0.0625,) * 3)
+    p0 = (g.x0[0] + 17 * 0.0625, g.x0[1] + 19 * 0.0625, g.x0[2] + 15 * 0.0625)
+    geo = SDFPlane(tuple(nrm), p0)
+    ref, normals, intercepts, valid = orc.plic(geo, 3, g.x0, g.d, g.n)
+    for variant, kz in ((2, 8), (2, 33), (0, 1)):
+        c = allocate_cache(g, PLICCutCell)
+        c.set_tuning(variant, kz, False)
+        update_cache(c, geo, g)
+        assert_cells_equal(c, ref)
+        assert same_bits(c.normals, normals) and same_bits(c.intercepts, intercepts)
+    assert shared_face_mismatches(ref["face_fraction"], g.n, 3) == 0
+
