@@ -17,7 +17,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liborc.so")
+# CUTCELL_ORACLE_LIB selects another build of the same sources (the sanitizer build of
+# run_sanitized.sh); the default is the optimised liborc.so beside this file.
+LIB_PATH = os.environ.get("CUTCELL_ORACLE_LIB", os.path.join(HERE, "liborc.so"))
 
 FIELDS = ("kind", "ambiguous", "volume_fraction", "centroid", "face_fraction",
           "face_centroid_local", "interface_centroid")
