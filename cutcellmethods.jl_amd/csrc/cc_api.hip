@@ -95,6 +95,8 @@ struct cc_cache {
     int64_t prims_cap = 0;
     std::vector<double> prims_host;
     hipStream_t stream = nullptr;
+    bool async_pending = false;    // cc_update_async left work on `stream` not yet waited for
+    hipEvent_t order_ev = nullptr;  // orders another stream (a batch's) after that work
     bool profiling = false;
     hipEvent_t ev[MAX_TIMED + 1] = {};
     const char *ev_name[MAX_TIMED] = {};
@@ -505,6 +507,7 @@ int update_impl(cc_cache *c, const cc_geometry *geo, const cc_grid *g, bool sync
     rc = dispatch_update(c, &gd, nullptr, 0, g);
     if (rc) return rc;
     if (sync) HIPC(hipStreamSynchronize(c->stream));
+    c->async_pending = !sync;
     return CC_OK;
 }
 
@@ -864,6 +867,8 @@ int diagnostics_impl(cc_cache *c, const cc_grid *grid, cc_diagnostics *out) {
 }
 
 void free_all(cc_cache *c) {
+    if (c->order_ev) (void)hipEventDestroy(c->order_ev);
+    c->order_ev = nullptr;
     if (c->dg) (void)hipFree(c->dg);
     if (c->dg_host) (void)hipHostFree(c->dg_host);
     for (void *p : c->ps_allocs) (void)hipFree(p);
@@ -1188,9 +1193,9 @@ extern "C" {
 
 const char *cc_last_error(void) { return g_err.c_str(); }
 
-void cc_internal_set_error(const char *msg) { g_err = msg ? msg : ""; }
+CC_HIDDEN void cc_internal_set_error(const char *msg) { g_err = msg ? msg : ""; }
 
-int cc_internal_body(cc_cache *c, int *method, int *dtype, int64_t *n, void **cols, void **cutlist,
+CC_HIDDEN int cc_internal_body(cc_cache *c, int *method, int *dtype, int64_t *n, void **cols, void **cutlist,
                      void **counter, int *kz) {
     if (!c) return CC_ERR_ARGUMENT;
     *method = c->method;
@@ -1204,11 +1209,20 @@ int cc_internal_body(cc_cache *c, int *method, int *dtype, int64_t *n, void **co
 }
 
 // Whether the cache is laid out on a quadtree's leaves.
-int cc_internal_leaves(const cc_cache *c) { return c && c->leaves ? 1 : 0; }
+CC_HIDDEN int cc_internal_leaves(const cc_cache *c) { return c && c->leaves ? 1 : 0; }
 
-// A body of a batched step was updated: a surface marched before describes an older field.
-void cc_internal_touch(cc_cache *c) {
+CC_HIDDEN void cc_internal_touch(cc_cache *c) {
     if (c) c->gen += 1;
+}
+
+CC_HIDDEN int cc_internal_order_after(cc_cache *c, hipStream_t s) {
+    if (!c || !c->async_pending) return CC_OK;
+    if (!c->order_ev && hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming) != hipSuccess)
+        return fail(CC_ERR_HIP, "could not create an ordering event");
+    HIPC(hipEventRecord(c->order_ev, c->stream));
+    HIPC(hipStreamWaitEvent(s, c->order_ev, 0));
+    c->async_pending = false;
+    return CC_OK;
 }
 
 const char *cc_version(void) { return "libcutcell_hip 0.1.0 (gfx950)"; }
@@ -1288,6 +1302,7 @@ int cc_update_async(cc_cache *c, const cc_geometry *geo, const cc_grid *grid) {
 int cc_synchronize(cc_cache *c) {
     if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
     HIPC(hipStreamSynchronize(c->stream));
+    c->async_pending = false;
     return CC_OK;
 }
 

@@ -100,11 +100,6 @@ __global__ __launch_bounds__(MC_CUT_THREADS) void k_batch_b(const BatchBody<T> *
 }  // namespace
 }  // namespace cc
 
-// The library's internal view of a cache (defined in cc_api.hip); only the accessors are used here.
-extern "C" int cc_internal_body(cc_cache *c, int *method, int *dtype, int64_t *n, void **cols,
-                                void **cutlist, void **counter, int *kz);
-extern "C" void cc_internal_touch(cc_cache *c);
-extern "C" void cc_internal_set_error(const char *msg);
 
 struct cc_batch {
     int nb = 0, dtype = 0;
@@ -320,7 +315,11 @@ int cc_batch_step(cc_batch *bt, const cc_geometry *geos, const cc_grid *grids, f
                                      : build_desc<double>(bt, geos, grids, nullptr, nullptr);
     }
     if (rc) return rc;
-    for (int b = 0; b < bt->nb; ++b) cc_internal_touch(bt->caches[b]);
+    for (int b = 0; b < bt->nb; ++b) {
+        cc_internal_touch(bt->caches[b]);
+        // an update a caller left running on a member cache (cc_update_async) finishes first
+        if (cc_internal_order_after(bt->caches[b], bt->stream) != CC_OK) return CC_ERR_HIP;
+    }
     if (hipMemcpyAsync(bt->desc_dev, bt->desc_host, bt->desc_bytes, hipMemcpyHostToDevice, bt->stream) !=
             hipSuccess ||
         hipEventRecord(bt->e0, bt->stream) != hipSuccess ||

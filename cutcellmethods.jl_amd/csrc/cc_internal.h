@@ -5,6 +5,21 @@
 
 #include "cc_device.h"
 
+// Hooks between the library's own translation units (the cache is defined in cc_api.hip).  Hidden:
+// they link inside libcutcell_hip.so and are not part of its exported C ABI.
+#define CC_HIDDEN __attribute__((visibility("hidden")))
+struct cc_cache;
+extern "C" {
+CC_HIDDEN void cc_internal_set_error(const char *msg);
+CC_HIDDEN int cc_internal_body(cc_cache *c, int *method, int *dtype, int64_t *n, void **cols, void **cutlist,
+                               void **counter, int *kz);
+CC_HIDDEN int cc_internal_leaves(const cc_cache *c);
+// A body of a batched step was updated: a surface marched before describes an older field.
+CC_HIDDEN void cc_internal_touch(cc_cache *c);
+// Makes stream s wait for the work cc_update_async left on the cache's stream (nothing if none).
+CC_HIDDEN int cc_internal_order_after(cc_cache *c, hipStream_t s);
+}
+
 namespace cc {
 
 // One device pointer per CutCellData column (src/cut_cell.jl:112-120).

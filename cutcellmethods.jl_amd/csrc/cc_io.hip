@@ -19,10 +19,6 @@
 #include "../../include/cutcell_hip.h"
 #include "cc_internal.h"
 
-extern "C" int cc_internal_body(cc_cache *c, int *method, int *dtype, int64_t *n, void **cols, void **cutlist,
-                                void **counter, int *kz);
-extern "C" void cc_internal_set_error(const char *msg);
-extern "C" int cc_internal_leaves(const cc_cache *c);
 
 namespace {
 

@@ -27,7 +27,6 @@
 #include "../../include/cutcell_hip.h"
 #include "cc_internal.h"
 
-extern "C" void cc_internal_set_error(const char *msg);
 
 namespace {
 

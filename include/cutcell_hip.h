@@ -307,7 +307,9 @@ int cc_mesh_cell_indices(cc_cache *cache, const cc_grid *grid, int64_t *cells, i
  * over every body's work items, one cut-cell/face pass.  Bodies must be analytic (sphere, torus,
  * box, plane).  Each step's grids (origins may move) and bodies are read from `grids[b]`/`geos[b]`;
  * every body's results land in its own cache's columns, identical to cc_update on that cache.
- * The caches must outlive the batch and must not be updated on their own while it runs. */
+ * The caches must outlive the batch and must not be updated on their own while it runs.  A step
+ * waits for any cc_update_async still running on a member cache; other work a caller queues on a
+ * member's stream (cc_stream) must be synchronised by the caller before the step. */
 typedef struct cc_batch cc_batch;
 int cc_batch_create(int nbodies, cc_cache *const *caches, cc_batch **out);
 int cc_batch_destroy(cc_batch *batch);
