@@ -515,6 +515,111 @@ void orc_ms_cells(const double *phi, const double *x0, const double *d, const in
 }
 
 /* =========================================================================================== */
+/* The 2D contour: generate_mesh(cache, grid), src/marching_squares/surface.jl:59-86, 123-177   */
+/* =========================================================================================== */
+
+/* _vertex_id! (:170-177): a vertex keyed by its exact coordinates (+0-normalised, x .+ zero(T)),
+ * numbered 1, 2, ... in first-reach order.  Open addressing over the two coordinates' bits. */
+typedef struct {
+    uint64_t *key; /* 2 per slot */
+    int64_t *id;   /* 0 = empty */
+    int64_t cap;   /* a power of two */
+    double *nodes; /* 2 per vertex, grown as needed */
+    int64_t nn, ncap;
+} vtx_map;
+
+static uint64_t dbits(double v)
+{
+    uint64_t u;
+    memcpy(&u, &v, 8);
+    return u;
+}
+
+static int64_t vertex_id(vtx_map *m, const double p[2])
+{
+    double k0 = p[0] + 0.0, k1 = p[1] + 0.0;
+    uint64_t a = dbits(k0), b = dbits(k1);
+    uint64_t h = (a * 0x9E3779B97F4A7C15ull) ^ (b + 0x632BE59BD9B4E019ull + (a << 6) + (a >> 2));
+    int64_t s = (int64_t)(h & (uint64_t)(m->cap - 1));
+    while (m->id[s]) {
+        if (m->key[2 * s] == a && m->key[2 * s + 1] == b) return m->id[s];
+        s = (s + 1) & (m->cap - 1);
+    }
+    if (m->nn == m->ncap) {
+        m->ncap = m->ncap ? 2 * m->ncap : 1024;
+        m->nodes = (double *)realloc(m->nodes, sizeof(double) * 2 * m->ncap);
+    }
+    m->nodes[2 * m->nn] = k0;
+    m->nodes[2 * m->nn + 1] = k1;
+    m->nn += 1;
+    m->key[2 * s] = a;
+    m->key[2 * s + 1] = b;
+    m->id[s] = m->nn;
+    return m->nn;
+}
+
+/* generate_mesh over a grid (:59-86): the cells whose corner signs differ (_ms_screen, :123-135; one
+ * byte a cell, screened in parallel), then serially in cell order each one's cell_interface
+ * (marching_squares.jl:343-351: the gap segments with their ends swapped) pushed as lines between
+ * welded vertices (_push_segments!, :153-161).  Two-call protocol: with nodes/lines NULL only the
+ * counts; nodes 2 per vertex, lines 2 (1-based ids) per segment.  Returns 0, or -1 when out of
+ * memory. */
+int orc_ms_contour(const double *phi, const double *x0, const double *d, const int64_t *n,
+                   int64_t *nnodes, int64_t *nlines, double *nodes, int64_t *lines)
+{
+    int64_t ncell = n[0] * n[1], m0 = n[0] + 1, ncut = 0;
+    uint8_t *cut = (uint8_t *)malloc((size_t)(ncell > 0 ? ncell : 1));
+    if (!cut) return -1;
+#pragma omp parallel for schedule(static) reduction(+ : ncut)
+    for (int64_t c = 0; c < ncell; ++c) {
+        int64_t i = c % n[0], j = c / n[0], b = i + m0 * j;
+        int s0 = phi[b] >= 0.0, s1 = phi[b + 1] >= 0.0, s2 = phi[b + m0 + 1] >= 0.0, s3 = phi[b + m0] >= 0.0;
+        cut[c] = !(s0 == s1 && s1 == s2 && s2 == s3);
+        ncut += cut[c];
+    }
+    vtx_map m = {0};
+    m.cap = 64;
+    while (m.cap < 8 * ncut + 16) m.cap *= 2;
+    m.key = (uint64_t *)calloc((size_t)(2 * m.cap), sizeof(uint64_t));
+    m.id = (int64_t *)calloc((size_t)m.cap, sizeof(int64_t));
+    if (!m.key || !m.id) {
+        free(cut); free(m.key); free(m.id);
+        return -1;
+    }
+    int64_t nl = 0;
+    static const int B[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}}; /* MS_NODE_BITS :68 */
+    for (int64_t c = 0; c < ncell; ++c) {
+        if (!cut[c]) continue;
+        int64_t i = c % n[0], j = c / n[0];
+        double cn[4][2], ph[4];
+        for (int k = 0; k < 4; ++k) {
+            int64_t ii = i + B[k][0], jj = j + B[k][1];
+            cn[k][0] = x0[0] + (double)ii * d[0];
+            cn[k][1] = x0[1] + (double)jj * d[1];
+            ph[k] = phi[ii + m0 * jj];
+        }
+        double fr[4], sa[4][2], sb[4][2], ga[2][2], gb[2][2];
+        int present[4];
+        boundary_walk(cn, ph, fr, present, sa, sb);
+        int ns = gap_segments(present, sa, sb, ga, gb);
+        for (int q = 0; q < ns; ++q) { /* cell_interface returns (n, b, a): from gb to ga */
+            int64_t i1 = vertex_id(&m, gb[q]);
+            int64_t i2 = vertex_id(&m, ga[q]);
+            if (lines) {
+                lines[2 * nl] = i1;
+                lines[2 * nl + 1] = i2;
+            }
+            nl += 1;
+        }
+    }
+    if (nodes) memcpy(nodes, m.nodes, sizeof(double) * 2 * (size_t)m.nn);
+    *nnodes = m.nn;
+    *nlines = nl;
+    free(cut); free(m.key); free(m.id); free(m.nodes);
+    return 0;
+}
+
+/* =========================================================================================== */
 /* Marching cubes: src/marching_cubes/marching_cubes.jl                                         */
 /* =========================================================================================== */
 

@@ -80,6 +80,10 @@ void orc_ms_cells(const double *phi, const double *x0, const double *d, const in
 /* ---- marching squares on a quadtree's leaves: src/marching_squares/cache.jl:129-137 ----------
  * Leaf c: level[c], lower corner coord[2c..2c+1] on its level's lattice (spacing d / 2^level from
  * the base origin x0).  phi4: the four corner values per leaf; out: one entry per leaf. */
+/* generate_mesh over a 2D grid (src/marching_squares/surface.jl:59-86, 123-177): the contour of phi
+ * with exactly welded vertices; counts only when nodes/lines are NULL.  0, or -1 out of memory. */
+int orc_ms_contour(const double *phi, const double *x0, const double *d, const int64_t *n,
+                   int64_t *nnodes, int64_t *nlines, double *nodes, int64_t *lines);
 void orc_ms_leaves(const orc_geometry *g, const double *x0, const double *d, int64_t n,
                    const int32_t *level, const int32_t *coord, double *phi4, const orc_cols *out);
 

@@ -56,6 +56,8 @@ def lib():
         vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p
         L.orc_sample_nodes.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.orc_ms_cells.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_ms_contour.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_ms_contour.restype = ctypes.c_int
         L.orc_mc_cells.argtypes = [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int64, vp]
         L.orc_mc_cell_case.argtypes = [vp]
         L.orc_mc_cell_case.restype = ctypes.c_int
@@ -171,6 +173,21 @@ def ms_cells(phi, x0, d, n):
     phi = _f64(phi)
     lib().orc_ms_cells(_p(phi), _p(_f64(x0, 3)), _p(_f64(d, 3)), _p(_i64(n)), ctypes.byref(s))
     return cols
+
+
+def ms_contour(phi, x0, d, n):
+    """generate_mesh over a 2D grid (src/marching_squares/surface.jl:59-86): (nodes (m, 2), lines
+    (k, 2) 1-based)."""
+    phi = _f64(phi)
+    args = (_p(phi), _p(_f64(x0, 3)), _p(_f64(d, 3)), _p(_i64(n)))
+    nv, nl = ctypes.c_int64(0), ctypes.c_int64(0)
+    if lib().orc_ms_contour(*args, ctypes.byref(nv), ctypes.byref(nl), None, None) != 0:
+        raise MemoryError("orc_ms_contour")
+    nodes = np.zeros((nv.value, 2))
+    lines = np.zeros((nl.value, 2), np.int64)
+    if lib().orc_ms_contour(*args, ctypes.byref(nv), ctypes.byref(nl), _p(nodes), _p(lines)) != 0:
+        raise MemoryError("orc_ms_contour")
+    return nodes, lines
 
 
 def ms_leaves(geo, x0, d, level, coord):

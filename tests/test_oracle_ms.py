@@ -201,3 +201,34 @@ def test_watertight_off_lattice():
         outside_by_cut += int((((a == 1) & (b == 2)) | ((a == 2) & (b == 1))).sum())
     assert outside_by_cut > 50
     assert np.count_nonzero(readers.closure_residual(c["face_fraction"], d)) == 0
+
+
+def test_contour_against_an_independent_restatement_and_its_shape():
+    # generate_mesh over a grid (src/marching_squares/surface.jl:59-177): the oracle's C stitch
+    # against the tests' own Python restatement (cell_interface per cut cell in cell order, exact
+    # +0-normalised weld), on the 200^2 circle, a random field on an offset anisotropic grid, and a
+    # field with exact zeros at nodes (crossings landing on corners, -0 against +0 in the weld)
+    from cutcell.grid import CartesianGrid, CartesianIsoGrid
+    from refrng import XorShift, randn_field
+    from test_gpu_parity import contour_reference
+    g = CartesianIsoGrid((-1.0, -1.0), (1.0, 1.0), sz=0.01)
+    phi = sample_vec(lambda x, y: np.sqrt(x * x + y * y) - 0.5, g.x0, g.d, g.n)
+    nodes, lines = orc.ms_contour(phi, g.x0, g.d, g.n)
+    wn, wl = contour_reference(phi, g)
+    assert np.array_equal(nodes, wn) and np.array_equal(lines, wl)
+    # one closed loop: every vertex ends two segments; wound with the body on the left of b - a
+    counts = np.bincount(lines.reshape(-1), minlength=nodes.shape[0] + 1)[1:]
+    assert np.all(counts == 2) and nodes.shape[0] == lines.shape[0] > 300
+    a, b = nodes[lines[:, 0] - 1], nodes[lines[:, 1] - 1]
+    v = b - a
+    assert np.all(np.sum(np.stack([v[:, 1], -v[:, 0]], 1) * (0.5 * (a + b)), axis=1) > 0)
+    assert np.all(np.abs(np.hypot(nodes[:, 0], nodes[:, 1]) - 0.5) < 1e-3)
+    ga = CartesianGrid((-1.3, -0.7), (37, 29), (0.07, 0.05))
+    rng = XorShift(4242)
+    for field in (randn_field(rng, ga.nodegrid_size()),
+                  np.where(np.arange(38 * 30) % 7 == 0, 0.0, randn_field(rng, ga.nodegrid_size())),
+                  np.where(np.arange(38 * 30) % 5 == 0, -0.0, randn_field(rng, ga.nodegrid_size()))):
+        nodes, lines = orc.ms_contour(field, ga.x0, ga.d, ga.n)
+        wn, wl = contour_reference(field, ga)
+        assert np.array_equal(nodes, wn) and np.array_equal(lines, wl)
+        assert lines.shape[0] > 100
