@@ -63,6 +63,46 @@ def workload(n, seed):
     return grid, SDFSphere(centre, 0.4), centre
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    return orc
+
+
+def host_cpu():
+    """(model name, logical CPUs) of the host the CPU baselines run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def _cpu_info(orc=None, threads=None):
+    """cores (the threads the baseline used: the oracle's OpenMP threads unless given), the host
+    CPU's model and its logical CPUs."""
+    model, nproc = host_cpu()
+    return {"cores": orc.num_threads() if threads is None else threads, "cpu_model": model, "nproc": nproc}
+
+
+def own_fill(cache, names, iters=3):
+    """The plain fill kernel over each of the cache's own columns in ``names`` (overwriting them):
+    GB/s per column and over all of them together -- the write rate these allocations reach."""
+    per, tb, tms = {}, 0, 0.0
+    for name in names:
+        gbs = cache.probe_fill(name, iters)
+        nb = cache.column_ptr(name)[1] // 16 * 16
+        per[name] = round(gbs, 1)
+        tb += nb
+        tms += nb / gbs / 1e6
+    return per, tb / tms / 1e6
+
+
 def cpu_baseline(grid, geo, budget_s):
     """The oracle (all host cores, OpenMP) over the same workload in slabs of 8 k-layers, from k = 0
     upward (wrapping), until about budget_s seconds of CPU update time have been spent."""
@@ -82,7 +122,7 @@ def cpu_baseline(grid, geo, budget_s):
         cells += n[0] * n[1] * (k1 - k)
         slabs += 1
         k = 0 if k1 >= n[2] else k1
-    return {"value": cells / spent, "unit": "cells/s", "cores": orc.num_threads(), "kind": "port",
+    return {"value": cells / spent, "unit": "cells/s", **_cpu_info(orc), "kind": "port",
             "sample": f"oracle update (sample_nodes + cell moments) of {slabs} slabs of 8 k-layers "
                       f"of the {n[0]}x{n[1]}x{n[2]} workload from k=0 up = {cells} cells in "
                       f"{spent:.2f} s"}
@@ -97,7 +137,7 @@ def _timed(fn, reps):
     return float(np.median(ts)), ts
 
 
-def secondary(seed):
+def secondary(seed, cpu=True):
     """The other workloads, each at its stated size: returns a dict of measurements."""
     from cutcell import (CartesianIsoGrid, MarchingCubesCutCell, MarchingSquaresCutCell, PLICCutCell,
                          SDFSphere, SDFTorus, allocate_cache, generate_mesh, random_rotation,
@@ -114,24 +154,71 @@ def secondary(seed):
     nodes, lines = generate_mesh(c2, g2)
     out["ms_200"] = {"update_us": round(t_up * 1e6, 1), "generate_mesh_ms": round(t_mesh * 1e3, 3),
                      "contour_segments": int(lines.shape[0]), "cells": g2.ncells}
+    if cpu:
+        orc = _oracle()
+        phi_buf = np.empty(int(np.prod(g2.nodegrid_size())))
+
+        def cpu_update():
+            orc.ms_cells(orc.sample_nodes(geo2, 2, g2.x0, g2.d, g2.n, out=phi_buf), g2.x0, g2.d, g2.n)
+
+        cpu_update()
+        t_cu, _ = _timed(cpu_update, 20)
+        phi2 = orc.sample_nodes(geo2, 2, g2.x0, g2.d, g2.n)
+        t_cm, _ = _timed(lambda: orc.ms_contour(phi2, g2.x0, g2.d, g2.n), 10)
+        cn, cl = orc.ms_contour(phi2, g2.x0, g2.d, g2.n)
+        out["ms_200"]["cpu_baseline"] = {
+            "update_us": round(t_cu * 1e6, 1), "generate_mesh_ms": round(t_cm * 1e3, 3), **_cpu_info(orc),
+            "kind": "port", "sample": "the oracle's update (sample_nodes + cell moments, OpenMP) and its "
+                                      "contour (threaded screen, serial weld) of the same 200x200 grid",
+            "same_contour": bool(np.array_equal(cn, nodes) and np.array_equal(cl, lines))}
     del c2
     # PLIC, torus R=0.5 r=0.15, seeded tilt, 384^3
     rng = np.random.default_rng(seed)
     g3 = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=384)
     tor = SDFTorus((0.0, 0.0, 0.0), 0.5, 0.15, random_rotation(rng))
     c3 = allocate_cache(g3, PLICCutCell)
+    # the write rate of the columns the sweep writes (the centroid columns keep their NaN seed)
+    fill_cols, fill_all = own_fill(c3, ("kind", "volume_fraction", "face_fraction", "normals", "intercepts",
+                                        "is_valid"))
     c3.set_profiling(True)
     for _ in range(3):
         update_cache(c3, tor, g3)
-    t_p, _ = _timed(lambda: update_cache(c3, tor, g3), 10)
-    kt = dict(c3.kernel_times())
+    kts = []
+
+    def plic_step():
+        update_cache(c3, tor, g3)
+        kts.append(dict(c3.kernel_times()))
+
+    t_p, _ = _timed(plic_step, 10)
+    kt = {k: float(np.median([v[k] for v in kts])) for k in kts[0]}
     n3 = g3.ncells
-    two_pass = 131 * n3
+    two_pass, fused = 131 * n3, 90 * n3
+    sweep_ms = kt.get("plic_fused")
     out["plic_384"] = {"update_ms": round(t_p * 1e3, 3), "cells_per_s": n3 / t_p,
                        "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
-                       "algorithmic_bytes_two_pass": two_pass,
+                       "algorithmic_bytes_fused": fused, "algorithmic_bytes_two_pass": two_pass,
                        "hbm_frac_two_pass": round(two_pass / t_p / 1e9 / HBM_PEAK_GBS, 4),
-                       "cut_cells": int((c3.read("kind") == 2).sum())}
+                       "cut_cells": int((c3.read("kind") == 2).sum()),
+                       "own_columns_fill_GBps": round(fill_all, 1), "column_fill_GBps": fill_cols}
+    if sweep_ms:
+        gbs = fused / (sweep_ms * 1e-3) / 1e9
+        out["plic_384"]["roofline_fused"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                             "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                             "frac_of_own_fill": round(gbs / fill_all, 4)}
+    if cpu:
+        orc = _oracle()
+        layers = 16
+        k0 = g3.n[2] // 2 - layers // 2
+        x0 = (g3.x0[0], g3.x0[1], g3.x0[2] + k0 * g3.d[2])
+        sub = (g3.n[0], g3.n[1], layers)
+        ns = sub[0] * sub[1] * layers
+        buf = (orc.alloc_cols(ns, 3, nan=True), np.zeros((ns, 3)), np.zeros(ns), np.zeros(ns, np.uint8))
+        orc.plic(tor, 3, x0, g3.d, sub, out=buf)
+        t_c, _ = _timed(lambda: orc.plic(tor, 3, x0, g3.d, sub, out=buf), 3)
+        out["plic_384"]["cpu_baseline"] = {
+            "value": round(t_c * g3.n[2] / layers * 1e3, 1), "unit": "ms per update", **_cpu_info(orc),
+            "kind": "port", "sample": f"the oracle's fit and face passes (OpenMP) over the {layers} middle "
+                                      f"k-layers ({ns} cells), scaled to the {g3.n[2]} layers"}
     del c3
     return out
 
@@ -177,7 +264,7 @@ def surface(seed, n=512, reps=8, cpu=True):
         t = time.perf_counter()
         nodes, tris, _ = orc.mc_mesh(phi, g.x0, g.d, g.n)
         t_cpu = time.perf_counter() - t
-        out["cpu_baseline"] = {"value": round(t_cpu * 1e3 / 2, 1), "unit": "ms per march", "cores": 1, "kind": "port",
+        out["cpu_baseline"] = {"value": round(t_cpu * 1e3 / 2, 1), "unit": "ms per march", **_cpu_info(threads=1), "kind": "port",
                                "sample": "the oracle's serial march of the same field (it marches twice, counting "
                                          "then filling; half the time is reported)",
                                "same_mesh": bool(len(nodes) == nv and len(tris) == nt)}
@@ -235,7 +322,7 @@ def quadtree(seed, level=12, reps=20, cpu=True):
         t = time.perf_counter()
         orc.ms_leaves(geo, base.x0, base.d, mesh.level, mesh.coord)
         t_cpu = time.perf_counter() - t
-        out["cpu_baseline"] = {"value": round(t_cpu * 1e3, 2), "unit": "ms per update", "cores": orc.num_threads(),
+        out["cpu_baseline"] = {"value": round(t_cpu * 1e3, 2), "unit": "ms per update", **_cpu_info(orc),
                                "kind": "port", "sample": "the oracle's leaf pass over the same tree (OpenMP)"}
     del cache
     return out
@@ -285,7 +372,7 @@ def plic_surface(seed, n=384, reps=5, cpu=True):
         orc.plic_surface(3, g.x0, g.d, g.n, nrm, ic)
         t_s = (time.perf_counter() - t) / 2  # it clips twice, counting then filling
         out["cpu_baseline"] = {"calc_volume_ms": round(t_v * 1e3, 1), "surface_ms": round(t_s * 1e3, 1),
-                               "cores": 1, "kind": "port",
+                               **_cpu_info(threads=1), "kind": "port",
                                "sample": "the oracle's serial calc_volume and surface over the same grid",
                                "volume_rel_diff": abs(float(vol) - v_ref) / v_ref}
     del c
@@ -341,7 +428,7 @@ def diagnostics_pass(seed, n=512, reps=8, cpu=True, layers=16):
             np.any(fcl[:-1, :, :, 5] != fcl[1:, :, :, 4], axis=-1).sum())
         t_cpu = time.perf_counter() - t
         out["cpu_baseline"] = {"value": round(t_cpu * grid.n[2] / layers * 1e3, 1), "unit": "ms per pass",
-                               "cores": 1, "kind": "host numpy readers",
+                               **_cpu_info(threads=1), "kind": "host numpy readers",
                                "sample": f"{layers} middle k-layers ({hi - lo} cells) already on the host, "
                                          f"scaled to the {n}^3 grid"}
     del c
@@ -429,7 +516,7 @@ def _compare_caches(exact, sur, grid, layers=16):
     return res
 
 
-def csg(seed, n=512, train_epochs=24, train_samples=1 << 22):
+def csg(seed, n=512, train_epochs=24, train_samples=1 << 22, cpu=True):
     """The composite body (smooth union, k = 0.02, of 512 spheres, centres U[-0.7, 0.7]^3, radii
     U[0.05, 0.15]) at n^3, moving 0.5 cell per step: the exact path (every node folds every sphere),
     the culled exact path (bitwise the same phi) and the learned surrogate (network trained here on
@@ -477,18 +564,17 @@ def csg(seed, n=512, train_epochs=24, train_samples=1 << 22):
         if src == "culled":
             st = cache.phi_stats()
             res[src]["mean_kept_spheres_per_brick"] = round(st["kept_sum"] / st["bricks"], 2)
-    sur = allocate_cache(grid, MarchingCubesCutCell)
-    sur.set_profiling(True)
-    sur.set_phi_source("surrogate", net, coarse=8, band_cells=3.0)
-    stune = tune_stream(sur, body, at(0), candidates=cands, repeats=2, use_cache=False)
-    update_cache(sur, body, at(0))
+    # the surrogate on the same cache allocation (where an allocation lands moves the stream pass)
+    cache.set_phi_source("surrogate", net, coarse=8, band_cells=3.0)
+    stune = tune_stream(cache, body, at(0), candidates=cands, repeats=2, use_cache=False)
+    update_cache(cache, body, at(0))
     ts, kts = [], []
     for s in range(8):
         t = time.perf_counter()
-        update_cache(sur, body, at(1 + s))
+        update_cache(cache, body, at(1 + s))
         ts.append(time.perf_counter() - t)
-        kts.append(dict(sur.kernel_times()))
-    st = sur.phi_stats()
+        kts.append(dict(cache.kernel_times()))
+    st = cache.phi_stats()
     km = {k: round(float(np.median([kt[k] for kt in kts])), 4) for k in kts[0]}
     nc = n // 8 + 2
     lat_points = nc ** 3
@@ -504,20 +590,39 @@ def csg(seed, n=512, train_epochs=24, train_samples=1 << 22):
         "certificate_evals": st["cert_evals"],
         "nn_lattice_tflops": round(nn_tflops, 1),
         "mfma_utilisation_bf16": round(nn_tflops / 2500.0, 4)}
-    # identical results: the same grid position through the exact path and the surrogate
-    cache.set_phi_source("naive")
+    # identical results: the same grid position through the exact path (a second cache) and the
+    # surrogate
+    sur = allocate_cache(grid, MarchingCubesCutCell)
     update_cache(cache, body, at(20))
     update_cache(sur, body, at(20))
-    res["agreement"] = _compare_caches(cache, sur, grid)
+    res["agreement"] = _compare_caches(sur, cache, grid)
     res["speedup_vs_naive"] = round(res["naive"]["update_ms"] / res["surrogate"]["update_ms"], 2)
     res["speedup_vs_culled"] = round(res["culled"]["update_ms"] / res["surrogate"]["update_ms"], 3)
+    res["timed_on"] = "one cache allocation for naive, culled and surrogate; a second for the agreement check"
+    if cpu:
+        # the oracle's exact fold (every node folds all 512 spheres) and cell pass on a slab of
+        # `layers` k-layers through the body, scaled to the grid
+        orc = _oracle()
+        layers = 2
+        k0 = n // 2 - layers // 2
+        g0 = at(0)
+        x0 = (g0.x0[0], g0.x0[1], g0.x0[2] + k0 * g0.d[2])
+        sub = (n, n, layers)
+        phi = np.empty((n + 1) * (n + 1) * (layers + 1))
+        cols = orc.alloc_cols(n * n * layers, 3)
+        t = time.perf_counter()
+        orc.mc_cells(orc.sample_nodes(body, 3, x0, g0.d, sub, out=phi), x0, g0.d, sub, cols=cols)
+        t_c = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": round(t_c * n / layers * 1e3, 1), "unit": "ms per update", **_cpu_info(orc),
+                               "kind": "port", "sample": f"the oracle's exact update of {layers} middle k-layers "
+                                                         f"({n * n * layers} cells), scaled to the {n} layers"}
     out.update(res)
     del cache, sur
     net.close()
     return out
 
 
-def batched(seed, steps=32):
+def batched(seed, steps=32, cpu=True, cpu_budget_s=10.0):
     from cutcell import MarchingCubesCutCell, PLICCutCell, allocate_cache
     from cutcell.autotune import tune_batch
     from cutcell.batch import Batch
@@ -543,13 +648,43 @@ def batched(seed, steps=32):
     ncell = wl.ncells()
     mc_cells = sum(int(np.prod(B["n"])) for B in wl.bodies if B["method"] == "mc")
     alg = BYTES_PER_CELL * mc_cells + 131 * (ncell - mc_cells)
-    return {"bodies": nb, "cells_per_step": ncell, "mc_cells": mc_cells,
-            "cells_per_s": ncell / (np.mean(lat) * 1e-3),
-            "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
-                                "p99": round(float(np.percentile(lat, 99)), 3)},
-            "graph_device_ms_p50": round(float(np.percentile(dev, 50)), 3),
-            "hbm_frac": round(alg / (float(np.percentile(dev, 50)) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "kz_by_body_class": sorted(set(kz.values()))}
+    out = {"bodies": nb, "cells_per_step": ncell, "mc_cells": mc_cells,
+           "cells_per_s": ncell / (np.mean(lat) * 1e-3),
+           "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                               "p99": round(float(np.percentile(lat, 99)), 3)},
+           "graph_device_ms_p50": round(float(np.percentile(dev, 50)), 3),
+           "hbm_frac": round(alg / (float(np.percentile(dev, 50)) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "kz_by_body_class": sorted(set(kz.values()))}
+    del batch, caches
+    if cpu:
+        # the oracle updates the bodies of the last step one after another (outputs allocated
+        # beforehand, untimed) until the budget is spent; the step time is the sum over bodies
+        orc = _oracle()
+        s = steps - 1
+        spent, cells, done = 0.0, 0, 0
+        for b in range(nb):
+            if spent >= cpu_budget_s:
+                break
+            g, geo = wl.grid(b, s), wl.geometry(b, s)
+            nc = g.ncells
+            if wl.bodies[b]["method"] == "mc":
+                phi = np.empty(int(np.prod(g.nodegrid_size())))
+                cols = orc.alloc_cols(nc, 3)
+                t = time.perf_counter()
+                orc.mc_cells(orc.sample_nodes(geo, 3, g.x0, g.d, g.n, out=phi), g.x0, g.d, g.n, cols=cols)
+            else:
+                buf = (orc.alloc_cols(nc, 3, nan=True), np.zeros((nc, 3)), np.zeros(nc), np.zeros(nc, np.uint8))
+                t = time.perf_counter()
+                orc.plic(geo, 3, g.x0, g.d, g.n, out=buf)
+            spent += time.perf_counter() - t
+            cells += nc
+            done += 1
+        rate = cells / spent
+        out["cpu_baseline"] = {"value": rate, "unit": "cells/s", "step_ms_estimate": round(ncell / rate * 1e3, 1),
+                               **_cpu_info(orc), "kind": "port",
+                               "sample": f"the oracle's update of bodies 0..{done - 1} of step {s} ({cells} cells "
+                                         f"in {spent:.2f} s); the step estimate scales to its {ncell} cells"}
+    return out
 
 
 def main():
@@ -562,9 +697,10 @@ def main():
     from cutcell import MarchingCubesCutCell, allocate_cache, update_cache
 
     if a.only:
-        fn = {"csg": lambda: csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples),
-              "batched": lambda: batched(a.seed),
-              "secondary": lambda: secondary(a.seed),
+        fn = {"csg": lambda: csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples,
+                                 cpu=not a.no_cpu),
+              "batched": lambda: batched(a.seed, cpu=not a.no_cpu),
+              "secondary": lambda: secondary(a.seed, cpu=not a.no_cpu),
               "surface": lambda: surface(a.seed, cpu=not a.no_cpu),
               "quadtree": lambda: quadtree(a.seed, cpu=not a.no_cpu),
               "plic_surface": lambda: plic_surface(a.seed, cpu=not a.no_cpu),
@@ -581,6 +717,10 @@ def main():
 
     from cutcell.cache import probe_bandwidth
     fill_gbs, copy_gbs = probe_bandwidth(8 << 30, 5)  # also brings the memory clocks up
+    # the write rate of the cache's own columns (where an allocation lands moves it); every one of
+    # them is rewritten by the updates below
+    col_fill, own_fill_gbs = own_fill(cache, ("kind", "ambiguous", "volume_fraction", "centroid", "face_fraction",
+                                              "face_centroid_local", "interface_centroid", "phi"))
     tuning = None
     if a.tune == "auto":
         from cutcell.autotune import tune_stream
@@ -622,11 +762,18 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "algorithmic_bytes": alg,
+                "traffic": traffic,
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this kernel on this workload, "
+                                  "scaled by the same counters on plain copy/fill kernels "
+                                  f"({os.path.relpath(a.traffic, ROOT)})" if traffic is not None else None,
+                "algorithmic_bytes": alg,
                 "kernel_ms": {k: round(v, 4) for k, v in avg.items()}}
     update_bytes = BYTES_PER_CELL * ncell + BYTES_PER_NODE * nnode
     roofline["probe_fill_GBps"] = round(fill_gbs, 1)
     roofline["probe_copy_GBps"] = round(copy_gbs, 1)
+    roofline["own_columns_fill_GBps"] = round(own_fill_gbs, 1)
+    roofline["face_centroid_local_fill_GBps"] = col_fill["face_centroid_local"]
+    roofline["frac_of_own_fill"] = round(achieved / own_fill_gbs, 4)
     sweep = None
     if a.sweep:
         sweep = []
@@ -650,7 +797,7 @@ def main():
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"MarchingCubesCutCell update_cache! {a.n}^3 Float64, sphere r=0.4 "
-                               f"centre {tuple(round(c, 4) for c in centre)}, moving 0.5 cell/step",
+                               f"centre {tuple(round(float(c), 4) for c in centre)}, moving 0.5 cell/step",
                    "cells": ncell, "cut_cells": cache.cut_count(),
                    "stream_tuning": {"variant": tv, "kz": tk, "nontemporal": bool(tn),
                                      "chosen_by": "autotuner" if tuning else "--tune"},
@@ -663,13 +810,14 @@ def main():
     if not a.quick:
         out["fp32"] = fp32_error(grid, geo, cache, "auto" if a.tune == "auto" else (tv, tk, bool(tn)))
         del cache
-        out.update(secondary(a.seed))
+        out.update(secondary(a.seed, cpu=not a.no_cpu))
         out["mc_surface_512"] = surface(a.seed, cpu=not a.no_cpu)
         out["ms_quadtree"] = quadtree(a.seed, cpu=not a.no_cpu)
         out["plic_surface_384"] = plic_surface(a.seed, cpu=not a.no_cpu)
         out["diagnostics_512"] = diagnostics_pass(a.seed, cpu=not a.no_cpu)
-        out["batched_64_bodies"] = batched(a.seed)
-        out["csg_512_spheres"] = csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples)
+        out["batched_64_bodies"] = batched(a.seed, cpu=not a.no_cpu)
+        out["csg_512_spheres"] = csg(a.seed, train_epochs=a.train_epochs, train_samples=a.train_samples,
+                                     cpu=not a.no_cpu)
     print(json.dumps(out))
 
 

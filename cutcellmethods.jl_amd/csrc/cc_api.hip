@@ -1446,6 +1446,37 @@ int cc_kernel_times(cc_cache *c, int max, float *ms, const char **names, int *co
     return CC_OK;
 }
 
+int cc_probe_fill_column(cc_cache *c, int field, int iters, float *ms) {
+    if (!c || !ms || iters < 1) return fail(CC_ERR_ARGUMENT, "NULL cache or result, or iters < 1");
+    if (field < 0 || field >= CC_NUM_FIELDS || !c->col[field])
+        return fail(CC_ERR_ARGUMENT, "field %d is not a column of this cache", field);
+    const size_t n = (size_t)c->colbytes[field] / 16;
+    if (n == 0) return fail(CC_ERR_ARGUMENT, "column %d is shorter than 16 bytes", field);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = CC_OK;
+    float best = 1e30f;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(CC_ERR_HIP, "probe events: %s", hipGetErrorString(hipGetLastError()));
+    } else {
+        for (int it = 0; it < iters + 1; ++it) {
+            float t = 0.0f;
+            (void)hipEventRecord(e0, c->stream);
+            hipLaunchKernelGGL(k_probe_fill, dim3(256 * 64), dim3(256), 0, c->stream, (f64x2 *)c->col[field], n, 0.0);
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&t, e0, e1);
+            if (it > 0 && t < best) best = t;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(CC_ERR_HIP, "probe failed: %s", hipGetErrorString(e));
+    }
+    *ms = best;
+    c->gen += 1;  // the column no longer holds the last update's values
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
+}
+
 int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms) {
     if (nbytes < (1 << 20) || iters < 1) return fail(CC_ERR_ARGUMENT, "need nbytes >= 1 MiB, iters >= 1");
     nbytes &= ~int64_t(4095);

@@ -123,6 +123,14 @@ class Cache:
         check(lib().cc_column(self._h, _lib.FIELDS[name], ctypes.byref(p), ctypes.byref(nb)))
         return p.value, nb.value
 
+    def probe_fill(self, name, iters=5):
+        """GB/s of a plain 16-byte-per-lane fill of column ``name`` -- the write rate this
+        allocation reaches.  Overwrites the column: update the cache before reading it again."""
+        ms = ctypes.c_float()
+        check(lib().cc_probe_fill_column(self._h, _lib.FIELDS[name], int(iters), ctypes.byref(ms)))
+        nb = self.column_ptr(name)[1] // 16 * 16
+        return nb / (ms.value * 1e-3) / 1e9
+
     def read(self, name, start=0, stop=None):
         """Column ``name`` copied to the host, optionally only leading-axis rows [start, stop)."""
         shape = self._shape(name)

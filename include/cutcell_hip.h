@@ -373,6 +373,10 @@ int cc_phi_stats(cc_cache *cache, int64_t *stats);
  * lane-contiguous accesses -- the achievable-bandwidth yardstick the cut-cell passes are read
  * against, and the plain copy kernel that scales rocprof's FETCH_SIZE/WRITE_SIZE counters. */
 int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms);
+/* The same fill kernel over one of a cache's own columns (its first nbytes/16*16 bytes), best time of
+ * `iters`: the write rate that allocation reaches, the yardstick for the passes writing it, since
+ * where an allocation lands moves it.  Overwrites the column: update the cache before reading it. */
+int cc_probe_fill_column(cc_cache *cache, int field, int iters, float *ms);
 /* Write-pattern study for the same yardstick: best time of a fill of `nbytes` with `blocks`
  * workgroups of 256 and store pattern `variant` (0 grid-stride, 1 non-temporal, 2 one contiguous
  * 64 KiB run per workgroup iteration, 3 four stores in flight per lane). */

@@ -270,7 +270,7 @@ void orc_sample_nodes(const orc_geometry *g, int dim, const double *x0, const do
                       const int64_t *n, double *phi)
 {
     int64_t m0 = n[0] + 1, m1 = n[1] + 1, m2 = dim == 3 ? n[2] + 1 : 1;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for collapse(2) schedule(static)
     for (int64_t k = 0; k < m2; ++k)
         for (int64_t j = 0; j < m1; ++j)
             for (int64_t i = 0; i < m0; ++i) {
@@ -1009,7 +1009,7 @@ void orc_mc_cells(const double *phi, const double *x0, const double *d, const in
                   int64_t k0, int64_t k1, const orc_cols *out)
 {
     int64_t m0 = n[0] + 1, m1 = n[1] + 1;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for collapse(2) schedule(static)
     for (int64_t k = k0; k < k1; ++k)
         for (int64_t j = 0; j < n[1]; ++j)
             for (int64_t i = 0; i < n[0]; ++i) {
@@ -1129,7 +1129,7 @@ void orc_plic_fit(const orc_geometry *g, int dim, const double *x0, const double
                   double *intercepts, uint8_t *is_valid)
 {
     int64_t n2 = dim == 3 ? n[2] : 1;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for collapse(2) schedule(static)
     for (int64_t k = 0; k < n2; ++k)
         for (int64_t j = 0; j < n[1]; ++j)
             for (int64_t i = 0; i < n[0]; ++i) {
@@ -1196,7 +1196,7 @@ void orc_plic_faces(int dim, const double *d, const int64_t *n, const double *no
 {
     int64_t n2 = dim == 3 ? n[2] : 1;
     int64_t st[3] = {1, n[0], n[0] * n[1]};
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for collapse(2) schedule(static)
     for (int64_t k = 0; k < n2; ++k)
         for (int64_t j = 0; j < n[1]; ++j)
             for (int64_t i = 0; i < n[0]; ++i) {

@@ -143,10 +143,11 @@ def _cols_struct(cols):
     return s
 
 
-def sample_nodes(geo, dim, x0, d, n):
-    """``phi`` at every node (src/marching_squares/cache.jl:67-70), flat, column-major."""
+def sample_nodes(geo, dim, x0, d, n, out=None):
+    """``phi`` at every node (src/marching_squares/cache.jl:67-70), flat, column-major (into ``out``
+    when given)."""
     nn = [int(v) + 1 for v in n[:dim]]
-    phi = np.empty(int(np.prod(nn)), np.float64)
+    phi = np.empty(int(np.prod(nn)), np.float64) if out is None else out
     g = make_geometry(geo)
     lib().orc_sample_nodes(ctypes.byref(g), dim, _p(_f64(x0, 3)), _p(_f64(d, 3)),
                            _p(_i64(n)), _p(phi))
@@ -205,12 +206,14 @@ def ms_leaves(geo, x0, d, level, coord):
     return phi4, cols
 
 
-def mc_cells(phi, x0, d, n, k0=0, k1=None):
-    """Cells with k in [k0, k1) of a 3D grid (src/marching_squares/cache.jl:75-78)."""
+def mc_cells(phi, x0, d, n, k0=0, k1=None, cols=None):
+    """Cells with k in [k0, k1) of a 3D grid (src/marching_squares/cache.jl:75-78); ``cols`` (from
+    alloc_cols) is written in place when given."""
     if k1 is None:
         k1 = int(n[2])
     ncell = int(n[0]) * int(n[1]) * (k1 - k0)
-    cols = alloc_cols(ncell, 3)
+    if cols is None:
+        cols = alloc_cols(ncell, 3)
     s = _cols_struct(cols)
     phi = _f64(phi)
     lib().orc_mc_cells(_p(phi), _p(_f64(x0, 3)), _p(_f64(d, 3)), _p(_i64(n)), k0, k1,
@@ -271,15 +274,18 @@ def volume_fraction(normal, a):
     return lib().orc_volume_fraction(len(normal), _p(nv), float(a))
 
 
-def plic(geo, dim, x0, d, n):
+def plic(geo, dim, x0, d, n, out=None):
     """A PLIC cache update (src/plic/cache.jl:77-92): returns (cols, normals, intercepts,
     is_valid); the centroid columns are NaN and ``ambiguous`` is false as allocate_cache seeds
-    them (src/plic/cache.jl:60-75)."""
+    them (src/plic/cache.jl:60-75).  ``out``, a tuple of that form, is written in place."""
     ncell = int(np.prod([int(v) for v in n[:dim]]))
-    cols = alloc_cols(ncell, dim, nan=True)
-    normals = np.zeros((ncell, dim))
-    intercepts = np.zeros(ncell)
-    valid = np.zeros(ncell, np.uint8)
+    if out is None:
+        cols = alloc_cols(ncell, dim, nan=True)
+        normals = np.zeros((ncell, dim))
+        intercepts = np.zeros(ncell)
+        valid = np.zeros(ncell, np.uint8)
+    else:
+        cols, normals, intercepts, valid = out
     g = make_geometry(geo)
     x0a, da, na = _f64(x0, 3), _f64(d, 3), _i64(n)
     lib().orc_plic_fit(ctypes.byref(g), dim, _p(x0a), _p(da), _p(na), _p(cols["volume_fraction"]),
