@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "libcutcell_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["cc_api.hip", "cc_mc.hip", "cc_ms.hip", "cc_plic.hip", "cc_batch.hip", "cc_nn.hip",
-           "cc_phi.hip", "cc_mesh.hip", "cc_plic_surf.hip", "cc_diag.hip", "cc_io.hip"]
+           "cc_phi.hip", "cc_mesh.hip", "cc_plic_surf.hip", "cc_diag.hip", "cc_io.hip", "cc_contour.hip"]
 
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

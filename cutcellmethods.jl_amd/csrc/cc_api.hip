@@ -155,6 +155,8 @@ struct cc_cache {
     // this update (gen) and this grid frame, or this tagged tree (arrays, count, version, base grid)
     std::vector<double> contour_nodes;
     std::vector<int64_t> contour_lines;
+    bool contour_dev = false;  // the kept contour is the device weld's (weld_nodes / weld_lines)
+    int64_t contour_dev_nn = 0, contour_dev_nl = 0;
     bool contour_ok = false;
     uint64_t contour_gen = 0;
     const void *contour_src[2] = {};
@@ -167,6 +169,14 @@ struct cc_cache {
     int64_t mark_phi_cap = 0;
     int32_t *mark_lc = nullptr;
     int64_t mark_lc_cap = 0;
+    // where the contour is welded (CC_WELD_*), and the device weld's scratch and outputs
+    int weld_mode = CC_WELD_AUTO;
+    char *weld_scratch = nullptr;
+    int64_t weld_scratch_cap = 0;
+    int64_t *weld_lines = nullptr;
+    int64_t weld_lines_cap = 0;
+    double *weld_nodes = nullptr;
+    int64_t weld_nodes_cap = 0;
 };
 
 namespace {
@@ -888,6 +898,9 @@ void free_all(cc_cache *c) {
     if (c->mark_idx) (void)hipFree(c->mark_idx);
     if (c->mark_phi) (void)hipFree(c->mark_phi);
     if (c->mark_lc) (void)hipFree(c->mark_lc);
+    if (c->weld_scratch) (void)hipFree(c->weld_scratch);
+    if (c->weld_lines) (void)hipFree(c->weld_lines);
+    if (c->weld_nodes) (void)hipFree(c->weld_nodes);
     if (c->leaf_level) (void)hipFree(c->leaf_level);
     if (c->leaf_coord) (void)hipFree(c->leaf_coord);
     for (auto &p : c->col)
@@ -1070,8 +1083,35 @@ template <typename T> int mark_cut_cells(cc_cache *c, int64_t n, int64_t n0, int
     return CC_OK;
 }
 
+// The device weld (cc_contour.hip) of ncut gathered cut cells into the cache's kept contour.
+template <typename T> int device_weld(cc_cache *c, const cc::ContourCells<T> &in, int64_t ncut) {
+    const int64_t bytes = (int64_t)cc::contour_weld_bytes<T>(ncut);
+    int rc = ensure(&c->weld_scratch, &c->weld_scratch_cap, bytes);
+    if (rc == CC_OK) rc = ensure(&c->weld_lines, &c->weld_lines_cap, 4 * ncut);
+    if (rc == CC_OK) rc = ensure(&c->weld_nodes, &c->weld_nodes_cap, 8 * ncut);
+    if (rc) return rc;
+    int64_t nseg = 0, nvert = 0;
+    const int w = cc::contour_weld<T>(in, ncut, c->weld_scratch, (size_t)c->weld_scratch_cap, c->weld_lines,
+                                      c->weld_nodes, &nseg, &nvert, c->stream);
+    if (w == -1) return fail(CC_ERR_HIP, "contour: the weld's scratch is too small");
+    if (w != 0) return fail(CC_ERR_HIP, "contour weld: %s", hipGetErrorString(hipGetLastError()));
+    // kept on the device: the fill call copies it straight into the caller's buffers
+    c->contour_dev = true;
+    c->contour_dev_nn = nvert;
+    c->contour_dev_nl = nseg;
+    return CC_OK;
+}
+
+// Whether a contour of ncut cut cells is welded on the device: from 4096 cut cells unless the cache
+// says otherwise (below that the host weld's single pass beats the device's dozen launches).
+bool weld_on_device(const cc_cache *c, int64_t ncut) {
+    if (c->weld_mode == CC_WELD_DEVICE) return true;
+    if (c->weld_mode == CC_WELD_HOST) return false;
+    return ncut >= 4096;
+}
+
 // The grid contour of a marching-squares cache into its kept contour: the cut cells' corners gathered
-// on the device, welded on the host.
+// on the device, welded there or on the host.
 template <typename T> int grid_contour(cc_cache *c, const cc_grid *g) {
     int64_t ncut = 0;
     int rc = mark_cut_cells<T>(c, c->ncell, c->n[0], &ncut);
@@ -1080,34 +1120,79 @@ template <typename T> int grid_contour(cc_cache *c, const cc_grid *g) {
     rc = ensure(&c->mark_idx, &c->mark_idx_cap, m);
     if (rc == CC_OK) rc = ensure(&c->mark_phi, &c->mark_phi_cap, 4 * (int64_t)sizeof(T) * m);
     if (rc) return rc;
-    std::vector<int64_t> idx(m);
-    std::vector<T> phi(4 * m);
+    c->contour_nodes.clear();
+    c->contour_lines.clear();
+    c->contour_dev = false;
     if (ncut > 0) {
         cc::launch_grid_gather<T>(c->mark_bits, c->mark_rank, (size_t)c->ncell, (const T *)c->col[CC_FIELD_PHI],
                                   (size_t)c->n[0], c->mark_idx, (T *)c->mark_phi, c->stream);
         HIPC(hipGetLastError());
+    }
+    if (ncut > 0 && weld_on_device(c, ncut)) {
+        const cc::ContourCells<T> in{c->mark_idx, nullptr, (const T *)c->mark_phi, c->n[0], T(g->x0[0]),
+                                     T(g->x0[1]), T(g->d[0]), T(g->d[1])};
+        return device_weld<T>(c, in, ncut);
+    }
+    std::vector<int64_t> idx(m);
+    std::vector<T> phi(4 * m);
+    if (ncut > 0) {
         HIPC(hipMemcpyAsync(idx.data(), c->mark_idx, sizeof(int64_t) * ncut, hipMemcpyDeviceToHost, c->stream));
         HIPC(hipMemcpyAsync(phi.data(), c->mark_phi, 4 * sizeof(T) * ncut, hipMemcpyDeviceToHost, c->stream));
         HIPC(hipStreamSynchronize(c->stream));
     }
-    c->contour_nodes.clear();
-    c->contour_lines.clear();
     contour<T>(ncut, idx.data(), phi.data(), g, c->contour_nodes, c->contour_lines);
     return CC_OK;
 }
 
-// Marks, ranks and gathers the cut leaves on the device, copies them to the host and contours them
-// into the cache's kept contour.
+// The device copy of a host leaf list, as run_leaves keeps it: copied again unless the caller's
+// version tag says the last copy holds these leaves.
+int leaves_on_device(cc_cache *c, const cc_leaves *t, const int32_t **level, const int32_t **coord) {
+    if (t->on_device) {
+        *level = t->level;
+        *coord = t->coord;
+        return CC_OK;
+    }
+    const bool same = t->version != 0 && t->version == c->leaf_version && c->leaf_src[0] == t->level &&
+                      c->leaf_src[1] == t->coord;
+    if (!same) {
+        HIPC(hipMemcpyAsync(c->leaf_level, t->level, sizeof(int32_t) * t->n, hipMemcpyHostToDevice, c->stream));
+        HIPC(hipMemcpyAsync(c->leaf_coord, t->coord, 2 * sizeof(int32_t) * t->n, hipMemcpyHostToDevice, c->stream));
+        c->leaf_src[0] = t->level;
+        c->leaf_src[1] = t->coord;
+        c->leaf_version = t->version;
+    }
+    *level = c->leaf_level;
+    *coord = c->leaf_coord;
+    return CC_OK;
+}
+
+// Marks, ranks and gathers the cut leaves on the device and contours them into the cache's kept
+// contour, welded on the device or on the host.
 template <typename T> int tree_contour(cc_cache *c, const cc_leaves *tree) {
     const int64_t n = tree->n;
     int64_t ncut = 0;
     int rc = mark_cut_cells<T>(c, n, 0, &ncut);
     if (rc) return rc;
     const int64_t m = ncut > 0 ? ncut : 1;
+    const bool dev = ncut > 0 && weld_on_device(c, ncut);
     rc = ensure(&c->mark_idx, &c->mark_idx_cap, m);
     if (rc == CC_OK) rc = ensure(&c->mark_phi, &c->mark_phi_cap, 4 * (int64_t)sizeof(T) * m);
-    if (rc == CC_OK && tree->on_device) rc = ensure(&c->mark_lc, &c->mark_lc_cap, 3 * m);
+    if (rc == CC_OK && (tree->on_device || dev)) rc = ensure(&c->mark_lc, &c->mark_lc_cap, 3 * m);
     if (rc) return rc;
+    c->contour_nodes.clear();
+    c->contour_lines.clear();
+    c->contour_dev = false;
+    if (dev) {
+        const int32_t *level = nullptr, *coord = nullptr;
+        rc = leaves_on_device(c, tree, &level, &coord);
+        if (rc) return rc;
+        cc::launch_leaf_gather<T>(c->mark_bits, c->mark_rank, (size_t)n, (const T *)c->col[CC_FIELD_PHI], level,
+                                  coord, c->mark_idx, (T *)c->mark_phi, c->mark_lc, c->stream);
+        HIPC(hipGetLastError());
+        const cc::ContourCells<T> in{c->mark_idx, c->mark_lc, (const T *)c->mark_phi, 0, T(tree->x0[0]),
+                                     T(tree->x0[1]), T(tree->d[0]), T(tree->d[1])};
+        return device_weld<T>(c, in, ncut);
+    }
     std::vector<int64_t> idx(m);
     std::vector<T> phi(4 * m);
     std::vector<int32_t> lc(tree->on_device ? 3 * m : 0);
@@ -1122,10 +1207,33 @@ template <typename T> int tree_contour(cc_cache *c, const cc_leaves *tree) {
             HIPC(hipMemcpyAsync(lc.data(), c->mark_lc, 3 * sizeof(int32_t) * ncut, hipMemcpyDeviceToHost, c->stream));
         HIPC(hipStreamSynchronize(c->stream));
     }
-    c->contour_nodes.clear();
-    c->contour_lines.clear();
     contour_leaves<T>(ncut, idx.data(), phi.data(), tree->on_device ? lc.data() : nullptr, tree, c->contour_nodes,
                       c->contour_lines);
+    return CC_OK;
+}
+
+// The kept contour's counts, and with buffers its nodes and lines (from the host weld's vectors or
+// straight from the device weld's arrays).
+int contour_out(cc_cache *c, double *nodes, int64_t *nnodes, int64_t *lines, int64_t *nlines) {
+    const int64_t nn = c->contour_dev ? c->contour_dev_nn : (int64_t)c->contour_nodes.size() / 2;
+    const int64_t nl = c->contour_dev ? c->contour_dev_nl : (int64_t)c->contour_lines.size() / 2;
+    if (nodes && lines) {
+        if (*nnodes < nn || *nlines < nl)
+            return fail(CC_ERR_ARGUMENT, "buffers hold %lld nodes / %lld lines, need %lld / %lld",
+                        (long long)*nnodes, (long long)*nlines, (long long)nn, (long long)nl);
+        if (c->contour_dev) {
+            if (nl > 0) {
+                HIPC(hipMemcpyAsync(nodes, c->weld_nodes, sizeof(double) * 2 * nn, hipMemcpyDeviceToHost, c->stream));
+                HIPC(hipMemcpyAsync(lines, c->weld_lines, sizeof(int64_t) * 2 * nl, hipMemcpyDeviceToHost, c->stream));
+                HIPC(hipStreamSynchronize(c->stream));
+            }
+        } else {
+            std::memcpy(nodes, c->contour_nodes.data(), c->contour_nodes.size() * sizeof(double));
+            std::memcpy(lines, c->contour_lines.data(), c->contour_lines.size() * sizeof(int64_t));
+        }
+    }
+    *nnodes = nn;
+    *nlines = nl;
     return CC_OK;
 }
 
@@ -1303,6 +1411,15 @@ int cc_synchronize(cc_cache *c) {
     if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
     HIPC(hipStreamSynchronize(c->stream));
     c->async_pending = false;
+    return CC_OK;
+}
+
+int cc_set_contour_weld(cc_cache *c, int mode) {
+    if (!c) return fail(CC_ERR_ARGUMENT, "cache is NULL");
+    if (mode != CC_WELD_AUTO && mode != CC_WELD_HOST && mode != CC_WELD_DEVICE)
+        return fail(CC_ERR_ARGUMENT, "unknown weld mode %d", mode);
+    c->weld_mode = mode;
+    c->contour_ok = false;
     return CC_OK;
 }
 
@@ -1577,19 +1694,7 @@ int cc_generate_mesh_ms(cc_cache *c, const cc_grid *grid, double *nodes, int64_t
         c->contour_gen = c->gen;
         std::memcpy(c->contour_frame, frame, sizeof frame);
     }
-    const std::vector<double> &vn = c->contour_nodes;
-    const std::vector<int64_t> &vl = c->contour_lines;
-    const int64_t nn = (int64_t)vn.size() / 2, nl = (int64_t)vl.size() / 2;
-    if (nodes && lines) {
-        if (*nnodes < nn || *nlines < nl)
-            return fail(CC_ERR_ARGUMENT, "buffers hold %lld nodes / %lld lines, need %lld / %lld",
-                        (long long)*nnodes, (long long)*nlines, (long long)nn, (long long)nl);
-        std::memcpy(nodes, vn.data(), vn.size() * sizeof(double));
-        std::memcpy(lines, vl.data(), vl.size() * sizeof(int64_t));
-    }
-    *nnodes = nn;
-    *nlines = nl;
-    return CC_OK;
+    return contour_out(c, nodes, nnodes, lines, nlines);
 }
 
 int cc_leaf_cache_create(int dtype, const cc_leaves *tree, cc_cache **out) {
@@ -1669,19 +1774,7 @@ int cc_generate_mesh_leaves(cc_cache *c, const cc_leaves *tree, double *nodes, i
         c->contour_version = tree->version;
         std::memcpy(c->contour_frame, frame, sizeof frame);
     }
-    const std::vector<double> &vn = c->contour_nodes;
-    const std::vector<int64_t> &vl = c->contour_lines;
-    const int64_t nn = (int64_t)vn.size() / 2, nl = (int64_t)vl.size() / 2;
-    if (nodes && lines) {
-        if (*nnodes < nn || *nlines < nl)
-            return fail(CC_ERR_ARGUMENT, "buffers hold %lld nodes / %lld lines, need %lld / %lld",
-                        (long long)*nnodes, (long long)*nlines, (long long)nn, (long long)nl);
-        std::memcpy(nodes, vn.data(), vn.size() * sizeof(double));
-        std::memcpy(lines, vl.data(), vl.size() * sizeof(int64_t));
-    }
-    *nnodes = nn;
-    *nlines = nl;
-    return CC_OK;
+    return contour_out(c, nodes, nnodes, lines, nlines);
 }
 
 int cc_calc_volume(int dtype, const cc_geometry *geo, const cc_grid *grid, double *volume) {

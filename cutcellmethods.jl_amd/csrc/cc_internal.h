@@ -244,6 +244,24 @@ void launch_leaf_gather(const unsigned long long *bits, const int64_t *wrank, si
                         const int32_t *level, const int32_t *coord, int64_t *idx, T *phi_out, int32_t *lc_out,
                         hipStream_t s);
 
+// The marching-squares contour welded on the device (cc_contour.hip): the cut cells as the gathers
+// above leave them, entry r being grid cell idx[r] (a grid n0 cells wide) or the leaf with level and
+// lattice corner lc[3r..3r+2] (lc not null), with corner values phi[4r..4r+3].  contour_weld writes
+// the reference's lines (two 1-based vertex numbers per segment, *nseg segments) and nodes (two
+// Float64 values per vertex, *nvert vertices) to device memory sized for 4 ncut ends; it returns 0,
+// -1 when the scratch (contour_weld_bytes) is too small, -2 on a HIP failure.  Blocks on stream s.
+template <typename T> struct ContourCells {
+    const int64_t *idx;
+    const int32_t *lc;
+    const T *phi;
+    int64_t n0;
+    T x0, y0, dx, dy;
+};
+template <typename T> size_t contour_weld_bytes(int64_t ncut);
+template <typename T>
+int contour_weld(const ContourCells<T> &in, int64_t ncut, void *scratch, size_t scratch_bytes, int64_t *lines_dev,
+                 double *nodes_dev, int64_t *nseg, int64_t *nvert, hipStream_t s);
+
 // Diagnostics over a cache's cells: counts[0..6] = cut, inside, outside, ambiguous, cells with a
 // nonzero closure residual, shared faces whose fractions differ, shared faces whose centroids differ;
 // counts[7] = the bits of the smallest positive cut volume fraction (1.0 if none); counts[8] = the bits

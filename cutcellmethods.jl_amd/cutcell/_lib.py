@@ -32,7 +32,7 @@ EXPORTS = ("cc_cache_create", "cc_cache_destroy", "cc_update", "cc_update_from_p
            "cc_stream", "cc_set_tuning", "cc_get_tuning", "cc_set_profiling", "cc_kernel_times", "cc_generate_mesh_ms",
            "cc_generate_mesh_mc", "cc_mesh_device", "cc_mesh_cell_indices",
            "cc_calc_volume", "cc_cut_cell_report", "cc_cache_diagnostics", "cc_write_cache_vtk", "cc_write_npy", "cc_generate_mesh_plic", "cc_extract_surface_plic", "cc_leaf_cache_create", "cc_update_leaves", "cc_update_leaves_from_phi", "cc_generate_mesh_leaves",
-           "cc_probe_bandwidth", "cc_probe_write", "cc_probe_fill_column", "cc_last_error", "cc_version",
+           "cc_probe_bandwidth", "cc_probe_write", "cc_probe_fill_column", "cc_set_contour_weld", "cc_last_error", "cc_version",
            "cc_mlp_create", "cc_mlp_destroy", "cc_mlp_num_params", "cc_mlp_get_params",
            "cc_mlp_set_params", "cc_mlp_eval", "cc_mlp_loss_grad", "cc_mlp_train",
            "cc_set_phi_source", "cc_phi_stats")
@@ -135,6 +135,7 @@ def lib():
         L.cc_probe_write.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_float)]
         L.cc_probe_fill_column.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        L.cc_set_contour_weld.argtypes = [vp, ctypes.c_int]
         fp = ctypes.POINTER(ctypes.c_float)
         L.cc_mlp_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_uint64, ctypes.POINTER(vp)]

@@ -197,6 +197,12 @@ class Cache:
         check(lib().cc_phi_stats(self._h, a))
         return dict(zip(self.PHI_STAT_NAMES, list(a)))
 
+    def set_contour_weld(self, mode="auto"):
+        """Where generate_mesh welds a marching-squares contour: ``"auto"`` (the device from 4096
+        cut cells), ``"host"`` or ``"device"``; the nodes and lines are the same in every mode."""
+        code = {"auto": 0, "host": 1, "device": 2}[mode]
+        check(lib().cc_set_contour_weld(self._h, code))
+
     def set_profiling(self, on=True):
         check(lib().cc_set_profiling(self._h, 1 if on else 0))
 

@@ -221,6 +221,14 @@ typedef struct cc_leaves {
                               the library keeps its device copy instead of copying the leaves again */
 } cc_leaves;
 
+/* Where generate_mesh welds a marching-squares contour (grid or tree): on the device from 4096 cut
+ * cells (CC_WELD_AUTO), always on the host, or always on the device.  No counterpart in the
+ * reference, whose weld is a serial host Dict; every mode gives the same nodes and lines. */
+#define CC_WELD_AUTO 0
+#define CC_WELD_HOST 1
+#define CC_WELD_DEVICE 2
+int cc_set_contour_weld(cc_cache *cache, int mode);
+
 /* allocate_cache(mesh::AdaptiveMesh{2}, MarchingSquaresCutCell()) -- src/marching_squares/cache.jl:100-106:
  * a marching-squares cache laid out on the leaves (one CutCellData record per leaf, in leaf order;
  * the phi column holds each leaf's four corner values in MS_NODE_BITS order), seeded as no body. */
