@@ -4,18 +4,22 @@
 // kernels in this file.  No counterpart in the reference (whose bodies are SDFLibrary closures).
 //
 // Kernels:
-//   k_mlp_forward   -- fused forward of a 64-point tile through every layer, activations kept in LDS
-//                      (ping-pong), weights streamed from L2; optionally saves the pre-activations z
-//                      (FP32) and activations a (BF16) of every layer for the backward pass.
-//   k_gemm          -- generic 64x64-tiled BF16 GEMM, C = op(A) op(B), for the backward pass:
-//                      dA = dZ W^T (with the SiLU' epilogue) and dW = A^T dZ (split over the batch,
-//                      FP32 atomics).
-//   k_out_*, k_colsum, k_adam -- the output layer, bias gradients and the optimiser step.
+//   k_mlp_infer     -- evaluation: a 128-point tile through every layer, activations kept in LDS.
+//   k_train_fwd     -- the training forward of a 64-point minibatch tile, gathered through the
+//                      epoch's permutation; saves, feature-major, what the backward needs and ends
+//                      the pass on the device (loss, dphi, the last layer's dz and column sums).
+//   k_train_bwd     -- one layer of the backward pass in one launch: the weight gradient as
+//                      partial 64x64 tiles over point chunks (no atomics) beside dz of the layer
+//                      below with its bias column sums.
+//   k_adam_all      -- Adam over the whole parameter vector in one launch.
+// A training step is 2 + depth launches, captured once in a graph and replayed; the step index
+// (minibatch offset, learning rate, bias corrections) is read on the device.
 //
 // MFMA 16x16x32 BF16 operand layout (wave64): lane l holds, for A (16 x 32, M x K), row l%16 and the
 // 8 consecutive k = 8*(l/16) .. +7; for B (32 x 16, K x N), column l%16 and the same 8 k; the FP32
 // result (16 x 16) gives lane l column l%16 and rows 4*(l/16) .. +3.
 #include <hip/hip_bf16.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -42,13 +46,6 @@ __device__ __forceinline__ u16 f2bf(float f) {  // round to nearest even
 }
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
-__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
-__device__ __forceinline__ float dsilu(float z) {
-    const float s = sigm(z);
-    return s * (1.0f + z * (1.0f - s));
-}
-
 __device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f32x4 &c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -73,121 +70,20 @@ struct MlpDev {
 constexpr int WIDTH = 256;        // hidden width the kernels are written for
 constexpr int LDW = WIDTH + 8;    // LDS row stride (BF16 elements) of the activation tile
 
-// ---- fused forward ------------------------------------------------------------------------------
-// One workgroup (256 threads, 4 waves) runs TILE points through every layer.  The activation tile
-// lives in one LDS buffer, [TILE][LDW] BF16 (33 KB): each wave accumulates its 64 output columns
-// (4 x 4 MFMA tiles, 64 accumulator registers) from the whole tile, the workgroup synchronises,
-// and the new activations overwrite the old in place.  B fragments come straight from the
-// transposed weights (L2-resident: 230 k parameters = 460 KB in BF16).
-// Points are either explicit (x: n x 3) or, for LAT, the nodes of a lattice: point p is node
-// (p % lat.n[0], p / lat.n[0] % lat.n[1], p / (lat.n[0] lat.n[1])) at lat.x0 + index * lat.h.
-template <bool SAVE, bool LAT>
-__global__ __launch_bounds__(256) void k_mlp_forward(MlpDev m, const float *__restrict__ x, int n,
-                                                    float *__restrict__ phi, u16 *__restrict__ feat,
-                                                    float *__restrict__ zs, u16 *__restrict__ as,
-                                                    cc::Lattice lat) {
-    __shared__ __attribute__((aligned(16))) u16 buf[TILE * LDW];
-    const int W = WIDTH, F2 = 2 * m.F, LDF = F2 + 8;
-    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const int p0 = blockIdx.x * TILE;
-    // Fourier features: [sin(2 pi B x), cos(2 pi B x)] for each of the TILE points
-    for (int e = t; e < TILE * m.F; e += 256) {
-        const int p = e / m.F, f = e - p * m.F;
-        float s = 0.0f, c = 1.0f;
-        if (p0 + p < n) {
-            float xq[3];
-            if (LAT) {
-                const int q = p0 + p, i = q % lat.n[0], r = q / lat.n[0];
-                const int j = r % lat.n[1], k = r / lat.n[1];
-                xq[0] = lat.x0[0] + (float)i * lat.h[0];
-                xq[1] = lat.x0[1] + (float)j * lat.h[1];
-                xq[2] = lat.x0[2] + (float)k * lat.h[2];
-            } else {
-                xq[0] = x[3 * (size_t)(p0 + p)];
-                xq[1] = x[3 * (size_t)(p0 + p) + 1];
-                xq[2] = x[3 * (size_t)(p0 + p) + 2];
-            }
-            const float u0 = (xq[0] - m.in_lo) * m.in_scale, u1 = (xq[1] - m.in_lo) * m.in_scale,
-                        u2 = (xq[2] - m.in_lo) * m.in_scale;
-            const float *b = m.ffB + 3 * f;
-            // sin and cos of 2 pi (B . u): the hardware sine and cosine take revolutions
-            const float rev = (b[0] * u0 + b[1] * u1) + b[2] * u2;
-            s = __builtin_amdgcn_sinf(rev);
-            c = __builtin_amdgcn_cosf(rev);
-        }
-        buf[p * LDF + f] = f2bf(s);
-        buf[p * LDF + m.F + f] = f2bf(c);
-        if (SAVE && p0 + p < n) {
-            feat[(size_t)(p0 + p) * F2 + f] = f2bf(s);
-            feat[(size_t)(p0 + p) * F2 + m.F + f] = f2bf(c);
-        }
-    }
-    __syncthreads();
-    int K = F2, LDin = LDF;
-    for (int l = 0; l < m.depth; ++l) {
-        // this wave's columns [64*wave, 64*wave + 64) of the layer
-        f32x4 acc[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const u16 *Wt = m.Wt[l];
-        for (int k0 = 0; k0 < K; k0 += KS) {
-            const int kk = k0 + 8 * (lane >> 4);
-            bf16x8 a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = load8(buf + (16 * i + (lane & 15)) * LDin + kk);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = load8(Wt + (size_t)(64 * wave + 16 * j + (lane & 15)) * K + kk);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
-        }
-        __syncthreads();  // every wave has read the whole input tile
-        const float *bias = m.bias[l];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int col = 64 * wave + 16 * j + (lane & 15);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * i + 4 * (lane >> 4) + r;
-                    const float z = acc[i][j][r] + bias[col];
-                    const u16 a16 = f2bf(silu(z));
-                    buf[row * LDW + col] = a16;
-                    if (SAVE && p0 + row < n) {
-                        zs[((size_t)l * n + p0 + row) * W + col] = z;
-                        as[((size_t)l * n + p0 + row) * W + col] = a16;
-                    }
-                }
-            }
-        __syncthreads();
-        K = W;
-        LDin = LDW;
-    }
-    // output layer: phi = sum_k a[k] w[k] + b, 4 lanes per point
-    const float bo = *m.bout;
-    for (int p = t >> 2; p < TILE; p += 64) {
-        float s = 0.0f;
-        for (int k = (t & 3); k < W; k += 4) s += bf2f(buf[p * LDW + k]) * bf2f(f2bf(m.wout[k]));
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        if ((t & 3) == 0 && p0 + p < n) phi[p0 + p] = s + bo;
-    }
-}
-
 // ---- inference forward --------------------------------------------------------------------------
-// The same network for evaluation only, on 128-point tiles, with the matrix-core operands swapped:
+// The network for evaluation only, on 128-point tiles, with the matrix-core operands swapped:
 // A = the transposed weights (rows = output features, 8 consecutive k per lane, read from L2 with
 // the next K step's fragments loaded while the current step's MFMAs run), B = the activation tile
 // (columns = points).  Each wave owns 64 output features for all 128 points (4 x 8 MFMA tiles), and
 // its accumulators hold 4 consecutive features of one point per lane, so the epilogue -- bias,
 // SiLU with the hardware reciprocal, one packed BF16 conversion -- ends in one 8-byte LDS store per
 // four values.  The activation tile is [128][LDW] BF16 (67.6 KB), updated in place between layers.
+// Points are either explicit (x: n x 3) or, for LAT, the nodes of a lattice: point p is node
+// (p % lat.n[0], p / lat.n[0] % lat.n[1], p / (lat.n[0] lat.n[1])) at lat.x0 + index * lat.h.
 constexpr int ITILE = 128;
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
@@ -283,183 +179,546 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(MlpDev m, const float *__r
 
 constexpr size_t INFER_LDS = (size_t)ITILE * LDW * sizeof(u16);
 
-// ---- generic GEMM for the backward pass ---------------------------------------------------------
-// C[M x N] = op(A)[M x K] op(B)[K x N]; op(A) = A (A stored M x K, row stride lda) or A^T (A stored
-// K x M); op(B) = B (stored K x N) or B^T (stored N x K).  M, N multiples of 64, K of 32.
-// EPI 0: C32 (+)= result (atomic when the K range is split over gridDim.z)
-// EPI 1: C16 = bf16(result * silu'(aux))      (backward through a SiLU layer)
-template <bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256) void k_gemm(const u16 *__restrict__ A, int lda, const u16 *__restrict__ B,
-                                              int ldb, float *__restrict__ C32, u16 *__restrict__ C16,
-                                              int ldc, int M, int N, int K, const float *__restrict__ aux,
-                                              int ldaux) {
-    __shared__ __attribute__((aligned(16))) u16 As[TILE][KS + 8];  // [m][k]
-    __shared__ __attribute__((aligned(16))) u16 Bs[TILE][KS + 8];  // [n][k]
-    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
-    const int kchunk = K / gridDim.z;
-    const int kb = blockIdx.z * kchunk, ke = kb + kchunk;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    f32x4 acc[2][2];
+// ---- training -----------------------------------------------------------------------------------
+// Minibatch order: epoch e visits the samples in the order of a keyed pseudo-random permutation of
+// [0, N) (N <= 2^30), a six-round balanced Feistel network on 2 hb bits (2^(2 hb) >= N) with cycle
+// walking.  The walk ends: it follows x's cycle of a bijection, on which x itself is below N.  The
+// round function is the "lowbias32" integer hash (C. Wellons, hash-prospector, public domain).
+__device__ __forceinline__ uint32_t hash32(uint32_t v) {
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t shuffle_index(uint32_t x, uint32_t N, int hb, const uint32_t *__restrict__ key) {
+    const uint32_t mask = (1u << hb) - 1u;
+    do {
+        uint32_t L = x >> hb, R = x & mask;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kb; k0 < ke; k0 += KS) {
-        // stage a 64 x 32 tile of op(A) and of op(B)^T: 2048 elements each, 8 per thread
-        {
-            const int r = t >> 2, c8 = (t & 3) * 8;  // non-transposed: row r, k c8..c8+7
-            if (!TA) {
-                *reinterpret_cast<bf16x8 *>(&As[r][c8]) = load8(A + (size_t)(m0 + r) * lda + k0 + c8);
-            } else {  // A stored K x M: 8 consecutive m for one k
-                const int kk = t >> 3, mm = (t & 7) * 8;
-                const bf16x8 v = load8(A + (size_t)(k0 + kk) * lda + m0 + mm);
-                const u16 *vs = reinterpret_cast<const u16 *>(&v);
-#pragma unroll
-                for (int q = 0; q < 8; ++q) As[mm + q][kk] = vs[q];
-            }
-            if (TB) {  // B stored N x K
-                *reinterpret_cast<bf16x8 *>(&Bs[r][c8]) = load8(B + (size_t)(n0 + r) * ldb + k0 + c8);
-            } else {  // B stored K x N: 8 consecutive n for one k
-                const int kk = t >> 3, nn = (t & 7) * 8;
-                const bf16x8 v = load8(B + (size_t)(k0 + kk) * ldb + n0 + nn);
-                const u16 *vs = reinterpret_cast<const u16 *>(&v);
-#pragma unroll
-                for (int q = 0; q < 8; ++q) Bs[nn + q][kk] = vs[q];
-            }
+        for (int r = 0; r < 6; ++r) {
+            const uint32_t f = hash32(R ^ key[r]) & mask;
+            const uint32_t nl = R;
+            R = L ^ f;
+            L = nl;
         }
-        __syncthreads();
-        const int kk = 8 * (lane >> 4);
-        bf16x8 a[2], b[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = load8(&As[wm + 16 * i + (lane & 15)][kk]);
-#pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = load8(&Bs[wn + 16 * j + (lane & 15)][kk]);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
-        __syncthreads();
+        x = (L << hb) | R;
+    } while (x >= N);
+    return x;
+}
+
+// Saved state of a minibatch of n points (n a multiple of 64), laid out for the matrix cores: the
+// weight gradient dW_l = a_{l-1}^T dz_l reduces over points, so its operands are kept feature-major
+// ([feature][point], 8 consecutive points per 16-byte fragment load); dz_{l-1} = dz_l W_l^T reduces
+// over features, so it reads dz point-major and W_l row-major.
+struct TrainBufs {
+    u16 *featT;           // [2F][n]        Fourier features (layer 0's input)
+    u16 *aT;              // [depth-1][W][n] hidden activations a_l, l < depth - 1
+    u16 *ds;              // [depth-1][W][n] silu'(z_l), l < depth - 1
+    u16 *dz, *dzT;        // dz of the last layer: [n][W] and [W][n]
+    float *gpart;         // [rows][n / 64]: every workgroup's sums, one row per bias, w_out and b_out
+                          // entry and one for the loss (no atomics: the optimiser sums the rows)
+    int *ctr;             // [0]: step of the next forward, [1]: step of the next Adam
+    int rblast, rwout, rbout, rloss;  // rows of the last layer's bias, w_out, b_out, the loss
+};
+
+__device__ __forceinline__ void store4(u16 *p, const u16 v[4]) {  // 8-byte aligned
+    uint2 w;
+    w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+    w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+    *reinterpret_cast<uint2 *>(p) = w;
+}
+
+// Two floats to packed BF16 (round to nearest even) with the hardware conversion; the first in the
+// low half.
+__device__ __forceinline__ uint32_t pk2(float a, float b) {
+    const bf16x2 h = __builtin_convertvector(f32x2{a, b}, bf16x2);
+    return __builtin_bit_cast(uint32_t, h);
+}
+__device__ __forceinline__ float lo_bf(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float hi_bf(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+
+// One workgroup (512 threads, 8 waves) runs a 64-point tile through every layer.  The activation
+// tile lives in one LDS buffer, [TILE][LDW] BF16, updated in place between layers; each wave owns
+// 32 output features (4 x 2 MFMA tiles: lane = one feature, 4 consecutive points), with the
+// transposed weights' fragments for the next K step loaded while the current step's MFMAs run.
+// Eight waves on 64 points keep two waves on every SIMD, so one wave's SiLU epilogue runs beside
+// another's matrix products.
+// Points: X/Y rows shuffle_index(b n + p) under epoch e's keys for step s = ctr[0] = e nb + b when
+// keys ([epoch][6]) are given, else rows p.
+constexpr int FWD_T = 512;
+__global__ __launch_bounds__(FWD_T) void k_train_fwd(MlpDev m, TrainBufs tb, const float *__restrict__ X,
+                                                     const float *__restrict__ Y, const uint32_t *__restrict__ keys,
+                                                     int hb, uint32_t N, int nb, int n) {
+    __shared__ __attribute__((aligned(16))) u16 buf[TILE * LDW];
+    __shared__ float pu[TILE][4];  // unit-box coordinates and target of each point
+    __shared__ float wo[WIDTH];    // BF16-rounded output weights
+    __shared__ float sdp[TILE];    // dphi of each point
+    __shared__ float red[2][FWD_T / 64];
+    const int W = WIDTH, F = m.F, F2 = 2 * F, LDF = F2 + 8;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15;
+    const int koff = 8 * (lane >> 4), g4 = 4 * (lane >> 4);
+    const int p0 = blockIdx.x * TILE;
+    if (t < TILE) {
+        size_t q = (size_t)p0 + t;
+        if (keys) {
+            const int s = tb.ctr[0];
+            const int e = s / nb, b = s - e * nb;
+            q = shuffle_index((uint32_t)((size_t)b * n + q), N, hb, keys + 6 * e);
+            if (blockIdx.x == 0 && t == 0) tb.ctr[1] = s;
+        }
+        pu[t][0] = (X[3 * q] - m.in_lo) * m.in_scale;
+        pu[t][1] = (X[3 * q + 1] - m.in_lo) * m.in_scale;
+        pu[t][2] = (X[3 * q + 2] - m.in_lo) * m.in_scale;
+        pu[t][3] = Y[q];
     }
+    if (t < WIDTH) wo[t] = bf2f(f2bf(m.wout[t]));
+    // this wave's weight fragments of a whole layer (32 features x K <= 256), loaded in one burst:
+    // layer 0's while the features are computed, layer l + 1's while layer l's epilogue runs
+    bf16x8 b[8][2];
+    auto load_layer = [&](int l, int K) {
+        const u16 *wrow = m.Wt[l] + (size_t)(32 * wave + lr) * K + koff;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+        for (int ks = 0; ks < 8; ++ks)
+            if (ks * KS < K) {
+                b[ks][0] = load8(wrow + ks * KS);
+                b[ks][1] = load8(wrow + 16 * (size_t)K + ks * KS);
+            }
+    };
+    load_layer(0, F2);
+    __syncthreads();
+    // Fourier features [sin(2 pi B u), cos(2 pi B u)]: the hardware sine and cosine take revolutions
+    for (int e = t; e < TILE * F; e += FWD_T) {
+        const int p = e & (TILE - 1), f = e >> 6;
+        const float *b = m.ffB + 3 * f;
+        const float rev = (b[0] * pu[p][0] + b[1] * pu[p][1]) + b[2] * pu[p][2];
+        const uint32_t sc = pk2(__builtin_amdgcn_sinf(rev), __builtin_amdgcn_cosf(rev));
+        const u16 s = (u16)(sc & 0xffffu), c = (u16)(sc >> 16);
+        buf[p * LDF + f] = s;
+        buf[p * LDF + F + f] = c;
+        tb.featT[(size_t)f * n + p0 + p] = s;
+        tb.featT[(size_t)(F + f) * n + p0 + p] = c;
+    }
+    __syncthreads();
+    int K = F2, LDin = LDF;
+    f32x4 acc[4][2];
+    for (int l = 0; l < m.depth; ++l) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks)
+            if (ks * KS < K) {
+                bf16x8 a[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = load8(buf + (16 * i + lr) * LDin + ks * KS + koff);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[ks][j], acc[i][j]);
+            }
+        if (l + 1 < m.depth) load_layer(l + 1, W);
+        __syncthreads();  // every wave has read the whole input tile
+        const bool last = l == m.depth - 1;
+        u16 *aTl = tb.aT + (size_t)l * W * n, *dsl = tb.ds + (size_t)l * W * n;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const int col = n0 + wn + 16 * j + (lane & 15);
+            const int col = 32 * wave + 16 * j + lr;
+            const float bj = m.bias[l][col];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
-                const float v = acc[i][j][r];
-                if (EPI == 0) {
-                    if (gridDim.z > 1) atomicAdd(&C32[(size_t)row * ldc + col], v);
-                    else C32[(size_t)row * ldc + col] = v;
-                } else {
-                    C16[(size_t)row * ldc + col] = f2bf(v * dsilu(aux[(size_t)row * ldaux + col]));
+            for (int i = 0; i < 4; ++i) {
+                float av[4], dv[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float z = acc[i][j][r] + bj;
+                    const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+                    av[r] = z * sg;
+                    dv[r] = sg * __builtin_fmaf(z, 1.0f - sg, 1.0f);  // silu'(z)
+                    acc[i][j][r] = dv[r];
+                }
+                const uint2 ap = {pk2(av[0], av[1]), pk2(av[2], av[3])};
+                u16 *bp = buf + (16 * i + g4) * LDW + col;
+                bp[0] = (u16)(ap.x & 0xffffu);
+                bp[LDW] = (u16)(ap.x >> 16);
+                bp[2 * LDW] = (u16)(ap.y & 0xffffu);
+                bp[3 * LDW] = (u16)(ap.y >> 16);
+                if (!last) {
+                    const size_t o = (size_t)col * n + p0 + 16 * i + g4;
+                    *reinterpret_cast<uint2 *>(aTl + o) = ap;
+                    *reinterpret_cast<uint2 *>(dsl + o) = uint2{pk2(dv[0], dv[1]), pk2(dv[2], dv[3])};
                 }
             }
         }
-}
-
-// Output layer backward and loss, in three coalesced passes:
-//   k_out_dphi: dphi = 2 (phi - y) / n per point; loss += sum (phi - y)^2 / n and db_out += sum dphi,
-//               one atomic per workgroup;
-//   k_out_dz:   dZ_last = bf16((dphi w_out) silu'(z_last)) elementwise;
-//   k_colsum (weighted by dphi): dw_out = sum_p dphi a_last.
-__global__ __launch_bounds__(256) void k_out_dphi(const float *__restrict__ phi, const float *__restrict__ y,
-                                                  int n, float *__restrict__ dphi, float *__restrict__ g_bout,
-                                                  float *__restrict__ loss) {
-    __shared__ float sl[4], sd[4];
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    float r2 = 0.0f, dp = 0.0f;
-    if (p < n) {
-        const float r = phi[p] - y[p];
-        dp = 2.0f * r / (float)n;
-        dphi[p] = dp;
-        r2 = r * r / (float)n;
+        __syncthreads();
+        K = W;
+        LDin = LDW;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        r2 += __shfl_xor(r2, o);
-        dp += __shfl_xor(dp, o);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        sl[threadIdx.x >> 6] = r2;
-        sd[threadIdx.x >> 6] = dp;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicAdd(loss, (sl[0] + sl[1]) + (sl[2] + sl[3]));
-        atomicAdd(g_bout, (sd[0] + sd[1]) + (sd[2] + sd[3]));
-    }
-}
-
-__global__ __launch_bounds__(256) void k_out_dz(const float *__restrict__ dphi, int n, int W,
-                                                const float *__restrict__ wout, const float *__restrict__ zlast,
-                                                u16 *__restrict__ dz) {
-    const size_t o = blockIdx.x * (size_t)256 + threadIdx.x;
-    if (o >= (size_t)n * W) return;
-    const int p = (int)(o / W), k = (int)(o - (size_t)p * W);
-    dz[o] = f2bf(dphi[p] * bf2f(f2bf(wout[k])) * dsilu(zlast[o]));
-}
-
-// db[col] += sum over rows of w[row] * dz[row][col] (w == nullptr: weight 1), W == WIDTH.  A
-// workgroup takes COLSUM_ROWS rows: each thread two adjacent columns (one 32-bit load) and every
-// fourth row, eight rows in flight; the four row phases meet in LDS; one atomic per column.
-constexpr int COLSUM_ROWS = 64;
-__global__ __launch_bounds__(512) void k_colsum(const u16 *__restrict__ dz, const float *__restrict__ w, int n,
-                                                float *__restrict__ db) {
-    __shared__ float part[4][WIDTH];
-    const int t = threadIdx.x, c2 = (t & 127) * 2, ph = t >> 7;
-    const int r0 = blockIdx.x * COLSUM_ROWS;
-    float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll 8
-    for (int q = 0; q < COLSUM_ROWS / 4; ++q) {
-        const int r = r0 + ph + 4 * q;
-        if (r < n) {
-            const uint32_t v = *reinterpret_cast<const uint32_t *>(dz + (size_t)r * WIDTH + c2);
-            const float wr = w ? w[r] : 1.0f;
-            s0 += wr * bf2f((u16)(v & 0xffffu));
-            s1 += wr * bf2f((u16)(v >> 16));
+    // output layer, 8 lanes per point: phi, the residual, dphi = 2 (phi - y) / n
+    {
+        const int p = t >> 3;
+        float s = 0.0f;
+        for (int k = t & 7; k < W; k += 8) s += bf2f(buf[p * LDW + k]) * wo[k];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        const float res = (s + *m.bout) - pu[p][3];
+        const float dp = 2.0f * res / (float)n;
+        float r2 = 0.0f, dq = 0.0f;
+        if ((t & 7) == 0) {
+            sdp[p] = dp;
+            r2 = res * res / (float)n;
+            dq = dp;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            r2 += __shfl_xor(r2, o);
+            dq += __shfl_xor(dq, o);
+        }
+        if (lane == 0) {
+            red[0][wave] = r2;
+            red[1][wave] = dq;
         }
     }
-    part[ph][c2] = s0;
-    part[ph][c2 + 1] = s1;
     __syncthreads();
-    if (t < WIDTH) atomicAdd(&db[t], (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]));
-}
-
-// Adam on the FP32 master parameters; refreshes the BF16 transposed copy of a weight matrix.
-__global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float *__restrict__ g,
-                                              float *__restrict__ m1, float *__restrict__ m2, int count,
-                                              float lr, float b1, float b2, float eps, float c1, float c2,
-                                              u16 *__restrict__ wt, int K, int N) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= count) return;
-    const float gi = g[i];
-    const float a = b1 * m1[i] + (1.0f - b1) * gi;
-    const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
-    m1[i] = a;
-    m2[i] = b;
-    const float v = p[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
-    p[i] = v;
-    if (wt) {  // p is W[K][N] row-major; wt is [N][K]
-        const int k = i / N, nn = i - k * N;
-        wt[(size_t)nn * K + k] = f2bf(v);
+    const size_t nblk = gridDim.x;
+    if (t == 0) {
+        float a = 0.0f, b = 0.0f;
+        for (int w = 0; w < FWD_T / 64; ++w) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        tb.gpart[tb.rloss * nblk + blockIdx.x] = a;
+        tb.gpart[tb.rbout * nblk + blockIdx.x] = b;
+    }
+    // dz of the last layer = bf16(dphi w_out silu'(z)); column sums for its bias and for w_out.
+    // Each (point, feature) of the tile belongs to one lane, which replaces a by dz in place.
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int col = 32 * wave + 16 * j + lr;
+        const float wj = wo[col];
+        float sb = 0.0f, sw = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float dv[4];
+            u16 *bp = buf + (16 * i + g4) * LDW + col;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float dp = sdp[16 * i + g4 + r];
+                dv[r] = dp * wj * acc[i][j][r];
+                sw += bf2f(bp[r * LDW]) * dp;
+            }
+            const uint2 dk = {pk2(dv[0], dv[1]), pk2(dv[2], dv[3])};
+            sb += (lo_bf(dk.x) + hi_bf(dk.x)) + (lo_bf(dk.y) + hi_bf(dk.y));
+            bp[0] = (u16)(dk.x & 0xffffu);
+            bp[LDW] = (u16)(dk.x >> 16);
+            bp[2 * LDW] = (u16)(dk.y & 0xffffu);
+            bp[3 * LDW] = (u16)(dk.y >> 16);
+            *reinterpret_cast<uint2 *>(tb.dzT + (size_t)col * n + p0 + 16 * i + g4) = dk;
+        }
+        sb += __shfl_xor(sb, 16);
+        sb += __shfl_xor(sb, 32);
+        sw += __shfl_xor(sw, 16);
+        sw += __shfl_xor(sw, 32);
+        if (lane < 16) {
+            tb.gpart[(tb.rblast + col) * nblk + blockIdx.x] = sb;
+            tb.gpart[(tb.rwout + col) * nblk + blockIdx.x] = sw;
+        }
+    }
+    __syncthreads();
+    for (int c = t; c < TILE * W / 8; c += FWD_T) {  // the point-major copy, 16 bytes a lane
+        const int row = c >> 5, c8 = (c & 31) * 8;
+        *reinterpret_cast<bf16x8 *>(tb.dz + (size_t)(p0 + row) * W + c8) = load8(buf + row * LDW + c8);
     }
 }
 
-__global__ void k_to_bf16_t(const float *__restrict__ w, u16 *__restrict__ wt, int K, int N) {
+// One layer l of the backward pass.  Workgroups [0, nA): the weight gradient, one 64x64 tile of
+// dW_l (features i of the layer's input x features j of its output) over one of S equal point
+// chunks; the four waves take interleaved K steps of the chunk and meet in LDS, and the tile is
+// written to its chunk's slice of part (the optimiser sums the slices in a fixed order).
+// Workgroups [nA, nA + n / 64) when l > 0: dz_{l-1} = bf16((dz_l W_l^T) silu'(z_{l-1})) for 64
+// points (each wave 64 features), written in both layouts, with the bias column sums of layer l-1.
+struct BwdLayer {
+    const u16 *ainT;   // [K][n] the layer's input, feature-major
+    const u16 *dz;     // [n][W] dz_l
+    const u16 *dzT;    // [W][n] dz_l
+    const u16 *w16;    // [K][W] W_l row-major (l > 0)
+    const u16 *ds;     // [W][n] silu'(z_{l-1}) (l > 0)
+    u16 *dzp, *dzpT;   // dz_{l-1} (l > 0)
+    float *part;       // [S][K][W]
+    float *gb;         // rows of layer l-1's bias in the workgroup sums (l > 0), n / 64 per row
+    int *ctr;          // non-null once per step: advances the forward's step counter
+    int n, K, S, nA;
+};
+
+// acc[i][j] += A_i(k) B_j(k) over the K steps k = k, k + ks, ... < kend, where A_i(k) = load8(Ar +
+// i ast + k) and B_j(k) = load8(Br + j bst + k) are MFMA fragments; the fragments are loaded two
+// steps ahead (the loads past the last step reload it).
+template <int NI, int NJ>
+__device__ __forceinline__ void mma_steps(f32x4 (&acc)[NI][NJ], const u16 *Ar, size_t ast, const u16 *Br, size_t bst,
+                                          int k, int kend, int ks) {
+    if (k >= kend) return;
+    const int klast = k + ((kend - 1 - k) / ks) * ks;
+    bf16x8 x0[NI], y0[NJ], x1[NI], y1[NJ];
+    auto ld = [&](bf16x8(&x)[NI], bf16x8(&y)[NJ], int kk) {
+        kk = kk < klast ? kk : klast;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) x[i] = load8(Ar + i * ast + kk);
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) y[j] = load8(Br + j * bst + kk);
+    };
+    auto mm = [&](const bf16x8(&x)[NI], const bf16x8(&y)[NJ]) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) acc[i][j] = mfma(x[i], y[j], acc[i][j]);
+    };
+    ld(x0, y0, k);
+    ld(x1, y1, k + ks);
+    for (;;) {
+        mm(x0, y0);
+        ld(x0, y0, k + 2 * ks);
+        if (k + ks > klast) break;
+        mm(x1, y1);
+        ld(x1, y1, k + 3 * ks);
+        k += 2 * ks;
+        if (k > klast) break;
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void k_train_bwd(BwdLayer a) {
+    // two partial weight-gradient tiles (32 KB), or the point-major dz tile ([TILE][LDW] BF16)
+    __shared__ __attribute__((aligned(16))) float smem[std::max(2 * TILE * TILE, TILE * LDW / 2)];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15;
+    const int koff = 8 * (lane >> 4), g4 = 4 * (lane >> 4);
+    if (a.ctr && blockIdx.x == 0 && t == 0) a.ctr[0] += 1;  // no forward reads it meanwhile
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if ((int)blockIdx.x < a.nA) {
+        constexpr int NT = WIDTH / TILE;
+        const int ntile = (a.K / TILE) * NT;
+        int tile, s;
+        if (a.S % 8 == 0) {
+            // workgroups go to the 8 XCDs round-robin: every tile of one chunk on one XCD, whose L2
+            // then serves the chunk's activation and dz slices to all of its tiles
+            const int x = blockIdx.x & 7, k = blockIdx.x >> 3;
+            tile = k % ntile;
+            s = 8 * (k / ntile) + x;
+        } else {
+            tile = blockIdx.x % ntile;
+            s = blockIdx.x / ntile;
+        }
+        const int i0 = (tile / NT) * TILE, j0 = (tile % NT) * TILE;
+        const int chunk = a.n / a.S, kb = s * chunk;
+        const size_t st = (size_t)16 * a.n;
+        mma_steps<4, 4>(acc, a.ainT + (size_t)(i0 + lr) * a.n + koff, st, a.dzT + (size_t)(j0 + lr) * a.n + koff, st,
+                        kb + KS * wave, kb + chunk, 4 * KS);
+        // the four partial tiles meet in LDS in two rounds, (w0 + w2) + (w1 + w3)
+        auto put = [&](float *slot) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) slot[(16 * i + g4 + r) * TILE + 16 * j + lr] = acc[i][j][r];
+        };
+        auto add = [&](const float *slot) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[i][j][r] += slot[(16 * i + g4 + r) * TILE + 16 * j + lr];
+        };
+        if (wave >= 2) put(smem + (wave - 2) * TILE * TILE);
+        __syncthreads();
+        if (wave < 2) add(smem + wave * TILE * TILE);
+        __syncthreads();
+        if (wave == 1) put(smem);
+        __syncthreads();
+        if (wave == 0) {
+            add(smem);
+            float *out = a.part + (size_t)s * a.K * WIDTH + (size_t)i0 * WIDTH + j0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) out[(size_t)(16 * i + g4 + r) * WIDTH + 16 * j + lr] = acc[i][j][r];
+        }
+        return;
+    }
+    const int p0 = (blockIdx.x - a.nA) * TILE;
+    const size_t st = (size_t)16 * WIDTH;
+    uint2 dsv[4][4];  // silu'(z_{l-1}) of this lane's outputs, loaded before the products
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            dsv[j][i] = *reinterpret_cast<const uint2 *>(a.ds + (size_t)(64 * wave + 16 * j + lr) * a.n + p0 + 16 * i + g4);
+    // rows: the block's points; columns: this wave's 64 features of layer l-1
+    mma_steps<4, 4>(acc, a.dz + (size_t)(p0 + lr) * WIDTH + koff, st, a.w16 + (size_t)(64 * wave + lr) * WIDTH + koff,
+                    st, 0, WIDTH, KS);
+    u16 *stage = reinterpret_cast<u16 *>(smem);  // [TILE][LDW] BF16, the point-major copy
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = 64 * wave + 16 * j + lr;
+        float sb = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const size_t o = (size_t)col * a.n + p0 + 16 * i + g4;
+            const uint2 d = dsv[j][i];
+            const uint2 v = {pk2(acc[i][j][0] * lo_bf(d.x), acc[i][j][1] * hi_bf(d.x)),
+                             pk2(acc[i][j][2] * lo_bf(d.y), acc[i][j][3] * hi_bf(d.y))};
+            sb += (lo_bf(v.x) + hi_bf(v.x)) + (lo_bf(v.y) + hi_bf(v.y));
+            u16 *bp = stage + (16 * i + g4) * LDW + col;
+            bp[0] = (u16)(v.x & 0xffffu);
+            bp[LDW] = (u16)(v.x >> 16);
+            bp[2 * LDW] = (u16)(v.y & 0xffffu);
+            bp[3 * LDW] = (u16)(v.y >> 16);
+            *reinterpret_cast<uint2 *>(a.dzpT + o) = v;
+        }
+        sb += __shfl_xor(sb, 16);
+        sb += __shfl_xor(sb, 32);
+        if (lane < 16) a.gb[(size_t)col * (a.n / TILE) + blockIdx.x - a.nA] = sb;
+    }
+    __syncthreads();
+    for (int c = t; c < TILE * WIDTH / 8; c += 256) {
+        const int row = c >> 5, c8 = (c & 31) * 8;
+        *reinterpret_cast<bf16x8 *>(a.dzp + (size_t)(p0 + row) * WIDTH + c8) = load8(stage + row * LDW + c8);
+    }
+}
+
+// Adam over the whole FP32 parameter vector.  Workgroups [0, nW / 256): one thread per weight, whose
+// gradient is the sum of the S partial slices of its layer, in slice order.  The workgroups after
+// them: one wave per other parameter (biases, w_out, b_out) and one for the loss, each summing its
+// row of workgroup sums in a fixed order.  A weight's new value refreshes both BF16 copies: wt
+// ([W][K], the forward's operand) and w16 ([K][W], the backward's).  update == 0 only writes the
+// gradient into g (the gradient entry point).  The loss is added to *loss.
+struct AdamArgs {
+    float *p, *g, *m1, *m2;
+    const float *part;   // layer l's slices start at part + 32 off_w[l]
+    const float *gpart;  // [nq][nblk] workgroup sums of the other parameters, then the loss
+    float *loss;
+    const float *sched;  // [step][3]: learning rate, 1 - beta1^t, 1 - beta2^t
+    const int *ctr;      // ctr[1]: this step's index into sched
+    size_t nparam, off_w[9];
+    int K[8], S[8];
+    u16 *wt[8], *w16[8];
+    int depth, update, nblk, nq;
+};
+
+__global__ __launch_bounds__(256) void k_adam_all(AdamArgs a) {
+    const size_t nW = a.off_w[a.depth];
+    float gi;
+    int l = -1;
+    size_t i, loc = 0;
+    if (blockIdx.x < nW / 256) {
+        i = blockIdx.x * (size_t)256 + threadIdx.x;
+        l = 0;
+        while (l + 1 < a.depth && i >= a.off_w[l + 1]) ++l;
+        loc = i - a.off_w[l];
+        const size_t sz = (size_t)a.K[l] * WIDTH;
+        const float *q = a.part + 32 * a.off_w[l] + loc;
+        gi = 0.0f;
+        int s = 0;
+        for (; s + 8 <= a.S[l]; s += 8) {  // eight loads in flight, summed in slice order
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = q[(s + u) * sz];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) gi += v[u];
+        }
+        for (; s < a.S[l]; ++s) gi += q[s * sz];
+    } else {
+        const int q = (int)(blockIdx.x - nW / 256) * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+        if (q >= a.nq) return;
+        const float *row = a.gpart + (size_t)q * a.nblk;
+        float sum = 0.0f;
+        for (int b = lane; b < a.nblk; b += 64) sum += row[b];
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if (lane != 0) return;
+        i = nW + q;
+        if (i >= a.nparam) {  // the loss row
+            *a.loss += sum;
+            return;
+        }
+        gi = sum;
+    }
+    if (!a.update) {
+        a.g[i] = gi;
+        return;
+    }
+    const int st = a.ctr[1];
+    const float lr = a.sched[3 * st], c1 = a.sched[3 * st + 1], c2 = a.sched[3 * st + 2];
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float mo = b1 * a.m1[i] + (1.0f - b1) * gi;
+    const float ve = b2 * a.m2[i] + (1.0f - b2) * gi * gi;
+    a.m1[i] = mo;
+    a.m2[i] = ve;
+    const float v = a.p[i] - lr * (mo / c1) / (sqrtf(ve / c2) + eps);
+    a.p[i] = v;
+    if (l >= 0) {
+        const int ii = (int)(loc / WIDTH), jj = (int)(loc - (size_t)ii * WIDTH);
+        const u16 h = f2bf(v);
+        a.w16[l][loc] = h;
+        a.wt[l][(size_t)jj * a.K[l] + ii] = h;
+    }
+}
+
+// Both BF16 copies of a weight matrix W[K][N] (row-major FP32): wt = W^T, w16 = W.
+__global__ void k_to_bf16_t(const float *__restrict__ w, u16 *__restrict__ wt, u16 *__restrict__ w16, int K, int N) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= K * N) return;
     const int k = i / N, nn = i - k * N;
-    wt[(size_t)nn * K + k] = f2bf(w[i]);
+    const u16 h = f2bf(w[i]);
+    wt[(size_t)nn * K + k] = h;
+    w16[i] = h;
 }
 
-// Exact samples for training: phi of the body at given points (FP64 evaluation, stored FP32).
-__global__ void k_exact_points(cc::Geo geo, const float *__restrict__ x, int n, float *__restrict__ y) {
+// Training samples, drawn on the device: candidate c has coordinates lo + (hi - lo) u_d, with u_d
+// the top 24 bits of splitmix64(key + (3 c + d) golden) (the SplitMix64 generator of Steele, Lea and
+// Flood, 2014), and the body's phi at it (FP64 evaluation, stored FP32).  ok (when given) flags the
+// candidates within band of the surface.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_candidates(cc::Geo geo, uint64_t key, uint64_t c0, int n, float lo, float hi, float band,
+                             float *__restrict__ x, float *__restrict__ y, uint8_t *__restrict__ ok) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const double p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
-    y[i] = (float)cc::sdf_value<double, 3>(geo, p);
+    const uint64_t c = c0 + (uint64_t)i;
+    float u[3];
+    double p[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const uint64_t h = splitmix64(key + (3 * c + d) * 0x9e3779b97f4a7c15ull);
+        u[d] = lo + (hi - lo) * ((float)(h >> 40) * 0x1p-24f);
+        p[d] = u[d];
+        x[3 * (size_t)i + d] = u[d];
+    }
+    const float v = (float)cc::sdf_value<double, 3>(geo, p);
+    y[i] = v;
+    if (ok) ok[i] = fabsf(v) < band;
+}
+
+// Appends the selected candidates sel[0..n) (in candidate order) to the samples.
+__global__ void k_place(const float *__restrict__ cx, const float *__restrict__ cy, const int *__restrict__ sel, int n,
+                        float *__restrict__ X, float *__restrict__ Y) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int c = sel[i];
+    X[3 * (size_t)i] = cx[3 * (size_t)c];
+    X[3 * (size_t)i + 1] = cx[3 * (size_t)c + 1];
+    X[3 * (size_t)i + 2] = cx[3 * (size_t)c + 2];
+    Y[i] = cy[c];
 }
 
 }  // namespace
@@ -478,13 +737,17 @@ struct cc_mlp {
     float *params = nullptr;  // device, nparam
     float *grads = nullptr;   // device, nparam
     float *m1 = nullptr, *m2 = nullptr;
-    u16 *wt[8] = {};          // device BF16 transposed weights per layer
+    u16 *wt[8] = {};          // device BF16 weights per layer, transposed ([N][K]: the forward's operand)
+    u16 *w16[8] = {};         // device BF16 weights per layer, row-major ([K][N]: the backward's operand)
     int step = 0;
     std::vector<float> ffB_host;
-    // training scratch, sized for the batch
+    // training state, sized for the batch
     int cap = 0;
-    float *x = nullptr, *y = nullptr, *phi = nullptr, *zs = nullptr, *loss = nullptr, *dphi = nullptr;
-    u16 *feat = nullptr, *as = nullptr, *dz0 = nullptr, *dz1 = nullptr;
+    float *x = nullptr, *y = nullptr;  // the gradient entry point's samples
+    u16 *featT = nullptr, *aT = nullptr, *ds = nullptr, *dz[2] = {}, *dzT[2] = {};
+    float *part = nullptr;   // weight-gradient partials: up to 32 slices of every weight matrix
+    float *gpart = nullptr;  // workgroup sums of the other gradients and the loss: [rows][batch / 64]
+    float *small = nullptr;  // [0]: loss accumulator; ints at [16], [17]: step counters
     int ecap = 0;  // inference scratch
     float *ex = nullptr, *ephi = nullptr;
     hipStream_t stream = nullptr;
@@ -519,99 +782,121 @@ template <typename P> void release(P *&p) {
     p = nullptr;
 }
 
+int *counters(cc_mlp *m) { return reinterpret_cast<int *>(m->small + 16); }
+
+// Rows of the workgroup sums: one per parameter after the weights, then the loss.
+int sum_rows(const cc_mlp *m) { return (int)(m->nparam - m->off_b[0]) + 1; }
+
 int ensure_cap(cc_mlp *m, int n) {
     if (n <= m->cap) return CC_OK;
-    release(m->x); release(m->y); release(m->phi); release(m->zs); release(m->feat); release(m->as);
-    release(m->dz0); release(m->dz1); release(m->loss); release(m->dphi);
+    release(m->x); release(m->y); release(m->featT); release(m->aT); release(m->ds);
+    release(m->dz[0]); release(m->dz[1]); release(m->dzT[0]); release(m->dzT[1]);
+    release(m->part); release(m->gpart); release(m->small);
     m->cap = 0;
-    const size_t W = m->width, D = m->depth, F2 = 2 * m->F;
-    if (hipMalloc(&m->x, 3 * sizeof(float) * n) || hipMalloc(&m->y, sizeof(float) * n) ||
-        hipMalloc(&m->phi, sizeof(float) * n) || hipMalloc(&m->zs, sizeof(float) * D * n * W) ||
-        hipMalloc(&m->feat, sizeof(u16) * n * F2) || hipMalloc(&m->as, sizeof(u16) * D * n * W) ||
-        hipMalloc(&m->dz0, sizeof(u16) * n * W) || hipMalloc(&m->dz1, sizeof(u16) * n * W) ||
-        hipMalloc(&m->loss, 64) || hipMalloc(&m->dphi, sizeof(float) * n))
-        return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate training buffers");
+    const size_t W = m->width, H = std::max(m->depth - 1, 1), F2 = 2 * m->F, N = n;
+    bool ok = hipMalloc(&m->x, 12 * N) == hipSuccess && hipMalloc(&m->y, 4 * N) == hipSuccess &&
+              hipMalloc(&m->featT, 2 * F2 * N) == hipSuccess && hipMalloc(&m->aT, 2 * H * W * N) == hipSuccess &&
+              hipMalloc(&m->ds, 2 * H * W * N) == hipSuccess &&
+              hipMalloc(&m->part, 4 * 32 * m->off_b[0]) == hipSuccess &&
+              hipMalloc(&m->gpart, 4 * sum_rows(m) * (N / TILE)) == hipSuccess && hipMalloc(&m->small, 256) == hipSuccess;
+    for (int c = 0; ok && c < 2; ++c)
+        ok = hipMalloc(&m->dz[c], 2 * N * W) == hipSuccess && hipMalloc(&m->dzT[c], 2 * N * W) == hipSuccess;
+    if (!ok) return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate training buffers");
     m->cap = n;
     return CC_OK;
 }
 
-// Forward + backward over n points already in m->x / m->y (n a multiple of 64), enqueued on the
-// model's stream: gradients into m->grads (zeroed here), the mean squared error added to *m->loss.
-void enqueue_loss_grad(cc_mlp *m, int n) {
+// Point chunks of a layer's weight gradient (K input features): about 256 workgroups, each chunk
+// a whole number of K steps.
+int chunks(int n, int K) {
+    int S = std::max(1, std::min(32, 256 / ((K / TILE) * (WIDTH / TILE))));
+    while ((n / KS) % S) --S;
+    return S;
+}
+
+// Forward and backward over one minibatch of n points (a multiple of 64), enqueued on the model's
+// stream: the rows of X / Y are taken in the epoch's shuffled order when keys are given (N samples,
+// nb minibatches an epoch, the step read from the device counters), else rows 0..n-1.  Leaves the
+// weight-gradient partials in m->part and the workgroup sums of the other gradients and of the loss
+// in m->gpart.
+void enqueue_forward_backward(cc_mlp *m, int n, const float *X, const float *Y, const uint32_t *keys, int hb,
+                              uint32_t N, int nb) {
     hipStream_t s = m->stream;
-    const int W = m->width, D = m->depth, F2 = 2 * m->F;
-    (void)hipMemsetAsync(m->grads, 0, sizeof(float) * m->nparam, s);
-    hipLaunchKernelGGL((k_mlp_forward<true, false>), dim3(n / TILE), dim3(256), 0, s, dev_view(m), m->x, n,
-                       m->phi, m->feat, m->zs, m->as, cc::Lattice{});
-    const size_t L = (size_t)n * W;
-    hipLaunchKernelGGL(k_out_dphi, dim3((n + 255) / 256), dim3(256), 0, s, m->phi, m->y, n, m->dphi,
-                       m->grads + m->off_bout, m->loss);
-    hipLaunchKernelGGL(k_out_dz, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, s, m->dphi, n, W,
-                       m->params + m->off_wout, m->zs + (D - 1) * L, m->dz0);
-    const dim3 gcs((n + COLSUM_ROWS - 1) / COLSUM_ROWS);
-    hipLaunchKernelGGL(k_colsum, gcs, dim3(512), 0, s, m->as + (D - 1) * L, (const float *)m->dphi, n,
-                       m->grads + m->off_wout);
-    u16 *dz = m->dz0, *dzp = m->dz1;
-    // split the batch of the dW products into equal chunks of whole K steps: split divides n / KS
-    int split = std::max(1, std::min(64, n / 256));
-    while ((n / KS) % split) --split;
-    for (int l = D - 1; l >= 0; --l) {
-        const int K = m->K[l];
-        const u16 *ain = l == 0 ? m->feat : m->as + (l - 1) * L;
-        const int lda = l == 0 ? F2 : W;
-        // dW[l] (K x W) = a_in^T (K x n) . dz (n x W)
-        hipLaunchKernelGGL((k_gemm<true, false, 0>), dim3(W / TILE, K / TILE, split), dim3(256), 0, s, ain,
-                           lda, dz, W, m->grads + m->off_w[l], (u16 *)nullptr, W, K, W, n,
-                           (const float *)nullptr, 0);
-        hipLaunchKernelGGL(k_colsum, gcs, dim3(512), 0, s, dz, (const float *)nullptr, n,
-                           m->grads + m->off_b[l]);
-        if (l > 0) {
-            // dz_prev (n x W) = (dz (n x W) . W[l]^T (W x W)) * silu'(z_prev); W[l]^T via wt[l] = [W][K]
-            hipLaunchKernelGGL((k_gemm<false, false, 1>), dim3(W / TILE, n / TILE, 1), dim3(256), 0, s, dz,
-                               W, m->wt[l], K, (float *)nullptr, dzp, W, n, K, W, m->zs + (l - 1) * L, W);
-            std::swap(dz, dzp);
-        }
+    const int D = m->depth, W = m->width;
+    TrainBufs tb;
+    tb.featT = m->featT;
+    tb.aT = m->aT;
+    tb.ds = m->ds;
+    tb.dz = m->dz[0];
+    tb.dzT = m->dzT[0];
+    tb.gpart = m->gpart;
+    tb.ctr = counters(m);
+    const size_t nW = m->off_b[0];
+    tb.rblast = (int)(m->off_b[D - 1] - nW);
+    tb.rwout = (int)(m->off_wout - nW);
+    tb.rbout = (int)(m->off_bout - nW);
+    tb.rloss = sum_rows(m) - 1;
+    hipLaunchKernelGGL(k_train_fwd, dim3(n / TILE), dim3(FWD_T), 0, s, dev_view(m), tb, X, Y, keys, hb, N, nb, n);
+    for (int l = D - 1, c = 0; l >= 0; --l, c ^= 1) {
+        BwdLayer a;
+        a.ainT = l == 0 ? m->featT : m->aT + (size_t)(l - 1) * W * n;
+        a.dz = m->dz[c];
+        a.dzT = m->dzT[c];
+        a.w16 = m->w16[l];
+        a.ds = l > 0 ? m->ds + (size_t)(l - 1) * W * n : nullptr;
+        a.dzp = m->dz[c ^ 1];
+        a.dzpT = m->dzT[c ^ 1];
+        a.part = m->part + 32 * m->off_w[l];
+        a.gb = l > 0 ? m->gpart + (m->off_b[l - 1] - nW) * (size_t)(n / TILE) : nullptr;
+        a.ctr = keys && l == D - 1 ? counters(m) : nullptr;
+        a.n = n;
+        a.K = m->K[l];
+        a.S = chunks(n, a.K);
+        a.nA = (a.K / TILE) * (W / TILE) * a.S;
+        const int nB = l > 0 ? n / TILE : 0;
+        hipLaunchKernelGGL(k_train_bwd, dim3(a.nA + nB), dim3(256), 0, s, a);
     }
 }
 
-// One forward + backward with its own loss, synchronised.
+void enqueue_adam(cc_mlp *m, int n, int update, const float *sched) {
+    AdamArgs a;
+    a.p = m->params;
+    a.g = m->grads;
+    a.m1 = m->m1;
+    a.m2 = m->m2;
+    a.part = m->part;
+    a.gpart = m->gpart;
+    a.loss = m->small;
+    a.nblk = n / TILE;
+    a.nq = sum_rows(m);
+    a.sched = sched;
+    a.ctr = counters(m);
+    a.nparam = m->nparam;
+    for (int l = 0; l < 8; ++l) {
+        const bool in = l < m->depth;
+        a.off_w[l] = in ? m->off_w[l] : 0;
+        a.K[l] = in ? m->K[l] : 0;
+        a.S[l] = in ? chunks(n, m->K[l]) : 0;
+        a.wt[l] = in ? m->wt[l] : nullptr;
+        a.w16[l] = in ? m->w16[l] : nullptr;
+    }
+    a.off_w[m->depth] = m->off_b[0];
+    a.depth = m->depth;
+    a.update = update;
+    hipLaunchKernelGGL(k_adam_all, dim3((unsigned)(m->off_b[0] / 256 + (a.nq + 3) / 4)), dim3(256), 0, m->stream, a);
+}
+
+// The loss and the full gradient (into m->grads) at the n samples in m->x / m->y, synchronised.
 int loss_grad(cc_mlp *m, int n, float *loss_out) {
-    (void)hipMemsetAsync(m->loss, 0, 4, m->stream);
-    enqueue_loss_grad(m, n);
+    (void)hipMemsetAsync(m->small, 0, 256, m->stream);
+    enqueue_forward_backward(m, n, m->x, m->y, nullptr, 0, 0, 1);
+    enqueue_adam(m, n, 0, nullptr);
     float lh = 0.0f;
-    (void)hipMemcpyAsync(&lh, m->loss, 4, hipMemcpyDeviceToHost, m->stream);
+    (void)hipMemcpyAsync(&lh, m->small, 4, hipMemcpyDeviceToHost, m->stream);
     if (hipStreamSynchronize(m->stream) != hipSuccess)
         return nfail(CC_ERR_HIP, std::string("surrogate step: ") + hipGetErrorString(hipGetLastError()));
     if (loss_out) *loss_out = lh;
     return CC_OK;
-}
-
-// Gathers minibatch rows idx[0..n) of the device-resident samples into the step buffers.
-__global__ void k_gather(const float *__restrict__ X, const float *__restrict__ Y, const uint32_t *__restrict__ idx,
-                         int n, float *__restrict__ x, float *__restrict__ y) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = idx[i];
-    x[3 * i] = X[3 * (size_t)s];
-    x[3 * i + 1] = X[3 * (size_t)s + 1];
-    x[3 * i + 2] = X[3 * (size_t)s + 2];
-    y[i] = Y[s];
-}
-
-void adam_step(cc_mlp *m, float lr) {
-    m->step += 1;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
-    for (int l = 0; l < m->depth; ++l) {
-        const int cnt = m->K[l] * m->width;
-        hipLaunchKernelGGL(k_adam, dim3((cnt + 255) / 256), dim3(256), 0, m->stream, m->params + m->off_w[l],
-                           m->grads + m->off_w[l], m->m1 + m->off_w[l], m->m2 + m->off_w[l], cnt, lr, b1,
-                           b2, eps, c1, c2, m->wt[l], m->K[l], m->width);
-    }
-    const int rest = (int)(m->nparam - m->off_b[0]);  // biases, wout, bout are contiguous after W
-    hipLaunchKernelGGL(k_adam, dim3((rest + 255) / 256), dim3(256), 0, m->stream, m->params + m->off_b[0],
-                       m->grads + m->off_b[0], m->m1 + m->off_b[0], m->m2 + m->off_b[0], rest, lr, b1, b2,
-                       eps, c1, c2, (u16 *)nullptr, 0, 0);
 }
 
 }  // namespace
@@ -691,7 +976,9 @@ int cc_mlp_create(int nfreq, int width, int depth, float sigma, uint64_t seed, c
     bool ok = hipMalloc(&m->ffB, 12 * nfreq) == hipSuccess && hipMalloc(&m->params, 4 * m->nparam) == hipSuccess &&
               hipMalloc(&m->grads, 4 * m->nparam) == hipSuccess && hipMalloc(&m->m1, 4 * m->nparam) == hipSuccess &&
               hipMalloc(&m->m2, 4 * m->nparam) == hipSuccess;
-    for (int l = 0; ok && l < depth; ++l) ok = hipMalloc(&m->wt[l], 2 * (size_t)m->K[l] * width) == hipSuccess;
+    for (int l = 0; ok && l < depth; ++l)
+        ok = hipMalloc(&m->wt[l], 2 * (size_t)m->K[l] * width) == hipSuccess &&
+             hipMalloc(&m->w16[l], 2 * (size_t)m->K[l] * width) == hipSuccess;
     if (!ok) {
         cc_mlp_destroy(m);
         return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate parameters");
@@ -704,7 +991,7 @@ int cc_mlp_create(int nfreq, int width, int depth, float sigma, uint64_t seed, c
     for (int l = 0; ok && l < depth; ++l) {
         const int cnt = m->K[l] * width;
         hipLaunchKernelGGL(k_to_bf16_t, dim3((cnt + 255) / 256), dim3(256), 0, m->stream,
-                           m->params + m->off_w[l], m->wt[l], m->K[l], width);
+                           m->params + m->off_w[l], m->wt[l], m->w16[l], m->K[l], width);
     }
     if (!ok || hipStreamSynchronize(m->stream) != hipSuccess) {
         const std::string msg = std::string("surrogate initialisation: ") + hipGetErrorString(hipGetLastError());
@@ -720,12 +1007,14 @@ int cc_mlp_destroy(cc_mlp *m) {
     (void)hipSetDevice(m->device);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (auto p : {(void *)m->ffB, (void *)m->params, (void *)m->grads, (void *)m->m1, (void *)m->m2,
-                   (void *)m->x, (void *)m->y, (void *)m->phi, (void *)m->zs, (void *)m->feat,
-                   (void *)m->as, (void *)m->dz0, (void *)m->dz1, (void *)m->loss, (void *)m->ex,
-                   (void *)m->ephi, (void *)m->dphi})
+                   (void *)m->x, (void *)m->y, (void *)m->featT, (void *)m->aT, (void *)m->ds,
+                   (void *)m->dz[0], (void *)m->dz[1], (void *)m->dzT[0], (void *)m->dzT[1],
+                   (void *)m->part, (void *)m->small, (void *)m->ex, (void *)m->ephi})
         if (p) (void)hipFree(p);
-    for (auto w : m->wt)
-        if (w) (void)hipFree(w);
+    for (int l = 0; l < 8; ++l) {
+        if (m->wt[l]) (void)hipFree(m->wt[l]);
+        if (m->w16[l]) (void)hipFree(m->w16[l]);
+    }
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return CC_OK;
@@ -757,7 +1046,7 @@ int cc_mlp_set_params(cc_mlp *m, const float *params) {
     for (int l = 0; l < m->depth; ++l) {
         const int cnt = m->K[l] * m->width;
         hipLaunchKernelGGL(k_to_bf16_t, dim3((cnt + 255) / 256), dim3(256), 0, m->stream,
-                           m->params + m->off_w[l], m->wt[l], m->K[l], m->width);
+                           m->params + m->off_w[l], m->wt[l], m->w16[l], m->K[l], m->width);
     }
     (void)hipStreamSynchronize(m->stream);
     return CC_OK;
@@ -838,90 +1127,132 @@ int cc_mlp_train(cc_mlp *m, const cc_geometry *geo, int64_t nsamples, float lo, 
         g.nprims = geo->n_prims;
         g.prims = prims;
     }
-    // samples: uniform half directly; near-surface half by rejection from uniform candidates
+    // samples, drawn on the device: the uniform half directly, the near-surface half by rejection
+    // from further uniform candidates, kept in candidate order
     std::mt19937_64 rng(seed);
-    std::uniform_real_distribution<float> ud(lo, hi);
-    const size_t N = (size_t)nsamples;
-    std::vector<float> X(3 * N), Y(N);
-    const size_t chunk = 1 << 20;
-    std::vector<float> cx(3 * chunk), cy(chunk);
-    float *dcx = nullptr, *dcy = nullptr;
-    if (hipMalloc(&dcx, 12 * chunk) != hipSuccess || hipMalloc(&dcy, 4 * chunk) != hipSuccess) {
-        (void)hipFree(dcx);
-        if (prims) (void)hipFree(prims);
-        return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate training: sample buffers");
-    }
-    size_t nu = 0, nn = 0, half = N / 2;
-    int guard = 0;
-    while ((nu < half || nn < N - half) && guard < 4096) {
-        ++guard;
-        for (auto &v : cx) v = ud(rng);
-        // every copy is ordered on the model's (non-blocking) stream with the kernel it feeds
-        (void)hipMemcpyAsync(dcx, cx.data(), 12 * chunk, hipMemcpyHostToDevice, m->stream);
-        hipLaunchKernelGGL(k_exact_points, dim3(chunk / 256), dim3(256), 0, m->stream, g, dcx, (int)chunk, dcy);
-        (void)hipMemcpyAsync(cy.data(), dcy, 4 * chunk, hipMemcpyDeviceToHost, m->stream);
-        if (hipStreamSynchronize(m->stream) != hipSuccess) {
-            (void)hipFree(dcx);
-            (void)hipFree(dcy);
-            if (prims) (void)hipFree(prims);
-            return nfail(CC_ERR_HIP, std::string("surrogate sampling: ") + hipGetErrorString(hipGetLastError()));
-        }
-        for (size_t i = 0; i < chunk; ++i) {
-            size_t slot;
-            if (nu < half) slot = nu++;
-            else if (std::fabs(cy[i]) < band && nn < N - half) slot = half + nn++;
-            else continue;
-            X[3 * slot] = cx[3 * i];
-            X[3 * slot + 1] = cx[3 * i + 1];
-            X[3 * slot + 2] = cx[3 * i + 2];
-            Y[slot] = cy[i];
-        }
-    }
-    (void)hipFree(dcx);
-    (void)hipFree(dcy);
-    if (prims) (void)hipFree(prims);
-    if (nn < N - half) return nfail(CC_ERR_ARGUMENT, "could not find enough near-surface samples");
-    int rc = ensure_cap(m, batch);
-    if (rc) return rc;
-    // the samples stay on the device; each epoch uploads one permutation and syncs once
-    float *dX = nullptr, *dY = nullptr;
-    uint32_t *dperm = nullptr;
-    if (hipMalloc(&dX, 12 * N) || hipMalloc(&dY, 4 * N) || hipMalloc(&dperm, 4 * N)) {
-        (void)hipFree(dX);
-        (void)hipFree(dY);
+    const uint64_t skey = rng();
+    const size_t N = (size_t)nsamples, half = N / 2, chunk = 1 << 20;
+    float *dX = nullptr, *dY = nullptr, *cx = nullptr, *cy = nullptr, *dsched = nullptr;
+    uint8_t *cok = nullptr;
+    int *sel = nullptr, *nsel = nullptr;
+    uint32_t *dkeys = nullptr;
+    void *tmp = nullptr;
+    size_t tmpb = 0;
+    (void)hipcub::DeviceSelect::Flagged(nullptr, tmpb, hipcub::CountingInputIterator<int>(0), (const uint8_t *)nullptr,
+                                        (int *)nullptr, (int *)nullptr, (int)chunk);
+    auto free_all = [&]() {
+        for (void *q : {(void *)dX, (void *)dY, (void *)cx, (void *)cy, (void *)dsched, (void *)cok, (void *)sel,
+                        (void *)nsel, (void *)dkeys, tmp, (void *)prims})
+            if (q) (void)hipFree(q);
+    };
+    if (hipMalloc(&dX, 12 * N) || hipMalloc(&dY, 4 * N) || hipMalloc(&cx, 12 * chunk) || hipMalloc(&cy, 4 * chunk) ||
+        hipMalloc(&cok, chunk) || hipMalloc(&sel, 4 * chunk) || hipMalloc(&nsel, 64) || hipMalloc(&tmp, tmpb)) {
+        free_all();
         return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate training samples");
     }
-    (void)hipMemcpyAsync(dX, X.data(), 12 * N, hipMemcpyHostToDevice, m->stream);
-    (void)hipMemcpyAsync(dY, Y.data(), 4 * N, hipMemcpyHostToDevice, m->stream);
-    std::vector<uint32_t> perm(N);
-    for (size_t i = 0; i < N; ++i) perm[i] = (uint32_t)i;
+    hipLaunchKernelGGL(k_candidates, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, m->stream, g, skey,
+                       (uint64_t)0, (int)half, lo, hi, band, dX, dY, (uint8_t *)nullptr);
+    size_t nn = 0;
+    uint64_t c0 = half;
+    for (int guard = 0; nn < N - half && guard < 4096; ++guard, c0 += chunk) {
+        int got = 0;
+        hipLaunchKernelGGL(k_candidates, dim3(chunk / 256), dim3(256), 0, m->stream, g, skey, c0, (int)chunk, lo, hi,
+                           band, cx, cy, cok);
+        if (hipcub::DeviceSelect::Flagged(tmp, tmpb, hipcub::CountingInputIterator<int>(0), cok, sel, nsel,
+                                          (int)chunk, m->stream) != hipSuccess ||
+            hipMemcpyAsync(&got, nsel, 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+            hipStreamSynchronize(m->stream) != hipSuccess) {
+            free_all();
+            return nfail(CC_ERR_HIP, std::string("surrogate sampling: ") + hipGetErrorString(hipGetLastError()));
+        }
+        const int take = (int)std::min<size_t>((size_t)got, N - half - nn);
+        if (take > 0)
+            hipLaunchKernelGGL(k_place, dim3((take + 255) / 256), dim3(256), 0, m->stream, cx, cy, sel, take,
+                               dX + 3 * (half + nn), dY + half + nn);
+        nn += (size_t)take;
+    }
+    if (nn < N - half) {
+        free_all();
+        return nfail(CC_ERR_ARGUMENT, "could not find enough near-surface samples");
+    }
+    int rc = ensure_cap(m, batch);
+    if (rc) {
+        free_all();
+        return rc;
+    }
     const size_t nbatch = N / batch;
     const size_t total_steps = nbatch * epochs;
-    size_t gstep = 0;
+    if (total_steps > (size_t)INT32_MAX) {
+        free_all();
+        return nfail(CC_ERR_ARGUMENT, "too many training steps");
+    }
+    // epoch e visits the samples in the order of a keyed permutation (six round keys an epoch)
+    int hb = 0;
+    while ((uint64_t(1) << (2 * hb)) < N) ++hb;
+    std::vector<uint32_t> keys(6 * (size_t)epochs);
+    for (auto &k : keys) k = (uint32_t)rng();
+    // the learning rate (cosine decay) and Adam's bias corrections of every step, read on the device
+    std::vector<float> sched(3 * total_steps);
+    for (size_t gs = 0; gs < total_steps; ++gs) {
+        const float t = (float)(m->step + 1 + gs);
+        sched[3 * gs] = 0.5f * lr * (1.0f + std::cos(3.14159265f * (float)gs / (float)total_steps));
+        sched[3 * gs + 1] = 1.0f - std::pow(0.9f, t);
+        sched[3 * gs + 2] = 1.0f - std::pow(0.999f, t);
+    }
+    if (hipMalloc(&dsched, 12 * total_steps) || hipMalloc(&dkeys, 4 * keys.size())) {
+        free_all();
+        return nfail(CC_ERR_OUT_OF_MEMORY, "surrogate training schedule");
+    }
+    (void)hipMemcpyAsync(dsched, sched.data(), 12 * total_steps, hipMemcpyHostToDevice, m->stream);
+    (void)hipMemcpyAsync(dkeys, keys.data(), 4 * keys.size(), hipMemcpyHostToDevice, m->stream);
+    (void)hipMemsetAsync(m->small, 0, 256, m->stream);  // loss and step counters
+    // one step: forward, backward, Adam.  Runs of GSTEPS steps and single steps are captured once as
+    // graphs and replayed (the step index lives on the device)
+    auto enqueue_step = [&]() {
+        enqueue_forward_backward(m, batch, dX, dY, dkeys, hb, (uint32_t)N, (int)nbatch);
+        enqueue_adam(m, batch, 1, dsched);
+    };
+    constexpr int GSTEPS = 8;
+    hipGraph_t graph[2] = {};
+    hipGraphExec_t exec[2] = {};
+    for (int k = 0; k < 2; ++k) {
+        if (hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) break;
+        for (int r = 0; r < (k == 0 ? GSTEPS : 1); ++r) enqueue_step();
+        if (hipStreamEndCapture(m->stream, &graph[k]) != hipSuccess ||
+            hipGraphInstantiate(&exec[k], graph[k], nullptr, nullptr, 0) != hipSuccess)
+            exec[k] = nullptr;  // the same kernels are then launched one by one
+    }
+    (void)hipGetLastError();
+    float *hloss = nullptr;
+    if (hipHostMalloc(&hloss, 64) != hipSuccess) hloss = nullptr;
+    float lbuf = 0.0f;
+    float *lh = hloss ? hloss : &lbuf;
     for (int e = 0; e < epochs && rc == CC_OK; ++e) {
-        std::shuffle(perm.begin(), perm.end(), rng);
-        (void)hipMemcpyAsync(dperm, perm.data(), 4 * N, hipMemcpyHostToDevice, m->stream);
-        (void)hipMemsetAsync(m->loss, 0, 4, m->stream);
-        for (size_t b = 0; b < nbatch; ++b, ++gstep) {
-            hipLaunchKernelGGL(k_gather, dim3((batch + 255) / 256), dim3(256), 0, m->stream, dX, dY,
-                               dperm + b * batch, batch, m->x, m->y);
-            enqueue_loss_grad(m, batch);
-            const float lrt = 0.5f * lr * (1.0f + std::cos(3.14159265f * (float)gstep / (float)total_steps));
-            adam_step(m, lrt);
+        (void)hipMemsetAsync(m->small, 0, 4, m->stream);
+        for (size_t b = 0; b < nbatch;) {
+            if (exec[0] && b + GSTEPS <= nbatch) {
+                (void)hipGraphLaunch(exec[0], m->stream);
+                b += GSTEPS;
+            } else {
+                if (exec[1]) (void)hipGraphLaunch(exec[1], m->stream);
+                else enqueue_step();
+                ++b;
+            }
         }
-        float l = 0.0f;
-        (void)hipMemcpyAsync(&l, m->loss, 4, hipMemcpyDeviceToHost, m->stream);
+        (void)hipMemcpyAsync(lh, m->small, 4, hipMemcpyDeviceToHost, m->stream);
         if (hipStreamSynchronize(m->stream) != hipSuccess)
             rc = nfail(CC_ERR_HIP, std::string("surrogate training: ") + hipGetErrorString(hipGetLastError()));
         else if (history)
-            history[e] = l / (float)nbatch;
+            history[e] = lh[0] / (float)nbatch;
     }
-    (void)hipFree(dX);
-    (void)hipFree(dY);
-    (void)hipFree(dperm);
+    for (int k = 0; k < 2; ++k) {
+        if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+        if (graph[k]) (void)hipGraphDestroy(graph[k]);
+    }
+    if (hloss) (void)hipHostFree(hloss);
+    free_all();
     if (rc) return rc;
-    if (hipStreamSynchronize(m->stream) != hipSuccess)
-        return nfail(CC_ERR_HIP, hipGetErrorString(hipGetLastError()));
+    m->step += (int)total_steps;
     return CC_OK;
 }
 

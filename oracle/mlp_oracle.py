@@ -3,7 +3,7 @@
 numpy restatement of the forward pass, mean-squared loss and backward pass of the library's
 Fourier-feature MLP (3 -> 2F sin/cos features -> depth x width SiLU -> 1), with the device's
 numerics emulated: every matrix-product operand rounded to bfloat16 (round to nearest even), FP32
-accumulation, FP32 bias and activation.  ``emulate=False`` runs the same algebra in float64 with no
+accumulation, FP32 bias and activation, and silu'(z) of the hidden layers stored in bfloat16.  ``emulate=False`` runs the same algebra in float64 with no
 rounding, which the finite-difference test pins.  The parameter vector is packed as the library
 packs it: W_l (K_l x width, row-major) for every layer, then every b_l, then w_out, then b_out.
 """
@@ -79,6 +79,7 @@ def loss_grad(p, B, x, y, F, width, depth, emulate=True):
         gW[l] = acts[l].astype(dt).T @ dz.astype(dt)
         gb[l] = np.sum(dz.astype(dt), axis=0)
         if l > 0:
-            dz = r((dz.astype(dt) @ r(Ws[l]).astype(dt).T) * dsil(zs[l - 1]))
+            # the device keeps silu'(z) of the hidden layers in bfloat16 between its passes
+            dz = r((dz.astype(dt) @ r(Ws[l]).astype(dt).T) * r(dsil(zs[l - 1])))
     g = np.concatenate([w.reshape(-1) for w in gW] + list(gb) + [g_wout, [g_bout]])
     return loss, g
