@@ -241,7 +241,8 @@ __device__ __forceinline__ float lo_bf(uint32_t w) { return __uint_as_float(w <<
 __device__ __forceinline__ float hi_bf(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 
 // One workgroup (512 threads, 8 waves) runs a 64-point tile through every layer.  The activation
-// tile lives in one LDS buffer, [TILE][LDW] BF16, updated in place between layers; each wave owns
+// tile lives in two LDS buffers, [TILE][LDW] BF16 each (dynamic LDS, FWD_LDS bytes): layer l reads
+// one and writes the other, so a layer needs one barrier; each wave owns
 // 32 output features (4 x 2 MFMA tiles: lane = one feature, 4 consecutive points), with the
 // transposed weights' fragments for the next K step loaded while the current step's MFMAs run.
 // Eight waves on 64 points keep two waves on every SIMD, so one wave's SiLU epilogue runs beside
@@ -249,14 +250,16 @@ __device__ __forceinline__ float hi_bf(uint32_t w) { return __uint_as_float(w & 
 // Points: X/Y rows shuffle_index(b n + p) under epoch e's keys for step s = ctr[0] = e nb + b when
 // keys ([epoch][6]) are given, else rows p.
 constexpr int FWD_T = 512;
+constexpr size_t FWD_LDS = 2 * (size_t)TILE * LDW * sizeof(u16);
 __global__ __launch_bounds__(FWD_T) void k_train_fwd(MlpDev m, TrainBufs tb, const float *__restrict__ X,
                                                      const float *__restrict__ Y, const uint32_t *__restrict__ keys,
                                                      int hb, uint32_t N, int nb, int n) {
-    __shared__ __attribute__((aligned(16))) u16 buf[TILE * LDW];
+    extern __shared__ __attribute__((aligned(16))) u16 fbuf[];  // 2 x [TILE][LDW]
     __shared__ float pu[TILE][4];  // unit-box coordinates and target of each point
     __shared__ float wo[WIDTH];    // BF16-rounded output weights
     __shared__ float sdp[TILE];    // dphi of each point
     __shared__ float red[2][FWD_T / 64];
+    u16 *buf = fbuf, *nxt = fbuf + TILE * LDW;
     const int W = WIDTH, F = m.F, F2 = 2 * F, LDF = F2 + 8;
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, lr = lane & 15;
     const int koff = 8 * (lane >> 4), g4 = 4 * (lane >> 4);
@@ -321,7 +324,6 @@ __global__ __launch_bounds__(FWD_T) void k_train_fwd(MlpDev m, TrainBufs tb, con
                     for (int j = 0; j < 2; ++j) acc[i][j] = mfma(a[i], b[ks][j], acc[i][j]);
             }
         if (l + 1 < m.depth) load_layer(l + 1, W);
-        __syncthreads();  // every wave has read the whole input tile
         const bool last = l == m.depth - 1;
         u16 *aTl = tb.aT + (size_t)l * W * n, *dsl = tb.ds + (size_t)l * W * n;
 #pragma unroll
@@ -340,7 +342,7 @@ __global__ __launch_bounds__(FWD_T) void k_train_fwd(MlpDev m, TrainBufs tb, con
                     acc[i][j][r] = dv[r];
                 }
                 const uint2 ap = {pk2(av[0], av[1]), pk2(av[2], av[3])};
-                u16 *bp = buf + (16 * i + g4) * LDW + col;
+                u16 *bp = nxt + (16 * i + g4) * LDW + col;
                 bp[0] = (u16)(ap.x & 0xffffu);
                 bp[LDW] = (u16)(ap.x >> 16);
                 bp[2 * LDW] = (u16)(ap.y & 0xffffu);
@@ -353,6 +355,9 @@ __global__ __launch_bounds__(FWD_T) void k_train_fwd(MlpDev m, TrainBufs tb, con
             }
         }
         __syncthreads();
+        u16 *const o = buf;
+        buf = nxt;
+        nxt = o;
         K = W;
         LDin = LDW;
     }
@@ -453,37 +458,35 @@ struct BwdLayer {
 };
 
 // acc[i][j] += A_i(k) B_j(k) over the K steps k = k, k + ks, ... < kend, where A_i(k) = load8(Ar +
-// i ast + k) and B_j(k) = load8(Br + j bst + k) are MFMA fragments; the fragments are loaded two
-// steps ahead (the loads past the last step reload it).
-template <int NI, int NJ>
+// i ast + k) and B_j(k) = load8(Br + j bst + k) are MFMA fragments, loaded DEPTH steps ahead into a
+// ring of register sets.
+template <int NI, int NJ, int DEPTH>
 __device__ __forceinline__ void mma_steps(f32x4 (&acc)[NI][NJ], const u16 *Ar, size_t ast, const u16 *Br, size_t bst,
                                           int k, int kend, int ks) {
     if (k >= kend) return;
-    const int klast = k + ((kend - 1 - k) / ks) * ks;
-    bf16x8 x0[NI], y0[NJ], x1[NI], y1[NJ];
-    auto ld = [&](bf16x8(&x)[NI], bf16x8(&y)[NJ], int kk) {
-        kk = kk < klast ? kk : klast;
+    const int nsteps = (kend - 1 - k) / ks + 1;
+    bf16x8 x[DEPTH][NI], y[DEPTH][NJ];
+    auto ld = [&](int d, int step) {
+        const int kk = k + step * ks;
 #pragma unroll
-        for (int i = 0; i < NI; ++i) x[i] = load8(Ar + i * ast + kk);
+        for (int i = 0; i < NI; ++i) x[d][i] = load8(Ar + i * ast + kk);
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) y[j] = load8(Br + j * bst + kk);
+        for (int j = 0; j < NJ; ++j) y[d][j] = load8(Br + j * bst + kk);
     };
-    auto mm = [&](const bf16x8(&x)[NI], const bf16x8(&y)[NJ]) {
 #pragma unroll
-        for (int i = 0; i < NI; ++i)
+    for (int d = 0; d < DEPTH; ++d)
+        if (d < nsteps) ld(d, d);
+    for (int s0 = 0; s0 < nsteps; s0 += DEPTH) {
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) acc[i][j] = mfma(x[i], y[j], acc[i][j]);
-    };
-    ld(x0, y0, k);
-    ld(x1, y1, k + ks);
-    for (;;) {
-        mm(x0, y0);
-        ld(x0, y0, k + 2 * ks);
-        if (k + ks > klast) break;
-        mm(x1, y1);
-        ld(x1, y1, k + 3 * ks);
-        k += 2 * ks;
-        if (k > klast) break;
+        for (int d = 0; d < DEPTH; ++d) {
+            if (s0 + d < nsteps) {
+#pragma unroll
+                for (int i = 0; i < NI; ++i)
+#pragma unroll
+                    for (int j = 0; j < NJ; ++j) acc[i][j] = mfma(x[d][i], y[d][j], acc[i][j]);
+                if (s0 + d + DEPTH < nsteps) ld(d, s0 + d + DEPTH);
+            }
+        }
     }
 }
 
@@ -515,8 +518,8 @@ __global__ __launch_bounds__(256, 2) void k_train_bwd(BwdLayer a) {
         const int i0 = (tile / NT) * TILE, j0 = (tile % NT) * TILE;
         const int chunk = a.n / a.S, kb = s * chunk;
         const size_t st = (size_t)16 * a.n;
-        mma_steps<4, 4>(acc, a.ainT + (size_t)(i0 + lr) * a.n + koff, st, a.dzT + (size_t)(j0 + lr) * a.n + koff, st,
-                        kb + KS * wave, kb + chunk, 4 * KS);
+        mma_steps<4, 4, 4>(acc, a.ainT + (size_t)(i0 + lr) * a.n + koff, st, a.dzT + (size_t)(j0 + lr) * a.n + koff,
+                           st, kb + KS * wave, kb + chunk, 4 * KS);
         // the four partial tiles meet in LDS in two rounds, (w0 + w2) + (w1 + w3)
         auto put = [&](float *slot) {
 #pragma unroll
@@ -553,17 +556,22 @@ __global__ __launch_bounds__(256, 2) void k_train_bwd(BwdLayer a) {
         return;
     }
     const int p0 = (blockIdx.x - a.nA) * TILE;
-    const size_t st = (size_t)16 * WIDTH;
+    u16 *stage = reinterpret_cast<u16 *>(smem);  // [TILE][LDW] BF16: dz_l rows, then dz_{l-1} rows
+    for (int c = t; c < TILE * WIDTH / 8; c += 256) {  // the block's dz_l rows in one burst
+        const int row = c >> 5, c8 = (c & 31) * 8;
+        *reinterpret_cast<bf16x8 *>(stage + row * LDW + c8) = load8(a.dz + (size_t)(p0 + row) * WIDTH + c8);
+    }
     uint2 dsv[4][4];  // silu'(z_{l-1}) of this lane's outputs, loaded before the products
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             dsv[j][i] = *reinterpret_cast<const uint2 *>(a.ds + (size_t)(64 * wave + 16 * j + lr) * a.n + p0 + 16 * i + g4);
-    // rows: the block's points; columns: this wave's 64 features of layer l-1
-    mma_steps<4, 4>(acc, a.dz + (size_t)(p0 + lr) * WIDTH + koff, st, a.w16 + (size_t)(64 * wave + lr) * WIDTH + koff,
-                    st, 0, WIDTH, KS);
-    u16 *stage = reinterpret_cast<u16 *>(smem);  // [TILE][LDW] BF16, the point-major copy
+    __syncthreads();
+    // rows: the block's points (from LDS); columns: this wave's 64 features of layer l-1
+    mma_steps<4, 4, 2>(acc, stage + lr * LDW + koff, (size_t)16 * LDW, a.w16 + (size_t)(64 * wave + lr) * WIDTH + koff,
+                       (size_t)16 * WIDTH, 0, WIDTH, KS);
+    __syncthreads();  // every wave has read the dz_l rows
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int col = 64 * wave + 16 * j + lr;
@@ -836,7 +844,7 @@ void enqueue_forward_backward(cc_mlp *m, int n, const float *X, const float *Y, 
     tb.rwout = (int)(m->off_wout - nW);
     tb.rbout = (int)(m->off_bout - nW);
     tb.rloss = sum_rows(m) - 1;
-    hipLaunchKernelGGL(k_train_fwd, dim3(n / TILE), dim3(FWD_T), 0, s, dev_view(m), tb, X, Y, keys, hb, N, nb, n);
+    hipLaunchKernelGGL(k_train_fwd, dim3(n / TILE), dim3(FWD_T), FWD_LDS, s, dev_view(m), tb, X, Y, keys, hb, N, nb, n);
     for (int l = D - 1, c = 0; l >= 0; --l, c ^= 1) {
         BwdLayer a;
         a.ainT = l == 0 ? m->featT : m->aT + (size_t)(l - 1) * W * n;
@@ -960,11 +968,13 @@ int cc_mlp_create(int nfreq, int width, int depth, float sigma, uint64_t seed, c
         std::uniform_real_distribution<float> ud(-a, a);
         for (int i = 0; i < width; ++i) p[m->off_wout + i] = ud(rng);
     }
-    // the inference kernel's activation tile exceeds the default 64 KB of dynamic LDS
+    // the inference and training forward kernels' activation tiles exceed the default 64 KB of LDS
     if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_infer<true>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)INFER_LDS) != hipSuccess ||
         hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_infer<false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)INFER_LDS) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&k_train_fwd),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)FWD_LDS) != hipSuccess ||
         hipGetDevice(&m->device) != hipSuccess) {
         delete m;
         return nfail(CC_ERR_HIP, std::string("surrogate kernel attributes: ") + hipGetErrorString(hipGetLastError()));
