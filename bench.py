@@ -543,8 +543,10 @@ def csg(seed, n=512, train_epochs=24, train_samples=1 << 22, cpu=True):
                        "batch": 16384, "final_loss": float(hist[-1]), "first_epoch_loss": float(hist[0]),
                        "approx_tflop": round(tflop, 2), "approx_tflops": round(tflop / tt, 1)}
     from cutcell.autotune import tune_stream
-    # the stream pass reads phi here: z-march depths that reuse node layers, or flat tiles
-    cands = [(1, 4, False), (1, 8, False), (1, 16, False), (1, 8, True), (0, 16, False)]
+    # the fused z-march evaluating the culled or surrogate phi from the bricks, or a phi pass and a
+    # stream pass reading it (z-march depths that reuse node layers, or flat tiles)
+    cands = [(2, 8, False), (2, 16, False), (2, 8, True), (2, 16, True),
+             (1, 8, False), (1, 16, False), (1, 8, True), (0, 16, False)]
     cache = allocate_cache(grid, MarchingCubesCutCell)
     cache.set_profiling(True)
     res = {}
@@ -560,7 +562,9 @@ def csg(seed, n=512, train_epochs=24, train_samples=1 << 22, cpu=True):
             kts.append(dict(cache.kernel_times()))
         res[src] = {"update_ms": round(1e3 * float(np.median(ts)), 3),
                     "kernel_ms": {k: round(float(np.median([kt[k] for kt in kts])), 4) for k in kts[0]},
-                    "stream_tuning": [tune["variant"], tune["kz"], tune["nontemporal"]]}
+                    "stream_tuning": [tune["variant"], tune["kz"], tune["nontemporal"]],
+                    "tuning_trials_ms": {f"{r['variant']},{r['kz']},{int(r['nontemporal'])}": round(r["ms"], 3)
+                                         for r in tune["trials"]}}
         if src == "culled":
             st = cache.phi_stats()
             res[src]["mean_kept_spheres_per_brick"] = round(st["kept_sum"] / st["bricks"], 2)
@@ -582,6 +586,8 @@ def csg(seed, n=512, train_epochs=24, train_samples=1 << 22, cpu=True):
     res["surrogate"] = {
         "update_ms": round(1e3 * float(np.median(ts)), 3), "kernel_ms": km,
         "stream_tuning": [stune["variant"], stune["kz"], stune["nontemporal"]],
+        "tuning_trials_ms": {f"{r['variant']},{r['kz']},{int(r['nontemporal'])}": round(r["ms"], 3)
+                             for r in stune["trials"]},
         "coarse_lattice": f"{nc}^3 (8 cells)", "band": "3h",
         "exact_node_fraction": round(st["exact"] / st["nodes"], 4),
         "uncertified_fraction": round(st["uncertified"] / st["nodes"], 4),
@@ -596,6 +602,21 @@ def csg(seed, n=512, train_epochs=24, train_samples=1 << 22, cpu=True):
     update_cache(cache, body, at(20))
     update_cache(sur, body, at(20))
     res["agreement"] = _compare_caches(sur, cache, grid)
+    # the analytic headline body (the sphere) on the same cache allocation, its own tuned shape: what
+    # the surrogate update costs beyond an update whose phi is free
+    _, sphere, _ = workload(n, seed)
+    cache.set_phi_source("naive")
+    atune = tune_stream(cache, sphere, at(0), candidates=[(2, 1, True), (2, 1, False), (2, 4, True), (2, 4, False)],
+                        repeats=2, use_cache=False)
+    ts = []
+    for s in range(8):
+        t = time.perf_counter()
+        update_cache(cache, sphere, at(1 + s))
+        ts.append(time.perf_counter() - t)
+    res["analytic_sphere_same_cache"] = {"update_ms": round(1e3 * float(np.median(ts)), 3),
+                                         "stream_tuning": [atune["variant"], atune["kz"], atune["nontemporal"]]}
+    res["surrogate_minus_analytic_ms"] = round(res["surrogate"]["update_ms"]
+                                               - res["analytic_sphere_same_cache"]["update_ms"], 3)
     res["speedup_vs_naive"] = round(res["naive"]["update_ms"] / res["surrogate"]["update_ms"], 2)
     res["speedup_vs_culled"] = round(res["culled"]["update_ms"] / res["surrogate"]["update_ms"], 3)
     res["timed_on"] = "one cache allocation for naive, culled and surrogate; a second for the agreement check"

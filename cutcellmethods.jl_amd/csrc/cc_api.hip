@@ -114,9 +114,13 @@ struct cc_cache {
     int64_t cnt_cap = 0;
     float *nn = nullptr;
     int64_t nn_cap = 0;
-    uint32_t *bstat = nullptr;  // per-brick counters of the last culled/surrogate update
+    uint32_t *bstat = nullptr;  // per-brick counters of the last culled/surrogate update (then, when
+                                // the stream pass evaluated phi, its per-work-item counters)
     int64_t bstat_cap = 0;
+    int8_t *cert = nullptr;     // sub-brick sign certificates for the fused surrogate pass
+    int64_t cert_cap = 0;
     int64_t phi_nbricks = 0;
+    int64_t phi_nwork = 0;  // work items of the stream pass that evaluated phi (their counters follow)
     // the whole-grid surface (marching cubes): work arrays, the last mesh, and what it was made from
     uint64_t gen = 0;  // updates so far
     cc::MeshBuffers mb = {};
@@ -328,7 +332,11 @@ template <typename T> int ensure(T **p, int64_t *cap, int64_t want) {
 
 // A sphere union's nodal phi through the cull pass and the brick pass (culled exact, or the
 // surrogate with certified signs).  Bitwise the plain fold at every node the surrogate does not keep.
-template <typename T> int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::GridT<T> &gt, T *phi) {
+// With bp (the fused stream pass, `nwork` work items) the phi pass is left to the stream pass: the
+// cull pass certifies the sub-bricks' signs and *bp receives what the stream pass evaluates from.
+template <typename T>
+int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::GridT<T> &gt, T *phi, cc::BrickPhi *bp = nullptr,
+               int64_t nwork = 0) {
     if (geo.nprims > 65535)
         return fail(CC_ERR_UNSUPPORTED, "culled and surrogate phi take at most 65535 primitives");
     int nb[3];
@@ -336,9 +344,11 @@ template <typename T> int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::
     const int64_t nbrick = (int64_t)nb[0] * nb[1] * nb[2];
     int rc = ensure(&c->cull_list, &c->cull_cap, nbrick * geo.nprims);
     if (!rc) rc = ensure(&c->cull_cnt, &c->cnt_cap, nbrick);
-    if (!rc) rc = ensure(&c->bstat, &c->bstat_cap, nbrick * cc::PHI_BSTAT);
+    if (!rc) rc = ensure(&c->bstat, &c->bstat_cap, (nbrick + (bp ? nwork : 0)) * cc::PHI_BSTAT);
+    if (!rc && bp && c->phi_source == CC_PHI_SURROGATE) rc = ensure(&c->cert, &c->cert_cap, nbrick * 64);
     if (rc) return rc;
     c->phi_nbricks = nbrick;
+    c->phi_nwork = bp ? nwork : 0;
     // magnitude bound for the rounding margins: coordinates of primitives and lattice, radii
     double scale = 1.0, pm = 0.0, gm = 0.0;
     for (int64_t i = 0; i < geo.nprims; ++i) {
@@ -371,8 +381,20 @@ template <typename T> int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::
         const double h = std::fmax((double)gt.d[0], std::fmax((double)gt.d[1], (double)gt.d[2]));
         sa.band = (float)(c->band_cells * h);
     }
-    cc::launch_cull<T>(gt, geo, scale, c->cull_list, c->cull_cnt, c->stream);
+    int8_t *cert = bp && sa.nn ? c->cert : nullptr;
+    cc::launch_cull<T>(gt, geo, scale, c->cull_list, c->cull_cnt, cert, bp ? c->bstat : nullptr, c->stream);
     mark(c, "cull");
+    if (bp) {
+        bp->list = c->cull_list;
+        bp->cnt = c->cull_cnt;
+        bp->cert = cert;
+        bp->sa = sa;
+        bp->nbx = nb[0];
+        bp->nby = nb[1];
+        bp->nprims = geo.nprims;
+        bp->wstat = c->bstat + nbrick * cc::PHI_BSTAT;
+        return CC_OK;
+    }
     cc::launch_phi_bricks<T>(phi, gt, geo, c->cull_list, c->cull_cnt, sa, c->bstat, c->stream);
     mark(c, "phi_bricks");
     return CC_OK;
@@ -390,12 +412,19 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
     if (c->method == CC_MARCHING_SQUARES || c->method == CC_MARCHING_CUBES) {
         // The marching-cubes stream pass samples an analytic body itself (phi written once, never
         // re-read); a sphere union is sampled once by its own pass and then read.
+        // A sphere union's culled or surrogate phi is evaluated by the fused pass from its bricks.
         cc::McTuning tn = c->tune;
+        const bool bricks = c->method == CC_MARCHING_CUBES && geo && geo->kind == cc::GEO_SPHERE_UNION &&
+                            c->phi_source != CC_PHI_NAIVE;
         const bool fuse = c->method == CC_MARCHING_CUBES && geo && tn.variant == cc::MC_MARCH_FUSED &&
-                          geo->kind != cc::GEO_SPHERE_UNION && (geo->kind & cc::GEO_KIND) != cc::GEO_UNION;
+                          (geo->kind != cc::GEO_SPHERE_UNION || bricks) &&
+                          (geo->kind & cc::GEO_KIND) != cc::GEO_UNION;
         if (!fuse && tn.variant == cc::MC_MARCH_FUSED) tn.variant = cc::MC_MARCH;
-        if (geo && !fuse && geo->kind == cc::GEO_SPHERE_UNION && c->phi_source != CC_PHI_NAIVE) {
-            const int rc = phi_bricks<T>(c, *geo, gt, phi);
+        cc::BrickPhi bp{};
+        if (bricks) {
+            uint32_t W, R, KZ, nseg, nrg, nkb;
+            cc::mc_brick_tile_shape(gt.n, tn.kz > 0 ? (uint32_t)tn.kz : 16u, &W, &R, &KZ, &nseg, &nrg, &nkb);
+            const int rc = phi_bricks<T>(c, *geo, gt, phi, fuse ? &bp : nullptr, (int64_t)nseg * nrg * nkb);
             if (rc) return rc;
         } else if (geo && !fuse) {
             cc::launch_sample<T>(phi, gt, *geo, c->dim, c->stream);
@@ -408,7 +437,7 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
         cc::launch_zero_u32(c->counter, c->stream);
         if (c->method == CC_MARCHING_CUBES) {
             cc::launch_mc_stream<T>(phi, gt, fuse ? geo : nullptr, cols, c->cutlist, c->counter, tn,
-                                    c->stream);
+                                    c->stream, fuse && bricks ? &bp : nullptr);
             mark(c, "mc_stream");
             cc::launch_mc_cut<T>(phi, gt, cols, c->cutlist, c->counter, c->stream);
             mark(c, "mc_cut");
@@ -912,6 +941,7 @@ void free_all(cc_cache *c) {
     if (c->cull_cnt) (void)hipFree(c->cull_cnt);
     if (c->nn) (void)hipFree(c->nn);
     if (c->bstat) (void)hipFree(c->bstat);
+    if (c->cert) (void)hipFree(c->cert);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1453,17 +1483,18 @@ int cc_phi_stats(cc_cache *c, int64_t *stats) {
     if (!c->phi_nbricks) return CC_OK;
     HIPC(hipSetDevice(c->device));
     HIPC(hipStreamSynchronize(c->stream));
-    const int64_t nb = c->phi_nbricks;
-    std::vector<uint32_t> cnt(nb), bs(nb * cc::PHI_BSTAT);
+    const int64_t nb = c->phi_nbricks, nr = nb + c->phi_nwork;
+    std::vector<uint32_t> cnt(nb), bs(nr * cc::PHI_BSTAT);
     HIPC(hipMemcpy(cnt.data(), c->cull_cnt, 4 * nb, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(bs.data(), c->bstat, 4 * nb * cc::PHI_BSTAT, hipMemcpyDeviceToHost));
-    // nodes, exact, certificate evaluations, kept primitives, uncertified, band, contradicted, bricks
-    for (int64_t b = 0; b < nb; ++b) {
-        const uint32_t *v = bs.data() + b * cc::PHI_BSTAT;
+    HIPC(hipMemcpy(bs.data(), c->bstat, 4 * nr * cc::PHI_BSTAT, hipMemcpyDeviceToHost));
+    // nodes, exact, certificate evaluations, kept primitives, uncertified, band, contradicted, bricks;
+    // the records are the bricks' and then, when the stream pass evaluated phi, its work items'
+    for (int64_t b = 0; b < nb; ++b) stats[3] += cnt[b];
+    for (int64_t r = 0; r < nr; ++r) {
+        const uint32_t *v = bs.data() + r * cc::PHI_BSTAT;
         stats[0] += v[0];
         stats[1] += v[1];
         stats[2] += v[2];
-        stats[3] += cnt[b];
         stats[4] += v[3];
         stats[5] += v[4];
         stats[6] += v[5];

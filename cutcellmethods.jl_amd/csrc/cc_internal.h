@@ -51,11 +51,14 @@ struct McTuning {
     int nontemporal = 0;           // streaming (non-temporal) stores for the column writes
 };
 
+struct BrickPhi;
 // The stream pass.  `geo` is the body when the variant samples it in-kernel (MC_MARCH_FUSED, phi is
-// then written by this pass); otherwise phi must already hold the nodal field.
+// then written by this pass); otherwise phi must already hold the nodal field.  With `bp` (and the
+// fused variant) a sphere union's culled or surrogate phi is evaluated in-kernel from the bricks.
 template <typename T>
 void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &cols,
-                      uint32_t *cutlist, uint32_t *counter, const McTuning &tune, hipStream_t s);
+                      uint32_t *cutlist, uint32_t *counter, const McTuning &tune, hipStream_t s,
+                      const BrickPhi *bp = nullptr);
 template <typename T>
 void launch_mc_cut(const T *phi, const GridT<T> &g, const Cols<T> &cols, const uint32_t *cutlist,
                    const uint32_t *counter, hipStream_t s);
@@ -107,6 +110,15 @@ void launch_zero_u32(uint32_t *p, hipStream_t s);
 // or the whole row when narrower), R whole rows per tile when rows are short, kz layers per item.
 void mc_tile_shape(const int *n, uint32_t kz, uint32_t *W, uint32_t *R, uint32_t *KZ, uint32_t *nseg,
                    uint32_t *nrg, uint32_t *nkb);
+// The brick-aligned tiles of the fused pass over a sphere union's culled or surrogate phi:
+// MC_BRICK_W cells along x (a multiple of the brick edge; fewer on a narrower grid) by a power of two
+// of rows dividing the brick edge, kz rounded down to a power of two dividing it.  One brick's
+// footprint (16 x 16 cells) measured fastest: the exact nodes cluster in 4^3-node sub-bricks, which
+// a compact tile meets in fewer waves than a strip (64-, 128- and 256-cell strips were 1.3-1.7x
+// slower on the composite body at 512^3).
+constexpr int MC_BRICK_W = 16;
+void mc_brick_tile_shape(const int *n, uint32_t kz, uint32_t *W, uint32_t *R, uint32_t *KZ, uint32_t *nseg,
+                         uint32_t *nrg, uint32_t *nkb);
 
 // ---- nodal phi of a sphere union, brick by brick ------------------------------------------------
 // The node lattice is cut into bricks of PHI_BRICK^3 nodes.  For each brick a cull pass keeps, in
@@ -146,9 +158,26 @@ inline void phi_brick_counts(const int *n, int *nb) {
     for (int a = 0; a < 3; ++a) nb[a] = (n[a] + 1 + PHI_BRICK - 1) / PHI_BRICK;
 }
 
+// The culled or surrogate phi as the fused stream pass evaluates it at a node: the node's brick
+// (nbx x nby bricks per layer) folds its kept primitives list[b * nprims ..][0, cnt[b]), unless the
+// sub-brick's sign certificate cert[64 b + sub] (null: culled exact) and the network value allow
+// keeping the latter -- the phi pass's decisions exactly.  wstat receives PHI_BSTAT counters per
+// work item of the stream pass, laid out as the phi pass's per-brick counters (no certificates).
+struct BrickPhi {
+    const uint16_t *list;
+    const uint32_t *cnt;
+    const int8_t *cert;
+    SurrogateArgs sa;
+    int nbx, nby, nprims;
+    uint32_t *wstat;
+};
+
+// The cull pass; with cert non-null it also certifies the signs of each brick's 64 sub-bricks
+// (int8 per sub-brick: +1, -1, or 0 when uncertified) and writes each brick's PHI_BSTAT counters
+// into bstat (certificate evaluations only).
 template <typename T>
 void launch_cull(const GridT<T> &g, const Geo &geo, double scale, uint16_t *list, uint32_t *cnt,
-                 hipStream_t s);
+                 int8_t *cert, uint32_t *bstat, hipStream_t s);
 template <typename T>
 void launch_phi_bricks(T *phi, const GridT<T> &g, const Geo &geo, const uint16_t *list,
                        const uint32_t *cnt, const SurrogateArgs &sa, uint32_t *bstat, hipStream_t s);

@@ -60,6 +60,19 @@ __global__ __launch_bounds__(MC_TILE) void k_mc_march(McBody<T> B, uint32_t nwor
     mc_march_item<T, FUSED, NT>(B, w, L);
 }
 
+// The fused pass for a sphere union's culled or surrogate phi: brick-aligned items (MC_BRICK_W cells
+// x MC_TILE / MC_BRICK_W rows, up to 16 layers) that stage their bricks' kept primitives.
+template <typename T, bool NT>
+__global__ __launch_bounds__(MC_TILE) void k_mc_march_bricks(McBody<T> B, uint32_t nwork) {
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (w >= nwork) return;
+    __shared__ MarchLDS<T> L;
+    __shared__ T sph[4 * MARCH_SPH];
+    __shared__ BrickLists<T> bl;
+    mc_march_item<T, true, NT, true>(B, w, L, sph, &bl);
+}
+
 template <typename T>
 __global__ __launch_bounds__(MC_CUT_THREADS) void k_mc_cut(const T *__restrict__ phi, GridT<T> g, Cols<T> cols,
                                                            const uint32_t *__restrict__ cutlist,
@@ -103,7 +116,8 @@ void launch_sample(T *phi, const GridT<T> &g, const Geo &geo, int dim, hipStream
 
 template <typename T>
 void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &cols,
-                      uint32_t *cutlist, uint32_t *counter, const McTuning &tune, hipStream_t s) {
+                      uint32_t *cutlist, uint32_t *counter, const McTuning &tune, hipStream_t s,
+                      const BrickPhi *bp) {
     const uint32_t ncell = (uint32_t)((uint64_t)g.n[0] * g.n[1] * g.n[2]);
     if (tune.variant == MC_FLAT || (geo == nullptr && tune.variant == MC_MARCH_FUSED)) {
         const uint32_t ntiles = (ncell + MC_TILE - 1) / MC_TILE;
@@ -123,9 +137,20 @@ void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &
     B.cols = cols;
     B.cutlist = cutlist;
     B.counter = counter;
-    mc_tile_shape(g.n, tune.kz > 0 ? tune.kz : 16, &B.W, &B.R, &B.kz, &B.nseg, &B.nrg, &B.nkb);
     B.fused = tune.variant == MC_MARCH_FUSED && geo != nullptr;
     B.nontemporal = tune.nontemporal;
+    B.bp = bp ? *bp : BrickPhi{};
+    if (B.fused && bp) {
+        mc_brick_tile_shape(g.n, tune.kz > 0 ? tune.kz : 16, &B.W, &B.R, &B.kz, &B.nseg, &B.nrg, &B.nkb);
+        const uint32_t nwork = B.nseg * B.nrg * B.nkb;
+        const uint32_t blocks = ((nwork + 7) / 8) * 8;
+        if (tune.nontemporal)
+            hipLaunchKernelGGL((k_mc_march_bricks<T, true>), dim3(blocks), dim3(MC_TILE), 0, s, B, nwork);
+        else
+            hipLaunchKernelGGL((k_mc_march_bricks<T, false>), dim3(blocks), dim3(MC_TILE), 0, s, B, nwork);
+        return;
+    }
+    mc_tile_shape(g.n, tune.kz > 0 ? tune.kz : 16, &B.W, &B.R, &B.kz, &B.nseg, &B.nrg, &B.nkb);
     const uint32_t nwork = B.nseg * B.nrg * B.nkb;
     const uint32_t blocks = ((nwork + 7) / 8) * 8;
 #define CC_LAUNCH_MARCH(F, N) \
@@ -156,6 +181,21 @@ void mc_tile_shape(const int *n, uint32_t kz, uint32_t *W, uint32_t *R, uint32_t
     *nkb = (n2 + kz - 1) / kz;
 }
 
+void mc_brick_tile_shape(const int *n, uint32_t kz, uint32_t *W, uint32_t *R, uint32_t *KZ, uint32_t *nseg,
+                         uint32_t *nrg, uint32_t *nkb) {
+    const uint32_t n0 = n[0], n1 = n[1], n2 = n[2];
+    *W = n0 < (uint32_t)MC_BRICK_W ? n0 : (uint32_t)MC_BRICK_W;
+    uint32_t r = PHI_BRICK;  // rows: a power of two dividing the brick edge, W R <= MC_TILE
+    while (r > 1 && r * *W > (uint32_t)MC_TILE) r >>= 1;
+    *R = n1 < r ? n1 : r;
+    uint32_t k = PHI_BRICK;  // the largest divisor of the brick edge not above kz
+    while (k > 1 && k > kz) k >>= 1;
+    *KZ = k;
+    *nseg = (n0 + *W - 1) / *W;
+    *nrg = (n1 + *R - 1) / *R;
+    *nkb = (n2 + k - 1) / k;
+}
+
 void launch_zero_u32(uint32_t *p, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, p);
 }
@@ -164,9 +204,9 @@ template void launch_sample<double>(double *, const GridT<double> &, const Geo &
 template void launch_sample<float>(float *, const GridT<float> &, const Geo &, int, hipStream_t);
 template void launch_mc_stream<double>(double *, const GridT<double> &, const Geo *,
                                        const Cols<double> &, uint32_t *, uint32_t *,
-                                       const McTuning &, hipStream_t);
+                                       const McTuning &, hipStream_t, const BrickPhi *);
 template void launch_mc_stream<float>(float *, const GridT<float> &, const Geo *, const Cols<float> &,
-                                      uint32_t *, uint32_t *, const McTuning &, hipStream_t);
+                                      uint32_t *, uint32_t *, const McTuning &, hipStream_t, const BrickPhi *);
 template void launch_mc_cut<double>(const double *, const GridT<double> &, const Cols<double> &,
                                     const uint32_t *, const uint32_t *, hipStream_t);
 template void launch_mc_cut<float>(const float *, const GridT<float> &, const Cols<float> &,

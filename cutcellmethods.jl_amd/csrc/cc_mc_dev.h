@@ -4,6 +4,7 @@
 // z-marching stream pass and of the cut-cell pass.  Included by exactly one place per translation
 // unit; every symbol is internal to that unit.
 #pragma once
+#include "cc_brick_dev.h"
 #include "cc_internal.h"
 
 #define CC_LUT_QUAL __constant__
@@ -581,6 +582,70 @@ __device__ __forceinline__ void stream_tile(const Cols<T> &cols, uint32_t c0, in
 }
 
 
+// One column of a tile of rc rows of wc cells, the rows rowstride cells apart: [rc x wc cells x M
+// components] from `f(q)` (q = flat element index within the tile, rows in order), 16-byte
+// lane-contiguous stores within each row when a row holds whole vectors.
+template <typename T, int M, bool NT, typename F>
+__device__ __forceinline__ void stream_column_rows(T *__restrict__ dst, int wc, int rc, size_t rowstride, F f) {
+    constexpr int V = 16 / sizeof(T);
+    typedef typename Vec16<T>::type vec;
+    const int per = wc * M;
+    if (per % V == 0) {
+        const int vper = per / V, nv = vper * rc;
+        for (int e = threadIdx.x; e < nv; e += blockDim.x) {
+            const int r = e / vper, ve = e - r * vper;
+            vec v;
+#pragma unroll
+            for (int u = 0; u < V; ++u) v[u] = f(r * per + ve * V + u);
+            put<NT>(reinterpret_cast<vec *>(dst + (size_t)r * rowstride * M + (size_t)ve * V), v);
+        }
+    } else {
+        for (int q = threadIdx.x; q < per * rc; q += blockDim.x) {
+            const int r = q / per;
+            put<NT>(dst + (size_t)r * rowstride * M + (q - r * per), f(q));
+        }
+    }
+}
+
+// Every column of a tile of rc rows of wc cells starting at cell c0, the rows n0 cells apart (a
+// brick-shaped tile), from the staged tile.  Cut cells get placeholder values, as in stream_tile.
+template <typename T, bool NT>
+__device__ __forceinline__ void stream_tile_rows(const Cols<T> &cols, uint32_t c0, int wc, int rc, uint32_t n0,
+                                                 const TileLDS<T> &s) {
+    const int t = threadIdx.x;
+    {
+        int8_t *dk = cols.kind + c0;
+        uint8_t *da = cols.amb + c0;
+        if (wc % 16 == 0) {
+            const int vper = wc / 16;
+            for (int e = t; e < vper * rc; e += blockDim.x) {
+                const int r = e / vper, ve = e - r * vper;
+                const size_t o = (size_t)r * n0 + 16 * ve;
+                put<NT>(reinterpret_cast<i32x4 *>(dk + o), *reinterpret_cast<const i32x4 *>(s.kind + r * wc + 16 * ve));
+                put<NT>(reinterpret_cast<i32x4 *>(da + o), i32x4{0, 0, 0, 0});
+            }
+        } else {
+            for (int q = t; q < wc * rc; q += blockDim.x) {
+                const int r = q / wc;
+                const size_t o = (size_t)r * n0 + (q - r * wc);
+                dk[o] = s.kind[q];
+                da[o] = 0;
+            }
+        }
+    }
+    stream_column_rows<T, 1, NT>(cols.vf + c0, wc, rc, n0, [&](int q) { return s.kind[q] == CELL_OUTSIDE ? T(1) : T(0); });
+    stream_column_rows<T, 3, NT>(cols.centroid + 3 * (size_t)c0, wc, rc, n0, [&](int q) { return s.cen[q]; });
+    stream_column_rows<T, 6, NT>(cols.ff + 6 * (size_t)c0, wc, rc, n0,
+                                 [&](int q) { return s.kind[q / 6] == CELL_OUTSIDE ? T(1) : T(0); });
+    stream_column_rows<T, 12, NT>(cols.fcl + 12 * (size_t)c0, wc, rc, n0, [&](int q) {
+        const int cell = q / 12, r = q - 12 * cell, dir = r >> 1, w = r & 1;
+        const int ax = dir >> 1;
+        const int a = w == 0 ? (ax == 0 ? 1 : 0) : (ax == 2 ? 1 : 2);
+        return s.hl[3 * cell + a];
+    });
+    stream_column_rows<T, 3, NT>(cols.ic + 3 * (size_t)c0, wc, rc, n0, [&](int q) { return s.cen[q]; });
+}
+
 // A marching-cubes cache as the kernels see it: one body of a batch, or the one cache of an update.
 template <typename T> struct McBody {
     GridT<T> g;
@@ -593,9 +658,20 @@ template <typename T> struct McBody {
     uint32_t nseg, nrg, nkb;  // tiles along x, row groups along y, layer blocks along z
     uint32_t fused;           // sample the body in-kernel (phi written) or read phi
     uint32_t nontemporal;
+    BrickPhi bp;              // a sphere union's culled or surrogate phi (the BRICKS items)
 };
 
 constexpr int MARCH_NODES = 2 * (MC_TILE + 1);  // max (R+1)(W+1) over R*W <= MC_TILE
+constexpr int MARCH_SPH = 256;  // kept primitives a brick-aligned item stages in LDS (more: read from memory)
+constexpr int MARCH_BX = MC_TILE / PHI_BRICK;  // bricks along x a brick-aligned item may span
+
+// The kept lists of the bricks a brick-shaped item spans along x (in LDS): count, list, and the LDS
+// copy of the primitives (null when they did not fit).
+template <typename T> struct BrickLists {
+    uint32_t nk[MARCH_BX];
+    const uint16_t *L[MARCH_BX];
+    T *sph[MARCH_BX];
+};
 
 // Cut cells of up to CUT_STEPS z-steps of one work item, held as the waves' ballot masks and
 // appended to the work list with ONE atomic per flush: a single global counter taking one atomic
@@ -612,6 +688,7 @@ template <typename T> struct MarchLDS {
     TileLDS<T> s;
     T rows[2][MARCH_NODES];  // [layer parity][(R+1) x (W+1) nodes, row-major]
     CutMasks cm;
+    uint32_t st[5];          // BRICKS: owned nodes, exact, uncertified, band, contradicted
 };
 
 // Appends the cut cells recorded in cm (steps [0, ns)) to the work list.  Called by every thread
@@ -643,10 +720,12 @@ __device__ __forceinline__ void flush_cuts(CutMasks &cm, int ns, int wc, uint32_
 // One work item of the z-marching stream pass: the R x W cells of rows [j0, j0+R) and columns
 // [i0, i0+W), marched over the layers [k0, k0+kz).  The node rows of the layer below stay in LDS, so
 // each step brings in only the layer above: computed from the body (FUSED: owned nodes written to
-// phi exactly once, phi never re-read) or read from phi.  The cells of one step are one contiguous
-// range of every column (R > 1 only for whole rows).
-template <typename T, bool FUSED, bool NT>
-__device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L) {
+// phi exactly once, phi never re-read; BRICKS: a sphere union's culled or surrogate phi from its
+// bricks, with the counters of the owned nodes written to B.bp.wstat[w]) or read from phi.  The
+// cells of one step are one contiguous range of every column (R > 1 only for whole rows).
+template <typename T, bool FUSED, bool NT, bool BRICKS = false>
+__device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L, T *sph = nullptr,
+                              BrickLists<T> *bl = nullptr) {
     const GridT<T> &g = B.g;
     const uint32_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     const uint32_t seg = w % B.nseg, rest = w / B.nseg, rg = rest % B.nrg, kb = rest / B.nrg;
@@ -657,6 +736,44 @@ __device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L) {
     const int t = threadIdx.x;
     const size_t sy = n0 + 1, sz = sy * (n1 + 1);
 
+    // BRICKS: the item spans BX = ceil(wc / PHI_BRICK) bricks along x and one brick along y and z
+    // (i0 a multiple of PHI_BRICK, j0 and k0 multiples of R and kz, which divide it), so the kept
+    // list of brick bx, exact over its box extended by one node, serves every node of the item with
+    // (gi - i0) / PHI_BRICK == bx (the last brick also the halo node i0 + wc); the lists are staged in
+    // LDS one after another while they fit in MARCH_SPH primitives, the rest read from memory
+    int nbxi = 1;
+    if (BRICKS) {
+        nbxi = (wc + PHI_BRICK - 1) / PHI_BRICK;
+        const uint32_t brow = (uint32_t)B.bp.nbx * (j0 / PHI_BRICK + (uint32_t)B.bp.nby * (k0 / PHI_BRICK));
+        if (t < nbxi) {
+            const uint32_t b = i0 / PHI_BRICK + t + brow;
+            bl->nk[t] = B.bp.cnt[b];
+            bl->L[t] = B.bp.list + (size_t)b * B.bp.nprims;
+        }
+        __syncthreads();
+        if (t == 0) {  // the lists one after another in the LDS budget; those past it stay in memory
+            uint32_t off = 0;
+            for (int bx = 0; bx < nbxi; ++bx) {
+                const bool fits = off + bl->nk[bx] <= (uint32_t)MARCH_SPH;
+                bl->sph[bx] = fits ? sph + 4 * (size_t)off : nullptr;
+                off += fits ? bl->nk[bx] : 0u;
+            }
+        }
+        __syncthreads();
+        for (int bx = 0; bx < nbxi; ++bx) {
+            T *dst = bl->sph[bx];
+            if (!dst) continue;
+            for (uint32_t e = t; e < bl->nk[bx]; e += blockDim.x) {
+                const double *q = B.geo.prims + 4 * (size_t)bl->L[bx][e];
+                dst[4 * e] = T(q[0]);
+                dst[4 * e + 1] = T(q[1]);
+                dst[4 * e + 2] = T(q[2]);
+                dst[4 * e + 3] = T(q[3]);
+            }
+        }
+        __syncthreads();
+    }
+    uint32_t st_own = 0, st_exact = 0, st_unc = 0, st_band = 0, st_sign = 0;  // BRICKS, owned nodes
     auto fill = [&](int par, uint32_t k) {
         const bool own_layer = k < k1 || k == n2;
         for (int v = t; v < nn; v += blockDim.x) {
@@ -667,9 +784,25 @@ __device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L) {
             if (FUSED) {
                 const T x[3] = {g.x0[0] + T(gi) * g.d[0], g.x0[1] + T(gj) * g.d[1],
                                 g.x0[2] + T(k) * g.d[2]};
-                val = sdf_value<T, 3, false>(B.geo, x);
+                int how = NODE_EXACT;
+                if (BRICKS) {
+                    int bx = ii / PHI_BRICK;
+                    bx = bx < nbxi ? bx : nbxi - 1;
+                    val = brick_node_phi<T>(B.bp, B.geo, gi, gj, k, x, &how, bl->sph[bx], bl->nk[bx], bl->L[bx]);
+                } else {
+                    val = sdf_value<T, 3, false>(B.geo, x);
+                }
                 const bool own = own_layer && (r < rc || gj == n1) && (ii < wc || gi == n0);
-                if (own) B.phi[idx] = val;
+                if (own) {
+                    B.phi[idx] = val;
+                    if (BRICKS) {
+                        st_own += 1u;
+                        st_exact += how != NODE_NET ? 1u : 0u;
+                        st_unc += how == NODE_EXACT && B.bp.cert ? 1u : 0u;
+                        st_band += how == NODE_BAND ? 1u : 0u;
+                        st_sign += how == NODE_CONTRADICTED ? 1u : 0u;
+                    }
+                }
             } else {
                 val = B.phi[idx];
             }
@@ -699,13 +832,30 @@ __device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L) {
         if (t == 0) L.cm.c0[ns] = c0;
         ++ns;
         __syncthreads();
-        stream_tile<T, NT>(B.cols, c0, tc, L.s);
+        if (BRICKS) stream_tile_rows<T, NT>(B.cols, c0, wc, rc, n0, L.s);
+        else stream_tile<T, NT>(B.cols, c0, tc, L.s);
         __syncthreads();
         if (ns == CUT_STEPS || k + 1 == k1) {
             flush_cuts(L.cm, ns, wc, n0, B.cutlist, B.counter);
             ns = 0;
         }
         par ^= 1;
+    }
+    if (BRICKS) {  // the item's counters, laid out as the phi pass's per-brick ones
+        if (t < 5) L.st[t] = 0u;
+        __syncthreads();
+        const uint32_t v5[5] = {st_own, st_exact, st_unc, st_band, st_sign};
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            uint32_t v = v5[q];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if ((t & 63) == 0 && v) atomicAdd(&L.st[q], v);
+        }
+        __syncthreads();
+        if (t < PHI_BSTAT) {
+            const uint32_t rec[PHI_BSTAT] = {L.st[0], L.st[1], 0u, L.st[2], L.st[3], L.st[4], 0u, 0u};
+            B.bp.wstat[(size_t)w * PHI_BSTAT + t] = rec[t];
+        }
     }
 }
 

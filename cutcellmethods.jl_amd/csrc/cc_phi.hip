@@ -13,12 +13,11 @@
 // certified sub-brick a node keeps the network's value unless |phi_nn| is under the band or its
 // sign contradicts the certificate; then it is evaluated exactly too.  Every node's sign is thus
 // the exact one, `kind` is identical everywhere and every cut cell sees its exact corners.
-#include "cc_internal.h"
+#include "cc_brick_dev.h"
 
 namespace cc {
 namespace {
 
-constexpr int CULL_LDS = 512;  // primitives staged in LDS per brick (more are read from memory)
 constexpr int BRICK_NODES = PHI_BRICK * PHI_BRICK * PHI_BRICK;
 
 __device__ __forceinline__ void brick_of(int b, int nbx, int nby, int *bi) {
@@ -29,16 +28,20 @@ __device__ __forceinline__ void brick_of(int b, int nbx, int nby, int *bi) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_cull(GridT<T> g, Geo geo, T delta, int nbx, int nby,
-                                              uint16_t *__restrict__ list, uint32_t *__restrict__ cnt) {
+                                              uint16_t *__restrict__ list, uint32_t *__restrict__ cnt,
+                                              int8_t *__restrict__ cert, T delta_c, uint32_t *__restrict__ bstat) {
     __shared__ T wmin[4];
     __shared__ uint32_t wcnt[4];
+    __shared__ uint32_t ncert;
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     int bi[3];
     brick_of(b, nbx, nby, bi);
+    // the brick's box extended by one node layer on its upper sides: the kept list is then exact for
+    // every node a brick-shaped stream tile touches, its halo included
     T lo[3], hi[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int i0 = bi[a] * PHI_BRICK, i1 = min(i0 + PHI_BRICK - 1, g.n[a]);
+        const int i0 = bi[a] * PHI_BRICK, i1 = min(i0 + PHI_BRICK, g.n[a]);
         lo[a] = g.x0[a] + T(i0) * g.d[a];
         hi[a] = g.x0[a] + T(i1) * g.d[a];
     }
@@ -93,43 +96,21 @@ __global__ __launch_bounds__(256) void k_cull(GridT<T> g, Geo geo, T delta, int 
         __syncthreads();
     }
     if (t == 0) cnt[b] = total;
-}
-
-// The sphere-union fold over a brick's kept primitives, in the order and with the arithmetic of
-// union_dist (cc_device.h, fold_sphere).  Primitive 0 is always kept.
-template <typename T>
-__device__ __forceinline__ T culled_fold(const T *sph, uint32_t nk, const uint16_t *L, const Geo &geo,
-                                         T x, T y, T z) {
-    const T k = T(geo.p[0]);
-    T acc;
-    if (nk <= (uint32_t)CULL_LDS) {
-        acc = sphere_dist<T, 3>(sph[0], sph[1], sph[2], sph[3], x, y, z);
-        for (uint32_t e = 1; e < nk; ++e)
-            acc = fold_sphere<T, 3>(acc, sph[4 * e], sph[4 * e + 1], sph[4 * e + 2], sph[4 * e + 3], x, y, z, k);
-    } else {
-        const double *s = geo.prims + 4 * (size_t)L[0];
-        acc = sphere_dist<T, 3>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), x, y, z);
-        for (uint32_t e = 1; e < nk; ++e) {
-            s = geo.prims + 4 * (size_t)L[e];
-            acc = fold_sphere<T, 3>(acc, T(s[0]), T(s[1]), T(s[2]), T(s[3]), x, y, z, k);
-        }
+    if (!bstat) return;
+    // the sign certificates of the brick's 64 sub-bricks (the kept list is complete: every thread
+    // of the workgroup wrote its entries before the last barrier)
+    if (t == 0) ncert = 0;
+    __syncthreads();
+    if (cert && t < 64) {
+        bool valid;
+        cert[64 * (size_t)b + t] =
+            sub_brick_sign<T>(g, geo, (const T *)nullptr, total, out, bi[0] * PHI_BRICK, bi[1] * PHI_BRICK,
+                              bi[2] * PHI_BRICK, t, delta_c, &valid);
+        const unsigned long long m = __ballot(valid);
+        if (t == 0) ncert = (uint32_t)__popcll(m);
     }
-    return acc;
-}
-
-__device__ __forceinline__ float nn_trilinear(const SurrogateArgs &sa, int gi, int gj, int gk) {
-    const int f = sa.f;
-    const int I = gi / f, J = gj / f, K = gk / f;
-    const float fx = (float)(gi - f * I) / (float)f, fy = (float)(gj - f * J) / (float)f,
-                fz = (float)(gk - f * K) / (float)f;
-    const size_t sx = 1, sy = (size_t)sa.nc[0], sz = (size_t)sa.nc[0] * sa.nc[1];
-    const float *p = sa.nn + I * sx + J * sy + K * sz;
-    const float c00 = p[0] + fx * (p[sx] - p[0]);
-    const float c10 = p[sy] + fx * (p[sy + sx] - p[sy]);
-    const float c01 = p[sz] + fx * (p[sz + sx] - p[sz]);
-    const float c11 = p[sz + sy] + fx * (p[sz + sy + sx] - p[sz + sy]);
-    const float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
-    return c0 + fz * (c1 - c0);
+    __syncthreads();
+    if (t < PHI_BSTAT) bstat[(size_t)b * PHI_BSTAT + t] = t == 2 ? ncert : 0u;
 }
 
 template <typename T, bool SURR>
@@ -160,29 +141,10 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
     __syncthreads();
     const size_t m0 = (size_t)g.n[0] + 1, m1 = (size_t)g.n[1] + 1;
     if (SURR && t < 64) {
-        // sign certificate of sub-brick t
-        const int s3[3] = {t & 3, (t >> 2) & 3, t >> 4};
-        const int base[3] = {i0, j0, k0};
-        bool valid = true;
-        T c[3], e2 = T(0), dg = T(0);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int slo = base[a] + PHI_SUB * s3[a];
-            const int shi = min(slo + PHI_SUB - 1, g.n[a]);
-            valid = valid && slo <= g.n[a];
-            const T xl = g.x0[a] + T(slo) * g.d[a], xh = g.x0[a] + T(shi) * g.d[a];
-            c[a] = (xl + xh) / T(2);
-            const T ea = fabs(c[a] - xl) > fabs(xh - c[a]) ? fabs(c[a] - xl) : fabs(xh - c[a]);
-            e2 = e2 + ea * ea;
-            dg = dg + g.d[a] * g.d[a];
-        }
-        int8_t sg = 0;
-        if (valid) {
-            const T pc = culled_fold<T>(sph, nk, L, geo, c[0], c[1], c[2]);
-            if (fabs(pc) - sqrt(e2) - sqrt(dg) > delta_c) sg = pc > T(0) ? 1 : -1;
-            atomicAdd(&st_cert, 1u);
-        }
-        ssign[t] = sg;
+        bool valid;
+        ssign[t] = sub_brick_sign<T>(g, geo, nk <= (uint32_t)CULL_LDS ? sph : nullptr, nk, L, i0, j0, k0, t, delta_c,
+                                     &valid);
+        if (valid) atomicAdd(&st_cert, 1u);
     }
     if (SURR) __syncthreads();
     // classify: nodes the surrogate may keep are written now; the rest are queued for the fold
@@ -199,9 +161,10 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
                 ++n_unc;
             } else {
                 const float v = nn_trilinear(sa, gi, gj, gk);
-                if (fabsf(v) < sa.band) {
+                const int how = keep_network(sa, sg, v);
+                if (how == NODE_BAND) {
                     ++n_band;
-                } else if ((v > 0.0f) != (sg > 0)) {
+                } else if (how == NODE_CONTRADICTED) {
                     ++n_sign;
                 } else {
                     phi[gi + m0 * (gj + m1 * gk)] = T(v);
@@ -228,7 +191,7 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
         const int idx = queue[e];
         const int gi = i0 + (idx & 15), gj = j0 + ((idx >> 4) & 15), gk = k0 + (idx >> 8);
         const T x = g.x0[0] + T(gi) * g.d[0], y = g.x0[1] + T(gj) * g.d[1], z = g.x0[2] + T(gk) * g.d[2];
-        phi[gi + m0 * (gj + m1 * gk)] = culled_fold<T>(sph, nk, L, geo, x, y, z);
+        phi[gi + m0 * (gj + m1 * gk)] = culled_fold<T>(nk <= (uint32_t)CULL_LDS ? sph : nullptr, nk, L, geo, x, y, z);
     }
     if (t < PHI_BSTAT) {  // this brick's counters (summed on the host when asked for)
         const uint32_t v[PHI_BSTAT] = {st_nodes, nq, st_cert, st_uncert, st_band, st_sign, 0u, 0u};
@@ -249,11 +212,12 @@ template <typename T> static T cert_margin(double scale, int nprims) {
 
 template <typename T>
 void launch_cull(const GridT<T> &g, const Geo &geo, double scale, uint16_t *list, uint32_t *cnt,
-                 hipStream_t s) {
+                 int8_t *cert, uint32_t *bstat, hipStream_t s) {
     int nb[3];
     phi_brick_counts(g.n, nb);
     hipLaunchKernelGGL((k_cull<T>), dim3(nb[0] * nb[1] * nb[2]), dim3(256), 0, s, g, geo,
-                       cull_margin<T>(scale), nb[0], nb[1], list, cnt);
+                       cull_margin<T>(scale), nb[0], nb[1], list, cnt, cert, cert_margin<T>(scale, geo.nprims),
+                       bstat);
 }
 
 template <typename T>
@@ -271,10 +235,10 @@ void launch_phi_bricks(T *phi, const GridT<T> &g, const Geo &geo, const uint16_t
                            nb[0], nb[1], bstat);
 }
 
-template void launch_cull<double>(const GridT<double> &, const Geo &, double, uint16_t *, uint32_t *,
-                                  hipStream_t);
-template void launch_cull<float>(const GridT<float> &, const Geo &, double, uint16_t *, uint32_t *,
-                                 hipStream_t);
+template void launch_cull<double>(const GridT<double> &, const Geo &, double, uint16_t *, uint32_t *, int8_t *,
+                                  uint32_t *, hipStream_t);
+template void launch_cull<float>(const GridT<float> &, const Geo &, double, uint16_t *, uint32_t *, int8_t *,
+                                 uint32_t *, hipStream_t);
 template void launch_phi_bricks<double>(double *, const GridT<double> &, const Geo &, const uint16_t *,
                                         const uint32_t *, const SurrogateArgs &, uint32_t *, hipStream_t);
 template void launch_phi_bricks<float>(float *, const GridT<float> &, const Geo &, const uint16_t *,
