@@ -141,6 +141,9 @@ int cc_stream(cc_cache *cache, void **stream);
  *   variant 0 = flat tiles of 256 consecutive cells reading phi,
  *           1 = z-march of 256-cell row runs over kz layers reading phi,
  *           2 = z-march sampling an analytic body in-kernel (phi written once, never re-read);
+ *               for a sphere union with a culled or surrogate phi source it evaluates that phi
+ *               in-kernel from the bricks (16 x 16-cell tiles, kz rounded down to a power of two
+ *               dividing 16) instead of running the separate brick pass;
  *   nontemporal != 0 streams the column stores past the caches.  For a 3D PLIC cache, variant 0
  *   is the reference's two passes (fit, then faces) and any other value the fused single sweep
  *   (fit + resolved faces, kz layers per work item).  Results are identical for every setting;
@@ -357,8 +360,9 @@ int cc_mlp_train(cc_mlp *net, const cc_geometry *geo, int64_t nsamples, float lo
 /* How cc_update produces a sphere union's nodal phi in a marching-cubes cache (sample_nodes,
  * src/marching_cubes/cache.jl:33-40, folds the whole body at every node):
  *   CC_PHI_NAIVE     -- every node folds every primitive (the plain exact path; the default);
- *   CC_PHI_CULLED    -- each 16^3-node brick folds only the primitives a bound over its box cannot
- *                       prove inert: bitwise the same phi;
+ *   CC_PHI_CULLED    -- each 16^3-node brick folds only the primitives a bound over its box
+ *                       (extended by one node on its upper sides) cannot prove inert: bitwise
+ *                       the same phi;
  *   CC_PHI_SURROGATE -- the network `net` on a lattice of spacing `coarse` cells, interpolated;
  *                       exact (culled) wherever a Lipschitz sign certificate is missing, where
  *                       |phi_nn| < band_cells * h, or where the network's sign contradicts the
