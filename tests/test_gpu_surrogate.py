@@ -47,11 +47,12 @@ def test_forward_matches_the_bf16_oracle():
     assert err_e < err_d
 
 
-def test_loss_and_gradients_match_the_bf16_oracle():
+@pytest.mark.parametrize("n", [512, 64])  # 64: one forward tile, one or two point chunks per dW tile
+def test_loss_and_gradients_match_the_bf16_oracle(n):
     rng = np.random.default_rng(1)
     net = Surrogate(nfreq=64, depth=4, sigma=2.0, seed=3)
     p, B = net.params()
-    x = rng.uniform(-1, 1, (512, 3)).astype(np.float32)
+    x = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
     y = (np.linalg.norm(x, axis=1) - 0.5).astype(np.float32)
     loss, g = net.loss_grad(x, y)
     lo, go = mo.loss_grad(p, B, x, y, 64, 256, 4, emulate=True)
@@ -87,6 +88,22 @@ def test_training_reduces_the_loss_and_fits_the_body():
     err = np.sqrt(np.mean((net.eval(x.astype(np.float32)) - exact) ** 2))
     print(f"held-out rms error {err:.3e}")
     assert err < 0.02
+
+
+def test_training_is_deterministic():
+    # no atomics anywhere in the step: the weight gradients are summed from per-chunk partial
+    # tiles and the other gradients from per-workgroup sums, in fixed orders, and the samples and
+    # the epochs' orders come from counter-based generators -- two runs agree to the last bit
+    body = csg_spheres(5, count=16, lo=-0.5, hi=0.5, rmin=0.1, rmax=0.25)
+    runs = []
+    for _ in range(2):
+        net = Surrogate(nfreq=64, depth=4, sigma=4.0, seed=11)
+        hist = net.train(body, nsamples=1 << 16, band=0.05, epochs=3, batch=4096, lr=5e-3, seed=2)
+        runs.append((hist, net.params()[0]))
+        net.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1].view(np.uint32), runs[1][1].view(np.uint32))
+    assert runs[0][0][-1] < runs[0][0][0]
 
 
 def union_case(n=(70, 45, 33), count=64, seed=4):
@@ -265,6 +282,27 @@ def test_phi_source_argument_errors():
     m = allocate_cache(g2, MarchingSquaresCutCell)
     with pytest.raises(Unsupported):
         m.set_phi_source("culled")
+
+
+def test_fused_phi_statistics_equal_the_brick_pass():
+    # the fused stream pass counts every node once (the owned ones of its work items) and the cull
+    # pass counts the certificates: the same counters as the separate brick pass, phi the same bits
+    body = csg_spheres(9, count=48, lo=-0.6, hi=0.6, rmin=0.08, rmax=0.25)
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=(45, 38, 70))
+    net = Surrogate(seed=4)
+    net.train(body, nsamples=1 << 16, epochs=3, batch=4096, seed=1)
+    for source in ("culled", "surrogate"):
+        out = []
+        for tune in ((1, 8, 0), (2, 8, 0), (2, 16, 1)):
+            c = allocate_cache(g, MarchingCubesCutCell)
+            c.set_tuning(*tune)
+            c.set_phi_source(source, net if source == "surrogate" else None, coarse=4)
+            update_cache(c, body, g)
+            out.append((c.phi_stats(), c.phi.copy()))
+        for st, phi in out[1:]:
+            assert st == out[0][0], (source, st, out[0][0])
+            assert np.array_equal(phi.view(np.uint64), out[0][1].view(np.uint64)), source
+        assert out[0][0]["nodes"] == int(np.prod(np.array(g.n) + 1))
 
 
 @pytest.mark.parametrize("tune", [(0, 16, 0), (1, 3, 0), (1, 16, 1), (2, 64, 0), (2, 100, 0)])
