@@ -181,6 +181,10 @@ def secondary(seed, cpu=True):
     fill_cols, fill_all = own_fill(c3, ("kind", "volume_fraction", "face_fraction", "normals", "intercepts",
                                         "is_valid"))
     c3.set_profiling(True)
+    from cutcell.autotune import tune_stream
+    # the sweep's z-depth and store kind, tuned on this GPU like the headline's stream shape
+    ptune = tune_stream(c3, tor, g3, candidates=[(2, 8, False), (2, 16, False), (2, 16, True), (2, 32, False)],
+                        repeats=2, use_cache=False)
     for _ in range(3):
         update_cache(c3, tor, g3)
     kts = []
@@ -196,6 +200,7 @@ def secondary(seed, cpu=True):
     sweep_ms = kt.get("plic_fused")
     out["plic_384"] = {"update_ms": round(t_p * 1e3, 3), "cells_per_s": n3 / t_p,
                        "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+                       "sweep_tuning": [ptune["variant"], ptune["kz"], ptune["nontemporal"]],
                        "algorithmic_bytes_fused": fused, "algorithmic_bytes_two_pass": two_pass,
                        "hbm_frac_two_pass": round(two_pass / t_p / 1e9 / HBM_PEAK_GBS, 4),
                        "cut_cells": int((c3.read("kind") == 2).sum()),

@@ -2,6 +2,7 @@
 // marching-squares contour.  Mirrors the reference's cache API (src/cache.jl:15-121,
 // src/marching_squares/cache.jl, src/marching_cubes/cache.jl, src/plic/cache.jl) with status codes
 // in place of exceptions.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -41,19 +42,26 @@ template <typename T> __global__ void k_fill(T *p, size_t n, T v) {
 
 constexpr int MAX_TIMED = 8;
 
-// Calibration kernels for the bandwidth probe: a write-only fill and a read+write copy, both with
-// 16-byte lane-contiguous accesses in a grid-stride loop.
+// Calibration kernels for the bandwidth probe: a write-only fill and a read+write copy, both one-shot
+// -- one 16-byte element per thread, a workgroup per 4 KiB, as many workgroups as the buffer needs.
+// Of the write patterns measured on MI355X this one reached the highest rate (6.9 TB/s on 2 and
+// 12 GiB buffers, where grid-stride loops of 2048-32768 workgroups reached 4.3-5.9 TB/s and two or
+// four stores per thread 3.7-6.2 TB/s), so the probe's rate is the yardstick a write stream is
+// read against.
 typedef double __attribute__((ext_vector_type(2))) f64x2;
+// Buffers beyond 32 GiB (2^31 elements) take a capped grid and loop, which keeps a launch's
+// work-items under 2^32.
+constexpr size_t PROBE_MAX_BLOCKS = size_t(1) << 23;
 __global__ __launch_bounds__(256) void k_probe_fill(f64x2 *dst, size_t n, double v) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = f64x2{v, v};
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = f64x2{v, v};
 }
 __global__ __launch_bounds__(256) void k_probe_copy(f64x2 *dst, const f64x2 *src, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
+// Workgroups of the probe kernels for n 16-byte elements: one element per thread up to the cap.
+inline unsigned probe_blocks(size_t n) { return (unsigned)std::min((n + 255) / 256, PROBE_MAX_BLOCKS); }
 
 // Write-pattern variants of the fill probe: 0 grid-stride, 1 grid-stride non-temporal,
 // 2 one contiguous 64 KiB run per workgroup iteration, 3 grid-stride with 4 stores in flight.
@@ -1395,9 +1403,9 @@ int cc_cache_create(int method, int dtype, const cc_grid *grid, cc_cache **out) 
     c->ncell = ncell;
     c->nnode = nnode;
     // The fused PLIC sweep fits the layer below and above each work item's z-range again, so it
-    // marches deeper than the nodal stream pass (measured on the 384^3 torus: kz 4 1.67 ms, 16
-    // 1.48 ms, 32 1.46 ms, 64 1.48 ms).
-    if (method == CC_PLIC) c->tune.kz = 32;
+    // marches deeper than the nodal stream pass (measured on the 384^3 torus: kz 1 2.09 ms, 2 1.46,
+    // 4 1.20, 8 1.11, 16 1.08, 32 1.13, 64 1.36 ms).
+    if (method == CC_PLIC) c->tune.kz = 16;
     (void)hipGetDevice(&c->device);
     int rc = check_grid(c, grid);
     if (rc) { delete c; return rc; }
@@ -1609,7 +1617,8 @@ int cc_probe_fill_column(cc_cache *c, int field, int iters, float *ms) {
         for (int it = 0; it < iters + 1; ++it) {
             float t = 0.0f;
             (void)hipEventRecord(e0, c->stream);
-            hipLaunchKernelGGL(k_probe_fill, dim3(256 * 64), dim3(256), 0, c->stream, (f64x2 *)c->col[field], n, 0.0);
+            hipLaunchKernelGGL(k_probe_fill, dim3(probe_blocks(n)), dim3(256), 0, c->stream, (f64x2 *)c->col[field], n,
+                               0.0);
             (void)hipEventRecord(e1, c->stream);
             (void)hipEventSynchronize(e1);
             (void)hipEventElapsedTime(&t, e0, e1);
@@ -1626,7 +1635,8 @@ int cc_probe_fill_column(cc_cache *c, int field, int iters, float *ms) {
 }
 
 int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms) {
-    if (nbytes < (1 << 20) || iters < 1) return fail(CC_ERR_ARGUMENT, "need nbytes >= 1 MiB, iters >= 1");
+    if (nbytes < (1 << 20) || nbytes > (int64_t(1) << 40) || iters < 1)
+        return fail(CC_ERR_ARGUMENT, "need 1 MiB <= nbytes <= 1 TiB, iters >= 1");
     nbytes &= ~int64_t(4095);
     void *a = nullptr, *b = nullptr;
     hipStream_t st = nullptr;
@@ -1634,7 +1644,7 @@ int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms
     int rc = CC_OK;
     float best_fill = 1e30f, best_copy = 1e30f;
     const size_t n = (size_t)nbytes / 16;
-    const int blocks = 256 * 64;
+    const unsigned blocks = probe_blocks(n);
     if (hipMalloc(&a, nbytes) != hipSuccess || hipMalloc(&b, nbytes) != hipSuccess ||
         hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -1671,7 +1681,9 @@ int cc_probe_bandwidth(int64_t nbytes, int iters, float *fill_ms, float *copy_ms
 }
 
 int cc_probe_write(int64_t nbytes, int iters, int variant, int blocks, float *ms) {
-    if (nbytes < (1 << 20) || iters < 1 || blocks < 1) return fail(CC_ERR_ARGUMENT, "bad probe size");
+    if (nbytes < (1 << 20) || nbytes > (int64_t(1) << 40) || iters < 1 || blocks < 1 ||
+        (size_t)blocks > PROBE_MAX_BLOCKS)
+        return fail(CC_ERR_ARGUMENT, "need 1 MiB <= nbytes <= 1 TiB and 1 <= blocks <= 2^23");
     nbytes &= ~int64_t(4095);
     void *a = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
