@@ -458,7 +458,7 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
         // passes, whose fit kernel carries its evaluation)
         cc::launch_plic_fused<T>(gt, *geo, cols, (T *)c->col[CC_FIELD_NORMALS],
                                  (T *)c->col[CC_FIELD_INTERCEPTS], (uint8_t *)c->col[CC_FIELD_IS_VALID],
-                                 (uint32_t)c->tune.kz, c->stream);
+                                 (uint32_t)c->tune.kz, c->tune.variant == cc::MC_MARCH_FUSED, c->stream);
         mark(c, "plic_fused");
     } else {
         cc::launch_plic_fit<T>(gt, *geo, c->dim, cols, (T *)c->col[CC_FIELD_NORMALS],

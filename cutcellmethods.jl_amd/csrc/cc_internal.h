@@ -99,10 +99,11 @@ constexpr size_t VOL_SCRATCH_PRIMS = VOL_SCRATCH_OUT + 256;
 constexpr size_t VOL_SCRATCH_MAX_PRIM_DOUBLES = (VOL_SCRATCH_BYTES - VOL_SCRATCH_PRIMS) / 8;
 int volume_scratch(void **ptr);
 
-// The fused 3D PLIC pass (fit + resolved faces in one sweep, no read-back).
+// The fused 3D PLIC pass (fit + resolved faces in one sweep, no read-back): `wide` picks 64 x 8
+// tiles with one wave per row, else 32 x 8 tiles.
 template <typename T>
 void launch_plic_fused(const GridT<T> &g, const Geo &geo, const Cols<T> &cols, T *normals,
-                       T *intercepts, uint8_t *valid, uint32_t kz, hipStream_t s);
+                       T *intercepts, uint8_t *valid, uint32_t kz, bool wide, hipStream_t s);
 
 void launch_zero_u32(uint32_t *p, hipStream_t s);
 

@@ -40,6 +40,17 @@ __global__ __launch_bounds__(256) void k_plic_fused(GridT<T> g, Geo geo, Cols<T>
     plic_fused_item<T>(g, geo, cols, normals, intercepts, valid, w, nx, ny, kz, L);
 }
 
+template <typename T>
+__global__ __launch_bounds__(PW_W * PW_R, 4) void k_plic_wide(GridT<T> g, Geo geo, Cols<T> cols, T *normals,
+                                                          T *intercepts, uint8_t *valid, uint32_t nx,
+                                                          uint32_t ny, uint32_t kz, uint32_t nwork) {
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (w >= nwork) return;
+    __shared__ PlicWideLDS<T, PW_R> L;
+    plic_wide_item<T, PW_R>(g, geo, cols, normals, intercepts, valid, w, nx, ny, kz, L);
+}
+
 // calc_volume (src/volume.jl:29-40): every cell's solid fraction from its plane (cell_plane, the fit
 // the PLIC cache stores), summed.  Block b sums a fixed contiguous range of cells, lane t the cells
 // t, t + 256, ... of it, then the block folds its 256 lane sums in a fixed tree; k_sum_parts folds
@@ -134,13 +145,18 @@ template void launch_plic_volume<float>(const GridT<float> &, const Geo &, int, 
 
 template <typename T>
 void launch_plic_fused(const GridT<T> &g, const Geo &geo, const Cols<T> &cols, T *normals,
-                       T *intercepts, uint8_t *valid, uint32_t kz, hipStream_t s) {
-    const uint32_t nx = (g.n[0] + PF_W - 1) / PF_W, ny = (g.n[1] + PF_R - 1) / PF_R;
+                       T *intercepts, uint8_t *valid, uint32_t kz, bool wide, hipStream_t s) {
+    const uint32_t tw = wide ? PW_W : PF_W, tr = wide ? PW_R : PF_R;
+    const uint32_t nx = (g.n[0] + tw - 1) / tw, ny = (g.n[1] + tr - 1) / tr;
     const uint32_t nkb = (g.n[2] + kz - 1) / kz;
     const uint32_t nwork = nx * ny * nkb;
     const uint32_t blocks = ((nwork + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_plic_fused<T>), dim3(blocks), dim3(256), 0, s, g, geo, cols, normals,
-                       intercepts, valid, nx, ny, kz, nwork);
+    if (wide)
+        hipLaunchKernelGGL((k_plic_wide<T>), dim3(blocks), dim3(PW_W * PW_R), 0, s, g, geo, cols, normals,
+                           intercepts, valid, nx, ny, kz, nwork);
+    else
+        hipLaunchKernelGGL((k_plic_fused<T>), dim3(blocks), dim3(256), 0, s, g, geo, cols, normals,
+                           intercepts, valid, nx, ny, kz, nwork);
 }
 
 template <typename T>
@@ -174,9 +190,9 @@ template void launch_plic_fit<double>(const GridT<double> &, const Geo &, int, c
 template void launch_plic_fit<float>(const GridT<float> &, const Geo &, int, const Cols<float> &,
                                      float *, float *, uint8_t *, hipStream_t);
 template void launch_plic_fused<double>(const GridT<double> &, const Geo &, const Cols<double> &,
-                                        double *, double *, uint8_t *, uint32_t, hipStream_t);
+                                        double *, double *, uint8_t *, uint32_t, bool, hipStream_t);
 template void launch_plic_fused<float>(const GridT<float> &, const Geo &, const Cols<float> &, float *,
-                                       float *, uint8_t *, uint32_t, hipStream_t);
+                                       float *, uint8_t *, uint32_t, bool, hipStream_t);
 template void launch_plic_faces<double>(const GridT<double> &, int, const Cols<double> &,
                                         const double *, const double *, const uint8_t *, hipStream_t);
 template void launch_plic_faces<float>(const GridT<float> &, int, const Cols<float> &, const float *,

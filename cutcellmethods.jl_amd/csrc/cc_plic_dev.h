@@ -337,5 +337,130 @@ __device__ void plic_fused_item(const GridT<T> &g, const Geo &geo, const Cols<T>
     }
 }
 
+// Tile of the wide fused 3D PLIC pass: PW_W cells along x, one per lane of a wave, by R rows, one
+// wave per row, marched in z.  Rows of 64 cells write each column in runs twice as long as the
+// 32 x 8 tile's, which the HBM write stream rewards (the same sweep's stores alone, 384^3: 0.88-0.94
+// against 0.98-1.13 ms), and the refitted ring is 144 cells per 512 (80 per 256 for 32 x 8).  Eight
+// rows at 128 VGPRs hold two blocks per CU, whose waves cover each other's barriers (16 rows, one
+// block per CU: 1.08 ms; 8 rows: 1.05 ms; 32 x 8: 1.11 ms, one box, one process).
+constexpr int PW_W = 64, PW_R = 8;
+
+template <typename T, int R> struct PlicWideLDS {
+    // per layer parity: for the row below the tile, the R rows and the row above, each lane's -y
+    // (slot 2) and +y (slot 3) candidates; then the left ring cell's +x and the right ring cell's -x
+    // candidate of each row
+    static constexpr int Y = (R + 2) * 2 * PW_W;
+    static constexpr int N = Y + 2 * R;
+    T v[2][N];
+};
+
+template <int R> __device__ __forceinline__ int wide_y(int row, int s, int lane) {
+    return (row * 2 + s) * PW_W + lane;
+}
+
+// One work item of the wide fused sweep: cells [i0, i0+64) x [j0, j0+R) over the layers
+// [k0, k0+kz).  The same per-cell functions as plic_fused_item, so every column is bitwise the
+// same; only who computes what differs:
+//  - wave r owns row j0 + r and lane l column i0 + l, and carries the planes' candidates up the
+//    z-march in registers (the -z and +z neighbours);
+//  - a layer's fit is stored as soon as it is computed (the next layer's, one step ahead), so only
+//    the candidates are carried;
+//  - the x neighbours' candidates come from the adjacent lanes; the y neighbours' through LDS,
+//    double-buffered by layer parity so one barrier per layer orders them (a wave writing parity p
+//    again at layer k + 2 has passed the barrier of layer k + 1, which every wave reaches only after
+//    its reads of layer k);
+//  - the ring just outside the tile is fitted again: the row below by wave 0, the row above by wave
+//    R - 1, the cells left and right of each row by lanes of wave 1, so three waves fit twice per
+//    layer and the four SIMDs share that work evenly.
+template <typename T, int R>
+__device__ void plic_wide_item(const GridT<T> &g, const Geo &geo, const Cols<T> &cols, T *normals,
+                               T *intercepts, uint8_t *valid, uint32_t w, uint32_t nx, uint32_t ny,
+                               uint32_t kz, PlicWideLDS<T, R> &L) {
+    static_assert(R >= 3 && 2 * R <= PW_W, "the ring duties need waves 0, 1 and R - 1 distinct");
+    constexpr int Y = PlicWideLDS<T, R>::Y;
+    const uint32_t n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    const uint32_t tx = w % nx, rest = w / nx, ty = rest % ny, kb = rest / ny;
+    const uint32_t i0 = tx * PW_W, j0 = ty * R, k0 = kb * kz, k1 = min(k0 + kz, n2);
+    const int wc = (int)min((uint32_t)PW_W, n0 - i0), rc = (int)min((uint32_t)R, n1 - j0);
+    const int lane = threadIdx.x % PW_W, r = threadIdx.x / PW_W;
+    const bool inside = lane < wc && r < rc;
+    const uint32_t i = i0 + lane, j = j0 + r;
+    T fullf[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        T f = T(1);
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            if (b != ax) f *= g.d[b];
+        fullf[ax] = f;
+    }
+    // this lane's ring cell, if any: its LDS slot, the face slot it supplies and its (i, j)
+    int hoff = -1, hslot = 0;
+    uint32_t hi_ = 0, hj = 0;
+    if (r == 0) {
+        if (lane < wc && j0 >= 1) { hoff = wide_y<R>(0, 1, lane); hslot = 3; hi_ = i0 + lane; hj = j0 - 1; }
+    } else if (r == R - 1) {
+        if (lane < wc && j0 + rc < n1) {
+            hoff = wide_y<R>(rc + 1, 0, lane); hslot = 2; hi_ = i0 + lane; hj = j0 + rc;
+        }
+    } else if (r == 1) {
+        if (lane < R) {
+            if (i0 >= 1 && lane < rc) { hoff = Y + lane; hslot = 1; hi_ = i0 - 1; hj = j0 + lane; }
+        } else if (lane < 2 * R) {
+            const int q = lane - R;
+            if (i0 + wc < n0 && q < rc) { hoff = Y + R + q; hslot = 0; hi_ = i0 + wc; hj = j0 + q; }
+        }
+    }
+    T cz_prev = T(0), cc[6], cn[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) cc[q] = cn[q] = T(0);
+    if (inside) {
+        if (k0 >= 1) cz_prev = plane_candidate(plic_plane<T, 3, false>(g, geo, i, j, k0 - 1), g.d, 5, fullf[2]);
+        const Plane<T> P = plic_plane<T, 3, false>(g, geo, i, j, k0);
+        plic_store_fit<T, 3>(P, cols, normals, intercepts, valid, i + (size_t)n0 * (j + (size_t)n1 * k0));
+#pragma unroll
+        for (int q = 0; q < 6; ++q) cc[q] = plane_candidate(P, g.d, q, fullf[q >> 1]);
+    }
+    for (uint32_t k = k0; k < k1; ++k) {
+        T *buf = L.v[k & 1];
+        if (inside && k + 1 < n2) {
+            const Plane<T> P = plic_plane<T, 3, false>(g, geo, i, j, k + 1);
+            if (k + 1 < k1)
+                plic_store_fit<T, 3>(P, cols, normals, intercepts, valid,
+                                     i + (size_t)n0 * (j + (size_t)n1 * (k + 1)));
+#pragma unroll
+            for (int q = 0; q < 6; ++q) cn[q] = plane_candidate(P, g.d, q, fullf[q >> 1]);
+        }
+        if (inside) {
+            buf[wide_y<R>(r + 1, 0, lane)] = cc[2];
+            buf[wide_y<R>(r + 1, 1, lane)] = cc[3];
+        }
+        if (hoff >= 0)
+            buf[hoff] = plane_candidate(plic_plane<T, 3, false>(g, geo, hi_, hj, k), g.d, hslot,
+                                        fullf[hslot >> 1]);
+        __syncthreads();
+        // every lane takes part in the shuffles; lanes past the tile's x edge hold zeros
+        const T left = __shfl_up(cc[1], 1, PW_W), right = __shfl_down(cc[0], 1, PW_W);
+        if (inside) {
+            const T lft = lane == 0 ? buf[Y + r] : left;
+            const T rgt = lane + 1 == wc ? buf[Y + R + r] : right;
+            T ff[6];
+            // _resolved_slot: low faces take the neighbour below's high candidate first
+            ff[0] = i >= 1 ? (lft + cc[0]) / T(2) : cc[0];
+            ff[1] = i + 2 <= n0 ? (cc[1] + rgt) / T(2) : cc[1];
+            ff[2] = j >= 1 ? (buf[wide_y<R>(r, 1, lane)] + cc[2]) / T(2) : cc[2];
+            ff[3] = j + 2 <= n1 ? (cc[3] + buf[wide_y<R>(r + 2, 0, lane)]) / T(2) : cc[3];
+            ff[4] = k >= 1 ? (cz_prev + cc[4]) / T(2) : cc[4];
+            ff[5] = k + 2 <= n2 ? (cc[5] + cn[4]) / T(2) : cc[5];
+            const size_t c = i + (size_t)n0 * (j + (size_t)n1 * k);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) cols.ff[6 * c + q] = ff[q];
+            cz_prev = cc[5];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) cc[q] = cn[q];
+        }
+    }
+}
+
 }  // namespace
 }  // namespace cc

@@ -483,12 +483,16 @@ def test_autotuner_picks_a_shape_and_results_stay_identical(tmp_path, monkeypatc
     assert (tmp_path / "t.json").exists()
 
 
-@pytest.mark.parametrize("variant,kz", [(0, 16), (2, 16), (2, 5), (2, 1)])
-def test_plic_fused_and_two_pass_bitwise(variant, kz):
-    # the fused single sweep and the reference's two passes give the oracle's bits, on a grid whose
-    # extents are not multiples of the 32 x 8 tile or of kz; torus and box (degenerate fits) bodies
+@pytest.mark.parametrize("variant,kz,n", [(0, 16, (70, 37, 29)), (1, 16, (70, 37, 29)), (1, 5, (70, 37, 29)),
+                                         (2, 16, (70, 37, 29)), (2, 5, (70, 37, 29)), (2, 1, (70, 37, 29)),
+                                         (2, 7, (131, 17, 20)), (2, 3, (40, 33, 9)), (2, 4, (64, 9, 6))])
+def test_plic_fused_and_two_pass_bitwise(variant, kz, n):
+    # the fused single sweeps (variant 1: 32 x 8 tiles, variant 2: 64 x 8 tiles, one wave per row)
+    # and the reference's two passes give the oracle's bits, on grids whose extents are not
+    # multiples of the tiles or of kz (a last tile row of one row, a last tile column of three
+    # cells, a grid narrower than one tile); torus and box (degenerate fits) bodies
     rng = np.random.default_rng(17)
-    g = CartesianGrid((-0.93, -1.07, -0.88), (70, 37, 29), (0.052,) * 3)
+    g = CartesianGrid((-0.93, -1.07, -0.88), n, (0.052,) * 3)
     for geo in (SDFTorus((0.05, -0.02, 0.03), 0.5, 0.15, random_rotation(rng)),
                 SDFBox((0.0, 0.0, 0.0), (0.4, 0.3, 0.35), random_rotation(rng)),
                 SDFBox((0.026, 0.0, 0.0), (0.4, 0.3, 0.35))):
@@ -513,7 +517,7 @@ def test_plic_fused_planes_through_lattice_nodes(nrm):
     p0 = (g.x0[0] + 17 * 0.0625, g.x0[1] + 19 * 0.0625, g.x0[2] + 15 * 0.0625)
     geo = SDFPlane(tuple(nrm), p0)
     ref, normals, intercepts, valid = orc.plic(geo, 3, g.x0, g.d, g.n)
-    for variant, kz in ((2, 8), (2, 33), (0, 1)):
+    for variant, kz in ((2, 8), (2, 33), (1, 8), (0, 1)):
         c = allocate_cache(g, PLICCutCell)
         c.set_tuning(variant, kz, False)
         update_cache(c, geo, g)
