@@ -169,7 +169,8 @@ class Cache:
         return s.value
 
     def set_tuning(self, variant=2, kz=4, nontemporal=False):
-        """Launch shape of the marching-cubes stream pass (0 flat, 1 z-march, 2 fused z-march)."""
+        """Launch shape of the marching-cubes stream pass (0 flat, 1 z-march, 2 fused z-march), or
+        of a 3D PLIC update (0 two passes, 1 fused sweep on 32 x 8 tiles, 2 on 64 x 8 tiles)."""
         check(lib().cc_set_tuning(self._h, int(variant), int(kz), 1 if nontemporal else 0))
 
     def get_tuning(self):

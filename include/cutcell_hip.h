@@ -145,9 +145,10 @@ int cc_stream(cc_cache *cache, void **stream);
  *               in-kernel from the bricks (16 x 16-cell tiles, kz rounded down to a power of two
  *               dividing 16) instead of running the separate brick pass;
  *   nontemporal != 0 streams the column stores past the caches.  For a 3D PLIC cache, variant 0
- *   is the reference's two passes (fit, then faces) and any other value the fused single sweep
- *   (fit + resolved faces, kz layers per work item).  Results are identical for every setting;
- *   only the speed differs. */
+ *   is the reference's two passes (fit, then faces), 1 the fused single sweep (fit + resolved
+ *   faces, kz layers per work item) on 32 x 8-cell tiles and 2 the same sweep on 64 x 8-cell
+ *   tiles, one wave per row; nontemporal has no effect there.  Results are identical for every
+ *   setting; only the speed differs. */
 #define CC_MC_FLAT 0
 #define CC_MC_MARCH 1
 #define CC_MC_MARCH_FUSED 2
