@@ -170,10 +170,12 @@ __device__ __forceinline__ T primitive_sdf(int kind, const double *pp, const T *
         T rho = sqrt(p[0] * p[0] + p[1] * p[1]);
         T qx = rho - T(pp[3]), qz = p[2];
         T nq = sqrt(qx * qx + qz * qz);
-        T u = qx / nq;
-        gl[0] = u * (p[0] / rho);
-        gl[1] = u * (p[1] / rho);
-        gl[2] = qz / nq;
+        // the gradient (qx p_x/rho, qx p_y/rho, qz) / nq over one common denominator: one division
+        // where the quotients took four (on the axis or the core circle it is NaN either way)
+        const T t = T(1) / (nq * rho);
+        gl[0] = (qx * p[0]) * t;
+        gl[1] = (qx * p[1]) * t;
+        gl[2] = (qz * rho) * t;
         from_local<T, 3>(r, gl, normal);
         return nq - T(pp[4]);
     }

@@ -120,10 +120,11 @@ static double primitive_sdf(int kind, const double *pp, int dim, const double *x
         double rho = sqrt(p[0] * p[0] + p[1] * p[1]);
         double qx = rho - pp[3], qz = p[2];
         double nq = sqrt(qx * qx + qz * qz);
-        double u = qx / nq;
-        gl[0] = u * (p[0] / rho);
-        gl[1] = u * (p[1] / rho);
-        gl[2] = qz / nq;
+        /* the gradient (qx p_x/rho, qx p_y/rho, qz) / nq over one common denominator: one division */
+        double t = 1.0 / (nq * rho);
+        gl[0] = (qx * p[0]) * t;
+        gl[1] = (qx * p[1]) * t;
+        gl[2] = (qz * rho) * t;
         from_local(r, gl, 3, normal);
         return nq - pp[4];
     }
