@@ -673,12 +673,16 @@ def batched(seed, steps=32, cpu=True, cpu_budget_s=10.0):
     lat = np.array(lat) * 1e3
     ncell = wl.ncells()
     mc_cells = sum(int(np.prod(B["n"])) for B in wl.bodies if B["method"] == "mc")
-    alg = BYTES_PER_CELL * mc_cells + 131 * (ncell - mc_cells)
+    mc_nodes = sum(int(np.prod([na + 1 for na in B["n"]])) for B in wl.bodies if B["method"] == "mc")
+    # what each body's update writes at least: a marching-cubes body its 202 B per cell and phi, a
+    # PLIC body (the fused sweep) its 90 B per cell
+    alg = BYTES_PER_CELL * mc_cells + BYTES_PER_NODE * mc_nodes + 90 * (ncell - mc_cells)
     out = {"bodies": nb, "cells_per_step": ncell, "mc_cells": mc_cells,
            "cells_per_s": ncell / (np.mean(lat) * 1e-3),
            "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                                "p99": round(float(np.percentile(lat, 99)), 3)},
            "graph_device_ms_p50": round(float(np.percentile(dev, 50)), 3),
+           "algorithmic_bytes": alg,
            "hbm_frac": round(alg / (float(np.percentile(dev, 50)) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "kz_by_body_class": sorted(set(kz.values()))}
     del batch, caches
