@@ -81,7 +81,7 @@ def tune_stream(cache, geo, grid, candidates=None, repeats=3, use_cache=True):
     return rec
 
 
-def tune_batch(batch, geos, grids, kz_candidates=(2, 4, 8, 16, 32), classes=3, repeats=2):
+def tune_batch(batch, geos, grids, kz_candidates=(1, 2, 4, 8, 16, 32), classes=3, repeats=2):
     """Per size class of the batch's marching-cubes bodies, and for its PLIC bodies as one more class,
     the slab depth that minimises the graph's device time (coordinate descent over the classes).
     Returns the chosen kz per body."""

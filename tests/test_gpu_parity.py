@@ -505,6 +505,37 @@ def test_plic_fused_and_two_pass_bitwise(variant, kz, n):
         assert np.array_equal(c.is_valid, valid)
 
 
+def test_plic_float32_3d_every_sweep_agrees():
+    # test/test_plic.jl:238-250 in 3D: a Float32 PLIC cache computes in Float32 end to end; the
+    # reference's two passes and both fused sweeps are the same expressions in the same order, so
+    # they agree bit for bit in Float32 too; faces single-valued, closure exactly zero, and the
+    # result close to the Float64 oracle's
+    rng = np.random.default_rng(29)
+    g = CartesianGrid((-0.93, -1.07, -0.88), (70, 37, 29), (0.052,) * 3)
+    names = ("kind", "volume_fraction", "face_fraction", "normals", "intercepts", "is_valid")
+    for geo in (SDFTorus((0.05, -0.02, 0.03), 0.5, 0.15, random_rotation(rng)),
+                SDFBox((0.0, 0.0, 0.0), (0.4, 0.3, 0.35), random_rotation(rng))):
+        runs = []
+        for variant, kz in ((0, 16), (1, 16), (2, 16), (2, 5)):
+            c = allocate_cache(g, PLICCutCell, np.float32)
+            c.set_tuning(variant, kz, False)
+            update_cache(c, geo, g)
+            runs.append({k: c.read(k) for k in names})
+        assert runs[0]["volume_fraction"].dtype == np.float32 and runs[0]["normals"].dtype == np.float32
+        for r in runs[1:]:
+            for k in names:
+                assert same_bits(r[k], runs[0][k]), k
+        ff = runs[0]["face_fraction"]
+        assert np.all((ff >= 0) & (ff <= 1))
+        assert shared_face_mismatches(ff, g.n, 3) == 0
+        assert np.count_nonzero(closure_residual(ff, np.float32(g.d))) == 0
+        ref, *_ = orc.plic(geo, 3, g.x0, g.d, g.n)
+        vf = runs[0]["volume_fraction"].astype(np.float64)
+        assert np.max(np.abs(vf - ref["volume_fraction"])) < 1e-4
+        assert np.max(np.abs(ff.astype(np.float64) - ref["face_fraction"])) < 1e-3
+        assert int(np.count_nonzero(runs[0]["kind"] != ref["kind"])) <= g.ncells // 1000
+
+
 @pytest.mark.parametrize("nrm", [(1.0, 0.0, 0.0), (0.0, -1.0, 0.0), (1.0, 1.0, 0.0), (1.0, 1.0, 1.0),
                                  (-1.0, 2.0, 2.0), (0.3, -0.4, 1e-17)])
 def test_plic_fused_planes_through_lattice_nodes(nrm):
