@@ -22,7 +22,7 @@ SDFLibrary is not vendored with the reference; these are the build's own signed-
   rectangle so).  Its normal is the fold's gradient: the winning child's own normal wherever the fold
   took a plain minimum, ``(1 - h/2) n_min + (h/2) n_max`` where it blended, scaled to unit length.
 * ``SDFRectangle(center, dims)``: a 2D box of side lengths ``dims`` (SDFLibrary's own reading of
-  ``dims`` is not available; side lengths, not half-widths, is this build's).
+  ``dims`` is not available; here they are side lengths, not half-widths).
 * ``EmptyGeo``: the constant field ``value`` (the reference seeds every cache with ``EmptyGeo(1)``,
   ``src/marching_squares/cache.jl:46``).
 * ``SDFBounded(child, pad_distance, fill_distance)``: the child within ``pad_distance`` (Euclidean)
