@@ -10,7 +10,10 @@
 namespace cc {
 namespace {
 
-constexpr int CULL_LDS = 512;  // primitives staged in LDS per brick (more are read from memory)
+// Primitives the phi pass stages in LDS per brick; a brick keeping more folds from memory.  A brick
+// of the 512-sphere composite body keeps 15 on average, and a smaller stage leaves room for more
+// blocks per CU.
+constexpr int CULL_LDS = 128;
 
 // The sphere-union fold over a brick's kept primitives, in the order and with the arithmetic of
 // union_dist (cc_device.h, fold_sphere).  Primitive 0 is always kept.  sph: the first

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
     __shared__ T sph[4 * CULL_LDS];
     __shared__ uint16_t queue[BRICK_NODES];
     __shared__ int8_t ssign[64];
-    __shared__ uint32_t qn, st_uncert, st_band, st_sign, st_nodes, st_cert;
+    __shared__ uint32_t wsum[4], st_uncert, st_band, st_sign, st_nodes, st_cert;
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63;
     int bi[3];
     brick_of(b, nbx, nby, bi);
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
         sph[4 * e + 2] = T(s[2]);
         sph[4 * e + 3] = T(s[3]);
     }
-    if (t == 0) qn = st_uncert = st_band = st_sign = st_nodes = st_cert = 0;
+    if (t == 0) st_uncert = st_band = st_sign = st_nodes = st_cert = 0;
     __syncthreads();
     const size_t m0 = (size_t)g.n[0] + 1, m1 = (size_t)g.n[1] + 1;
     if (SURR && t < 64) {
@@ -147,8 +147,11 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
         if (valid) atomicAdd(&st_cert, 1u);
     }
     if (SURR) __syncthreads();
-    // classify: nodes the surrogate may keep are written now; the rest are queued for the fold
-    uint32_t n_unc = 0, n_band = 0, n_sign = 0, n_nodes = 0;
+    // classify: nodes the surrogate may keep are written now; the rest are marked in a per-thread
+    // mask and queued for the fold below (no per-node synchronisation, so the network's loads of
+    // consecutive nodes overlap)
+    uint32_t n_unc = 0, n_band = 0, n_sign = 0, n_nodes = 0, pushm = 0;
+#pragma unroll 4
     for (int q = 0; q < BRICK_NODES / 256; ++q) {
         const int idx = t + 256 * q;
         const int li = idx & 15, lj = (idx >> 4) & 15, lk = idx >> 8;
@@ -173,12 +176,18 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
             }
         }
         n_nodes += in ? 1u : 0u;
-        const unsigned long long m = __ballot(push);
-        uint32_t qb = 0;
-        if (lane == 0 && m) qb = atomicAdd(&qn, (uint32_t)__popcll(m));
-        qb = __shfl(qb, 0);
-        if (push) queue[qb + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)idx;
+        pushm |= (push ? 1u : 0u) << q;
     }
+    // the queue: each thread's marked nodes at its offset, an exclusive scan of the counts over the
+    // block (within the wave by shuffles, over the four waves through LDS)
+    const uint32_t mine = (uint32_t)__popc(pushm);
+    uint32_t incl = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+    }
+    if (lane == 63) wsum[t >> 6] = incl;
     if (SURR) {
         atomicAdd(&st_uncert, n_unc);
         atomicAdd(&st_band, n_band);
@@ -186,7 +195,12 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
     }
     atomicAdd(&st_nodes, n_nodes);
     __syncthreads();
-    const uint32_t nq = qn;
+    uint32_t off = incl - mine;
+    for (int w = 0; w < (t >> 6); ++w) off += wsum[w];
+    const uint32_t nq = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+    for (int q = 0; q < BRICK_NODES / 256; ++q)
+        if ((pushm >> q) & 1u) queue[off++] = (uint16_t)(t + 256 * q);
+    __syncthreads();
     for (uint32_t e = t; e < nq; e += 256) {
         const int idx = queue[e];
         const int gi = i0 + (idx & 15), gj = j0 + ((idx >> 4) & 15), gk = k0 + (idx >> 8);
