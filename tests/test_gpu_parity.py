@@ -485,7 +485,8 @@ def test_autotuner_picks_a_shape_and_results_stay_identical(tmp_path, monkeypatc
 
 @pytest.mark.parametrize("variant,kz,n", [(0, 16, (70, 37, 29)), (1, 16, (70, 37, 29)), (1, 5, (70, 37, 29)),
                                          (2, 16, (70, 37, 29)), (2, 5, (70, 37, 29)), (2, 1, (70, 37, 29)),
-                                         (2, 7, (131, 17, 20)), (2, 3, (40, 33, 9)), (2, 4, (64, 9, 6))])
+                                         (2, 7, (131, 17, 20)), (2, 3, (40, 33, 9)), (2, 4, (64, 9, 6)),
+                                         (2, 2, (5, 3, 4)), (1, 3, (5, 3, 4)), (2, 1, (1, 1, 1))])
 def test_plic_fused_and_two_pass_bitwise(variant, kz, n):
     # the fused single sweeps (variant 1: 32 x 8 tiles, variant 2: 64 x 8 tiles, one wave per row)
     # and the reference's two passes give the oracle's bits, on grids whose extents are not
