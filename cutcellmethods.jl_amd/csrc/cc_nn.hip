@@ -77,10 +77,13 @@ constexpr int LDW = WIDTH + 8;    // LDS row stride (BF16 elements) of the activ
 // (columns = points).  Each wave owns 64 output features for all 128 points (4 x 8 MFMA tiles), and
 // its accumulators hold 4 consecutive features of one point per lane, so the epilogue -- bias,
 // SiLU with the hardware reciprocal, one packed BF16 conversion -- ends in one 8-byte LDS store per
-// four values.  The activation tile is [128][LDW] BF16 (67.6 KB), updated in place between layers.
-// Points are either explicit (x: n x 3) or, for LAT, the nodes of a lattice: point p is node
+// four values.  The activation tile is [ITILE][LDW] BF16 (67.6 KB), updated in place between
+// layers.  Points are either explicit (x: n x 3) or, for LAT, the nodes of a lattice: point p is node
 // (p % lat.n[0], p / lat.n[0] % lat.n[1], p / (lat.n[0] lat.n[1])) at lat.x0 + index * lat.h.
+// 64-point tiles (three blocks per CU instead of two) were slower: 0.33 against 0.28 ms on the 66^3
+// lattice, each weight fragment then feeding half as many MFMAs.
 constexpr int ITILE = 128;
+constexpr int INJ = ITILE / 16;  // 16-point column tiles of a wave's accumulators
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -126,27 +129,34 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(MlpDev m, const float *__r
     int K = F2, LDin = LDF;
     const int arow = 64 * wave + (lane & 15), koff = 8 * (lane >> 4);
     for (int l = 0; l < m.depth; ++l) {
-        f32x4 acc[4][8];
+        f32x4 acc[4][INJ];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int j = 0; j < INJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
         const u16 *Wt = m.Wt[l];
-        bf16x8 a[4], an[4];
+        // weight fragments two K steps ahead: one step's 32 MFMAs do not cover an L2 round trip
+        bf16x8 a[4], an[4], an2[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = load8(Wt + (size_t)(arow + 16 * i) * K + koff);
+        for (int i = 0; i < 4; ++i) {
+            a[i] = load8(Wt + (size_t)(arow + 16 * i) * K + koff);
+            an[i] = load8(Wt + (size_t)(arow + 16 * i) * K + (KS < K ? KS : 0) + koff);
+        }
         for (int k0 = 0; k0 < K; k0 += KS) {
-            const int kn = k0 + KS < K ? k0 + KS : k0;  // next step's fragments (the last reloads its own)
+            const int kn2 = k0 + 2 * KS < K ? k0 + 2 * KS : k0;  // (past the end: reload this step's own)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) an[i] = load8(Wt + (size_t)(arow + 16 * i) * K + kn + koff);
+            for (int i = 0; i < 4; ++i) an2[i] = load8(Wt + (size_t)(arow + 16 * i) * K + kn2 + koff);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
+            for (int j = 0; j < INJ; ++j) {
                 const bf16x8 b = load8(ibuf + (16 * j + (lane & 15)) * LDin + k0 + koff);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) acc[i][j] = mfma(a[i], b, acc[i][j]);
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = an[i];
+            for (int i = 0; i < 4; ++i) {
+                a[i] = an[i];
+                an[i] = an2[i];
+            }
         }
         __syncthreads();  // every wave has read the whole input tile
         const float *bias = m.bias[l];
@@ -155,7 +165,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(MlpDev m, const float *__r
             const int f0 = 64 * wave + 16 * i + 4 * (lane >> 4);
             const f32x4 bv = *reinterpret_cast<const f32x4 *>(bias + f0);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
+            for (int j = 0; j < INJ; ++j) {
                 const int p = 16 * j + (lane & 15);
                 const bf16x4 pk = {(__bf16)silu_fast(acc[i][j][0] + bv[0]), (__bf16)silu_fast(acc[i][j][1] + bv[1]),
                                    (__bf16)silu_fast(acc[i][j][2] + bv[2]), (__bf16)silu_fast(acc[i][j][3] + bv[3])};
@@ -166,9 +176,10 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(MlpDev m, const float *__r
         K = WIDTH;
         LDin = LDW;
     }
-    // output layer: two lanes per point, 128 of the 256 k each
+    // output layer: two lanes per point, 128 of the 256 k each (the first 2 ITILE threads; a whole
+    // number of waves)
     const float bo = *m.bout;
-    {
+    if (t < 2 * ITILE) {
         const int p = t >> 1, h = t & 1;
         float sacc = 0.0f;
         for (int k = h; k < WIDTH; k += 2) sacc += bf2f(ibuf[p * LDW + k]) * wo[k];
