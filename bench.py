@@ -549,9 +549,10 @@ def csg(seed, n=512, train_epochs=24, train_samples=1 << 22, cpu=True):
                        "approx_tflop": round(tflop, 2), "approx_tflops": round(tflop / tt, 1)}
     from cutcell.autotune import tune_stream
     # the fused z-march evaluating the culled or surrogate phi from the bricks, or a phi pass and a
-    # stream pass reading it (z-march depths that reuse node layers, or flat tiles)
+    # stream pass reading it (the z-march reads the phi pass's sign bitmap, so shallow depths are
+    # cheap; flat tiles read phi)
     cands = [(2, 8, False), (2, 16, False), (2, 8, True), (2, 16, True),
-             (1, 8, False), (1, 16, False), (1, 8, True), (0, 16, False)]
+             (1, 2, False), (1, 4, False), (1, 8, False), (1, 2, True), (0, 16, False)]
     cache = allocate_cache(grid, MarchingCubesCutCell)
     cache.set_profiling(True)
     res = {}

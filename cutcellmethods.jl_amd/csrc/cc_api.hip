@@ -127,6 +127,8 @@ struct cc_cache {
     int64_t bstat_cap = 0;
     int8_t *cert = nullptr;     // sub-brick sign certificates for the fused surrogate pass
     int64_t cert_cap = 0;
+    uint64_t *signs = nullptr;  // the phi pass's brick-major sign bitmap, for the stream pass reading it
+    int64_t signs_cap = 0;
     int64_t phi_nbricks = 0;
     int64_t phi_nwork = 0;  // work items of the stream pass that evaluated phi (their counters follow)
     // the whole-grid surface (marching cubes): work arrays, the last mesh, and what it was made from
@@ -344,7 +346,7 @@ template <typename T> int ensure(T **p, int64_t *cap, int64_t want) {
 // cull pass certifies the sub-bricks' signs and *bp receives what the stream pass evaluates from.
 template <typename T>
 int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::GridT<T> &gt, T *phi, cc::BrickPhi *bp = nullptr,
-               int64_t nwork = 0) {
+               int64_t nwork = 0, bool want_signs = false) {
     if (geo.nprims > 65535)
         return fail(CC_ERR_UNSUPPORTED, "culled and surrogate phi take at most 65535 primitives");
     int nb[3];
@@ -354,6 +356,7 @@ int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::GridT<T> &gt, T *phi, 
     if (!rc) rc = ensure(&c->cull_cnt, &c->cnt_cap, nbrick);
     if (!rc) rc = ensure(&c->bstat, &c->bstat_cap, (nbrick + (bp ? nwork : 0)) * cc::PHI_BSTAT);
     if (!rc && bp && c->phi_source == CC_PHI_SURROGATE) rc = ensure(&c->cert, &c->cert_cap, nbrick * 64);
+    if (!rc && want_signs) rc = ensure(&c->signs, &c->signs_cap, nbrick * (cc::PHI_BRICK * cc::PHI_BRICK * cc::PHI_BRICK / 64));
     if (rc) return rc;
     c->phi_nbricks = nbrick;
     c->phi_nwork = bp ? nwork : 0;
@@ -403,7 +406,8 @@ int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::GridT<T> &gt, T *phi, 
         bp->wstat = c->bstat + nbrick * cc::PHI_BSTAT;
         return CC_OK;
     }
-    cc::launch_phi_bricks<T>(phi, gt, geo, c->cull_list, c->cull_cnt, sa, c->bstat, c->stream);
+    cc::launch_phi_bricks<T>(phi, gt, geo, c->cull_list, c->cull_cnt, sa, c->bstat, want_signs ? c->signs : nullptr,
+                             c->stream);
     mark(c, "phi_bricks");
     return CC_OK;
 }
@@ -429,10 +433,13 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
                           (geo->kind & cc::GEO_KIND) != cc::GEO_UNION;
         if (!fuse && tn.variant == cc::MC_MARCH_FUSED) tn.variant = cc::MC_MARCH;
         cc::BrickPhi bp{};
+        // the z-march reading a brick pass's field reads its sign bitmap instead of phi: it needs only
+        // each node's side, and a cut cell's exact corners are read from phi by the cut pass
+        const bool signs = bricks && !fuse && tn.variant == cc::MC_MARCH;
         if (bricks) {
             uint32_t W, R, KZ, nseg, nrg, nkb;
             cc::mc_brick_tile_shape(gt.n, tn.kz > 0 ? (uint32_t)tn.kz : 16u, &W, &R, &KZ, &nseg, &nrg, &nkb);
-            const int rc = phi_bricks<T>(c, *geo, gt, phi, fuse ? &bp : nullptr, (int64_t)nseg * nrg * nkb);
+            const int rc = phi_bricks<T>(c, *geo, gt, phi, fuse ? &bp : nullptr, (int64_t)nseg * nrg * nkb, signs);
             if (rc) return rc;
         } else if (geo && !fuse) {
             cc::launch_sample<T>(phi, gt, *geo, c->dim, c->stream);
@@ -445,7 +452,7 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
         cc::launch_zero_u32(c->counter, c->stream);
         if (c->method == CC_MARCHING_CUBES) {
             cc::launch_mc_stream<T>(phi, gt, fuse ? geo : nullptr, cols, c->cutlist, c->counter, tn,
-                                    c->stream, fuse && bricks ? &bp : nullptr);
+                                    c->stream, fuse && bricks ? &bp : nullptr, signs ? c->signs : nullptr);
             mark(c, "mc_stream");
             cc::launch_mc_cut<T>(phi, gt, cols, c->cutlist, c->counter, c->stream);
             mark(c, "mc_cut");
@@ -950,6 +957,7 @@ void free_all(cc_cache *c) {
     if (c->nn) (void)hipFree(c->nn);
     if (c->bstat) (void)hipFree(c->bstat);
     if (c->cert) (void)hipFree(c->cert);
+    if (c->signs) (void)hipFree(c->signs);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -38,6 +38,24 @@ __device__ __forceinline__ T culled_fold(const T *sph, uint32_t nk, const uint16
     return acc;
 }
 
+// The brick-major sign bitmap the phi pass leaves beside phi for the stream pass, which needs only
+// each node's side (a cut cell's exact corners are read from phi by the cut pass): node (gi, gj, gk)
+// is bit li + 16 (lj % 4) of word 64 b + lj / 4 + 4 lk, with b its brick and (li, lj, lk) its place
+// in it, so the 64 nodes one wave of the phi pass holds (consecutive local indices) fill one word.
+// The bit is set when the node's value, zeros nudged to +eps as marching_cubes.jl:240-243 does,
+// is >= 0 (NaN: clear).
+template <typename T> __device__ __forceinline__ bool node_outside(T v) {
+    return fabs(v) < Eps<T>::v || v >= T(0);
+}
+
+__device__ __forceinline__ bool sign_bit(const uint64_t *__restrict__ signs, int nbx, int nby, uint32_t gi,
+                                         uint32_t gj, uint32_t gk) {
+    const uint32_t b = (gi / PHI_BRICK) + (uint32_t)nbx * ((gj / PHI_BRICK) + (uint32_t)nby * (gk / PHI_BRICK));
+    const uint32_t li = gi % PHI_BRICK, lj = gj % PHI_BRICK, lk = gk % PHI_BRICK;
+    const uint64_t w = signs[64 * (size_t)b + (lj >> 2) + 4 * lk];
+    return (w >> (li + 16 * (lj & 3))) & 1ull;
+}
+
 __device__ __forceinline__ float nn_trilinear(const SurrogateArgs &sa, int gi, int gj, int gk) {
     const int f = sa.f;
     const int I = gi / f, J = gj / f, K = gk / f;

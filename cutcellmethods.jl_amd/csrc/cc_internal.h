@@ -55,10 +55,12 @@ struct BrickPhi;
 // The stream pass.  `geo` is the body when the variant samples it in-kernel (MC_MARCH_FUSED, phi is
 // then written by this pass); otherwise phi must already hold the nodal field.  With `bp` (and the
 // fused variant) a sphere union's culled or surrogate phi is evaluated in-kernel from the bricks.
+// With `signs` (the z-march reading a field, MC_MARCH) each node's side comes from the phi pass's
+// sign bitmap instead of phi.
 template <typename T>
 void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &cols,
                       uint32_t *cutlist, uint32_t *counter, const McTuning &tune, hipStream_t s,
-                      const BrickPhi *bp = nullptr);
+                      const BrickPhi *bp = nullptr, const uint64_t *signs = nullptr);
 template <typename T>
 void launch_mc_cut(const T *phi, const GridT<T> &g, const Cols<T> &cols, const uint32_t *cutlist,
                    const uint32_t *counter, hipStream_t s);
@@ -179,9 +181,12 @@ struct BrickPhi {
 template <typename T>
 void launch_cull(const GridT<T> &g, const Geo &geo, double scale, uint16_t *list, uint32_t *cnt,
                  int8_t *cert, uint32_t *bstat, hipStream_t s);
+// The phi pass; with signs non-null it also leaves the brick-major sign bitmap of phi there (64
+// words per brick; cc_brick_dev.h sign_bit) for the stream pass to read instead of phi.
 template <typename T>
 void launch_phi_bricks(T *phi, const GridT<T> &g, const Geo &geo, const uint16_t *list,
-                       const uint32_t *cnt, const SurrogateArgs &sa, uint32_t *bstat, hipStream_t s);
+                       const uint32_t *cnt, const SurrogateArgs &sa, uint32_t *bstat, uint64_t *signs,
+                       hipStream_t s);
 
 // ---- the whole-grid marching-cubes surface (generate_mesh) ---------------------------------------
 // Sizes of the march's per-grid work arrays for a grid of n cells: node rows (j, k) of (n0+1)

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(MC_TILE) void k_mc_stream(const T *__restrict__ phi
     stream_tile<T, NT>(cols, c0, (int)min((uint32_t)MC_TILE, ncell - c0), s);
 }
 
-template <typename T, bool FUSED, bool NT>
+template <typename T, bool FUSED, bool NT, bool SIGNS = false>
 __global__ __launch_bounds__(MC_TILE) void k_mc_march(McBody<T> B, uint32_t nwork) {
     // XCD-aware order: the hardware deals workgroups round-robin over the 8 XCDs, so give each XCD
     // one contiguous run of work items; neighbouring rows then share node rows in that XCD's L2.
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(MC_TILE) void k_mc_march(McBody<T> B, uint32_t nwor
     const uint32_t w = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (w >= nwork) return;
     __shared__ MarchLDS<T> L;
-    mc_march_item<T, FUSED, NT>(B, w, L);
+    mc_march_item<T, FUSED, NT, false, SIGNS>(B, w, L);
 }
 
 // The fused pass for a sphere union's culled or surrogate phi: brick-aligned items (MC_BRICK_W cells
@@ -117,7 +117,7 @@ void launch_sample(T *phi, const GridT<T> &g, const Geo &geo, int dim, hipStream
 template <typename T>
 void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &cols,
                       uint32_t *cutlist, uint32_t *counter, const McTuning &tune, hipStream_t s,
-                      const BrickPhi *bp) {
+                      const BrickPhi *bp, const uint64_t *signs) {
     const uint32_t ncell = (uint32_t)((uint64_t)g.n[0] * g.n[1] * g.n[2]);
     if (tune.variant == MC_FLAT || (geo == nullptr && tune.variant == MC_MARCH_FUSED)) {
         const uint32_t ntiles = (ncell + MC_TILE - 1) / MC_TILE;
@@ -140,6 +140,13 @@ void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &
     B.fused = tune.variant == MC_MARCH_FUSED && geo != nullptr;
     B.nontemporal = tune.nontemporal;
     B.bp = bp ? *bp : BrickPhi{};
+    B.signs = B.fused ? nullptr : signs;
+    if (B.signs) {
+        int nb[3];
+        phi_brick_counts(g.n, nb);
+        B.sbx = nb[0];
+        B.sby = nb[1];
+    }
     if (B.fused && bp) {
         mc_brick_tile_shape(g.n, tune.kz > 0 ? tune.kz : 16, &B.W, &B.R, &B.kz, &B.nseg, &B.nrg, &B.nkb);
         const uint32_t nwork = B.nseg * B.nrg * B.nkb;
@@ -153,12 +160,14 @@ void launch_mc_stream(T *phi, const GridT<T> &g, const Geo *geo, const Cols<T> &
     mc_tile_shape(g.n, tune.kz > 0 ? tune.kz : 16, &B.W, &B.R, &B.kz, &B.nseg, &B.nrg, &B.nkb);
     const uint32_t nwork = B.nseg * B.nrg * B.nkb;
     const uint32_t blocks = ((nwork + 7) / 8) * 8;
-#define CC_LAUNCH_MARCH(F, N) \
-    hipLaunchKernelGGL((k_mc_march<T, F, N>), dim3(blocks), dim3(MC_TILE), 0, s, B, nwork)
-    if (B.fused && tune.nontemporal) CC_LAUNCH_MARCH(true, true);
-    else if (B.fused) CC_LAUNCH_MARCH(true, false);
-    else if (tune.nontemporal) CC_LAUNCH_MARCH(false, true);
-    else CC_LAUNCH_MARCH(false, false);
+#define CC_LAUNCH_MARCH(F, N, S) \
+    hipLaunchKernelGGL((k_mc_march<T, F, N, S>), dim3(blocks), dim3(MC_TILE), 0, s, B, nwork)
+    if (B.fused && tune.nontemporal) CC_LAUNCH_MARCH(true, true, false);
+    else if (B.fused) CC_LAUNCH_MARCH(true, false, false);
+    else if (B.signs && tune.nontemporal) CC_LAUNCH_MARCH(false, true, true);
+    else if (B.signs) CC_LAUNCH_MARCH(false, false, true);
+    else if (tune.nontemporal) CC_LAUNCH_MARCH(false, true, false);
+    else CC_LAUNCH_MARCH(false, false, false);
 #undef CC_LAUNCH_MARCH
 }
 
@@ -204,9 +213,10 @@ template void launch_sample<double>(double *, const GridT<double> &, const Geo &
 template void launch_sample<float>(float *, const GridT<float> &, const Geo &, int, hipStream_t);
 template void launch_mc_stream<double>(double *, const GridT<double> &, const Geo *,
                                        const Cols<double> &, uint32_t *, uint32_t *,
-                                       const McTuning &, hipStream_t, const BrickPhi *);
+                                       const McTuning &, hipStream_t, const BrickPhi *, const uint64_t *);
 template void launch_mc_stream<float>(float *, const GridT<float> &, const Geo *, const Cols<float> &,
-                                      uint32_t *, uint32_t *, const McTuning &, hipStream_t, const BrickPhi *);
+                                      uint32_t *, uint32_t *, const McTuning &, hipStream_t, const BrickPhi *,
+                                      const uint64_t *);
 template void launch_mc_cut<double>(const double *, const GridT<double> &, const Cols<double> &,
                                     const uint32_t *, const uint32_t *, hipStream_t);
 template void launch_mc_cut<float>(const float *, const GridT<float> &, const Cols<float> &,

@@ -659,6 +659,8 @@ template <typename T> struct McBody {
     uint32_t fused;           // sample the body in-kernel (phi written) or read phi
     uint32_t nontemporal;
     BrickPhi bp;              // a sphere union's culled or surrogate phi (the BRICKS items)
+    const uint64_t *signs;    // reading a field: its nodes' sides from a phi pass's bitmap, not phi
+    int sbx, sby;             // that bitmap's bricks along x and y
 };
 
 constexpr int MARCH_NODES = 2 * (MC_TILE + 1);  // max (R+1)(W+1) over R*W <= MC_TILE
@@ -723,7 +725,7 @@ __device__ __forceinline__ void flush_cuts(CutMasks &cm, int ns, int wc, uint32_
 // phi exactly once, phi never re-read; BRICKS: a sphere union's culled or surrogate phi from its
 // bricks, with the counters of the owned nodes written to B.bp.wstat[w]) or read from phi.  The
 // cells of one step are one contiguous range of every column (R > 1 only for whole rows).
-template <typename T, bool FUSED, bool NT, bool BRICKS = false>
+template <typename T, bool FUSED, bool NT, bool BRICKS = false, bool SIGNS = false>
 __device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L, T *sph = nullptr,
                               BrickLists<T> *bl = nullptr) {
     const GridT<T> &g = B.g;
@@ -803,6 +805,8 @@ __device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L, T 
                         st_sign += how == NODE_CONTRADICTED ? 1u : 0u;
                     }
                 }
+            } else if (SIGNS) {  // the side is all the cells of this pass read (+-1 keeps it)
+                val = sign_bit(B.signs, B.sbx, B.sby, gi, gj, k) ? T(1) : T(-1);
             } else {
                 val = B.phi[idx];
             }

@@ -118,8 +118,9 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
                                                     const uint16_t *__restrict__ list,
                                                     const uint32_t *__restrict__ cnt, SurrogateArgs sa,
                                                     T delta_c, int nbx, int nby,
-                                                    uint32_t *__restrict__ bstat) {
+                                                    uint32_t *__restrict__ bstat, uint64_t *__restrict__ signs) {
     __shared__ T sph[4 * CULL_LDS];
+    __shared__ uint64_t sbits[BRICK_NODES / 64];  // the brick's sign bitmap words (node_outside)
     __shared__ uint16_t queue[BRICK_NODES];
     __shared__ int8_t ssign[64];
     __shared__ uint32_t wsum[4], st_uncert, st_band, st_sign, st_nodes, st_cert;
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
         const int li = idx & 15, lj = (idx >> 4) & 15, lk = idx >> 8;
         const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
         const bool in = gi <= g.n[0] && gj <= g.n[1] && gk <= g.n[2];
-        bool push = in;
+        bool push = in, kept_out = false;
         if (SURR && in) {
             const int8_t sg = ssign[(li >> 2) + 4 * (lj >> 2) + 16 * (lk >> 2)];
             if (sg == 0) {
@@ -172,11 +173,18 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
                 } else {
                     phi[gi + m0 * (gj + m1 * gk)] = T(v);
                     push = false;
+                    kept_out = node_outside(T(v));
                 }
             }
         }
         n_nodes += in ? 1u : 0u;
         pushm |= (push ? 1u : 0u) << q;
+        // a wave's 64 nodes (local indices 64 w .. 64 w + 63) are one word of the sign bitmap: the
+        // kept network values' bits now, the folded nodes' below
+        if (signs) {
+            const unsigned long long sw = __ballot(kept_out);
+            if (lane == 0) sbits[idx >> 6] = sw;
+        }
     }
     // the queue: each thread's marked nodes at its offset, an exclusive scan of the counts over the
     // block (within the wave by shuffles, over the four waves through LDS)
@@ -205,7 +213,14 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
         const int idx = queue[e];
         const int gi = i0 + (idx & 15), gj = j0 + ((idx >> 4) & 15), gk = k0 + (idx >> 8);
         const T x = g.x0[0] + T(gi) * g.d[0], y = g.x0[1] + T(gj) * g.d[1], z = g.x0[2] + T(gk) * g.d[2];
-        phi[gi + m0 * (gj + m1 * gk)] = culled_fold<T>(nk <= (uint32_t)CULL_LDS ? sph : nullptr, nk, L, geo, x, y, z);
+        const T v = culled_fold<T>(nk <= (uint32_t)CULL_LDS ? sph : nullptr, nk, L, geo, x, y, z);
+        phi[gi + m0 * (gj + m1 * gk)] = v;
+        if (signs && node_outside(v))
+            atomicOr(reinterpret_cast<unsigned int *>(sbits) + (idx >> 5), 1u << (idx & 31));
+    }
+    if (signs) {
+        __syncthreads();
+        if (t < BRICK_NODES / 64) signs[(size_t)b * (BRICK_NODES / 64) + t] = sbits[t];
     }
     if (t < PHI_BSTAT) {  // this brick's counters (summed on the host when asked for)
         const uint32_t v[PHI_BSTAT] = {st_nodes, nq, st_cert, st_uncert, st_band, st_sign, 0u, 0u};
@@ -236,17 +251,18 @@ void launch_cull(const GridT<T> &g, const Geo &geo, double scale, uint16_t *list
 
 template <typename T>
 void launch_phi_bricks(T *phi, const GridT<T> &g, const Geo &geo, const uint16_t *list,
-                       const uint32_t *cnt, const SurrogateArgs &sa, uint32_t *bstat, hipStream_t s) {
+                       const uint32_t *cnt, const SurrogateArgs &sa, uint32_t *bstat, uint64_t *signs,
+                       hipStream_t s) {
     int nb[3];
     phi_brick_counts(g.n, nb);
     const dim3 grid(nb[0] * nb[1] * nb[2]);
     const T dc = cert_margin<T>(sa.scale, geo.nprims);
     if (sa.nn)
         hipLaunchKernelGGL((k_phi_bricks<T, true>), grid, dim3(256), 0, s, phi, g, geo, list, cnt, sa, dc,
-                           nb[0], nb[1], bstat);
+                           nb[0], nb[1], bstat, signs);
     else
         hipLaunchKernelGGL((k_phi_bricks<T, false>), grid, dim3(256), 0, s, phi, g, geo, list, cnt, sa, dc,
-                           nb[0], nb[1], bstat);
+                           nb[0], nb[1], bstat, signs);
 }
 
 template void launch_cull<double>(const GridT<double> &, const Geo &, double, uint16_t *, uint32_t *, int8_t *,
@@ -254,8 +270,10 @@ template void launch_cull<double>(const GridT<double> &, const Geo &, double, ui
 template void launch_cull<float>(const GridT<float> &, const Geo &, double, uint16_t *, uint32_t *, int8_t *,
                                  uint32_t *, hipStream_t);
 template void launch_phi_bricks<double>(double *, const GridT<double> &, const Geo &, const uint16_t *,
-                                        const uint32_t *, const SurrogateArgs &, uint32_t *, hipStream_t);
+                                        const uint32_t *, const SurrogateArgs &, uint32_t *, uint64_t *,
+                                        hipStream_t);
 template void launch_phi_bricks<float>(float *, const GridT<float> &, const Geo &, const uint16_t *,
-                                       const uint32_t *, const SurrogateArgs &, uint32_t *, hipStream_t);
+                                       const uint32_t *, const SurrogateArgs &, uint32_t *, uint64_t *,
+                                       hipStream_t);
 
 }  // namespace cc

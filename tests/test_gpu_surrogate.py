@@ -305,9 +305,10 @@ def test_fused_phi_statistics_equal_the_brick_pass():
         assert out[0][0]["nodes"] == int(np.prod(np.array(g.n) + 1))
 
 
-@pytest.mark.parametrize("tune", [(0, 16, 0), (1, 3, 0), (1, 16, 1), (2, 64, 0), (2, 100, 0)])
+@pytest.mark.parametrize("tune", [(0, 16, 0), (1, 1, 0), (1, 3, 0), (1, 16, 1), (2, 64, 0), (2, 100, 0)])
 def test_culled_and_surrogate_under_every_stream_shape(tune):
-    """Every launch shape of the stream pass reads the phi the brick pass wrote, update after update."""
+    """Every launch shape of the stream pass reads what the brick pass wrote (phi, or for the z-march
+    reading a field the sign bitmap beside it), update after update."""
     body = csg_spheres(9, count=48, lo=-0.6, hi=0.6, rmin=0.08, rmax=0.25)
     g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=(40, 36, 150))  # several 64-layer slabs
     plain = allocate_cache(g, MarchingCubesCutCell)
