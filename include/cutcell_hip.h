@@ -139,7 +139,9 @@ int cc_stream(cc_cache *cache, void **stream);
 /* Launch shape of the marching-cubes stream pass, chosen per cache (the autotuner's knob; no
  * counterpart in the reference, whose KernelAbstractions launch is fixed at workgroup 64):
  *   variant 0 = flat tiles of 256 consecutive cells reading phi,
- *           1 = z-march of 256-cell row runs over kz layers reading phi,
+ *           1 = z-march of 256-cell row runs over kz layers reading phi (for a sphere union's
+ *               culled or surrogate phi, the sign bitmap its phi pass leaves beside it: the cells of
+ *               this pass need only each node's side, and the cut cells read phi),
  *           2 = z-march sampling an analytic body in-kernel (phi written once, never re-read);
  *               for a sphere union with a culled or surrogate phi source it evaluates that phi
  *               in-kernel from the bricks (16 x 16-cell tiles, kz rounded down to a power of two
