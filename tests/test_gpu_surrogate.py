@@ -262,6 +262,11 @@ def test_surrogate_on_a_float32_cache():
     assert np.array_equal(pe > 0, ps > 0)
     corners = cut_corner_nodes(plain.read("kind"), g.n)
     assert np.array_equal(pe[corners].view(np.uint32), ps[corners].view(np.uint32))
+    # the phi pass and the z-march reading its sign bitmap, in Float32 too
+    sur.set_tuning(1, 4, False)
+    update_cache(sur, body, g)
+    for name in COLS:
+        assert np.array_equal(plain.read(name).view(np.uint8), sur.read(name).view(np.uint8)), name
 
 
 def test_phi_source_argument_errors():
