@@ -441,7 +441,7 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
             cc::mc_brick_tile_shape(gt.n, tn.kz > 0 ? (uint32_t)tn.kz : 16u, &W, &R, &KZ, &nseg, &nrg, &nkb);
             const int rc = phi_bricks<T>(c, *geo, gt, phi, fuse ? &bp : nullptr, (int64_t)nseg * nrg * nkb, signs);
             if (rc) return rc;
-        } else if (geo && !fuse) {
+        } else if (geo && !fuse && c->method == CC_MARCHING_CUBES) {
             cc::launch_sample<T>(phi, gt, *geo, c->dim, c->stream);
             mark(c, "sample_nodes");
         } else if (phi_in) {
@@ -456,6 +456,9 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
             mark(c, "mc_stream");
             cc::launch_mc_cut<T>(phi, gt, cols, c->cutlist, c->counter, c->stream);
             mark(c, "mc_cut");
+        } else if (geo) {  // the cell pass samples the body itself
+            cc::launch_ms_cells_fused<T>(*geo, phi, gt, cols, c->counter, c->stream);
+            mark(c, "ms_cells");
         } else {
             cc::launch_ms_cells<T>(phi, gt, cols, c->counter, c->stream);
             mark(c, "ms_cells");

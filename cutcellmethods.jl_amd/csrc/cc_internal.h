@@ -68,6 +68,10 @@ void launch_mc_cut(const T *phi, const GridT<T> &g, const Cols<T> &cols, const u
 template <typename T>
 void launch_ms_cells(const T *phi, const GridT<T> &g, const Cols<T> &cols, uint32_t *counter,
                      hipStream_t s);
+// The 2D cell pass sampling an analytic body itself (phi written once, bitwise the sampling pass's).
+template <typename T>
+void launch_ms_cells_fused(const Geo &geo, T *phi, const GridT<T> &g, const Cols<T> &cols, uint32_t *counter,
+                           hipStream_t s);
 
 // The quadtree pass: leaf c at `level[c]`, lower corner `coord[2c..2c+1]` on its level's lattice
 // (spacing d / 2^level from x0); samples `geo` at each leaf's corners into phi4 (four per leaf) --

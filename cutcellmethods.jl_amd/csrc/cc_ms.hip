@@ -29,6 +29,38 @@ __global__ __launch_bounds__(256) void k_ms_cells(const T *__restrict__ phi, Gri
     add_block_cuts(cut ? 1u : 0u, counter);
 }
 
+// The same pass sampling an analytic body itself (_sample_nodes_kernel! and _nodal_cells_kernel!,
+// src/marching_squares/cache.jl:67-78, in one launch): each cell evaluates its four corners at the
+// lattice points the sampling pass uses, x0 + T(i) d, so its values are phi's bit for bit, and
+// writes the nodes it owns -- its lower-left corner, and on the grid's far edges the corners
+// beyond -- so phi is written once.  A 200 x 200 update is launch-bound; one launch fewer.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ms_cells_fused(Geo geo, T *__restrict__ phi, GridT<T> g, Cols<T> cols,
+                                                        uint32_t *__restrict__ counter) {
+    const uint32_t n0 = g.n[0], n1 = g.n[1];
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    bool cut = false;
+    if (c < n0 * n1) {
+        const uint32_t i = c % n0, j = c / n0;
+        const size_t sy = n0 + 1, base = i + sy * j;
+        const T xl = g.x0[0] + T(i) * g.d[0], xh = g.x0[0] + T(i + 1) * g.d[0];
+        const T yl = g.x0[1] + T(j) * g.d[1], yh = g.x0[1] + T(j + 1) * g.d[1];
+        const T nx[4] = {xl, xh, xh, xl}, ny[4] = {yl, yl, yh, yh};
+        T ph[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T x[3] = {nx[q], ny[q], T(0)};
+            ph[q] = sdf_value<T, 2>(geo, x);
+        }
+        phi[base] = ph[0];
+        if (i + 1 == n0) phi[base + 1] = ph[1];
+        if (j + 1 == n1) phi[base + sy] = ph[3];
+        if (i + 1 == n0 && j + 1 == n1) phi[base + sy + 1] = ph[2];
+        cut = ms_cell(nx, ny, ph, g.d[0], g.d[1], cols, c);
+    }
+    add_block_cuts(cut ? 1u : 0u, counter);
+}
+
 // _leaf_cells_kernel! (src/marching_squares/cache.jl:129-137): a thread per quadtree leaf (a
 // grid-stride loop past 2048 blocks, so the cut count takes at most 2048 atomics).  Its corners
 // come off the integer lattice, x0 + T(coord + bit) h with h = d / 2^level
@@ -200,6 +232,14 @@ void launch_ms_cells(const T *phi, const GridT<T> &g, const Cols<T> &cols, uint3
 }
 
 template <typename T>
+void launch_ms_cells_fused(const Geo &geo, T *phi, const GridT<T> &g, const Cols<T> &cols, uint32_t *counter,
+                           hipStream_t s) {
+    const uint32_t ncell = (uint32_t)g.n[0] * g.n[1];
+    hipLaunchKernelGGL((k_ms_cells_fused<T>), dim3((ncell + 255) / 256), dim3(256), 0, s, geo, phi, g, cols,
+                       counter);
+}
+
+template <typename T>
 void launch_leaf_cells(const Geo *geo, const int32_t *level, const int32_t *coord, uint32_t n, const double *x0,
                        const double *d, T *phi4, const Cols<T> &cols, uint32_t *counter, hipStream_t s) {
     if (n == 0) return;
@@ -217,6 +257,10 @@ template void launch_leaf_cells<double>(const Geo *, const int32_t *, const int3
                                         const double *, double *, const Cols<double> &, uint32_t *, hipStream_t);
 template void launch_leaf_cells<float>(const Geo *, const int32_t *, const int32_t *, uint32_t, const double *,
                                        const double *, float *, const Cols<float> &, uint32_t *, hipStream_t);
+template void launch_ms_cells_fused<double>(const Geo &, double *, const GridT<double> &, const Cols<double> &,
+                                            uint32_t *, hipStream_t);
+template void launch_ms_cells_fused<float>(const Geo &, float *, const GridT<float> &, const Cols<float> &,
+                                           uint32_t *, hipStream_t);
 template void launch_ms_cells<double>(const double *, const GridT<double> &, const Cols<double> &,
                                       uint32_t *, hipStream_t);
 template void launch_ms_cells<float>(const float *, const GridT<float> &, const Cols<float> &,
