@@ -422,6 +422,28 @@ def test_mc_stream_variants_bitwise(variant, kz, nt):
     assert_cells_equal(c, orc.mc_cells(phi_ref, g.x0, g.d, g.n))
 
 
+def test_batched_tiny_bodies_match_single_updates():
+    # bodies with fewer work items than XCDs (a single cell, a 2 x 3 x 1 PLIC grid, a row of cells)
+    # leave some of the eight per-XCD lists empty; each body still gets exactly its own update
+    from cutcell.batch import Batch
+    specs = [((-0.5, -0.5, -0.5), (1, 1, 1), 1.0, MarchingCubesCutCell, SDFSphere((0.1, 0.0, 0.0), 0.4)),
+             ((-0.3, -0.45, -0.15), (2, 3, 1), 0.3, PLICCutCell, SDFSphere((0.05, 0.0, 0.0), 0.35)),
+             ((-1.0, -0.1, -0.1), (40, 1, 1), 0.05, MarchingCubesCutCell, SDFSphere((0.2, 0.0, 0.0), 0.5)),
+             ((-0.5, -0.5, -0.5), (9, 5, 7), 0.11, PLICCutCell,
+              SDFTorus((0.0, 0.0, 0.0), 0.3, 0.1, random_rotation(np.random.default_rng(2))))]
+    grids = [CartesianGrid(x0, n, (d,) * 3) for x0, n, d, _, _ in specs]
+    caches = [allocate_cache(g, m) for g, (_, _, _, m, _) in zip(grids, specs)]
+    geos = [s[4] for s in specs]
+    batch = Batch(caches)
+    for step in range(2):
+        moved = [g.moved((0.01 * step, 0.0, -0.005 * step)) for g in grids]
+        batch.step(geos, moved)
+        for c, g, geo, (_, _, _, m, _) in zip(caches, moved, geos, specs):
+            solo = update_cache(allocate_cache(g, m), geo, g)
+            for name in COLS:
+                assert same_bits(c.read(name), solo.read(name)), (step, g.n, name)
+
+
 def test_batched_bodies_match_single_updates_and_oracle():
     # mixed marching-cubes and PLIC bodies on uneven grids, moving: one graph launch per step gives
     # every body exactly what cc_update gives it alone
