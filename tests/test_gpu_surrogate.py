@@ -251,6 +251,26 @@ def test_single_primitive_union_and_moved_grid():
             assert np.array_equal(plain.read(name).view(np.uint8), culled.read(name).view(np.uint8)), name
 
 
+def test_exact_zero_nodes_under_every_phi_route():
+    # a dyadic grid and a sphere centred on a node with a radius of four cells: six nodes read phi
+    # exactly 0, which the cells nudge to +eps (outside, marching_cubes.jl:225-243); the brick pass's
+    # sign bitmap must classify them the same way for the z-march that reads it
+    g = CartesianIsoGrid((-1.0,) * 3, (1.0,) * 3, n=(32, 32, 32))
+    body = SDFSphereUnion(np.array([[0.0, 0.0, 0.0], [0.75, 0.75, 0.75]]), np.array([0.25, 0.1]), 0.02)
+    plain = update_cache(allocate_cache(g, MarchingCubesCutCell), body, g)
+    assert int(np.count_nonzero(plain.phi == 0.0)) == 6
+    net = Surrogate(seed=3)  # untrained: the exact fallback decides the nodes near the surface
+    for source in ("culled", "surrogate"):
+        for tune in ((1, 2, False), (1, 16, False), (2, 16, False)):
+            c = allocate_cache(g, MarchingCubesCutCell)
+            c.set_tuning(*tune)
+            c.set_phi_source(source, net if source == "surrogate" else None, coarse=4)
+            update_cache(c, body, g)
+            for name in COLS:
+                assert np.array_equal(plain.read(name).view(np.uint8), c.read(name).view(np.uint8)), \
+                    (source, tune, name)
+
+
 def test_surrogate_on_a_float32_cache():
     body, g = union_case(n=(48, 40, 36))
     net = Surrogate(seed=5)
