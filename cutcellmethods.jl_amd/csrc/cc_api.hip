@@ -389,6 +389,8 @@ int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::GridT<T> &gt, T *phi, 
         mark(c, "nn_lattice");
         sa.nn = c->nn;
         sa.f = f;
+        sa.fshift = (f & (f - 1)) == 0 ? __builtin_ctz((unsigned)f) : -1;
+        sa.finv = 1.0f / (float)f;
         const double h = std::fmax((double)gt.d[0], std::fmax((double)gt.d[1], (double)gt.d[2]));
         sa.band = (float)(c->band_cells * h);
     }

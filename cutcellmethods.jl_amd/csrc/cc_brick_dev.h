@@ -15,15 +15,55 @@ namespace {
 // blocks per CU.
 constexpr int CULL_LDS = 128;
 
+// A single-precision screen for the double fold.  A primitive leaves the fold unchanged when
+// fold_sphere's own skip test holds (|x - c| >= acc + k + r up to its margin: the smooth minimum
+// then returns acc bit for bit); the screen proves that test true from float data, so a wave whose
+// lanes all pass it skips the primitive's double work.  Per staged primitive (screen_record): the
+// centre minus an origin o, rounded to nearest, and w >= (k + r) + 1e-6 (|k + r| + R + |c - o|),
+// where R bounds |x - o| over every point the block folds.  For a point x, with xf = fl32(x - o)
+// and u = 2^-24: the float difference vector is within 3.7 u (|x - o| + |c - o|) of x - c, so
+// s = |xf - cf|^2 (fused multiply-adds) and tp = w + accf + 2^-18 |accf| (accf = fl32(acc)) satisfy
+// tp > (acc + k + r) + 5.9 u (|acc| + k + r) + 4 u (R + |c - o|) > 0 -- and s >= fl32(tp * tp)
+// gives |x - c| > acc + k + r + 3.6 u (|acc| + k + r), far above the skip test's margin of
+// 35 ulp(double) of the same sum; tp <= 0 gives acc + k + r < 0, where the skip test holds too.
+// |tp| >= 1e18 or (positive) below 2^-60 (where tp * tp could underflow), infinities, NaNs, negative
+// radii and w outside [1e-18, 1e30] (w = inf) never skip.
+__device__ __forceinline__ float4 screen_record(const double *s, const double *o, double k, double R) {
+    const double cx = s[0] - o[0], cy = s[1] - o[1], cz = s[2] - o[2];
+    const double kr = k + s[3];
+    double w = kr + 1e-6 * ((fabs(kr) + R) + sqrt((cx * cx + cy * cy) + cz * cz));
+    if (!(s[3] >= 0.0) || !(w < 1e30) || !(w >= 1e-18)) w = INFINITY;
+    return make_float4((float)cx, (float)cy, (float)cz, (float)w);
+}
+
+__device__ __forceinline__ bool screen_skips(const float4 &q, float xf, float yf, float zf, float accf) {
+    const float dx = xf - q.x, dy = yf - q.y, dz = zf - q.z;
+    const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float tp = fmaf(fabsf(accf), 0x1p-18f, accf + q.w);
+    return fabsf(tp) < 1e18f && (tp <= 0.0f || (tp >= 0x1p-60f && s >= tp * tp));
+}
+
 // The sphere-union fold over a brick's kept primitives, in the order and with the arithmetic of
 // union_dist (cc_device.h, fold_sphere).  Primitive 0 is always kept.  sph: the first
-// nk kept primitives staged as T (null: read every one through L).
+// nk kept primitives staged as T (null: read every one through L).  scr (double only, with sph):
+// their screen records about the origin o; a primitive the screen does not skip is folded as the
+// oracle folds every one, smin_poly(acc, |x - c| - r, k).
 template <typename T>
 __device__ __forceinline__ T culled_fold(const T *sph, uint32_t nk, const uint16_t *L, const Geo &geo,
-                                         T x, T y, T z) {
+                                         T x, T y, T z, const float4 *scr = nullptr, const T *o = nullptr) {
     const T k = T(geo.p[0]);
     T acc;
-    if (sph) {
+    if (sizeof(T) == 8 && sph && scr) {
+        acc = sphere_dist<T, 3>(sph[0], sph[1], sph[2], sph[3], x, y, z);
+        const float xf = (float)(x - o[0]), yf = (float)(y - o[1]), zf = (float)(z - o[2]);
+        float accf = (float)acc;
+        for (uint32_t e = 1; e < nk; ++e) {
+            if (screen_skips(scr[e], xf, yf, zf, accf)) continue;
+            acc = smin_poly(acc, sphere_dist<T, 3>(sph[4 * e], sph[4 * e + 1], sph[4 * e + 2], sph[4 * e + 3], x, y, z),
+                            k);
+            accf = (float)acc;
+        }
+    } else if (sph) {
         acc = sphere_dist<T, 3>(sph[0], sph[1], sph[2], sph[3], x, y, z);
         for (uint32_t e = 1; e < nk; ++e)
             acc = fold_sphere<T, 3>(acc, sph[4 * e], sph[4 * e + 1], sph[4 * e + 2], sph[4 * e + 3], x, y, z, k);
@@ -56,18 +96,34 @@ __device__ __forceinline__ bool sign_bit(const uint64_t *__restrict__ signs, int
     return (w >> (li + 16 * (lj & 3))) & 1ull;
 }
 
+// The network lattice's cell and fraction of node index g along one axis: g / f and
+// (g - f (g / f)) / f -- with a shift, a mask and the exact reciprocal when f is a power of two, the
+// same values.
+__device__ __forceinline__ int nn_axis(const SurrogateArgs &sa, int g, float *fr) {
+    if (sa.fshift >= 0) {
+        *fr = (float)(g & (sa.f - 1)) * sa.finv;
+        return g >> sa.fshift;
+    }
+    const int c = g / sa.f;
+    *fr = (float)(g - sa.f * c) / (float)sa.f;
+    return c;
+}
+
+// The bilinear value in lattice plane K at cell (I, J) and fractions (fx, fy).
+__device__ __forceinline__ float nn_plane(const SurrogateArgs &sa, int I, int J, int K, float fx, float fy) {
+    const size_t sy = (size_t)sa.nc[0];
+    const float *p = sa.nn + I + J * sy + (size_t)K * sy * (size_t)sa.nc[1];
+    const float c00 = p[0] + fx * (p[1] - p[0]);
+    const float c10 = p[sy] + fx * (p[sy + 1] - p[sy]);
+    return c00 + fy * (c10 - c00);
+}
+
+// The network's trilinear value at node (gi, gj, gk): the bilinear values of planes K and K + 1,
+// then along z.
 __device__ __forceinline__ float nn_trilinear(const SurrogateArgs &sa, int gi, int gj, int gk) {
-    const int f = sa.f;
-    const int I = gi / f, J = gj / f, K = gk / f;
-    const float fx = (float)(gi - f * I) / (float)f, fy = (float)(gj - f * J) / (float)f,
-                fz = (float)(gk - f * K) / (float)f;
-    const size_t sx = 1, sy = (size_t)sa.nc[0], sz = (size_t)sa.nc[0] * sa.nc[1];
-    const float *p = sa.nn + I * sx + J * sy + K * sz;
-    const float c00 = p[0] + fx * (p[sx] - p[0]);
-    const float c10 = p[sy] + fx * (p[sy + sx] - p[sy]);
-    const float c01 = p[sz] + fx * (p[sz + sx] - p[sz]);
-    const float c11 = p[sz + sy] + fx * (p[sz + sy + sx] - p[sz + sy]);
-    const float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
+    float fx, fy, fz;
+    const int I = nn_axis(sa, gi, &fx), J = nn_axis(sa, gj, &fy), K = nn_axis(sa, gk, &fz);
+    const float c0 = nn_plane(sa, I, J, K, fx, fy), c1 = nn_plane(sa, I, J, K + 1, fx, fy);
     return c0 + fz * (c1 - c0);
 }
 
@@ -79,7 +135,8 @@ __device__ __forceinline__ float nn_trilinear(const SurrogateArgs &sa, int gi, i
 template <typename T>
 __device__ __forceinline__ int8_t sub_brick_sign(const GridT<T> &g, const Geo &geo, const T *sph, uint32_t nk,
                                                  const uint16_t *L, int i0, int j0, int k0, int s, T delta_c,
-                                                 bool *valid) {
+                                                 bool *valid, const float4 *scr = nullptr,
+                                                 const T *o = nullptr) {
     const int s3[3] = {s & 3, (s >> 2) & 3, s >> 4};
     const int base[3] = {i0, j0, k0};
     bool ok = true;
@@ -97,7 +154,7 @@ __device__ __forceinline__ int8_t sub_brick_sign(const GridT<T> &g, const Geo &g
     }
     *valid = ok;
     if (!ok) return 0;
-    const T pc = culled_fold<T>(sph, nk, L, geo, c[0], c[1], c[2]);
+    const T pc = culled_fold<T>(sph, nk, L, geo, c[0], c[1], c[2], scr, o);
     if (fabs(pc) - sqrt(e2) - sqrt(dg) > delta_c) return pc > T(0) ? 1 : -1;
     return 0;
 }

@@ -157,6 +157,8 @@ struct SurrogateArgs {
     const float *nn;
     int nc[3];
     int f;
+    int fshift;    // log2 f when f is a power of two, else -1
+    float finv;    // 1 / f (exact when f is a power of two)
     float band;    // nodes with |phi_nn| < band are evaluated exactly
     double scale;  // bound on the magnitudes (coordinates, radii, distances) the fold rounds
 };

@@ -120,6 +120,7 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
                                                     T delta_c, int nbx, int nby,
                                                     uint32_t *__restrict__ bstat, uint64_t *__restrict__ signs) {
     __shared__ T sph[4 * CULL_LDS];
+    __shared__ float4 scr[sizeof(T) == 8 ? CULL_LDS : 1];  // screen records (double fold only)
     __shared__ uint64_t sbits[BRICK_NODES / 64];  // the brick's sign bitmap words (node_outside)
     __shared__ uint16_t queue[BRICK_NODES];
     __shared__ int8_t ssign[64];
@@ -131,51 +132,76 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
     const uint32_t nk = cnt[b];
     const uint16_t *L = list + (size_t)b * geo.nprims;
     const uint32_t nstage = nk < (uint32_t)CULL_LDS ? nk : (uint32_t)CULL_LDS;
+    // the screen's origin is the brick's first node; every point folded here (its nodes, its
+    // sub-bricks' centres) is within R of it
+    const T o[3] = {g.x0[0] + T(i0) * g.d[0], g.x0[1] + T(j0) * g.d[1], g.x0[2] + T(k0) * g.d[2]};
+    const double od[3] = {double(o[0]), double(o[1]), double(o[2])};
+    const double R = 16.0 * sqrt((double(g.d[0]) * double(g.d[0]) + double(g.d[1]) * double(g.d[1])) +
+                                 double(g.d[2]) * double(g.d[2]));
     for (uint32_t e = t; e < nstage; e += 256) {
         const double *s = geo.prims + 4 * (size_t)L[e];
         sph[4 * e] = T(s[0]);
         sph[4 * e + 1] = T(s[1]);
         sph[4 * e + 2] = T(s[2]);
         sph[4 * e + 3] = T(s[3]);
+        if (sizeof(T) == 8) scr[e] = screen_record(s, od, geo.p[0], R);
     }
+    const bool staged = nk <= (uint32_t)CULL_LDS;
+    const float4 *scrp = sizeof(T) == 8 && staged ? scr : nullptr;
     if (t == 0) st_uncert = st_band = st_sign = st_nodes = st_cert = 0;
     __syncthreads();
     const size_t m0 = (size_t)g.n[0] + 1, m1 = (size_t)g.n[1] + 1;
     if (SURR && t < 64) {
         bool valid;
-        ssign[t] = sub_brick_sign<T>(g, geo, nk <= (uint32_t)CULL_LDS ? sph : nullptr, nk, L, i0, j0, k0, t, delta_c,
-                                     &valid);
-        if (valid) atomicAdd(&st_cert, 1u);
+        ssign[t] = sub_brick_sign<T>(g, geo, staged ? sph : nullptr, nk, L, i0, j0, k0, t, delta_c, &valid, scrp, o);
+        const unsigned long long vm = __ballot(valid);
+        if (t == 0) st_cert = (uint32_t)__popcll(vm);
     }
     if (SURR) __syncthreads();
     // classify: nodes the surrogate may keep are written now; the rest are marked in a per-thread
     // mask and queued for the fold below (no per-node synchronisation, so the network's loads of
     // consecutive nodes overlap)
     uint32_t n_unc = 0, n_band = 0, n_sign = 0, n_nodes = 0, pushm = 0;
+    // a thread's nodes are one column of the brick, (li, lj) = (t % 16, t / 16) and lk = q, so the
+    // layer is the same for the whole block: the network's bilinear values in lattice planes K and
+    // K + 1 (those of nn_trilinear) serve every node of the column with gk / f == K, K never
+    // decreases along it, and K, Kc and the window's updates are uniform
+    const int li = t & 15, lj = t >> 4, gi = i0 + li, gj = j0 + lj;
+    const bool col_in = gi <= g.n[0] && gj <= g.n[1];
+    const size_t colix = (size_t)gi + m0 * (size_t)gj, sz = m0 * m1;
+    float nfx = 0.0f, nfy = 0.0f, b0 = 0.0f, b1 = 0.0f;
+    int nI = 0, nJ = 0, Kc = -2;
+    if (SURR) {
+        nI = nn_axis(sa, gi, &nfx);
+        nJ = nn_axis(sa, gj, &nfy);
+    }
 #pragma unroll 4
     for (int q = 0; q < BRICK_NODES / 256; ++q) {
-        const int idx = t + 256 * q;
-        const int li = idx & 15, lj = (idx >> 4) & 15, lk = idx >> 8;
-        const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
-        const bool in = gi <= g.n[0] && gj <= g.n[1] && gk <= g.n[2];
+        const int idx = t + 256 * q, lk = q, gk = k0 + lk;
+        const bool in = col_in && gk <= g.n[2];
         bool push = in, kept_out = false;
-        if (SURR && in) {
-            const int8_t sg = ssign[(li >> 2) + 4 * (lj >> 2) + 16 * (lk >> 2)];
-            if (sg == 0) {
-                ++n_unc;
-            } else {
-                const float v = nn_trilinear(sa, gi, gj, gk);
-                const int how = keep_network(sa, sg, v);
-                if (how == NODE_BAND) {
-                    ++n_band;
-                } else if (how == NODE_CONTRADICTED) {
-                    ++n_sign;
-                } else {
-                    phi[gi + m0 * (gj + m1 * gk)] = T(v);
-                    push = false;
-                    kept_out = node_outside(T(v));
+        float fz = 0.0f;
+        if (SURR && gk <= g.n[2]) {
+            const int K = nn_axis(sa, gk, &fz);
+            if (K != Kc) {
+                if (col_in) {
+                    b0 = K == Kc + 1 ? b1 : nn_plane(sa, nI, nJ, K, nfx, nfy);
+                    b1 = nn_plane(sa, nI, nJ, K + 1, nfx, nfy);
                 }
+                Kc = K;
             }
+        }
+        if (SURR) {  // keep_network's decision as selects: one masked store, no nested branches
+            const int8_t sg = ssign[(li >> 2) + 4 * (lj >> 2) + 16 * (lk >> 2)];
+            const float v = b0 + fz * (b1 - b0);
+            const bool cert = in && sg != 0, band = fabsf(v) < sa.band, contra = (v > 0.0f) != (sg > 0);
+            const bool keep = cert && !band && !contra;
+            n_unc += in && sg == 0 ? 1u : 0u;
+            n_band += cert && band ? 1u : 0u;
+            n_sign += cert && !band && contra ? 1u : 0u;
+            if (keep) phi[colix + sz * (size_t)gk] = T(v);
+            push = in && !keep;
+            kept_out = keep && node_outside(T(v));
         }
         n_nodes += in ? 1u : 0u;
         pushm |= (push ? 1u : 0u) << q;
@@ -196,12 +222,23 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
         if (lane >= o) incl += u;
     }
     if (lane == 63) wsum[t >> 6] = incl;
-    if (SURR) {
-        atomicAdd(&st_uncert, n_unc);
-        atomicAdd(&st_band, n_band);
-        atomicAdd(&st_sign, n_sign);
+    // the counters summed over the wave by shuffles (two per word: a wave's sums stay below 2^11) and
+    // added by its first lane -- an atomic from every lane would be expanded into a loop over the
+    // lanes on the scalar unit
+    uint32_t c01 = n_nodes | (n_unc << 16), c23 = n_band | (n_sign << 16);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        c01 += __shfl_xor(c01, o);
+        if (SURR) c23 += __shfl_xor(c23, o);
     }
-    atomicAdd(&st_nodes, n_nodes);
+    if (lane == 0) {
+        atomicAdd(&st_nodes, c01 & 0xffffu);
+        if (SURR) {
+            atomicAdd(&st_uncert, c01 >> 16);
+            atomicAdd(&st_band, c23 & 0xffffu);
+            atomicAdd(&st_sign, c23 >> 16);
+        }
+    }
     __syncthreads();
     uint32_t off = incl - mine;
     for (int w = 0; w < (t >> 6); ++w) off += wsum[w];
@@ -213,7 +250,7 @@ __global__ __launch_bounds__(256) void k_phi_bricks(T *__restrict__ phi, GridT<T
         const int idx = queue[e];
         const int gi = i0 + (idx & 15), gj = j0 + ((idx >> 4) & 15), gk = k0 + (idx >> 8);
         const T x = g.x0[0] + T(gi) * g.d[0], y = g.x0[1] + T(gj) * g.d[1], z = g.x0[2] + T(gk) * g.d[2];
-        const T v = culled_fold<T>(nk <= (uint32_t)CULL_LDS ? sph : nullptr, nk, L, geo, x, y, z);
+        const T v = culled_fold<T>(staged ? sph : nullptr, nk, L, geo, x, y, z, scrp, o);
         phi[gi + m0 * (gj + m1 * gk)] = v;
         if (signs && node_outside(v))
             atomicOr(reinterpret_cast<unsigned int *>(sbits) + (idx >> 5), 1u << (idx & 31));
