@@ -40,7 +40,8 @@ __device__ __forceinline__ bool screen_skips(const float4 &q, float xf, float yf
     const float dx = xf - q.x, dy = yf - q.y, dz = zf - q.z;
     const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const float tp = fmaf(fabsf(accf), 0x1p-18f, accf + q.w);
-    return fabsf(tp) < 1e18f && (tp <= 0.0f || (tp >= 0x1p-60f && s >= tp * tp));
+    // evaluated whole (no short circuit): three compares and two mask operations, no branches
+    return (fabsf(tp) < 1e18f) & ((tp <= 0.0f) | ((tp >= 0x1p-60f) & (s >= tp * tp)));
 }
 
 // The sphere-union fold over a brick's kept primitives, in the order and with the arithmetic of
