@@ -94,7 +94,7 @@ template <typename T>
 __global__ __launch_bounds__(MC_CUT_THREADS) void k_batch_b(const BatchBody<T> *__restrict__ bodies,
                                                             const uint32_t *__restrict__ begin, int nb,
                                                             uint32_t total) {
-    __shared__ T pts[39 * MC_CUT_THREADS];
+    __shared__ T pts[MC_CUT_SLOTS * MC_CUT_THREADS];
     const uint32_t w = xcd_order(blockIdx.x);
     if (w >= total) return;
     const int b = find_body(begin, nb, w);

@@ -35,10 +35,13 @@ template <typename T> struct Cols {
 
 // Tile width of the marching-cubes stream pass: cells per workgroup, one per thread.
 constexpr int MC_TILE = 256;
-// Workgroups of the cut-cell pass (a grid-stride loop over the device-side count) and their size
-// (each lane stages 13 crossing points, 39 values, in LDS).
+// Workgroups of the cut-cell pass (a grid-stride loop over the device-side count) and their size.
+// Each lane stages its cell's 13 points in MC_CUT_SLOTS values of LDS: one per edge crossing (its
+// coordinate along the edge; the other two are the edge's corner coordinates) and three for the
+// interior vertex.
 constexpr int MC_CUT_BLOCKS = 4096;
 constexpr int MC_CUT_THREADS = 128;
+constexpr int MC_CUT_SLOTS = 15;
 
 template <typename T>
 void launch_sample(T *phi, const GridT<T> &g, const Geo &geo, int dim, hipStream_t s);

@@ -74,10 +74,10 @@ __global__ __launch_bounds__(MC_TILE) void k_mc_march_bricks(McBody<T> B, uint32
 }
 
 template <typename T>
-__global__ __launch_bounds__(MC_CUT_THREADS) void k_mc_cut(const T *__restrict__ phi, GridT<T> g, Cols<T> cols,
+__global__ __launch_bounds__(MC_CUT_THREADS, 3) void k_mc_cut(const T *__restrict__ phi, GridT<T> g, Cols<T> cols,
                                                            const uint32_t *__restrict__ cutlist,
                                                            const uint32_t *__restrict__ counter) {
-    __shared__ T pts[39 * MC_CUT_THREADS];
+    __shared__ T pts[MC_CUT_SLOTS * MC_CUT_THREADS];
     const uint32_t count = *counter;
     for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < count; w += gridDim.x * blockDim.x)
         mc_cut_item(phi, g, cols, cutlist[w], pts + threadIdx.x);

@@ -334,10 +334,15 @@ __device__ __forceinline__ void face_clip(const T *phi, const FaceTests &ft, con
     }
 }
 
-// Crossing of edge E (edge_crossings, marching_cubes.jl:306-330), outside-anchored, into the
-// lane's LDS slots; zero where the edge is not crossed.  acc and ncr gather the interior vertex.
+// The axis edge E runs along (edges 0-7 alternate x and y, 8-11 run along z).
+__device__ __forceinline__ constexpr int edge_axis(int e) { return e < 8 ? (e & 1) : 2; }
+
+// Crossing of edge E (edge_crossings, marching_cubes.jl:306-330), outside-anchored.  Its coordinate
+// along the edge goes to the lane's LDS slot E (zero where the edge is not crossed, which sets no
+// bit of `cmask`); on the two other axes the point is pa + s (pb - pa) with pa = pb there, which
+// cut_point restores.  acc and ncr gather the interior vertex.
 template <int E, typename T>
-__device__ __forceinline__ void edge_point(const T *phi, const T *L, T *P, T *acc, int &ncr) {
+__device__ __forceinline__ void edge_point(const T *phi, const T *L, T *P, T *acc, int &ncr, unsigned &cmask) {
     constexpr int a = edge_node(E, 0), b = edge_node(E, 1);
     const bool crossed = (phi[a] >= T(0)) != (phi[b] >= T(0));
     const T pa[3] = {bitx(a) ? L[0] : T(0), bity(a) ? L[1] : T(0), bitz(a) ? L[2] : T(0)};
@@ -351,9 +356,34 @@ __device__ __forceinline__ void edge_point(const T *phi, const T *L, T *P, T *ac
             acc[x] += q[x];
         }
         ncr += 1;
+        cmask |= 1u << E;
     }
+    P[E * MC_CUT_THREADS] = q[edge_axis(E)];
+}
+
+// Point e of a cut cell (0-11 an edge crossing, 12 the interior vertex) from the lane's slots as
+// edge_point and mc_cut_cell left them.  On a crossed edge's two other axes the point is the edge's
+// corner coordinate c (0 or L) plus s * 0 -- c itself for a finite fraction s -- and (q - q) * 0
+// gives the same for every q the slot can hold (NaN or infinite q, from a NaN or infinite s, give
+// NaN there as s * 0 does); an edge not crossed is the zero point.
+template <typename T>
+__device__ __forceinline__ void cut_point(int e, const T *P, const T *L, unsigned crossed, T *v) {
+    if (e == 12) {
 #pragma unroll
-    for (int x = 0; x < 3; ++x) P[(3 * E + x) * MC_CUT_THREADS] = q[x];
+        for (int x = 0; x < 3; ++x) v[x] = P[(12 + x) * MC_CUT_THREADS];
+        return;
+    }
+    if (!((crossed >> e) & 1u)) {
+        v[0] = v[1] = v[2] = T(0);
+        return;
+    }
+    const T q = P[e * MC_CUT_THREADS];
+    const T z = (q - q) * T(0);
+    const int a = (int)((0x321047540310ull >> (4 * e)) & 15u);  // edge_node(e, 0), four bits per edge
+    const int ax = edge_axis(e);
+    v[0] = ax == 0 ? q : (bitx(a) ? L[0] : T(0)) + z;
+    v[1] = ax == 1 ? q : (bity(a) ? L[1] : T(0)) + z;
+    v[2] = ax == 2 ? q : (bitz(a) ? L[2] : T(0)) + z;
 }
 
 // The whole construction for one cut cell: cut_cell_moments (moments.jl:275-346) with
@@ -375,26 +405,25 @@ __device__ __forceinline__ void mc_cut_cell(const T *phi, int lut, const T *xl, 
 
     // ---- edge crossings on cell-local corners ----
     // The tiling codes index the 13 points at run time, so they live in this lane's LDS slots
-    // P[(3 e + x) * MC_CUT_THREADS] (lane-major: consecutive lanes, consecutive banks), not in a
-    // private array (which would go to scratch).
-    auto pts = [&](int e, int x) -> T & { return P[(3 * e + x) * MC_CUT_THREADS]; };
+    // P[slot * MC_CUT_THREADS] (lane-major: consecutive lanes, consecutive banks), not in a private
+    // array (which would go to scratch): MC_CUT_SLOTS values, read back by cut_point.
     T acc[3] = {T(0), T(0), T(0)};
     int ncr = 0;
-    edge_point<0>(phi, L, P, acc, ncr);
-    edge_point<1>(phi, L, P, acc, ncr);
-    edge_point<2>(phi, L, P, acc, ncr);
-    edge_point<3>(phi, L, P, acc, ncr);
-    edge_point<4>(phi, L, P, acc, ncr);
-    edge_point<5>(phi, L, P, acc, ncr);
-    edge_point<6>(phi, L, P, acc, ncr);
-    edge_point<7>(phi, L, P, acc, ncr);
-    edge_point<8>(phi, L, P, acc, ncr);
-    edge_point<9>(phi, L, P, acc, ncr);
-    edge_point<10>(phi, L, P, acc, ncr);
-    edge_point<11>(phi, L, P, acc, ncr);
-    pts(12, 0) = ncr > 0 ? acc[0] / T(ncr) : T(0);
-    pts(12, 1) = ncr > 0 ? acc[1] / T(ncr) : T(0);
-    pts(12, 2) = ncr > 0 ? acc[2] / T(ncr) : T(0);
+    unsigned crossed = 0;
+    edge_point<0>(phi, L, P, acc, ncr, crossed);
+    edge_point<1>(phi, L, P, acc, ncr, crossed);
+    edge_point<2>(phi, L, P, acc, ncr, crossed);
+    edge_point<3>(phi, L, P, acc, ncr, crossed);
+    edge_point<4>(phi, L, P, acc, ncr, crossed);
+    edge_point<5>(phi, L, P, acc, ncr, crossed);
+    edge_point<6>(phi, L, P, acc, ncr, crossed);
+    edge_point<7>(phi, L, P, acc, ncr, crossed);
+    edge_point<8>(phi, L, P, acc, ncr, crossed);
+    edge_point<9>(phi, L, P, acc, ncr, crossed);
+    edge_point<10>(phi, L, P, acc, ncr, crossed);
+    edge_point<11>(phi, L, P, acc, ncr, crossed);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) P[(12 + x) * MC_CUT_THREADS] = ncr > 0 ? acc[x] / T(ncr) : T(0);
 
     // ---- the closed polyhedron, by the divergence theorem ----
     T vol3 = T(0), cm[3] = {T(0), T(0), T(0)};
@@ -414,9 +443,10 @@ __device__ __forceinline__ void mc_cut_cell(const T *phi, int lut, const T *xl, 
     T iarea = T(0), icm[3] = {T(0), T(0), T(0)};
     for (int t = 0; t < ntri; ++t) {
         const int ia = tile[3 * t], ib = tile[3 * t + 1], id = tile[3 * t + 2];
-        const T a[3] = {pts(ia, 0), pts(ia, 1), pts(ia, 2)};
-        const T b[3] = {pts(ib, 0), pts(ib, 1), pts(ib, 2)};
-        const T d[3] = {pts(id, 0), pts(id, 1), pts(id, 2)};
+        T a[3], b[3], d[3];
+        cut_point(ia, P, L, crossed, a);
+        cut_point(ib, P, L, crossed, b);
+        cut_point(id, P, L, crossed, d);
         T u[3], v[3], w[3], xc[3];
 #pragma unroll
         for (int x = 0; x < 3; ++x) { u[x] = b[x] - a[x]; v[x] = d[x] - a[x]; }
@@ -864,7 +894,7 @@ __device__ void mc_march_item(const McBody<T> &B, uint32_t w, MarchLDS<T> &L, T 
 }
 
 // One entry of the cut-cell work list (if it exists): the full construction for that cell.  P is
-// the calling lane's slot in a workgroup array of 39 * MC_CUT_THREADS T in LDS.
+// the calling lane's slot in a workgroup array of MC_CUT_SLOTS * MC_CUT_THREADS T in LDS.
 template <typename T>
 __device__ __forceinline__ void mc_cut_item(const T *__restrict__ phi, const GridT<T> &g,
                                             const Cols<T> &cols, uint32_t c, T *P) {
