@@ -91,7 +91,7 @@ __global__ __launch_bounds__(MC_TILE) void k_batch_a(const BatchBody<T> *__restr
 }
 
 template <typename T>
-__global__ __launch_bounds__(MC_CUT_THREADS) void k_batch_b(const BatchBody<T> *__restrict__ bodies,
+__global__ __launch_bounds__(MC_CUT_THREADS, 3) void k_batch_b(const BatchBody<T> *__restrict__ bodies,
                                                             const uint32_t *__restrict__ begin, int nb,
                                                             uint32_t total) {
     __shared__ T pts[MC_CUT_SLOTS * MC_CUT_THREADS];
