@@ -24,8 +24,8 @@ constexpr int CULL_LDS = 128;
 // and u = 2^-24: the float difference vector is within 3.7 u (|x - o| + |c - o|) of x - c, so
 // s = |xf - cf|^2 (fused multiply-adds) and tp = w + accf + 2^-18 |accf| (accf = fl32(acc)) satisfy
 // tp > (acc + k + r) + 5.9 u (|acc| + k + r) + 4 u (R + |c - o|) > 0 -- and s >= fl32(tp * tp)
-// gives |x - c| > acc + k + r + 3.6 u (|acc| + k + r), far above the skip test's margin of
-// 35 ulp(double) of the same sum; tp <= 0 gives acc + k + r < 0, where the skip test holds too.
+// gives |x - c| > acc + k + r + 3.6 u (|acc| + k + r), far above the skip test's margin (under 40
+// double epsilons of the same sum); tp <= 0 gives acc + k + r < 0, where the skip test holds too.
 // |tp| >= 1e18 or (positive) below 2^-60 (where tp * tp could underflow), infinities, NaNs, negative
 // radii and w outside [1e-18, 1e30] (w = inf) never skip.
 __device__ __forceinline__ float4 screen_record(const double *s, const double *o, double k, double R) {
