@@ -101,6 +101,8 @@ struct cc_cache {
     uint32_t *counter = nullptr;
     double *prims_dev = nullptr;
     int64_t prims_cap = 0;
+    float4 *scr_dev = nullptr;  // a sphere union's screen records for the plain exact fold
+    int64_t scr_cap = 0;
     std::vector<double> prims_host;
     hipStream_t stream = nullptr;
     bool async_pending = false;    // cc_update_async left work on `stream` not yet waited for
@@ -414,6 +416,21 @@ int phi_bricks(cc_cache *c, const cc::Geo &geo, const cc::GridT<T> &gt, T *phi, 
     return CC_OK;
 }
 
+// The screen's margin 1e-6 R for a grid whose nodes all lie within R of the origin (the largest
+// corner norm), rounded up to a float.
+template <typename T> float screen_margin(const cc::GridT<T> &gt) {
+    double R = 0.0;
+    for (int corner = 0; corner < 8; ++corner) {
+        double s = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const double v = (corner >> a) & 1 ? (double)gt.x0[a] + (double)gt.n[a] * (double)gt.d[a] : (double)gt.x0[a];
+            s += v * v;
+        }
+        R = std::fmax(R, std::sqrt(s));
+    }
+    return std::nextafter((float)(1e-6 * R * (1.0 + 1e-9)), INFINITY);
+}
+
 template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void *phi_in, int on_dev,
                                      const cc_grid *g) {
     const cc::GridT<T> gt = grid_t<T>(g, c->dim);
@@ -444,7 +461,16 @@ template <typename T> int run_update(cc_cache *c, const cc::Geo *geo, const void
             const int rc = phi_bricks<T>(c, *geo, gt, phi, fuse ? &bp : nullptr, (int64_t)nseg * nrg * nkb, signs);
             if (rc) return rc;
         } else if (geo && !fuse && c->method == CC_MARCHING_CUBES) {
-            cc::launch_sample<T>(phi, gt, *geo, c->dim, c->stream);
+            cc::Geo gs = *geo;
+            if (sizeof(T) == 8 && c->dim == 3 && (geo->kind & cc::GEO_KIND) == cc::GEO_SPHERE_UNION &&
+                geo->nprims > 1) {  // the double fold screened in single precision
+                const int rc = ensure(&c->scr_dev, &c->scr_cap, (int64_t)geo->nprims);
+                if (rc) return rc;
+                cc::launch_screen_records(geo->prims, geo->nprims, geo->p[0], c->scr_dev, c->stream);
+                gs.scr = c->scr_dev;
+                gs.scr_m = screen_margin(gt);
+            }
+            cc::launch_sample<T>(phi, gt, gs, c->dim, c->stream);
             mark(c, "sample_nodes");
         } else if (phi_in) {
             HIPC(hipMemcpyAsync(phi, phi_in, c->colbytes[CC_FIELD_PHI],
@@ -957,6 +983,7 @@ void free_all(cc_cache *c) {
     if (c->cutlist) (void)hipFree(c->cutlist);
     if (c->counter) (void)hipFree(c->counter);
     if (c->prims_dev) (void)hipFree(c->prims_dev);
+    if (c->scr_dev) (void)hipFree(c->scr_dev);
     if (c->cull_list) (void)hipFree(c->cull_list);
     if (c->cull_cnt) (void)hipFree(c->cull_cnt);
     if (c->nn) (void)hipFree(c->nn);

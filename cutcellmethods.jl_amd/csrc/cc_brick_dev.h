@@ -15,34 +15,8 @@ namespace {
 // blocks per CU.
 constexpr int CULL_LDS = 128;
 
-// A single-precision screen for the double fold.  A primitive leaves the fold unchanged when
-// fold_sphere's own skip test holds (|x - c| >= acc + k + r up to its margin: the smooth minimum
-// then returns acc bit for bit); the screen proves that test true from float data, so a wave whose
-// lanes all pass it skips the primitive's double work.  Per staged primitive (screen_record): the
-// centre minus an origin o, rounded to nearest, and w >= (k + r) + 1e-6 (|k + r| + R + |c - o|),
-// where R bounds |x - o| over every point the block folds.  For a point x, with xf = fl32(x - o)
-// and u = 2^-24: the float difference vector is within 3.7 u (|x - o| + |c - o|) of x - c, so
-// s = |xf - cf|^2 (fused multiply-adds) and tp = w + accf + 2^-18 |accf| (accf = fl32(acc)) satisfy
-// tp > (acc + k + r) + 5.9 u (|acc| + k + r) + 4 u (R + |c - o|) > 0 -- and s >= fl32(tp * tp)
-// gives |x - c| > acc + k + r + 3.6 u (|acc| + k + r), far above the skip test's margin (under 40
-// double epsilons of the same sum); tp <= 0 gives acc + k + r < 0, where the skip test holds too.
-// |tp| >= 1e18 or (positive) below 2^-60 (where tp * tp could underflow), infinities, NaNs, negative
-// radii and w outside [1e-18, 1e30] (w = inf) never skip.
-__device__ __forceinline__ float4 screen_record(const double *s, const double *o, double k, double R) {
-    const double cx = s[0] - o[0], cy = s[1] - o[1], cz = s[2] - o[2];
-    const double kr = k + s[3];
-    double w = kr + 1e-6 * ((fabs(kr) + R) + sqrt((cx * cx + cy * cy) + cz * cz));
-    if (!(s[3] >= 0.0) || !(w < 1e30) || !(w >= 1e-18)) w = INFINITY;
-    return make_float4((float)cx, (float)cy, (float)cz, (float)w);
-}
-
-__device__ __forceinline__ bool screen_skips(const float4 &q, float xf, float yf, float zf, float accf) {
-    const float dx = xf - q.x, dy = yf - q.y, dz = zf - q.z;
-    const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    const float tp = fmaf(fabsf(accf), 0x1p-18f, accf + q.w);
-    // evaluated whole (no short circuit): three compares and two mask operations, no branches
-    return (fabsf(tp) < 1e18f) & ((tp <= 0.0f) | ((tp >= 0x1p-60f) & (s >= tp * tp)));
-}
+// The brick pass screens its fold with screen_record / screen_skips (cc_device.h), the records
+// about the brick's (or the work item's) first node.
 
 // The sphere-union fold over a brick's kept primitives, in the order and with the arithmetic of
 // union_dist (cc_device.h, fold_sphere).  Primitive 0 is always kept.  sph: the first

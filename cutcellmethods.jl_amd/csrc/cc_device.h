@@ -28,6 +28,10 @@ struct Geo {
     int nprims;
     double p[24];
     const double *prims;  // device pointer: nprims x (cx, cy, cz, r), or nprims x UNION_REC
+    // a 3D sphere union's float screen records (screen_record about the origin), or null, and the
+    // margin 1e-6 R (rounded up) for |x| <= R over the points evaluated
+    const float4 *scr = nullptr;
+    float scr_m = 0.0f;
 };
 
 // The grid as the kernels see it, converted to the cache's element type once on the host
@@ -100,6 +104,37 @@ __device__ __forceinline__ void from_local(const double *r, const T *gl, T *n) {
     }
 }
 
+// A single-precision screen for the double fold.  A primitive leaves the fold unchanged when
+// fold_sphere's own skip test holds (|x - c| >= acc + k + r up to its margin: the smooth minimum
+// then returns acc bit for bit); the screen proves that test true from float data, so a wave whose
+// lanes all pass it skips the primitive's double work.  Per primitive (screen_record): the centre
+// minus an origin o, rounded to nearest, and w >= (k + r) + 1e-6 (|k + r| + R + |c - o|), where R
+// bounds |x - o| over every point folded (R may instead come as the separate margin m >= 1e-6 R).
+// For a point x, with xf = fl32(x - o) and u = 2^-24: the float difference vector is within
+// 3.7 u (|x - o| + |c - o|) of x - c, so s = |xf - cf|^2 (fused multiply-adds) and
+// tp = w + m + accf + 2^-18 |accf| (accf = fl32(acc)) satisfy
+// tp > (acc + k + r) + 5.9 u (|acc| + k + r) + 4 u (R + |c - o|) > 0 -- and s >= fl32(tp * tp)
+// gives |x - c| > acc + k + r + 3.6 u (|acc| + k + r), far above the skip test's margin (under 40
+// double epsilons of the same sum); tp <= 0 gives acc + k + r < 0, where the skip test holds too.
+// |tp| >= 1e18 or (positive) below 2^-60 (where tp * tp could underflow), infinities, NaNs, negative
+// radii and w outside [1e-18, 1e30] (w = inf) never skip.
+__device__ __forceinline__ float4 screen_record(const double *s, const double *o, double k, double R) {
+    const double cx = s[0] - o[0], cy = s[1] - o[1], cz = s[2] - o[2];
+    const double kr = k + s[3];
+    double w = kr + 1e-6 * ((fabs(kr) + R) + sqrt((cx * cx + cy * cy) + cz * cz));
+    if (!(s[3] >= 0.0) || !(w < 1e30) || !(w >= 1e-18)) w = INFINITY;
+    return make_float4((float)cx, (float)cy, (float)cz, (float)w);
+}
+
+__device__ __forceinline__ bool screen_skips(const float4 &q, float xf, float yf, float zf, float accf,
+                                             float m = 0.0f) {
+    const float dx = xf - q.x, dy = yf - q.y, dz = zf - q.z;
+    const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float tp = fmaf(fabsf(accf), 0x1p-18f, accf + q.w) + m;
+    // evaluated whole (no short circuit): three compares and two mask operations, no branches
+    return (fabsf(tp) < 1e18f) & ((tp <= 0.0f) | ((tp >= 0x1p-60f) & (s >= tp * tp)));
+}
+
 // One step acc <- smin_poly(acc, |x - c| - r, k) of the sphere-union fold.  s is the squared norm
 // sphere_dist forms.  When sqrt(s) >= acc + k + r plus a rounding margin (or acc + k + r is below
 // minus the margin, so that even the centre is k above acc), the computed distance d satisfies
@@ -124,6 +159,17 @@ __device__ __forceinline__ T union_dist(const Geo &g, const T *x) {
     const double *s = g.prims;
     const T k = T(g.p[0]);
     T acc = sphere_dist<T, D>(T(s[0]), T(s[1]), T(s[2]), T(s[3]), x[0], x[1], z);
+    if (sizeof(T) == 8 && D == 3 && g.scr) {  // screened: a primitive not skipped folds as the oracle's
+        const float xf = (float)x[0], yf = (float)x[1], zf = (float)z;
+        float accf = (float)acc;
+        for (int i = 1; i < g.nprims; ++i) {
+            if (screen_skips(g.scr[i], xf, yf, zf, accf, g.scr_m)) continue;
+            const double *si = s + 4 * i;
+            acc = smin_poly(acc, sphere_dist<T, D>(T(si[0]), T(si[1]), T(si[2]), T(si[3]), x[0], x[1], z), k);
+            accf = (float)acc;
+        }
+        return acc;
+    }
     for (int i = 1; i < g.nprims; ++i) {
         const double *si = s + 4 * i;
         acc = fold_sphere<T, D>(acc, T(si[0]), T(si[1]), T(si[2]), T(si[3]), x[0], x[1], z, k);

@@ -45,6 +45,9 @@ constexpr int MC_CUT_SLOTS = 15;
 
 template <typename T>
 void launch_sample(T *phi, const GridT<T> &g, const Geo &geo, int dim, hipStream_t s);
+// A 3D sphere union's screen records about the origin (screen_record with R = 0): out[i] for the n
+// primitives at prims, smoothing k.
+void launch_screen_records(const double *prims, int n, double k, float4 *out, hipStream_t s);
 
 // Variants of the marching-cubes stream pass (picked per cache; the autotuner sweeps them).
 enum McVariant { MC_FLAT = 0, MC_MARCH = 1, MC_MARCH_FUSED = 2 };

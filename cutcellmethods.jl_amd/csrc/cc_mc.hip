@@ -101,6 +101,13 @@ __global__ __launch_bounds__(256) void k_sample(T *__restrict__ phi, GridT<T> g,
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0; }
 
+__global__ __launch_bounds__(256) void k_screen_records(const double *__restrict__ prims, int n, double k,
+                                                        float4 *__restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const double o[3] = {0.0, 0.0, 0.0};
+    if (i < n) out[i] = screen_record(prims + 4 * (size_t)i, o, k, 0.0);
+}
+
 }  // namespace
 
 template <typename T>
@@ -207,6 +214,10 @@ void mc_brick_tile_shape(const int *n, uint32_t kz, uint32_t *W, uint32_t *R, ui
 
 void launch_zero_u32(uint32_t *p, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, p);
+}
+
+void launch_screen_records(const double *prims, int n, double k, float4 *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_screen_records, dim3((n + 255) / 256), dim3(256), 0, s, prims, n, k, out);
 }
 
 template void launch_sample<double>(double *, const GridT<double> &, const Geo &, int, hipStream_t);

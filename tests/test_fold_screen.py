@@ -1,4 +1,4 @@
-"""The single-precision screen of the brick pass's double fold (cc_brick_dev.h screen_record and
+"""The single-precision screen of the sphere-union double fold (cc_device.h screen_record and
 screen_skips), restated in numpy, against the double skip test it must imply (cc_device.h
 fold_sphere): wherever the screen skips a primitive, fold_sphere's own test skips it too, so the
 fold is bitwise the oracle's.  The samples sit on both sides of the skip boundary, closer to it than
