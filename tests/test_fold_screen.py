@@ -27,10 +27,13 @@ def fma32(a, b, c):
     return (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(f32)
 
 
-def screen_skips(acc, c, r, x, k, o, R):
+def screen_skips(acc, c, r, x, k, o, R, split=False):
+    """split: R left out of the record and added as the separate margin m = fl32up(1e-6 R), as the
+    plain fold of the whole grid does (records about the origin, R the grid's largest corner norm)."""
     cx = c - o
     kr = k + r
-    w = kr + 1e-6 * ((np.abs(kr) + R) + np.sqrt((cx[:, 0] * cx[:, 0] + cx[:, 1] * cx[:, 1]) + cx[:, 2] * cx[:, 2]))
+    Rw = 0.0 if split else R
+    w = kr + 1e-6 * ((np.abs(kr) + Rw) + np.sqrt((cx[:, 0] * cx[:, 0] + cx[:, 1] * cx[:, 1]) + cx[:, 2] * cx[:, 2]))
     with np.errstate(invalid="ignore"):
         bad = ~(r >= 0.0) | ~(w < 1e30) | ~(w >= 1e-18)
     w = np.where(bad, np.inf, w)
@@ -42,6 +45,9 @@ def screen_skips(acc, c, r, x, k, o, R):
         d = xf - q
         s = fma32(d[:, 2], d[:, 2], fma32(d[:, 1], d[:, 1], d[:, 0] * d[:, 0]))
         tp = fma32(np.abs(accf), np.full_like(accf, 2.0 ** -18), accf + qw)
+        if split:
+            m = np.nextafter((1e-6 * np.asarray(R) * (1.0 + 1e-9)).astype(f32), f32(np.inf))
+            tp = tp + m
         return (np.abs(tp) < f32(1e18)) & ((tp <= f32(0)) | ((tp >= f32(2.0 ** -60)) & (s >= tp * tp)))
 
 
@@ -66,9 +72,12 @@ def samples(rng, n, scale):
 
 def test_the_screen_implies_the_double_skip_test():
     rng = np.random.default_rng(7)
-    for scale in (0.5, 2.0, 1e3):
+    for scale, split in ((0.5, False), (2.0, False), (1e3, False), (2.0, True), (1e3, True)):
         acc, c, r, x, k, o, R = samples(rng, 400_000, scale)
-        sk = screen_skips(acc, c, r, x, k, o, R)
+        if split:  # records about the origin: R bounds |x|
+            R = np.linalg.norm(x, axis=1) * (1.0 + 1e-12)
+            o = np.zeros_like(o)
+        sk = screen_skips(acc, c, r, x, k, o, R, split)
         ok = fp64_skip(acc, c, r, x, k)
         assert not np.any(sk & ~ok), (scale, int(np.count_nonzero(sk & ~ok)))
         # nothing inside the screen's margin of the boundary is skipped, and it is not vacuous:
