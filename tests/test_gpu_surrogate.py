@@ -132,6 +132,31 @@ def test_culled_fold_is_bitwise_the_plain_fold(dtype):
         assert np.array_equal(culled.phi.view(np.uint64), np.asarray(ref).view(np.uint64))
 
 
+def test_screened_folds_far_from_the_origin_and_with_degenerate_spheres():
+    # the plain and culled folds screen primitives in single precision with margins that grow with
+    # the coordinates' magnitude: a body around (1000, -500, 250), spheres of radius 0 and one of
+    # negative radius (never screened), and a tiny smoothing; phi and every column bitwise the oracle
+    rng = np.random.default_rng(17)
+    c0 = np.array([1000.0, -500.0, 250.0])
+    cen = c0 + rng.uniform(-0.5, 0.5, (40, 3))
+    rad = rng.uniform(0.02, 0.2, 40)
+    rad[3] = 0.0
+    rad[7] = -0.01
+    body = SDFSphereUnion(cen, rad, 1e-3)
+    g = CartesianIsoGrid(tuple(c0 - 0.7), tuple(c0 + 0.7), n=(36, 40, 33))
+    ref = orc.sample_nodes(body, 3, g.x0, g.d, g.n)
+    cols = orc.mc_cells(ref, g.x0, g.d, g.n)
+    for source in ("naive", "culled"):
+        c = allocate_cache(g, MarchingCubesCutCell)
+        c.set_phi_source(source)
+        update_cache(c, body, g)
+        assert np.array_equal(c.phi.view(np.uint64), np.asarray(ref).view(np.uint64)), source
+        for name in COLS:
+            got = np.ascontiguousarray(c.read(name)).view(np.uint8).ravel()
+            want = np.ascontiguousarray(cols[name]).view(np.uint8).ravel()
+            assert np.array_equal(got, want), (source, name)
+
+
 def cut_corner_nodes(kind, n):
     """Node linear indices of every corner of every cut cell."""
     n0, n1, n2 = n
